@@ -1,0 +1,130 @@
+"""Synthetic chunk-pair workloads (numpy PCG64, seeded).
+
+Workloads, named by what they are:
+  * ``single_pair``      one 4096-bp target x 4096-bp query, GC 0.41, one 500-bp
+                         homologous segment at 85% identity (uniform substitutions,
+                         no indels), forward strand only.
+  * ``uniform_pairs``    P chunk pairs of 4096 bp, GC 0.41; even-indexed pairs carry
+                         one planted homology (length U{50..2000}, identity
+                         U(0.70, 0.95), 1% per-base indel starts with geometric
+                         lengths of mean 2); both strands.
+  * ``uneven_pairs``     P pairs with query and target lengths log-uniform in
+                         [256, 16384], GC U(0.35, 0.55) per pair, planted as above,
+                         10% with a tandem repeat (period 1-6, length 100-1000).
+
+Base probabilities at GC = g: P(C) = P(G) = g/2, P(A) = P(T) = (1 - g)/2.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+ALPHA = np.frombuffer(b"ACGT", dtype=np.uint8)
+
+
+def random_seq(rng: np.random.Generator, n: int, gc: float) -> np.ndarray:
+    p = np.array([(1 - gc) / 2, gc / 2, gc / 2, (1 - gc) / 2])
+    return ALPHA[rng.choice(4, size=n, p=p)]
+
+
+def mutate(rng: np.random.Generator, seg: np.ndarray, ident: float, indel_rate: float = 0.0) -> np.ndarray:
+    """Substitute each base with probability 1-ident (uniformly among the other 3);
+    with indel_rate > 0, start an insertion or deletion (50/50, geometric length,
+    p = 0.5) after a base with that probability."""
+    seg = seg.copy()
+    sub = rng.random(seg.size) < (1.0 - ident)
+    if sub.any():
+        cur = np.searchsorted(ALPHA, seg[sub])
+        shift = rng.integers(1, 4, size=cur.size)
+        seg[sub] = ALPHA[(cur + shift) % 4]
+    if indel_rate <= 0:
+        return seg
+    out = []
+    i = 0
+    n = seg.size
+    while i < n:
+        out.append(seg[i])
+        i += 1
+        if rng.random() < indel_rate:
+            ln = int(rng.geometric(0.5))
+            if rng.random() < 0.5:
+                out.extend(ALPHA[rng.integers(0, 4, size=ln)].tolist())
+            else:
+                i += ln
+    return np.array(out, dtype=np.uint8)
+
+
+def single_pair(seed: int = 0, length: int = 4096, seg_len: int = 500, ident: float = 0.85, gc: float = 0.41):
+    rng = np.random.default_rng(seed)
+    q = random_seq(rng, length, gc)
+    t = random_seq(rng, length, gc)
+    qo = int(rng.integers(0, length - seg_len + 1))
+    to = int(rng.integers(0, length - seg_len + 1))
+    t[to: to + seg_len] = mutate(rng, q[qo: qo + seg_len], ident)
+    return t.tobytes(), q.tobytes(), {"qo": qo, "to": to}
+
+
+def _plant(rng, q: np.ndarray, t: np.ndarray):
+    L = int(rng.integers(50, 2001))
+    ident = float(rng.uniform(0.70, 0.95))
+    L = min(L, q.size, t.size)
+    qo = int(rng.integers(0, q.size - L + 1))
+    seg = mutate(rng, q[qo: qo + L], ident, indel_rate=0.01)
+    seg = seg[: t.size]
+    to = int(rng.integers(0, t.size - seg.size + 1))
+    t[to: to + seg.size] = seg
+    return {"L": L, "ident": ident, "qo": qo, "to": to}
+
+
+def uniform_pairs(n_pairs: int, seed: int = 1, length: int = 4096, gc: float = 0.41):
+    """Returns (targets, queries) lists of bytes, each entry one 4096-bp record."""
+    rng = np.random.default_rng(seed)
+    T, Q = [], []
+    for k in range(n_pairs):
+        q = random_seq(rng, length, gc)
+        t = random_seq(rng, length, gc)
+        if k % 2 == 0:
+            _plant(rng, q, t)
+        T.append(t.tobytes())
+        Q.append(q.tobytes())
+    return T, Q
+
+
+def uneven_pairs(n_pairs: int, seed: int = 2, lo: int = 256, hi: int = 16384):
+    rng = np.random.default_rng(seed)
+    T, Q = [], []
+    for k in range(n_pairs):
+        lq = int(np.exp(rng.uniform(np.log(lo), np.log(hi))))
+        lt = int(np.exp(rng.uniform(np.log(lo), np.log(hi))))
+        gc = float(rng.uniform(0.35, 0.55))
+        q = random_seq(rng, lq, gc)
+        t = random_seq(rng, lt, gc)
+        if k % 2 == 0:
+            _plant(rng, q, t)
+        if rng.random() < 0.10:
+            period = int(rng.integers(1, 7))
+            rl = int(rng.integers(100, 1001))
+            unit = random_seq(rng, period, gc)
+            rep = np.resize(unit, rl)
+            tgt = q if rng.random() < 0.5 else t
+            rl = min(rl, tgt.size)
+            off = int(rng.integers(0, tgt.size - rl + 1))
+            tgt[off: off + rl] = rep[:rl]
+        T.append(t.tobytes())
+        Q.append(q.tobytes())
+    return T, Q
+
+
+def fft_len(q_len: int, t_len: int) -> int:
+    n = 1
+    while n < q_len + t_len:
+        n *= 2
+    return n
+
+
+def write_fasta(path: str, records, names=None, width: int = 60) -> None:
+    with open(path, "wb") as f:
+        for i, s in enumerate(records):
+            name = names[i] if names else f"r{i}"
+            f.write(b">" + name.encode() + b"\n")
+            for k in range(0, len(s), width):
+                f.write(s[k: k + width] + b"\n")

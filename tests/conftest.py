@@ -1,0 +1,62 @@
+import os
+import sys
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (runs through the HIP C-ABI library)")
+
+
+@pytest.fixture(scope="session")
+def golden():
+    def load(name):
+        return np.load(os.path.join(GOLDEN, name), allow_pickle=False)
+
+    return load
+
+
+CODE = np.frombuffer(b"ACGT", np.uint8)
+
+
+def unpack2(packed: np.ndarray, n: int) -> bytes:
+    """Inverse of tests/golden/make_golden.py:pack2."""
+    p = np.asarray(packed, np.uint8)
+    c = np.stack([(p >> s) & 3 for s in (0, 2, 4, 6)], 1).reshape(-1)[:n]
+    return CODE[c].tobytes()
+
+
+def lcg_noise(n: int, seed: int) -> np.ndarray:
+    out = np.empty(n, np.float32)
+    x = seed & 0xFFFFFFFFFFFFFFFF
+    for i in range(n):
+        x = (x * 6364136223846793005 + 1442695040888963407) & 0xFFFFFFFFFFFFFFFF
+        out[i] = np.float32((x >> 40) / float(1 << 24) - 0.5)
+    return out
+
+
+def chunk_records(records, size, overlap):
+    """ChunkManager::ChunkItSelect (analysis/SeqChunk.cc:72-165) on uppercased records:
+    returns list of (record_id, start, bytes) with empty / all-N chunks cleared."""
+    out = []
+    for rid, s in enumerate(records):
+        L = len(s)
+        if L < 6:
+            continue
+        step = size - overlap
+        for j in range(1 + L // step):
+            start = j * step
+            end = min((j + 1) * step + overlap, L)
+            seg = s[start:end]
+            nn = seg.count(b"N") + seg.count(b"X")
+            if nn >= len(seg):
+                seg = b""
+            out.append((rid, start, seg))
+    return out
