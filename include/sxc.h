@@ -1,0 +1,147 @@
+/*
+ * sxc.h -- C ABI of the MI355X chunk-pair cross-correlation library (libsxc_hip.so).
+ *
+ * This is the drop-in boundary for Satsuma2's cross-correlation path: each entry point
+ * replaces one reference interface (cited below) and computes the same results on the
+ * GPU.  Plain pointers and sizes only; all host buffers are caller-owned; device memory
+ * is owned by the context.  Every function returns SXC_OK (0) or a negative status;
+ * sxc_last_error() describes the last failure of the calling thread.
+ *
+ * Sequences are byte strings of uppercase ASCII letters (< 128), as the reference holds
+ * them after vecDNAVector::Read (analysis/DNAVector.cc:1176-1293, ToUpper at :586).
+ * FFT lengths N are powers of two in [512, 16384] (2 * t_chunk in the slave,
+ * analysis/HomologyByXCorrSlave.cc:259); sequence lengths must not exceed N.
+ */
+#ifndef SXC_H
+#define SXC_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  SXC_OK = 0,
+  SXC_ERR_ARG = -1,       /* bad argument (size, N, byte >= 128, null pointer) */
+  SXC_ERR_HIP = -2,       /* a HIP runtime call failed */
+  SXC_ERR_CAPACITY = -3,  /* output capacity too small; *n_out holds the required count */
+  SXC_ERR_NOMEM = -4
+};
+
+typedef struct sxc_ctx sxc_ctx;
+
+/* One diagonal run found by the scan: SeqMatch (analysis/CrossCorr.h:165-196). */
+typedef struct {
+  int32_t start_target;
+  int32_t start_query;
+  int32_t len;
+} sxc_seqmatch;
+
+/* t_result (analysis/WorkQueue.h:23-33), byte-compatible on LP64 (72 bytes). */
+typedef struct {
+  uint64_t query_id;
+  uint64_t target_id;
+  uint64_t query_size;
+  uint64_t qstart;
+  uint64_t tstart;
+  uint64_t len;
+  uint8_t reverse;
+  uint8_t pad_[7];
+  double prob;
+  double ident;
+} sxc_result;
+
+/* Slave parameters (analysis/HomologyByXCorrSlave.cc:334-384 flags). */
+typedef struct {
+  int32_t N;            /* FFT length = 2 * t_chunk */
+  int32_t q_chunk;      /* -q_chunk (RCQuery) */
+  int32_t min_len;      /* -l */
+  int32_t strands;      /* 2 = forward then reverse complement (Align), 1 = forward only */
+  double cutoff;        /* -cutoff */
+  double cutoff_fast;   /* -cutoff_fast */
+  double min_prob;      /* keep threshold: 0.99 on the slave path (m_minProb, :76) */
+  double target_total;  /* sum of target record lengths (:405-408) */
+} sxc_params;
+
+/* One chunk pair: chunk bytes and the chunk metadata FilterMatches uses (:168-219). */
+typedef struct {
+  const char* tseq;
+  int32_t tlen;
+  const char* qseq;
+  int32_t qlen;
+  int32_t t_chunk_start; /* SeqChunk::GetStart of the target chunk */
+  int32_t q_chunk_start; /* SeqChunk::GetStart of the query chunk */
+  int32_t t_record;      /* t_result.target_id */
+  int32_t q_record;      /* t_result.query_id */
+  int32_t q_record_len;  /* t_result.query_size; RCQuery's qLen */
+  int32_t fast;          /* t_pair.fast: use cutoff_fast */
+} sxc_pair;
+
+/* Context on HIP device `device`; builds FFT twiddle, codec and entropy tables. */
+int sxc_create(sxc_ctx** out, int device);
+void sxc_destroy(sxc_ctx* ctx);
+const char* sxc_last_error(void);
+const char* sxc_version(void);
+
+/* CCSignal::SetSequence(seq, N) (analysis/CrossCorr.cc:179-206): out = 4*N floats, channels A, C, G, T. */
+int sxc_encode_signal(sxc_ctx* ctx, const char* seq, int32_t len, int32_t N, float* out);
+
+/* FFTReal<float>::do_fft (extern/RealFFT/FFTReal.hpp:146) on `batch` signals of N floats;
+ * spectra in FFTReal's packed layout. */
+int sxc_fft_forward(sxc_ctx* ctx, const float* in, float* out, int32_t N, int32_t batch);
+
+/* CrossCorrelation::CrossCorrelate(out, target, query) (analysis/CrossCorr.cc:386-403):
+ * tsig and qsig are 4*N encoded signals; xc receives N floats. */
+int sxc_xcorr(sxc_ctx* ctx, const float* tsig, const float* qsig, int32_t N, float* xc);
+
+/* SeqAnalyzer::MatchUp(out, query, target, xc) (analysis/CrossCorr.cc:583-605) with
+ * SetTopCutoff(cutoff): runs of every accepted lag in reference order.  *n_out receives
+ * the number found; SXC_ERR_CAPACITY if it exceeds cap. */
+int sxc_matchup(sxc_ctx* ctx, const char* q, int32_t lq, const char* t, int32_t lt, const float* xc, int32_t N,
+                double cutoff, sxc_seqmatch* out, int32_t cap, int32_t* n_out);
+
+/* GetMatchProbability(target, query, sT, sQ, len, targetSize) (analysis/AlignProbability.cc:42-127)
+ * and PrintMatch's identity (analysis/CrossCorr.cc:967-1037) for n matches. */
+int sxc_match_probability(sxc_ctx* ctx, const char* t, int32_t lt, const char* q, int32_t lq,
+                          const sxc_seqmatch* m, int32_t n, double target_size, double* prob, double* ident);
+
+/* HomologyByXCorr::Align + FilterMatches for one chunk pair (analysis/HomologyByXCorrSlave.cc:
+ * 221-253, 168-219): the plain drop-in, one pair per call.  Results in reference order
+ * (forward strand, then reverse; lags ascending; runs ascending). */
+int sxc_align_pair(sxc_ctx* ctx, const sxc_pair* pair, const sxc_params* prm, sxc_result* out, int32_t cap,
+                   int32_t* n_out);
+
+/* Many chunk pairs in one fused launch (align_target over a block of t_pairs,
+ * :270-300).  Results grouped by pair in input order, each pair as sxc_align_pair. */
+int sxc_align_batch(sxc_ctx* ctx, const sxc_pair* pairs, int32_t n, const sxc_params* prm, sxc_result* out,
+                    int64_t cap, int64_t* n_out);
+
+/* Resident batch: the inputs of sxc_align_batch uploaded once to HBM, so the device work
+ * can be run (and timed) repeatedly without host transfers.  Same results as
+ * sxc_align_batch. */
+typedef struct sxc_batch sxc_batch;
+int sxc_batch_create(sxc_ctx* ctx, const sxc_pair* pairs, int32_t n, const sxc_params* prm, sxc_batch** out);
+int sxc_batch_run(sxc_batch* batch);
+int sxc_batch_results(sxc_batch* batch, sxc_result* out, int64_t cap, int64_t* n_out);
+/* Accepted lags (FindTop) and diagonal runs (DoOne) of the last run, summed over the batch. */
+int sxc_batch_stats(sxc_batch* batch, int64_t* candidates, int64_t* runs);
+void sxc_batch_destroy(sxc_batch* batch);
+
+/* Device time of the last sxc_align_batch / sxc_align_pair / sxc_batch_run, by phase (milliseconds). */
+typedef struct {
+  double total_ms;
+  double upload_ms;
+  double spectra_ms;
+  double pair_ms;
+  double finalize_ms;
+  double download_ms;
+  int64_t candidates;
+  int64_t runs;
+} sxc_timing;
+int sxc_last_timing(sxc_ctx* ctx, sxc_timing* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,303 @@
+"""Python mirror of the reference's chunk-pair interfaces, computed on the GPU.
+
+Names and argument meaning follow the reference classes so code and tests read like
+the reference's own:
+
+  CCSignal.SetSequence(seq, size)          analysis/CrossCorr.h:16, CrossCorr.cc:179-206
+  CrossCorrelation.CrossCorrelate(t, q)    analysis/CrossCorr.h:151, CrossCorr.cc:386-403
+  SeqAnalyzer.MatchUp(query, target, xc)   analysis/CrossCorr.h:261, CrossCorr.cc:583-605
+  GetMatchProbability(t, q, sT, sQ, len, targetSize)   analysis/AlignProbability.h:10-15
+  PrintMatch(q, t, m) -> identity          analysis/CrossCorr.h:293, CrossCorr.cc:967-1037
+  HomologyByXCorr.Align / align_target     analysis/HomologyByXCorrSlave.cc:221-300
+
+Every call goes through libsxc_hip.so; a missing library or device raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+from typing import List, Sequence
+
+import numpy as np
+
+from . import _lib
+from ._lib import RESULT_DTYPE, Pair, Params, Result, SeqMatch, Timing, check
+
+
+def _fp(a: np.ndarray):
+    assert a.dtype == np.float32 and a.flags.c_contiguous
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def _b(s) -> bytes:
+    return s if isinstance(s, bytes) else bytes(s)
+
+
+class Context:
+    """One HIP device, its stream and constant tables (sxc_create)."""
+
+    def __init__(self, device: int = 0):
+        self.L = _lib.load()
+        self._h = C.c_void_p()
+        check(self.L.sxc_create(C.byref(self._h), device))
+
+    def close(self):
+        if self._h:
+            self.L.sxc_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def handle(self):
+        return self._h
+
+    # ---- plain entry points (one reference call each)
+    def encode(self, seq: bytes, N: int) -> np.ndarray:
+        out = np.zeros((4, N), np.float32)
+        check(self.L.sxc_encode_signal(self._h, _b(seq), len(seq), N, _fp(out)))
+        return out
+
+    def fft_forward(self, x: np.ndarray) -> np.ndarray:
+        x = np.ascontiguousarray(np.atleast_2d(x), np.float32)
+        out = np.zeros_like(x)
+        check(self.L.sxc_fft_forward(self._h, _fp(x), _fp(out), x.shape[1], x.shape[0]))
+        return out
+
+    def xcorr(self, tsig: np.ndarray, qsig: np.ndarray) -> np.ndarray:
+        tsig = np.ascontiguousarray(tsig, np.float32)
+        qsig = np.ascontiguousarray(qsig, np.float32)
+        N = tsig.shape[1]
+        xc = np.zeros(N, np.float32)
+        check(self.L.sxc_xcorr(self._h, _fp(tsig), _fp(qsig), N, _fp(xc)))
+        return xc
+
+    def matchup(self, q: bytes, t: bytes, xc: np.ndarray, cutoff: float) -> np.ndarray:
+        xc = np.ascontiguousarray(xc, np.float32)
+        cap = 1 << 14
+        while True:
+            buf = (SeqMatch * cap)()
+            n = C.c_int32(0)
+            rc = self.L.sxc_matchup(self._h, _b(q), len(q), _b(t), len(t), _fp(xc), xc.size, cutoff, buf, cap,
+                                    C.byref(n))
+            if rc == _lib.SXC_ERR_CAPACITY and n.value > cap:
+                cap = n.value
+                continue
+            check(rc)
+            break
+        return np.frombuffer(buf, np.int32, 3 * n.value).reshape(-1, 3).copy()
+
+    def match_probability(self, t: bytes, q: bytes, runs: np.ndarray, target_size: float):
+        runs = np.ascontiguousarray(runs, np.int32).reshape(-1, 3)
+        n = runs.shape[0]
+        prob = np.zeros(n, np.float64)
+        ident = np.zeros(n, np.float64)
+        check(self.L.sxc_match_probability(self._h, _b(t), len(t), _b(q), len(q),
+                                           runs.ctypes.data_as(C.POINTER(SeqMatch)), n, target_size,
+                                           prob.ctypes.data_as(C.POINTER(C.c_double)),
+                                           ident.ctypes.data_as(C.POINTER(C.c_double))))
+        return prob, ident
+
+    def timing(self) -> Timing:
+        t = Timing()
+        check(self.L.sxc_last_timing(self._h, C.byref(t)))
+        return t
+
+
+@dataclass
+class SlaveParams:
+    """HomologyByXCorrSlave flags (analysis/HomologyByXCorrSlave.cc:334-384)."""
+
+    t_chunk: int = 4096
+    q_chunk: int = 4096
+    min_len: int = 0
+    cutoff: float = 1.8
+    cutoff_fast: float = 2.9
+    min_prob: float = 0.99  # m_minProb is fixed at 0.99 on the slave path (:76)
+    strands: int = 2
+    target_total: float = 0.0
+
+    def to_c(self) -> Params:
+        return Params(2 * self.t_chunk, self.q_chunk, self.min_len, self.strands, self.cutoff, self.cutoff_fast,
+                      self.min_prob, self.target_total)
+
+
+@dataclass
+class ChunkPair:
+    tseq: bytes
+    qseq: bytes
+    t_chunk_start: int = 0
+    q_chunk_start: int = 0
+    t_record: int = 0
+    q_record: int = 0
+    q_record_len: int = -1
+    fast: int = 0
+
+
+def _pairs_array(pairs: Sequence[ChunkPair]):
+    arr = (Pair * max(1, len(pairs)))()
+    keep = []
+    for k, p in enumerate(pairs):
+        keep += [p.tseq, p.qseq]
+        qrl = p.q_record_len if p.q_record_len >= 0 else len(p.qseq)
+        arr[k] = Pair(p.tseq, len(p.tseq), p.qseq, len(p.qseq), p.t_chunk_start, p.q_chunk_start, p.t_record,
+                      p.q_record, qrl, p.fast)
+    return arr, keep
+
+
+class HomologyByXCorr:
+    """The slave's per-pair work (Align + FilterMatches) as plain, batched and resident calls."""
+
+    def __init__(self, ctx: Context, params: SlaveParams):
+        self.ctx = ctx
+        self.params = params
+
+    def align_pair(self, pair: ChunkPair) -> np.ndarray:
+        arr, keep = _pairs_array([pair])
+        prm = self.params.to_c()
+        cap = 1024
+        while True:
+            out = np.zeros(cap, RESULT_DTYPE)
+            n = C.c_int32(0)
+            rc = self.ctx.L.sxc_align_pair(self.ctx.handle, arr, C.byref(prm), out.ctypes.data_as(C.POINTER(Result)),
+                                           cap, C.byref(n))
+            if rc == _lib.SXC_ERR_CAPACITY and n.value > cap:
+                cap = n.value
+                continue
+            check(rc)
+            return out[: n.value].copy()
+
+    def align_batch(self, pairs: Sequence[ChunkPair]) -> np.ndarray:
+        arr, keep = _pairs_array(pairs)
+        prm = self.params.to_c()
+        cap = max(1024, 4 * len(pairs))
+        while True:
+            out = np.zeros(cap, RESULT_DTYPE)
+            n = C.c_int64(0)
+            rc = self.ctx.L.sxc_align_batch(self.ctx.handle, arr, len(pairs), C.byref(prm),
+                                            out.ctypes.data_as(C.POINTER(Result)), cap, C.byref(n))
+            if rc == _lib.SXC_ERR_CAPACITY and n.value > cap:
+                cap = n.value
+                continue
+            check(rc)
+            return out[: n.value].copy()
+
+
+class ResidentBatch:
+    """Chunk pairs uploaded once to HBM (sxc_batch_create); run() repeats the device work."""
+
+    def __init__(self, ctx: Context, pairs: Sequence[ChunkPair], params: SlaveParams):
+        self.ctx = ctx
+        self._arr, self._keep = _pairs_array(pairs)
+        self._prm = params.to_c()
+        self._h = C.c_void_p()
+        check(ctx.L.sxc_batch_create(ctx.handle, self._arr, len(pairs), C.byref(self._prm), C.byref(self._h)))
+
+    def run(self):
+        check(self.ctx.L.sxc_batch_run(self._h))
+
+    def results(self) -> np.ndarray:
+        cap = 1 << 12
+        while True:
+            out = np.zeros(cap, RESULT_DTYPE)
+            n = C.c_int64(0)
+            rc = self.ctx.L.sxc_batch_results(self._h, out.ctypes.data_as(C.POINTER(Result)), cap, C.byref(n))
+            if rc == _lib.SXC_ERR_CAPACITY and n.value > cap:
+                cap = n.value
+                continue
+            check(rc)
+            return out[: n.value].copy()
+
+    def stats(self):
+        c, r = C.c_int64(0), C.c_int64(0)
+        check(self.ctx.L.sxc_batch_stats(self._h, C.byref(c), C.byref(r)))
+        return c.value, r.value
+
+    def close(self):
+        if self._h:
+            self.ctx.L.sxc_batch_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# ---------------------------------------------------------------- reference-named mirror
+
+_default_ctx = None
+
+
+def default_context() -> Context:
+    global _default_ctx
+    if _default_ctx is None:
+        _default_ctx = Context(0)
+    return _default_ctx
+
+
+class CCSignal:
+    """CCSignal (analysis/CrossCorr.h:10-71): four float channels A, C, G, T."""
+
+    def __init__(self, ctx: Context | None = None):
+        self.ctx = ctx or default_context()
+        self._sig = np.zeros((4, 0), np.float32)
+
+    def SetSequence(self, seq: bytes, size: int) -> None:
+        self._sig = self.ctx.encode(seq, size)
+
+    def GetFullSize(self) -> int:
+        return self._sig.shape[1]
+
+    def GetCount(self) -> int:
+        return 4
+
+    def Get(self, i: int) -> np.ndarray:
+        return self._sig[i]
+
+    @property
+    def channels(self) -> np.ndarray:
+        return self._sig
+
+
+class CrossCorrelation:
+    """CrossCorrelation (analysis/CrossCorr.h:146-161)."""
+
+    def __init__(self, ctx: Context | None = None):
+        self.ctx = ctx or default_context()
+
+    def CrossCorrelate(self, one: CCSignal, two: CCSignal) -> np.ndarray:
+        return self.ctx.xcorr(one.channels, two.channels)
+
+
+class SeqAnalyzer:
+    """SeqAnalyzer (analysis/CrossCorr.h:257-290)."""
+
+    def __init__(self, ctx: Context | None = None):
+        self.ctx = ctx or default_context()
+        self.m_topCutoff = 1.8
+
+    def SetTopCutoff(self, c: float) -> None:
+        self.m_topCutoff = c
+
+    def MatchUp(self, query: bytes, target: bytes, xc: np.ndarray) -> np.ndarray:
+        return self.ctx.matchup(query, target, xc, self.m_topCutoff)
+
+
+def GetMatchProbability(target: bytes, query: bytes, startTarget: int, startQuery: int, length: int,
+                        targetSize: float, ctx: Context | None = None) -> float:
+    ctx = ctx or default_context()
+    prob, _ = ctx.match_probability(target, query, np.array([[startTarget, startQuery, length]], np.int32), targetSize)
+    return float(prob[0])
+
+
+def PrintMatch(query: bytes, target: bytes, m, ctx: Context | None = None) -> float:
+    """Identity returned by PrintMatch(query, target, m, true); m = (startTarget, startQuery, len)."""
+    ctx = ctx or default_context()
+    _, ident = ctx.match_probability(target, query, np.array([list(m)], np.int32), 0.0)
+    return float(ident[0])

@@ -1,0 +1,626 @@
+// sxc_api.hip -- C ABI of libsxc_hip.so (declared in include/sxc.h).
+//
+// Host side of the drop-in: validates and packs caller buffers, owns device memory,
+// launches the gfx950 kernels on one HIP stream per context, and returns results in
+// the reference's order.  All compute runs on the GPU; there is no host fallback.
+#include <hip/hip_runtime.h>
+#include <string.h>
+
+#include <algorithm>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/sxc.h"
+#include "sxc_batch_host.h"
+#include "sxc_kernels.h"
+#include "sxc_tables.h"
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIPCHK(expr)                                                                          \
+  do {                                                                                        \
+    hipError_t e_ = (expr);                                                                   \
+    if (e_ != hipSuccess) return fail(SXC_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t cap = 0;
+  hipError_t ensure(size_t bytes) {
+    if (bytes <= cap && p) return hipSuccess;
+    if (p) {
+      hipError_t e = hipFree(p);
+      if (e != hipSuccess) return e;
+      p = nullptr;
+      cap = 0;
+    }
+    size_t want = std::max<size_t>(bytes, 256);
+    want = (want + 4095) & ~(size_t)4095;
+    hipError_t e = hipMalloc(&p, want);
+    if (e == hipSuccess) cap = want;
+    return e;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+  }
+  template <class T>
+  T* as() const {
+    return (T*)p;
+  }
+};
+
+}  // namespace
+
+struct sxc_batch_dev {
+  DevBuf seq, tch, qch, pairs, spectra, which, slots, keys, counts, offsets, total, maxc, final_, stats;
+  sxc::HostBatch hb;
+  sxc_params prm{};
+  int nb = 0;
+  int slot_cap = 64;
+  int64_t n_results = 0;
+  void release() {
+    for (DevBuf* b : {&seq, &tch, &qch, &pairs, &spectra, &which, &slots, &keys, &counts, &offsets, &total, &maxc,
+                      &final_, &stats})
+      b->release();
+  }
+};
+
+struct sxc_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  sxc_tables* d_tab = nullptr;
+  float* d_osc = nullptr;
+  double* d_ent[16] = {};
+  hipEvent_t ev[8] = {};
+  sxc_timing last{};
+  std::mutex mu;
+  sxc_batch_dev scratch;  // buffers reused by the one-shot entry points
+  DevBuf p0, p1, p2, p3, p4, p5, p6;  // plain-API staging
+};
+
+namespace {
+
+int check_N(int N, int* nb) {
+  *nb = sxc::nb_of(N);
+  if (*nb < SXC_MIN_NB || *nb > 14) return fail(SXC_ERR_ARG, "N must be a power of two in [512, 16384]");
+  return SXC_OK;
+}
+
+int set_device(sxc_ctx* ctx) {
+  HIPCHK(hipSetDevice(ctx->device));
+  return SXC_OK;
+}
+
+float elapsed(hipEvent_t a, hipEvent_t b) {
+  float ms = 0.f;
+  if (hipEventElapsedTime(&ms, a, b) != hipSuccess) return 0.f;
+  return ms;
+}
+
+// Upload a packed batch (inputs become resident in HBM).
+int batch_upload(sxc_ctx* ctx, sxc_batch_dev& b) {
+  const sxc::HostBatch& hb = b.hb;
+  const int N = b.prm.N;
+  hipStream_t s = ctx->stream;
+  HIPCHK(b.seq.ensure(hb.seq.size()));
+  HIPCHK(b.tch.ensure(hb.tchunks.size() * sizeof(sxc_chunk)));
+  HIPCHK(b.qch.ensure(hb.qchunks.size() * sizeof(sxc_chunk)));
+  HIPCHK(b.pairs.ensure(hb.pairs.size() * sizeof(sxc_pair_dev)));
+  HIPCHK(b.spectra.ensure(hb.tchunks.size() * 4 * (size_t)N * sizeof(float)));
+  HIPCHK(b.which.ensure(hb.tchunks.size() * sizeof(int32_t)));
+  if (!hb.seq.empty()) HIPCHK(hipMemcpyAsync(b.seq.p, hb.seq.data(), hb.seq.size(), hipMemcpyHostToDevice, s));
+  if (!hb.tchunks.empty())
+    HIPCHK(hipMemcpyAsync(b.tch.p, hb.tchunks.data(), hb.tchunks.size() * sizeof(sxc_chunk), hipMemcpyHostToDevice, s));
+  if (!hb.qchunks.empty())
+    HIPCHK(hipMemcpyAsync(b.qch.p, hb.qchunks.data(), hb.qchunks.size() * sizeof(sxc_chunk), hipMemcpyHostToDevice, s));
+  if (!hb.pairs.empty())
+    HIPCHK(hipMemcpyAsync(b.pairs.p, hb.pairs.data(), hb.pairs.size() * sizeof(sxc_pair_dev), hipMemcpyHostToDevice, s));
+  std::vector<int32_t> idx(hb.tchunks.size());
+  for (size_t i = 0; i < idx.size(); ++i) idx[i] = (int32_t)i;
+  if (!idx.empty()) HIPCHK(hipMemcpyAsync(b.which.p, idx.data(), idx.size() * sizeof(int32_t), hipMemcpyHostToDevice, s));
+  HIPCHK(hipStreamSynchronize(s));  // idx is a host temporary
+  return SXC_OK;
+}
+
+int launch_pairs(sxc_ctx* ctx, sxc_batch_dev& b) {
+  const int total = (int)b.hb.pairs.size() * b.prm.strands;
+  HIPCHK(b.slots.ensure((size_t)total * b.slot_cap * sizeof(sxc_result_dev)));
+  HIPCHK(b.keys.ensure((size_t)total * b.slot_cap * sizeof(uint64_t)));
+  HIPCHK(b.counts.ensure((size_t)total * sizeof(int32_t)));
+  HIPCHK(b.stats.ensure((size_t)total * 2 * sizeof(int32_t)));
+  sxc::PairArgs a{};
+  a.tab = ctx->d_tab;
+  a.osc = ctx->d_osc;
+  a.ent_lut = ctx->d_ent[b.nb];
+  a.seq = b.seq.as<unsigned char>();
+  a.tchunks = b.tch.as<sxc_chunk>();
+  a.qchunks = b.qch.as<sxc_chunk>();
+  a.pairs = b.pairs.as<sxc_pair_dev>();
+  a.spectra = b.spectra.as<float>();
+  a.npairs = (int)b.hb.pairs.size();
+  a.strands = b.prm.strands;
+  a.min_len = b.prm.min_len;
+  a.q_chunk = b.prm.q_chunk;
+  a.cutoff = b.prm.cutoff;
+  a.cutoff_fast = b.prm.cutoff_fast;
+  a.min_prob = b.prm.min_prob;
+  a.target_total = b.prm.target_total;
+  a.out = b.slots.as<sxc_result_dev>();
+  a.keys = b.keys.as<uint64_t>();
+  a.counts = b.counts.as<int32_t>();
+  a.cap = b.slot_cap;
+  a.xc_out = nullptr;
+  a.stats = b.stats.as<int32_t>();
+  HIPCHK(sxc::launch_pair(b.nb, a, total, ctx->stream));
+  return SXC_OK;
+}
+
+// Device-only work of a resident batch: target spectra, fused pair kernel, ordering.
+int batch_run(sxc_ctx* ctx, sxc_batch_dev& b) {
+  hipStream_t s = ctx->stream;
+  const int total = (int)b.hb.pairs.size() * b.prm.strands;
+  HIPCHK(hipEventRecord(ctx->ev[1], s));
+  if (!b.hb.tchunks.empty()) {
+    sxc::SpecArgs sa{};
+    sa.tab = ctx->d_tab;
+    sa.osc = ctx->d_osc;
+    sa.ent_lut = ctx->d_ent[b.nb];
+    sa.seq = b.seq.as<unsigned char>();
+    sa.chunks = b.tch.as<sxc_chunk>();
+    sa.which = b.which.as<int32_t>();
+    sa.n = (int)b.hb.tchunks.size();
+    sa.spectra = b.spectra.as<float>();
+    sa.dst = b.which.as<int32_t>();
+    sa.signals = nullptr;
+    HIPCHK(sxc::launch_spectra(b.nb, sa, s));
+  }
+  HIPCHK(hipEventRecord(ctx->ev[2], s));
+  if (total == 0) {
+    b.n_results = 0;
+    HIPCHK(hipEventRecord(ctx->ev[3], s));
+    HIPCHK(hipEventRecord(ctx->ev[4], s));
+    HIPCHK(hipEventSynchronize(ctx->ev[4]));
+    return SXC_OK;
+  }
+  int rc = launch_pairs(ctx, b);
+  if (rc) return rc;
+  HIPCHK(hipEventRecord(ctx->ev[3], s));
+  HIPCHK(b.offsets.ensure((size_t)total * sizeof(int64_t)));
+  HIPCHK(b.total.ensure(sizeof(int64_t)));
+  HIPCHK(b.maxc.ensure(sizeof(int32_t)));
+  HIPCHK(sxc::launch_finalize(b.counts.as<int32_t>(), total, b.slot_cap, b.offsets.as<int64_t>(), b.total.as<int64_t>(),
+                              b.maxc.as<int32_t>(), nullptr, nullptr, nullptr, s, true));
+  int32_t maxc = 0;
+  int64_t tot = 0;
+  HIPCHK(hipMemcpyAsync(&maxc, b.maxc.p, sizeof maxc, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipMemcpyAsync(&tot, b.total.p, sizeof tot, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  if (maxc > b.slot_cap) {
+    // some (pair, strand) kept more records than its slot holds: enlarge and recompute
+    b.slot_cap = maxc;
+    rc = launch_pairs(ctx, b);
+    if (rc) return rc;
+    HIPCHK(sxc::launch_finalize(b.counts.as<int32_t>(), total, b.slot_cap, b.offsets.as<int64_t>(),
+                                b.total.as<int64_t>(), b.maxc.as<int32_t>(), nullptr, nullptr, nullptr, s, true));
+    HIPCHK(hipMemcpyAsync(&tot, b.total.p, sizeof tot, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+  }
+  HIPCHK(b.final_.ensure((size_t)std::max<int64_t>(tot, 1) * sizeof(sxc_result_dev)));
+  HIPCHK(sxc::launch_finalize(b.counts.as<int32_t>(), total, b.slot_cap, b.offsets.as<int64_t>(), nullptr, nullptr,
+                              b.slots.as<sxc_result_dev>(), b.keys.as<uint64_t>(), b.final_.as<sxc_result_dev>(), s,
+                              false));
+  HIPCHK(hipEventRecord(ctx->ev[4], s));
+  HIPCHK(hipEventSynchronize(ctx->ev[4]));
+  b.n_results = tot;
+  return SXC_OK;
+}
+
+int batch_download(sxc_ctx* ctx, sxc_batch_dev& b, sxc_result* out, int64_t cap, int64_t* n_out) {
+  *n_out = b.n_results;
+  if (b.n_results > cap) return fail(SXC_ERR_CAPACITY, "output capacity too small");
+  if (b.n_results > 0)
+    HIPCHK(hipMemcpyAsync(out, b.final_.p, (size_t)b.n_results * sizeof(sxc_result), hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  return SXC_OK;
+}
+
+int batch_prepare(sxc_batch_dev& b, const sxc_pair* pairs, int32_t n, const sxc_params* prm) {
+  if (!prm || n < 0 || (n > 0 && !pairs)) return fail(SXC_ERR_ARG, "null argument");
+  if (prm->strands != 1 && prm->strands != 2) return fail(SXC_ERR_ARG, "strands must be 1 or 2");
+  int nb;
+  int rc = check_N(prm->N, &nb);
+  if (rc) return rc;
+  std::string e = sxc::pack_batch(pairs, n, prm->N, b.hb);
+  if (!e.empty()) return fail(SXC_ERR_ARG, e);
+  b.prm = *prm;
+  b.nb = nb;
+  return SXC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* sxc_last_error(void) { return g_err.c_str(); }
+const char* sxc_version(void) { return "sxc 0.1 gfx950"; }
+
+int sxc_create(sxc_ctx** out, int device) {
+  if (!out) return fail(SXC_ERR_ARG, "null out");
+  *out = nullptr;
+  int ndev = 0;
+  HIPCHK(hipGetDeviceCount(&ndev));
+  if (device < 0 || device >= ndev) return fail(SXC_ERR_ARG, "no such HIP device");
+  sxc_ctx* c = new sxc_ctx();
+  c->device = device;
+  HIPCHK(hipSetDevice(device));
+  HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  for (auto& e : c->ev) HIPCHK(hipEventCreate(&e));
+  sxc_tables tab;
+  sxc::build_tables(&tab);
+  HIPCHK(hipMalloc(&c->d_tab, sizeof tab));
+  HIPCHK(hipMemcpy(c->d_tab, &tab, sizeof tab, hipMemcpyHostToDevice));
+  std::vector<float> osc(24576);
+  sxc::build_osc(osc.data());
+  HIPCHK(hipMalloc(&c->d_osc, osc.size() * sizeof(float)));
+  HIPCHK(hipMemcpy(c->d_osc, osc.data(), osc.size() * sizeof(float), hipMemcpyHostToDevice));
+  for (int nb = SXC_MIN_NB; nb <= SXC_MAX_NB; ++nb) {
+    const int len = std::max(1, sxc::ent_lut_len(nb));
+    std::vector<double> v(len, 0.);
+    if (sxc::ent_lut_len(nb) > 0) sxc::build_ent_lut(nb, v.data());
+    HIPCHK(hipMalloc(&c->d_ent[nb], len * sizeof(double)));
+    HIPCHK(hipMemcpy(c->d_ent[nb], v.data(), len * sizeof(double), hipMemcpyHostToDevice));
+  }
+  *out = c;
+  return SXC_OK;
+}
+
+void sxc_destroy(sxc_ctx* c) {
+  if (!c) return;
+  (void)hipSetDevice(c->device);
+  (void)hipStreamSynchronize(c->stream);
+  c->scratch.release();
+  for (DevBuf* b : {&c->p0, &c->p1, &c->p2, &c->p3, &c->p4, &c->p5, &c->p6}) b->release();
+  (void)hipFree(c->d_tab);
+  (void)hipFree(c->d_osc);
+  for (auto p : c->d_ent)
+    if (p) (void)hipFree(p);
+  for (auto e : c->ev)
+    if (e) (void)hipEventDestroy(e);
+  (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+int sxc_align_batch(sxc_ctx* ctx, const sxc_pair* pairs, int32_t n, const sxc_params* prm, sxc_result* out,
+                    int64_t cap, int64_t* n_out) {
+  if (!ctx || !n_out || (cap > 0 && !out)) return fail(SXC_ERR_ARG, "null argument");
+  std::lock_guard<std::mutex> g(ctx->mu);
+  int rc = set_device(ctx);
+  if (rc) return rc;
+  sxc_batch_dev& b = ctx->scratch;
+  rc = batch_prepare(b, pairs, n, prm);
+  if (rc) return rc;
+  HIPCHK(hipEventRecord(ctx->ev[0], ctx->stream));
+  rc = batch_upload(ctx, b);
+  if (rc) return rc;
+  rc = batch_run(ctx, b);
+  if (rc) return rc;
+  rc = batch_download(ctx, b, out, cap, n_out);
+  HIPCHK(hipEventRecord(ctx->ev[5], ctx->stream));
+  HIPCHK(hipEventSynchronize(ctx->ev[5]));
+  ctx->last.total_ms = elapsed(ctx->ev[0], ctx->ev[5]);
+  ctx->last.upload_ms = elapsed(ctx->ev[0], ctx->ev[1]);
+  ctx->last.spectra_ms = elapsed(ctx->ev[1], ctx->ev[2]);
+  ctx->last.pair_ms = elapsed(ctx->ev[2], ctx->ev[3]);
+  ctx->last.finalize_ms = elapsed(ctx->ev[3], ctx->ev[4]);
+  ctx->last.download_ms = elapsed(ctx->ev[4], ctx->ev[5]);
+  return rc;
+}
+
+int sxc_align_pair(sxc_ctx* ctx, const sxc_pair* pair, const sxc_params* prm, sxc_result* out, int32_t cap,
+                   int32_t* n_out) {
+  if (!n_out) return fail(SXC_ERR_ARG, "null n_out");
+  int64_t n = 0;
+  const int rc = sxc_align_batch(ctx, pair, 1, prm, out, cap, &n);
+  *n_out = (int32_t)n;
+  return rc;
+}
+
+int sxc_last_timing(sxc_ctx* ctx, sxc_timing* out) {
+  if (!ctx || !out) return fail(SXC_ERR_ARG, "null argument");
+  *out = ctx->last;
+  return SXC_OK;
+}
+
+// ---------------------------------------------------------------- resident batches
+
+typedef struct sxc_batch {
+  sxc_ctx* ctx;
+  sxc_batch_dev b;
+} sxc_batch;
+
+int sxc_batch_create(sxc_ctx* ctx, const sxc_pair* pairs, int32_t n, const sxc_params* prm, sxc_batch** out) {
+  if (!ctx || !out) return fail(SXC_ERR_ARG, "null argument");
+  *out = nullptr;
+  std::lock_guard<std::mutex> g(ctx->mu);
+  int rc = set_device(ctx);
+  if (rc) return rc;
+  sxc_batch* bt = new sxc_batch();
+  bt->ctx = ctx;
+  rc = batch_prepare(bt->b, pairs, n, prm);
+  if (!rc) rc = batch_upload(ctx, bt->b);
+  if (rc) {
+    bt->b.release();
+    delete bt;
+    return rc;
+  }
+  *out = bt;
+  return SXC_OK;
+}
+
+int sxc_batch_run(sxc_batch* bt) {
+  if (!bt) return fail(SXC_ERR_ARG, "null batch");
+  std::lock_guard<std::mutex> g(bt->ctx->mu);
+  int rc = set_device(bt->ctx);
+  if (rc) return rc;
+  HIPCHK(hipEventRecord(bt->ctx->ev[0], bt->ctx->stream));
+  HIPCHK(hipEventRecord(bt->ctx->ev[1], bt->ctx->stream));
+  rc = batch_run(bt->ctx, bt->b);
+  sxc_ctx* c = bt->ctx;
+  c->last.upload_ms = 0;
+  c->last.spectra_ms = elapsed(c->ev[1], c->ev[2]);
+  c->last.pair_ms = elapsed(c->ev[2], c->ev[3]);
+  c->last.finalize_ms = elapsed(c->ev[3], c->ev[4]);
+  c->last.download_ms = 0;
+  c->last.total_ms = elapsed(c->ev[1], c->ev[4]);
+  return rc;
+}
+
+int sxc_batch_results(sxc_batch* bt, sxc_result* out, int64_t cap, int64_t* n_out) {
+  if (!bt || !n_out) return fail(SXC_ERR_ARG, "null argument");
+  std::lock_guard<std::mutex> g(bt->ctx->mu);
+  int rc = set_device(bt->ctx);
+  if (rc) return rc;
+  return batch_download(bt->ctx, bt->b, out, cap, n_out);
+}
+
+int sxc_batch_stats(sxc_batch* bt, int64_t* candidates, int64_t* runs) {
+  if (!bt || !candidates || !runs) return fail(SXC_ERR_ARG, "null argument");
+  std::lock_guard<std::mutex> g(bt->ctx->mu);
+  const int total = (int)bt->b.hb.pairs.size() * bt->b.prm.strands;
+  std::vector<int32_t> st((size_t)total * 2);
+  if (total > 0) {
+    HIPCHK(hipMemcpyAsync(st.data(), bt->b.stats.p, st.size() * sizeof(int32_t), hipMemcpyDeviceToHost, bt->ctx->stream));
+    HIPCHK(hipStreamSynchronize(bt->ctx->stream));
+  }
+  int64_t c = 0, r = 0;
+  for (int i = 0; i < total; ++i) {
+    c += st[2 * i];
+    r += st[2 * i + 1];
+  }
+  *candidates = c;
+  *runs = r;
+  return SXC_OK;
+}
+
+void sxc_batch_destroy(sxc_batch* bt) {
+  if (!bt) return;
+  {
+    std::lock_guard<std::mutex> g(bt->ctx->mu);
+    (void)hipSetDevice(bt->ctx->device);
+    (void)hipStreamSynchronize(bt->ctx->stream);
+    bt->b.release();
+  }
+  delete bt;
+}
+
+// ---------------------------------------------------------------- plain API
+
+int sxc_encode_signal(sxc_ctx* ctx, const char* seq, int32_t len, int32_t N, float* out) {
+  if (!ctx || !out || len < 0 || (len > 0 && !seq)) return fail(SXC_ERR_ARG, "null argument");
+  int nb;
+  int rc = check_N(N, &nb);
+  if (rc) return rc;
+  if (len > N) return fail(SXC_ERR_ARG, "sequence longer than N");
+  std::lock_guard<std::mutex> g(ctx->mu);
+  rc = set_device(ctx);
+  if (rc) return rc;
+  sxc_chunk ck{0, len, 0};
+  for (int i = 0; i < len; ++i) {
+    const unsigned char b = (unsigned char)seq[i];
+    if (b >= 128) return fail(SXC_ERR_ARG, "sequence byte >= 128");
+    if (sxc::class_has_thirds(sxc::class_of(b))) ck.flags |= 1;
+  }
+  hipStream_t s = ctx->stream;
+  HIPCHK(ctx->p0.ensure((size_t)len + 16));
+  HIPCHK(ctx->p1.ensure(sizeof ck));
+  HIPCHK(ctx->p2.ensure((size_t)4 * N * sizeof(float)));
+  if (len) HIPCHK(hipMemcpyAsync(ctx->p0.p, seq, len, hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(ctx->p1.p, &ck, sizeof ck, hipMemcpyHostToDevice, s));
+  sxc::EncodeArgs a{ctx->d_tab, ctx->d_ent[nb], ctx->p0.as<unsigned char>(), ctx->p1.as<sxc_chunk>(), 0,
+                    ctx->p2.as<float>()};
+  HIPCHK(sxc::launch_encode(nb, a, 1, s));
+  HIPCHK(hipMemcpyAsync(out, ctx->p2.p, (size_t)4 * N * sizeof(float), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  return SXC_OK;
+}
+
+int sxc_fft_forward(sxc_ctx* ctx, const float* in, float* out, int32_t N, int32_t batch) {
+  if (!ctx || !in || !out || batch < 0) return fail(SXC_ERR_ARG, "null argument");
+  int nb;
+  int rc = check_N(N, &nb);
+  if (rc) return rc;
+  if (batch == 0) return SXC_OK;
+  std::lock_guard<std::mutex> g(ctx->mu);
+  rc = set_device(ctx);
+  if (rc) return rc;
+  hipStream_t s = ctx->stream;
+  // each signal is transformed as channel A of a 4-channel item
+  const size_t item = (size_t)4 * N;
+  HIPCHK(ctx->p0.ensure(item * batch * sizeof(float)));
+  HIPCHK(ctx->p1.ensure(item * batch * sizeof(float)));
+  HIPCHK(ctx->p2.ensure((size_t)batch * 2 * sizeof(int32_t)));
+  HIPCHK(ctx->p3.ensure(sizeof(sxc_chunk)));
+  HIPCHK(hipMemsetAsync(ctx->p0.p, 0, item * batch * sizeof(float), s));
+  HIPCHK(hipMemcpy2DAsync(ctx->p0.p, item * sizeof(float), in, (size_t)N * sizeof(float), (size_t)N * sizeof(float),
+                          batch, hipMemcpyHostToDevice, s));
+  std::vector<int32_t> idx(batch);
+  for (int i = 0; i < batch; ++i) idx[i] = i;
+  std::vector<int32_t> zero(batch, 0);
+  sxc_chunk ck{0, 0, 0};
+  HIPCHK(hipMemcpyAsync(ctx->p3.p, &ck, sizeof ck, hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(ctx->p2.p, zero.data(), batch * sizeof(int32_t), hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(ctx->p2.as<int32_t>() + batch, idx.data(), batch * sizeof(int32_t), hipMemcpyHostToDevice, s));
+  sxc::SpecArgs a{};
+  a.tab = ctx->d_tab;
+  a.osc = ctx->d_osc;
+  a.ent_lut = ctx->d_ent[nb];
+  a.seq = nullptr;
+  a.chunks = ctx->p3.as<sxc_chunk>();
+  a.which = ctx->p2.as<int32_t>();
+  a.n = batch;
+  a.spectra = ctx->p1.as<float>();
+  a.dst = ctx->p2.as<int32_t>() + batch;
+  a.signals = ctx->p0.as<float>();
+  HIPCHK(sxc::launch_spectra(nb, a, s));
+  HIPCHK(hipMemcpy2DAsync(out, (size_t)N * sizeof(float), ctx->p1.p, item * sizeof(float), (size_t)N * sizeof(float),
+                          batch, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  return SXC_OK;
+}
+
+int sxc_xcorr(sxc_ctx* ctx, const float* tsig, const float* qsig, int32_t N, float* xc) {
+  if (!ctx || !tsig || !qsig || !xc) return fail(SXC_ERR_ARG, "null argument");
+  int nb;
+  int rc = check_N(N, &nb);
+  if (rc) return rc;
+  std::lock_guard<std::mutex> g(ctx->mu);
+  rc = set_device(ctx);
+  if (rc) return rc;
+  hipStream_t s = ctx->stream;
+  const size_t sig = (size_t)4 * N * sizeof(float);
+  HIPCHK(ctx->p0.ensure(sig));
+  HIPCHK(ctx->p1.ensure(sig));
+  HIPCHK(ctx->p2.ensure(sig));
+  HIPCHK(ctx->p3.ensure((size_t)N * sizeof(float)));
+  HIPCHK(ctx->p4.ensure(2 * sizeof(int32_t) + sizeof(sxc_chunk)));
+  HIPCHK(hipMemcpyAsync(ctx->p0.p, tsig, sig, hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(ctx->p1.p, qsig, sig, hipMemcpyHostToDevice, s));
+  int32_t zz[2] = {0, 0};
+  HIPCHK(hipMemcpyAsync(ctx->p4.p, zz, sizeof zz, hipMemcpyHostToDevice, s));
+  sxc_chunk ck{0, 0, 0};
+  HIPCHK(hipMemcpyAsync((char*)ctx->p4.p + 8, &ck, sizeof ck, hipMemcpyHostToDevice, s));
+  sxc::SpecArgs sa{};
+  sa.tab = ctx->d_tab;
+  sa.osc = ctx->d_osc;
+  sa.ent_lut = ctx->d_ent[nb];
+  sa.chunks = (const sxc_chunk*)((char*)ctx->p4.p + 8);
+  sa.which = ctx->p4.as<int32_t>();
+  sa.n = 1;
+  sa.spectra = ctx->p2.as<float>();
+  sa.dst = ctx->p4.as<int32_t>() + 1;
+  sa.signals = ctx->p0.as<float>();
+  HIPCHK(sxc::launch_spectra(nb, sa, s));
+  sxc::XcorrArgs xa{ctx->d_tab, ctx->d_osc, ctx->p2.as<float>(), ctx->p1.as<float>(), ctx->p3.as<float>()};
+  HIPCHK(sxc::launch_xcorr(nb, xa, 1, s));
+  HIPCHK(hipMemcpyAsync(xc, ctx->p3.p, (size_t)N * sizeof(float), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  return SXC_OK;
+}
+
+int sxc_matchup(sxc_ctx* ctx, const char* q, int32_t lq, const char* t, int32_t lt, const float* xc, int32_t N,
+                double cutoff, sxc_seqmatch* out, int32_t cap, int32_t* n_out) {
+  if (!ctx || !xc || !n_out || (cap > 0 && !out)) return fail(SXC_ERR_ARG, "null argument");
+  int nb;
+  int rc = check_N(N, &nb);
+  if (rc) return rc;
+  sxc_pair p{t, lt, q, lq, 0, 0, 0, 0, lq, 0};
+  sxc::HostBatch hb;
+  std::string e = sxc::pack_batch(&p, 1, N, hb);
+  if (!e.empty()) return fail(SXC_ERR_ARG, e);
+  std::lock_guard<std::mutex> g(ctx->mu);
+  rc = set_device(ctx);
+  if (rc) return rc;
+  hipStream_t s = ctx->stream;
+  // runs per lag are bounded by the diagonal length / 2 + 1
+  const int slot = std::max(16, (int)std::min<int64_t>((int64_t)N * (N / 4 + 1), 1 << 22));
+  HIPCHK(ctx->p0.ensure(hb.seq.size()));
+  HIPCHK(ctx->p1.ensure(2 * sizeof(sxc_chunk)));
+  HIPCHK(ctx->p2.ensure((size_t)N * sizeof(float)));
+  HIPCHK(ctx->p3.ensure((size_t)slot * 3 * sizeof(int32_t)));
+  HIPCHK(ctx->p4.ensure((size_t)slot * sizeof(uint64_t)));
+  HIPCHK(ctx->p5.ensure(sizeof(int32_t)));
+  HIPCHK(hipMemcpyAsync(ctx->p0.p, hb.seq.data(), hb.seq.size(), hipMemcpyHostToDevice, s));
+  sxc_chunk cks[2] = {hb.tchunks[0], hb.qchunks[0]};
+  HIPCHK(hipMemcpyAsync(ctx->p1.p, cks, sizeof cks, hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(ctx->p2.p, xc, (size_t)N * sizeof(float), hipMemcpyHostToDevice, s));
+  sxc::MatchupArgs a{};
+  a.tab = ctx->d_tab;
+  a.ent_lut = ctx->d_ent[nb];
+  a.seq = ctx->p0.as<unsigned char>();
+  a.tchunks = ctx->p1.as<sxc_chunk>();
+  a.qchunks = ctx->p1.as<sxc_chunk>() + 1;
+  a.xc = ctx->p2.as<float>();
+  a.cutoff = cutoff;
+  a.runs = ctx->p3.as<int32_t>();
+  a.keys = ctx->p4.as<uint64_t>();
+  a.counts = ctx->p5.as<int32_t>();
+  a.cap = slot;
+  HIPCHK(sxc::launch_matchup(nb, a, 1, s));
+  int32_t cnt = 0;
+  HIPCHK(hipMemcpyAsync(&cnt, ctx->p5.p, sizeof cnt, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  *n_out = cnt;
+  if (cnt > cap) return fail(SXC_ERR_CAPACITY, "output capacity too small");
+  if (cnt > slot) return fail(SXC_ERR_CAPACITY, "run buffer overflow");
+  // runs come back unordered across lanes; order them by (lag rank, start) on the GPU
+  HIPCHK(ctx->p6.ensure((size_t)std::max(cnt, 1) * 3 * sizeof(int32_t)));
+  HIPCHK(sxc::launch_order_runs(ctx->p3.as<int32_t>(), ctx->p4.as<uint64_t>(), cnt, ctx->p6.as<int32_t>(), s));
+  if (cnt) HIPCHK(hipMemcpyAsync(out, ctx->p6.p, (size_t)cnt * 3 * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  return SXC_OK;
+}
+
+int sxc_match_probability(sxc_ctx* ctx, const char* t, int32_t lt, const char* q, int32_t lq, const sxc_seqmatch* m,
+                          int32_t n, double target_size, double* prob, double* ident) {
+  if (!ctx || n < 0 || (n > 0 && (!m || !prob || !ident || !t || !q))) return fail(SXC_ERR_ARG, "null argument");
+  for (int i = 0; i < n; ++i) {
+    if (m[i].len <= 0 || m[i].start_target < 0 || m[i].start_query < 0 || m[i].start_target + m[i].len > lt ||
+        m[i].start_query + m[i].len > lq)
+      return fail(SXC_ERR_ARG, "match outside the sequences");
+  }
+  for (int i = 0; i < lt; ++i)
+    if ((unsigned char)t[i] >= 128) return fail(SXC_ERR_ARG, "sequence byte >= 128");
+  for (int i = 0; i < lq; ++i)
+    if ((unsigned char)q[i] >= 128) return fail(SXC_ERR_ARG, "sequence byte >= 128");
+  if (n == 0) return SXC_OK;
+  std::lock_guard<std::mutex> g(ctx->mu);
+  int rc = set_device(ctx);
+  if (rc) return rc;
+  hipStream_t s = ctx->stream;
+  HIPCHK(ctx->p0.ensure((size_t)lt + 16));
+  HIPCHK(ctx->p1.ensure((size_t)lq + 16));
+  HIPCHK(ctx->p2.ensure((size_t)n * 3 * sizeof(int32_t)));
+  HIPCHK(ctx->p3.ensure((size_t)n * 2 * sizeof(double)));
+  if (lt) HIPCHK(hipMemcpyAsync(ctx->p0.p, t, lt, hipMemcpyHostToDevice, s));
+  if (lq) HIPCHK(hipMemcpyAsync(ctx->p1.p, q, lq, hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(ctx->p2.p, m, (size_t)n * 3 * sizeof(int32_t), hipMemcpyHostToDevice, s));
+  sxc::ProbArgs a{ctx->d_tab, ctx->p0.as<unsigned char>(), ctx->p1.as<unsigned char>(), ctx->p2.as<int32_t>(), n,
+                  target_size, ctx->p3.as<double>(), ctx->p3.as<double>() + n};
+  HIPCHK(sxc::launch_prob(a, s));
+  HIPCHK(hipMemcpyAsync(prob, ctx->p3.p, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipMemcpyAsync(ident, ctx->p3.as<double>() + n, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  return SXC_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,360 @@
+// sxc_fft.h -- FFTReal<float>'s exact dataflow, blocked for a CDNA4 workgroup.
+//
+// FFTReal (extern/RealFFT/FFTReal.hpp) computes a length-N real DFT with a bit-reversed
+// radix-4 first pass (pass 1+2, :466-495), an 8-point pass (pass 3, :499-528) and one
+// radix-2 level per bit above that (pass_n, n = 3..nbits-1, :553-656).  Every output
+// element is produced by a fixed sequence of IEEE float ops; any schedule that applies
+// the same ops to the same operands gives the same bits.  This file regroups the
+// butterflies so one thread owns a closed set of positions for several levels:
+//
+//   stage A : levels pass1+2, pass3, pass_3, pass_4 on 32 contiguous positions
+//             (the bit-reversed gather of pass 1+2 is done while loading);
+//   rounds  : levels pass_m .. pass_{m+K-1} (K <= 3) on "units" of 2^(K+2) positions.
+//
+// A unit at level m is butterfly index j and h_m - j (h_m = 2^(m-1)) in 2^(K-1)
+// neighbouring blocks; level m+1 butterflies j' and 2h_m - j' read exactly what level-m
+// butterfly j wrote in two blocks, so the set closes over K levels and is in place.
+// Index j = 0 and j = h_m/2 form one more unit of the same size.  The inverse
+// (FFTReal.hpp:661-906) runs the same units with levels in reverse order.
+//
+// Twiddles: FFTReal's per-level LUT (float)cos(i*PI/2^L) equals the level-12 table at
+// stride 2^(12-L) (same double argument), so one 2048-entry table serves every level
+// <= 12.  Levels > 12 use FFTReal's float oscillator (OscSinCos.hpp:88-104); its values
+// depend only on the butterfly index and are tabulated by the host.
+//
+// All arithmetic here must be compiled without FP contraction (no FMA): FFTReal is built
+// with plain -O3 and no -march, so it never fuses a multiply into an add.
+#pragma once
+#include <type_traits>
+
+#include "sxc_types.h"
+
+#if defined(__HIP__)
+#define SXC_HD __host__ __device__
+#else
+#define SXC_HD
+#endif
+
+#pragma clang fp contract(off)
+
+namespace sxc {
+
+template <int B, int E, class F>
+SXC_HD inline void static_for(F&& f) {
+  if constexpr (B < E) {
+    f(std::integral_constant<int, B>{});
+    static_for<B + 1, E>(f);
+  }
+}
+
+// LDS index of FFT position p: one pad word per 32 floats keeps the 32-float rows of
+// stage A on distinct banks.
+SXC_HD constexpr int pad(int p) { return p + (p >> 5); }
+template <int NB>
+struct Geo {
+  static constexpr int N = 1 << NB;
+  static constexpr int H = N / 2;
+  static constexpr int NT = (N / 32 < 64) ? 64 : N / 32;  // threads per workgroup
+  static constexpr int YLEN = N + N / 32;                  // padded LDS floats
+  static constexpr int GROUPS = N / 32;                    // stage-A units
+};
+
+SXC_HD inline unsigned bitrev(unsigned x, int bits) {
+  unsigned r = 0;
+  for (int b = 0; b < bits; ++b) r |= ((x >> b) & 1u) << (bits - 1 - b);
+  return r;
+}
+
+// FFTReal's sqrt2_2 = DataType(SQRT2 * 0.5) (FFTReal.hpp:506, def.h:43)
+#define SXC_SQRT2_2 ((float)(1.41421356237309514547462185873883 * 0.5))
+
+struct Twiddle {
+  const float* lut;  // level-12 cosine table (LDS copy)
+  const float* osc;  // {cos, sin} after i oscillator steps: level 13 at [0, 8192), level 14 at [8192, 24576)
+  template <int LEVEL>
+  SXC_HD inline void get(int beta, int hbeta, float& c, float& s) const {
+    if constexpr (LEVEL <= 12) {
+      c = lut[beta << (12 - LEVEL)];
+      s = lut[hbeta << (12 - LEVEL)];
+    } else {
+      const float* t = osc + (LEVEL == 13 ? 0 : 8192) + 2 * beta;
+      c = t[0];
+      s = t[1];
+    }
+  }
+};
+
+// ---------------------------------------------------------------- butterflies
+
+// compute_direct_pass_n body (FFTReal.hpp:589-595): inputs at block positions
+// i, h+i, nbr+i, nbr+h+i; outputs at i, nbr-i, nbr+i, 2nbr-i.
+SXC_HD inline void bfly_fwd(float sf1r, float sf1i, float sf2r, float sf2i, float c, float s, float& o0,
+                            float& o1, float& o2, float& o3) {
+  float v = sf2r * c - sf2i * s;
+  o0 = sf1r + v;
+  o1 = sf1r - v;
+  v = sf2r * s + sf2i * c;
+  o2 = v + sf1i;
+  o3 = v - sf1i;
+}
+
+// compute_inverse_pass_n body (FFTReal.hpp:761-770): inputs at i, nbr-i, nbr+i, 2nbr-i;
+// outputs at i, h+i, nbr+i, nbr+h+i.
+SXC_HD inline void bfly_inv(float xr, float xm, float xp, float x2, float c, float s, float& o0, float& o1,
+                            float& o2, float& o3) {
+  o0 = xr + xm;
+  o1 = xp - x2;
+  const float vr = xr - xm;
+  const float vi = xp + x2;
+  o2 = vr * c + vi * s;
+  o3 = vi * c - vr * s;
+}
+
+// One radix-2 level (pass_n with nbr = 2^P) over NBLK contiguous blocks held in v.
+template <int P, int NBLK, bool INV>
+SXC_HD inline void level_contig(float* v, const Twiddle& tw) {
+  constexpr int nbr = 1 << P, h = nbr >> 1;
+  float o[NBLK * 2 * nbr];
+  static_for<0, NBLK * 2 * nbr>([&](auto I) { o[decltype(I)::value] = v[decltype(I)::value]; });
+  static_for<0, NBLK>([&](auto B) {
+    constexpr int ci = decltype(B)::value * 2 * nbr;
+    o[ci] = v[ci] + v[ci + nbr];
+    o[ci + nbr] = v[ci] - v[ci + nbr];
+    if constexpr (INV) {
+      o[ci + h] = v[ci + h] * 2;
+      o[ci + nbr + h] = v[ci + nbr + h] * 2;
+    }
+    static_for<1, h>([&](auto I) {
+      constexpr int i = decltype(I)::value;
+      float c, s;
+      tw.template get<P>(i, h - i, c, s);
+      if constexpr (!INV)
+        bfly_fwd(v[ci + i], v[ci + h + i], v[ci + nbr + i], v[ci + nbr + h + i], c, s, o[ci + i], o[ci + nbr - i],
+                 o[ci + nbr + i], o[ci + 2 * nbr - i]);
+      else
+        bfly_inv(v[ci + i], v[ci + nbr - i], v[ci + nbr + i], v[ci + 2 * nbr - i], c, s, o[ci + i], o[ci + h + i],
+                 o[ci + nbr + i], o[ci + nbr + h + i]);
+    });
+  });
+  static_for<0, NBLK * 2 * nbr>([&](auto I) { v[decltype(I)::value] = o[decltype(I)::value]; });
+}
+
+// ---------------------------------------------------------------- stage A
+
+// Time position read by slot r of stage-A group g (pass 1+2 reads x[br[4k+m]]).
+template <int NB>
+SXC_HD inline int stageA_pos(int g, int r) {
+  return (int)((bitrev((unsigned)r, 5) << (NB - 5)) | bitrev((unsigned)g, NB - 5));
+}
+
+// Forward levels pass1+2, pass3, pass_3, pass_4 (FFTReal.hpp:466-528, 553-601).
+// v[r] holds the time sample at stageA_pos(g, r) on entry, spectrum positions 32g+r on exit.
+SXC_HD inline void stageA_forward(float* v, const Twiddle& tw) {
+  float d[32];
+  static_for<0, 8>([&](auto K) {
+    constexpr int b = 4 * decltype(K)::value;
+    d[b + 1] = v[b + 0] - v[b + 1];
+    d[b + 3] = v[b + 2] - v[b + 3];
+    const float s0 = v[b + 0] + v[b + 1];
+    const float s2 = v[b + 2] + v[b + 3];
+    d[b + 0] = s0 + s2;
+    d[b + 2] = s0 - s2;
+  });
+  const float sq = SXC_SQRT2_2;
+  static_for<0, 4>([&](auto K) {
+    constexpr int c = 8 * decltype(K)::value;
+    float t;
+    v[c + 0] = d[c + 0] + d[c + 4];
+    v[c + 4] = d[c + 0] - d[c + 4];
+    v[c + 2] = d[c + 2];
+    v[c + 6] = d[c + 6];
+    t = (d[c + 5] - d[c + 7]) * sq;
+    v[c + 1] = d[c + 1] + t;
+    v[c + 3] = d[c + 1] - t;
+    t = (d[c + 5] + d[c + 7]) * sq;
+    v[c + 5] = t + d[c + 3];
+    v[c + 7] = t - d[c + 3];
+  });
+  level_contig<3, 2, false>(v, tw);
+  level_contig<4, 1, false>(v, tw);
+}
+
+// Inverse of stage A: pass_4, pass_3 inverse (FFTReal.hpp:729-776), inverse pass 3
+// (:834-862), inverse pass 1+2 (:866-906).  On exit out[r] is the time sample for
+// position stageA_pos(g, r) (the bit-reversed store of pass 1+2).
+SXC_HD inline void stageA_inverse(float* v, float* out, const Twiddle& tw) {
+  level_contig<4, 1, true>(v, tw);
+  level_contig<3, 2, true>(v, tw);
+  const float sq = SXC_SQRT2_2;
+  float d[32];
+  static_for<0, 4>([&](auto K) {
+    constexpr int c = 8 * decltype(K)::value;
+    d[c + 0] = v[c + 0] + v[c + 4];
+    d[c + 4] = v[c + 0] - v[c + 4];
+    d[c + 2] = v[c + 2] * 2;
+    d[c + 6] = v[c + 6] * 2;
+    d[c + 1] = v[c + 1] + v[c + 3];
+    d[c + 3] = v[c + 5] - v[c + 7];
+    const float vr = v[c + 1] - v[c + 3];
+    const float vi = v[c + 5] + v[c + 7];
+    d[c + 5] = (vr + vi) * sq;
+    d[c + 7] = (vi - vr) * sq;
+  });
+  static_for<0, 8>([&](auto K) {
+    constexpr int b = 4 * decltype(K)::value;
+    const float b0 = d[b + 0] + d[b + 2];
+    const float b2 = d[b + 0] - d[b + 2];
+    const float b1 = d[b + 1] * 2;
+    const float b3 = d[b + 3] * 2;
+    out[b + 0] = b0 + b1;
+    out[b + 1] = b0 - b1;
+    out[b + 2] = b2 + b3;
+    out[b + 3] = b2 - b3;
+  });
+}
+
+// ---------------------------------------------------------------- rounds
+
+SXC_HD constexpr int slot(int A, int s) { return 2 * A - (s < 0 ? 1 : 0); }
+
+// Regular unit j (1 <= j < h_m/2) of a K-level round starting at level M.  Slot (A, s)
+// holds position A*h_m + s*j of the unit's big block.
+template <int K, int M, int L, bool INV>
+SXC_HD inline void unit_level_regular(float* v, int j, const Twiddle& tw) {
+  constexpr int S = 1 << (K + 2);
+  constexpr int HM = 1 << (M - 1);
+  constexpr int HL = HM << L;
+  float o[S];
+  static_for<0, (1 << (K - 1 - L))>([&](auto C) {
+    constexpr int base = decltype(C)::value << (L + 2);
+    static_for<0, (1 << L) + 1>([&](auto AL) {
+      constexpr int al = decltype(AL)::value;
+      static_for<0, 2>([&](auto SGI) {
+        constexpr int sg = decltype(SGI)::value == 0 ? 1 : -1;
+        if constexpr (!((al == 0 && sg < 0) || (al == (1 << L) && sg > 0))) {
+          const int beta = al * HM + sg * j;
+          float c, s;
+          tw.template get<M + L>(beta, HL - beta, c, s);
+          constexpr int i0 = slot(base + al, sg);
+          constexpr int i1 = slot(base + al + (1 << L), sg);
+          constexpr int i2 = slot(base + al + (2 << L), sg);
+          constexpr int i3 = slot(base + al + 3 * (1 << L), sg);
+          constexpr int o0 = slot(base + al, sg);
+          constexpr int o1 = slot(base + (2 << L) - al, -sg);
+          constexpr int o2 = slot(base + (2 << L) + al, sg);
+          constexpr int o3 = slot(base + (4 << L) - al, -sg);
+          if constexpr (!INV)
+            bfly_fwd(v[i0], v[i1], v[i2], v[i3], c, s, o[o0], o[o1], o[o2], o[o3]);
+          else
+            bfly_inv(v[o0], v[o1], v[o2], v[o3], c, s, o[i0], o[i1], o[i2], o[i3]);
+        }
+      });
+    });
+  });
+  static_for<0, S>([&](auto I) { v[decltype(I)::value] = o[decltype(I)::value]; });
+}
+
+// The unit made of butterflies 0 and h_m/2 (and their descendants): slot q holds
+// position q * h_m/2 of the big block.
+template <int K, int M, int L, bool INV>
+SXC_HD inline void unit_level_special(float* v, const Twiddle& tw) {
+  constexpr int S = 1 << (K + 2);
+  constexpr int HM = 1 << (M - 1);
+  constexpr int HL = HM << L;
+  float o[S];
+  static_for<0, S>([&](auto I) { o[decltype(I)::value] = v[decltype(I)::value]; });
+  static_for<0, (1 << (K - 1 - L))>([&](auto C) {
+    constexpr int base = decltype(C)::value << (L + 3);
+    constexpr int q0 = base, q1 = base + (4 << L);
+    // extreme coefficients (FFTReal.hpp:575-578 / 751-754)
+    o[q0] = v[q0] + v[q1];
+    o[q1] = v[q0] - v[q1];
+    if constexpr (INV) {
+      constexpr int qa = base + (2 << L), qb = base + 3 * (2 << L);
+      o[qa] = v[qa] * 2;
+      o[qb] = v[qb] * 2;
+    }
+    static_for<1, (2 << L)>([&](auto G) {
+      constexpr int g = decltype(G)::value;
+      constexpr int beta = g * (HM / 2);
+      float c, s;
+      tw.template get<M + L>(beta, HL - beta, c, s);
+      constexpr int i0 = base + g, i1 = base + g + (2 << L), i2 = base + g + (4 << L), i3 = base + g + 3 * (2 << L);
+      constexpr int o0 = base + g, o1 = base + (4 << L) - g, o2 = base + (4 << L) + g, o3 = base + (8 << L) - g;
+      if constexpr (!INV)
+        bfly_fwd(v[i0], v[i1], v[i2], v[i3], c, s, o[o0], o[o1], o[o2], o[o3]);
+      else
+        bfly_inv(v[o0], v[o1], v[o2], v[o3], c, s, o[i0], o[i1], o[i2], o[i3]);
+    });
+  });
+  static_for<0, S>([&](auto I) { v[decltype(I)::value] = o[decltype(I)::value]; });
+}
+
+// Round geometry for NB with K levels from M.
+template <int NB, int K, int M>
+struct RoundGeo {
+  static constexpr int N = 1 << NB;
+  static constexpr int HM = 1 << (M - 1);
+  static constexpr int UPB = HM / 2;                  // units per big block
+  static constexpr int BIG = (1 << (K + 1)) * HM;     // big block length
+  static constexpr int NBIG = N / BIG;
+  static constexpr int NREG = NBIG * (UPB - 1);       // regular units
+  static constexpr int UNITS = NBIG * UPB;            // = N / 2^(K+2)
+  static constexpr int S = 1 << (K + 2);
+};
+
+// Runs unit u of a round on the padded LDS array Y.  Units [0, NREG) are regular,
+// [NREG, UNITS) special, so only one wavefront per round mixes the two code paths.
+template <int NB, int K, int M, bool INV>
+SXC_HD inline void round_unit(float* Y, int u, const Twiddle& tw) {
+  using RG = RoundGeo<NB, K, M>;
+  float v[RG::S];
+  if (u < RG::NREG) {
+    const int b = u / (RG::UPB - 1);
+    const int j = 1 + u % (RG::UPB - 1);
+    const int off = b * RG::BIG;
+    static_for<0, RG::S>([&](auto I) {
+      constexpr int q = decltype(I)::value;
+      constexpr int A = (q + 1) >> 1;
+      constexpr int sg = (q & 1) ? -1 : 1;
+      v[q] = Y[pad(off + A * RG::HM + sg * j)];
+    });
+    if constexpr (!INV) {
+      static_for<0, K>([&](auto L) { unit_level_regular<K, M, decltype(L)::value, false>(v, j, tw); });
+    } else {
+      static_for<0, K>([&](auto L) { unit_level_regular<K, M, K - 1 - decltype(L)::value, true>(v, j, tw); });
+    }
+    static_for<0, RG::S>([&](auto I) {
+      constexpr int q = decltype(I)::value;
+      constexpr int A = (q + 1) >> 1;
+      constexpr int sg = (q & 1) ? -1 : 1;
+      Y[pad(off + A * RG::HM + sg * j)] = v[q];
+    });
+  } else {
+    const int off = (u - RG::NREG) * RG::BIG;
+    static_for<0, RG::S>([&](auto I) {
+      constexpr int q = decltype(I)::value;
+      v[q] = Y[pad(off + q * (RG::HM / 2))];
+    });
+    if constexpr (!INV) {
+      static_for<0, K>([&](auto L) { unit_level_special<K, M, decltype(L)::value, false>(v, tw); });
+    } else {
+      static_for<0, K>([&](auto L) { unit_level_special<K, M, K - 1 - decltype(L)::value, true>(v, tw); });
+    }
+    static_for<0, RG::S>([&](auto I) {
+      constexpr int q = decltype(I)::value;
+      Y[pad(off + q * (RG::HM / 2))] = v[q];
+    });
+  }
+}
+
+// Level schedule above stage A: levels 5 .. NB-1 in rounds of 3, the remainder last.
+template <int NB>
+struct Schedule {
+  static constexpr int R = NB - 5;
+  static constexpr int NR = (R + 2) / 3;
+  static constexpr int k(int r) { return (r < R / 3) ? 3 : (R - 3 * (R / 3)); }
+  static constexpr int m(int r) { return 5 + 3 * r; }
+};
+
+}  // namespace sxc

@@ -1,0 +1,246 @@
+// sxc_kernels.hip -- gfx950 kernels of the chunk-pair path and their launchers.
+//
+// Build: hipcc --offload-arch=gfx950 -O3 -ffp-contract=off (FFTReal and the reference's
+// double arithmetic are never contracted into FMA; see sxc_fft.h).
+#include <hip/hip_runtime.h>
+
+#include "sxc_kernels.h"
+#include "sxc_pipeline.h"
+
+namespace sxc {
+
+// Device executor: one HIP thread per emulated thread; per-thread state in registers.
+template <class R>
+struct DevExec {
+  R& r;
+  template <class F>
+  __device__ __forceinline__ void each(F&& f) {
+    f((int)threadIdx.x);
+  }
+  __device__ __forceinline__ void sync() { __syncthreads(); }
+  template <class T>
+  __device__ __forceinline__ T& reg(int) {
+    return r;
+  }
+  __device__ __forceinline__ int atomic_add_shared(int* p, int v) { return atomicAdd(p, v); }
+};
+
+// Workgroup -> (pair, strand): the two strands of a pair read the same resident target
+// spectra, so they are placed 8 workgroups apart, which the dispatcher sends to the same
+// XCD (and its L2) under round-robin XCD assignment.
+__device__ __forceinline__ int xcd_pair_strand(int b, int strands, int total) {
+  if (strands != 2) return b;
+  const int grp = b / 16, r = b % 16;
+  const int ps = (grp * 8 + (r % 8)) * 2 + (r / 8);
+  return ps < total ? ps : -1;
+}
+
+template <int NB>
+__global__ __launch_bounds__(Geo<NB>::NT) void k_pair(PairArgs a, int total, int grid) {
+  __shared__ __align__(16) char sm[Smem<NB>::BYTES];
+  Regs<NB> r;
+  DevExec<Regs<NB>> ex{r};
+  // grid is rounded up to a multiple of 16 for the strand interleave
+  const int ps = xcd_pair_strand((int)blockIdx.x, a.strands, total);
+  if (ps < 0) return;
+  (void)grid;
+  pair_body<NB>(ex, sm, a, ps);
+}
+
+template <int NB>
+__global__ __launch_bounds__(Geo<NB>::NT) void k_spectra(SpecArgs a) {
+  __shared__ __align__(16) char sm[Smem<NB>::BYTES];
+  Regs<NB> r;
+  DevExec<Regs<NB>> ex{r};
+  spectra_body<NB>(ex, sm, a, (int)blockIdx.x);
+}
+
+// Plain-API kernels ----------------------------------------------------------------
+
+// CCSignal::SetSequence: the 4*N encoded signal of one chunk.
+template <int NB>
+__global__ __launch_bounds__(Geo<NB>::NT) void k_encode(EncodeArgs a) {
+  __shared__ __align__(16) char sm[Smem<NB>::BYTES];
+  Regs<NB> r;
+  DevExec<Regs<NB>> ex{r};
+  const sxc_chunk ck = a.chunks[blockIdx.x];
+  phase_prepare<NB>(ex, sm, a.tab, a.ent_lut, a.seq + ck.off, ck.len, ck.flags & 1, a.strand);
+  for (int ch = 0; ch < 4; ++ch)
+    for (int p = threadIdx.x; p < Geo<NB>::N; p += Geo<NB>::NT)
+      a.out[((size_t)blockIdx.x * 4 + ch) * Geo<NB>::N + p] = encode_at<NB>(sm, ck.len, p, ch);
+}
+
+// CrossCorrelation::CrossCorrelate of given signals against resident target spectra.
+template <int NB>
+__global__ __launch_bounds__(Geo<NB>::NT) void k_xcorr(XcorrArgs a) {
+  __shared__ __align__(16) char sm[Smem<NB>::BYTES];
+  Regs<NB> r;
+  DevExec<Regs<NB>> ex{r};
+  using SM = Smem<NB>;
+  float* lut = (float*)(sm + SM::LUT);
+  for (int i = threadIdx.x; i < 2048; i += Geo<NB>::NT) lut[i] = a.tab->lut12[i];
+  __syncthreads();
+  const Twiddle tw{lut, a.osc};
+  const int k = blockIdx.x;
+  correlate<NB>(ex, sm, a.tab, tw, 0, a.spectra + (size_t)k * 4 * Geo<NB>::N, a.qsig + (size_t)k * 4 * Geo<NB>::N);
+  const float* Y = (const float*)(sm + SM::Y);
+  for (int p = threadIdx.x; p < Geo<NB>::N; p += Geo<NB>::NT) a.xc[(size_t)k * Geo<NB>::N + p] = Y[pad(p)];
+}
+
+// SeqAnalyzer::MatchUp on a given xc: peaks + scan, runs written instead of filtered.
+template <int NB>
+__global__ __launch_bounds__(Geo<NB>::NT) void k_matchup(MatchupArgs a) {
+  __shared__ __align__(16) char sm[Smem<NB>::BYTES];
+  Regs<NB> r;
+  DevExec<Regs<NB>> ex{r};
+  matchup_body<NB>(ex, sm, a, (int)blockIdx.x);
+}
+
+// GetMatchProbabilityEx + PrintMatch identity for given runs (one thread per run).
+__global__ void k_prob(ProbArgs a) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= a.n) return;
+  run_tail_direct(a, k);
+}
+
+// Exclusive scan of per-(pair, strand) kept counts (clipped to cap) -> offsets, total.
+__global__ __launch_bounds__(1024) void k_scan_counts(const int32_t* counts, int n, int cap, int64_t* offsets,
+                                                      int64_t* total, int32_t* max_count) {
+  __shared__ int64_t part[1024];
+  __shared__ int mx[1024];
+  const int t = threadIdx.x;
+  const int per = (n + 1023) / 1024;
+  const int b = t * per, e = (b + per < n) ? b + per : n;
+  int64_t s = 0;
+  int m = 0;
+  for (int i = b; i < e; ++i) {
+    const int c = counts[i];
+    s += c < cap ? c : cap;
+    m = c > m ? c : m;
+  }
+  part[t] = s;
+  mx[t] = m;
+  __syncthreads();
+  if (t == 0) {
+    int64_t acc = 0;
+    int mm = 0;
+    for (int i = 0; i < 1024; ++i) {
+      const int64_t v = part[i];
+      part[i] = acc;
+      acc += v;
+      mm = mx[i] > mm ? mx[i] : mm;
+    }
+    *total = acc;
+    *max_count = mm;
+  }
+  __syncthreads();
+  int64_t acc = part[t];
+  for (int i = b; i < e; ++i) {
+    offsets[i] = acc;
+    const int c = counts[i];
+    acc += c < cap ? c : cap;
+  }
+}
+
+// Orders each (pair, strand)'s kept records by (lag rank, run start) -- the order
+// FilterMatches pushes them -- and packs them at their offsets.
+__global__ __launch_bounds__(256) void k_finalize(const sxc_result_dev* slots, const uint64_t* keys,
+                                                  const int32_t* counts, const int64_t* offsets, int cap,
+                                                  sxc_result_dev* out) {
+  const int ps = blockIdx.x;
+  const int n = counts[ps] < cap ? counts[ps] : cap;
+  const size_t base = (size_t)ps * cap;
+  for (int e = threadIdx.x; e < n; e += blockDim.x) {
+    const uint64_t ke = keys[base + e];
+    int rank = 0;
+    for (int k = 0; k < n; ++k) rank += keys[base + k] < ke;
+    out[offsets[ps] + rank] = slots[base + e];
+  }
+}
+
+// Orders MatchUp runs by (lag rank, run start): rank by counting smaller keys.
+__global__ __launch_bounds__(256) void k_order_runs(const int32_t* runs, const uint64_t* keys, int n, int32_t* out) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  const uint64_t ke = keys[e];
+  int rank = 0;
+  for (int k = 0; k < n; ++k) rank += keys[k] < ke;
+  out[3 * rank] = runs[3 * e];
+  out[3 * rank + 1] = runs[3 * e + 1];
+  out[3 * rank + 2] = runs[3 * e + 2];
+}
+
+// ---------------------------------------------------------------- launchers
+
+hipError_t launch_order_runs(const int32_t* runs, const uint64_t* keys, int n, int32_t* out, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_order_runs, dim3((n + 255) / 256), dim3(256), 0, s, runs, keys, n, out);
+  return hipGetLastError();
+}
+
+#define SXC_NB_SWITCH(NB, CALL)      \
+  switch (NB) {                      \
+    case 9: CALL(9); break;          \
+    case 10: CALL(10); break;        \
+    case 11: CALL(11); break;        \
+    case 12: CALL(12); break;        \
+    case 13: CALL(13); break;        \
+    case 14: CALL(14); break;        \
+    default: return hipErrorInvalidValue; \
+  }
+
+hipError_t launch_pair(int nb, const PairArgs& a, int total, hipStream_t s) {
+  const int grid = (a.strands == 2) ? ((total + 15) / 16) * 16 : total;
+#define CALL(X) hipLaunchKernelGGL(k_pair<X>, dim3(grid), dim3(Geo<X>::NT), 0, s, a, total, grid)
+  SXC_NB_SWITCH(nb, CALL)
+#undef CALL
+  return hipGetLastError();
+}
+
+hipError_t launch_spectra(int nb, const SpecArgs& a, hipStream_t s) {
+#define CALL(X) hipLaunchKernelGGL(k_spectra<X>, dim3(a.n), dim3(Geo<X>::NT), 0, s, a)
+  SXC_NB_SWITCH(nb, CALL)
+#undef CALL
+  return hipGetLastError();
+}
+
+hipError_t launch_encode(int nb, const EncodeArgs& a, int n, hipStream_t s) {
+#define CALL(X) hipLaunchKernelGGL(k_encode<X>, dim3(n), dim3(Geo<X>::NT), 0, s, a)
+  SXC_NB_SWITCH(nb, CALL)
+#undef CALL
+  return hipGetLastError();
+}
+
+hipError_t launch_xcorr(int nb, const XcorrArgs& a, int n, hipStream_t s) {
+#define CALL(X) hipLaunchKernelGGL(k_xcorr<X>, dim3(n), dim3(Geo<X>::NT), 0, s, a)
+  SXC_NB_SWITCH(nb, CALL)
+#undef CALL
+  return hipGetLastError();
+}
+
+hipError_t launch_matchup(int nb, const MatchupArgs& a, int n, hipStream_t s) {
+#define CALL(X) hipLaunchKernelGGL(k_matchup<X>, dim3(n), dim3(Geo<X>::NT), 0, s, a)
+  SXC_NB_SWITCH(nb, CALL)
+#undef CALL
+  return hipGetLastError();
+}
+
+hipError_t launch_prob(const ProbArgs& a, hipStream_t s) {
+  if (a.n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_prob, dim3((a.n + 255) / 256), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_finalize(const int32_t* counts, int n, int cap, int64_t* offsets, int64_t* total,
+                           int32_t* max_count, const sxc_result_dev* slots, const uint64_t* keys,
+                           sxc_result_dev* out, hipStream_t s, bool scan_only) {
+  if (n <= 0) return hipSuccess;
+  if (scan_only) {
+    hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(1024), 0, s, counts, n, cap, offsets, total, max_count);
+  } else {
+    hipLaunchKernelGGL(k_finalize, dim3(n), dim3(256), 0, s, slots, keys, counts, offsets, cap, out);
+  }
+  return hipGetLastError();
+}
+
+}  // namespace sxc

@@ -1,0 +1,743 @@
+// sxc_pipeline.h -- the chunk-pair path as workgroup phases.
+//
+// Each kernel body is written once against an executor `Ex` that provides
+//   ex.each(f)   run f(tid) for every thread of the workgroup
+//   ex.sync()    workgroup barrier
+//   ex.reg(tid)  per-thread persistent state (registers on the GPU)
+//   ex.atomic_add_shared / ex.atomic_add_global
+// The HIP kernels instantiate it with the device executor (sxc_kernels.hip); the CPU
+// workgroup emulator used by the tests instantiates it with a sequential one.
+//
+// Phases and the reference functions they restate:
+//   prepare     CCSignal::ComputeEntropy + SeqToPCM means (analysis/CrossCorr.cc:35-134)
+//   spectrum    encode (SetSequence, :179-206) fused into FFTReal do_fft
+//   correlate   DoOne product (:477-492) + do_ifft + rescale + fftshift (:493-505), summed
+//               over A, C, G, T in that order (:395-403)
+//   peaks       SeqAnalyzer::FindTop (:878-944)
+//   scan        SeqAnalyzer::DoOne, OPTIM_DOONE (:667-724) over every accepted lag
+//   tail        GetMatchProbabilityEx (analysis/AlignProbability.cc:62-127), PrintMatch
+//               identity (analysis/CrossCorr.cc:967-1037), FilterMatches
+//               (analysis/HomologyByXCorrSlave.cc:168-219)
+#pragma once
+#include <math.h>
+
+#include "sxc_fft.h"
+
+namespace sxc {
+
+struct PairArgs {
+  const sxc_tables* tab;
+  const float* osc;          // oscillator twiddles for levels 13, 14
+  const double* ent_lut;     // entropy of quarter-exact windows for this N, [k-1][q]
+  const unsigned char* seq;  // all chunk bytes
+  const sxc_chunk* tchunks;
+  const sxc_chunk* qchunks;
+  const sxc_pair_dev* pairs;
+  const float* spectra;      // resident target spectra, 4*N floats per target chunk
+  int32_t npairs;
+  int32_t strands;           // 1 = forward only, 2 = forward then reverse complement
+  int32_t min_len;           // -l
+  int32_t q_chunk;           // -q_chunk (RCQuery chunk size)
+  double cutoff;
+  double cutoff_fast;
+  double min_prob;
+  double target_total;
+  sxc_result_dev* out;       // cap records per (pair, strand)
+  uint64_t* keys;            // ordering key per record
+  int32_t* counts;           // kept records per (pair, strand) (may exceed cap)
+  int32_t cap;
+  float* xc_out;             // optional: N floats per (pair, strand)
+  int32_t* stats;            // optional: [candidates, runs] per (pair, strand)
+};
+
+struct SpecArgs {
+  const sxc_tables* tab;
+  const float* osc;
+  const double* ent_lut;
+  const unsigned char* seq;
+  const sxc_chunk* chunks;
+  const int32_t* which;      // chunk indices to transform
+  int32_t n;
+  float* spectra;            // output: 4*N floats per entry of `which`, at spectra + dst[k]*4N
+  const int32_t* dst;        // destination slot per entry
+  const float* signals;      // optional: given 4*N signals instead of encoding chunk bytes
+};
+
+struct EncodeArgs {
+  const sxc_tables* tab;
+  const double* ent_lut;
+  const unsigned char* seq;
+  const sxc_chunk* chunks;
+  int32_t strand;
+  float* out;                // 4*N floats per chunk
+};
+
+struct XcorrArgs {
+  const sxc_tables* tab;
+  const float* osc;
+  const float* spectra;      // 4*N target spectra per item
+  const float* qsig;         // 4*N query signals per item
+  float* xc;                 // N floats per item
+};
+
+struct MatchupArgs {
+  const sxc_tables* tab;
+  const double* ent_lut;
+  const unsigned char* seq;
+  const sxc_chunk* tchunks;  // one target chunk per item
+  const sxc_chunk* qchunks;  // one query chunk per item
+  const float* xc;           // N floats per item
+  double cutoff;
+  int32_t* runs;             // cap x {start_target, start_query, len} per item
+  uint64_t* keys;
+  int32_t* counts;
+  int32_t cap;
+};
+
+struct ProbArgs {
+  const sxc_tables* tab;
+  const unsigned char* t;
+  const unsigned char* q;
+  const int32_t* runs;       // n x {start_target, start_query, len}
+  int32_t n;
+  double target_size;
+  double* prob;
+  double* ident;
+};
+
+// ---------------------------------------------------------------- shared memory plan
+
+template <int NB>
+struct Smem {
+  using G = Geo<NB>;
+  static constexpr int N = G::N;
+  static constexpr int NT = G::NT;
+  static constexpr int NWIN = 512;  // max entropy windows (L <= N, win = N/512)
+  // byte offsets
+  static constexpr int Y = 0;                                  // float[YLEN]: FFT / xc / scan area
+  static constexpr int LUT = Y + 4 * G::YLEN;                  // float[2048]
+  static constexpr int ENT = LUT + 4 * 2048;                   // float[NWIN]
+  static constexpr int ENV = ENT + 4 * NWIN;                   // double[max(N/256, 1)]
+  static constexpr int NENV = (N / 256) > 0 ? N / 256 : 1;
+  static constexpr int OFF = ENV + 8 * NENV;                   // double[4] channel means
+  static constexpr int DSUM = OFF + 32;                        // double[4] sequential sums
+  static constexpr int MISC = DSUM + 32;                       // int32[16]
+  static constexpr int CNT = MISC + 64;                        // int32[NT] per-thread counts
+  static constexpr int CODEC = CNT + 4 * NT;                   // double[18][4]
+  static constexpr int EQUAL = CODEC + 8 * 4 * SXC_NCLASS;     // double[18][18]
+  static constexpr int GC = EQUAL + 8 * SXC_NCLASS * SXC_NCLASS;  // double[18]
+  static constexpr int CLS = GC + 8 * SXC_NCLASS;              // uchar[128] byte -> class
+  static constexpr int SCORE = CLS + 128;                      // uchar[18*18]
+  static constexpr int QUART = SCORE + SXC_NCLASS * SXC_NCLASS;  // uchar[18][4]
+  static constexpr int QSEQ = (QUART + 4 * SXC_NCLASS + 15) & ~15;  // uchar[N] strand query
+  static constexpr int BYTES = (QSEQ + N + 15) & ~15;
+  // scan-phase carve-up of Y (after the peaks phase)
+  static constexpr int CAND = Y;                               // uint16[N]
+  static constexpr int TSEQ = Y + 2 * N;                       // uchar[N]
+  static constexpr int SUMM = TSEQ + N;                        // int2[NT]
+  static_assert(SUMM + 8 * NT <= Y + 4 * G::YLEN, "scan carve-up exceeds the FFT area");
+};
+
+enum { MISC_L = 0, MISC_THIRDS = 1, MISC_NCAND = 2, MISC_KEPT = 3, MISC_Q4 = 4 /* 4 ints */, MISC_RUNS = 8 };
+
+template <int NB>
+struct Regs {
+  float acc[32];  // correlation accumulated over channels, stage-A output order
+};
+
+// ---------------------------------------------------------------- helpers
+
+SXC_HD inline double ent_term(double p) {
+  // Ent (analysis/CrossCorr.cc:28-33)
+  if (p < 0.001) return 0;
+  return p * log(p) / 0.69314718056;
+}
+
+SXC_HD inline int score_of(const unsigned char* cls, const unsigned char* score, unsigned char tb, unsigned char qb) {
+  // LookupMatch::precalc_scores = (int)(100 * DNA_EqualAmb + 0.5) (analysis/CrossCorr.cc:547-553);
+  // DNA_EqualAmb scores identical letters other than 'N' as 1 (analysis/DNAVector.cc:372-387)
+  if (tb == qb && tb != 'N') return 100;
+  return score[cls[tb] * SXC_NCLASS + cls[qb]];
+}
+
+// Loads one chunk's bytes (reverse-complemented for strand 1) to LDS and computes the
+// per-window entropy weights and the four channel means.
+template <int NB, class Ex>
+SXC_HD inline void phase_prepare(Ex& ex, char* sm, const sxc_tables* tab, const double* ent_lut,
+                                 const unsigned char* src, int L, int thirds, int strand) {
+  using SM = Smem<NB>;
+  using G = Geo<NB>;
+  float* lut = (float*)(sm + SM::LUT);
+  unsigned char* qs = (unsigned char*)(sm + SM::QSEQ);
+  unsigned char* cls = (unsigned char*)(sm + SM::CLS);
+  unsigned char* score = (unsigned char*)(sm + SM::SCORE);
+  unsigned char* quart = (unsigned char*)(sm + SM::QUART);
+  double* codec = (double*)(sm + SM::CODEC);
+  double* equal = (double*)(sm + SM::EQUAL);
+  double* gcv = (double*)(sm + SM::GC);
+  int* misc = (int*)(sm + SM::MISC);
+  ex.each([&](int tid) {
+    for (int i = tid; i < 2048; i += G::NT) lut[i] = tab->lut12[i];
+    for (int i = tid; i < 128; i += G::NT) cls[i] = tab->cls[i];
+    for (int i = tid; i < SXC_NCLASS * SXC_NCLASS; i += G::NT) {
+      score[i] = (&tab->score[0][0])[i];
+      equal[i] = (&tab->equal[0][0])[i];
+    }
+    for (int i = tid; i < 4 * SXC_NCLASS; i += G::NT) {
+      quart[i] = (&tab->quarters[0][0])[i];
+      codec[i] = (&tab->codec[0][0])[i];
+    }
+    for (int i = tid; i < SXC_NCLASS; i += G::NT) gcv[i] = tab->gc[i];
+    // DNAVector::ReverseComplement (analysis/DNAVector.cc:482-521)
+    for (int i = tid; i < L; i += G::NT) qs[i] = strand ? tab->rc[src[L - 1 - i] & 127] : src[i];
+    if (tid < 16) misc[tid] = 0;
+  });
+  ex.sync();
+  float* ent = (float*)(sm + SM::ENT);
+  double* dsum = (double*)(sm + SM::DSUM);
+  const int win = G::N / 512;
+  ex.each([&](int tid) {
+    if (L < 1024) return;  // entropy weight is 1 (analysis/CrossCorr.cc:39-44)
+    const int nwin = (L + win - 1) / win;
+    for (int w = tid; w < nwin; w += G::NT) {
+      const int i = w * win;
+      const int k = (L - i < win) ? (L - i) : win;
+      double s;
+      if (!thirds) {
+        int q[4] = {0, 0, 0, 0};
+        for (int j = i; j < i + k; ++j) {
+          const unsigned char* qq = quart + 4 * cls[qs[j]];
+          q[0] += qq[0];
+          q[1] += qq[1];
+          q[2] += qq[2];
+          q[3] += qq[3];
+        }
+        const double* row = ent_lut + (size_t)(k - 1) * (4 * win + 1);
+        s = row[q[0]] + row[q[1]] + row[q[2]] + row[q[3]];
+      } else {
+        double a = 0, c = 0, g = 0, t = 0;
+        for (int j = i; j < i + k; ++j) {
+          const double* cd = codec + 4 * cls[qs[j]];
+          a += cd[0];
+          c += cd[1];
+          g += cd[2];
+          t += cd[3];
+        }
+        a /= (double)k;
+        c /= (double)k;
+        g /= (double)k;
+        t /= (double)k;
+        s = ent_term(a) + ent_term(c) + ent_term(g) + ent_term(t);
+      }
+      float e = (float)(-s);
+      if (e < 0.f) e = 0.f;
+      ent[w] = e;
+    }
+  });
+  // channel sums: exact in quarters unless a 1/3-valued letter is present, in which case
+  // the reference's sequential double order is replayed (analysis/CrossCorr.cc:108-116)
+  ex.each([&](int tid) {
+    if (!thirds) {
+      int q[4] = {0, 0, 0, 0};
+      for (int i = tid; i < L; i += G::NT) {
+        const unsigned char* qq = quart + 4 * cls[qs[i]];
+        q[0] += qq[0];
+        q[1] += qq[1];
+        q[2] += qq[2];
+        q[3] += qq[3];
+      }
+      for (int ch = 0; ch < 4; ++ch)
+        if (q[ch]) ex.atomic_add_shared(&misc[MISC_Q4 + ch], q[ch]);
+    } else if (tid < 4) {
+      double s = 0;
+      for (int i = 0; i < L; ++i) s += codec[4 * cls[qs[i]] + tid];
+      dsum[tid] = s;
+    }
+  });
+  ex.sync();
+  double* off = (double*)(sm + SM::OFF);
+  ex.each([&](int tid) {
+    if (tid < 4) {
+      const double sum = thirds ? dsum[tid] : (double)misc[MISC_Q4 + tid] * 0.25;
+      off[tid] = sum / (double)L;
+    }
+  });
+  ex.sync();
+}
+
+// Signal value of channel ch at time position p (CCSignal::SeqToPCM, :118-131).
+template <int NB>
+SXC_HD inline float encode_at(const char* sm, int L, int p, int ch) {
+  using SM = Smem<NB>;
+  if (p >= L) return 0.f;
+  const unsigned char* qs = (const unsigned char*)(sm + SM::QSEQ);
+  const unsigned char* cls = (const unsigned char*)(sm + SM::CLS);
+  const double* codec = (const double*)(sm + SM::CODEC);
+  const float e = (L < 1024) ? 1.f : ((const float*)(sm + SM::ENT))[p / (Geo<NB>::N / 512)];
+  const double off = ((const double*)(sm + SM::OFF))[ch];
+  return (float)((double)e * (codec[4 * cls[qs[p]] + ch] - off));
+}
+
+// Forward FFT of one channel into the padded LDS array: stage A (encode fused into
+// the bit-reversed load, or loaded from a given signal) then the rounds.
+template <int NB, int R, class Ex>
+SXC_HD inline void fwd_rounds(Ex& ex, float* Y, const Twiddle& tw) {
+  using S = Schedule<NB>;
+  if constexpr (R < S::NR) {
+    constexpr int K = S::k(R), M = S::m(R);
+    using RG = RoundGeo<NB, K, M>;
+    ex.each([&](int tid) {
+      for (int u = tid; u < RG::UNITS; u += Geo<NB>::NT) round_unit<NB, K, M, false>(Y, u, tw);
+    });
+    ex.sync();
+    fwd_rounds<NB, R + 1>(ex, Y, tw);
+  }
+}
+
+template <int NB, int R, class Ex>
+SXC_HD inline void inv_rounds(Ex& ex, float* Y, const Twiddle& tw) {
+  using S = Schedule<NB>;
+  if constexpr (R >= 0) {
+    constexpr int K = S::k(R), M = S::m(R);
+    using RG = RoundGeo<NB, K, M>;
+    ex.each([&](int tid) {
+      for (int u = tid; u < RG::UNITS; u += Geo<NB>::NT) round_unit<NB, K, M, true>(Y, u, tw);
+    });
+    ex.sync();
+    inv_rounds<NB, R - 1>(ex, Y, tw);
+  }
+}
+
+template <int NB, class Ex>
+SXC_HD inline void forward_channel(Ex& ex, char* sm, const sxc_tables* tab, const Twiddle& tw, int L, int ch,
+                                   const float* given) {
+  using SM = Smem<NB>;
+  using G = Geo<NB>;
+  float* Y = (float*)(sm + SM::Y);
+  ex.each([&](int g) {
+    if (g >= G::GROUPS) return;
+    float v[32];
+    static_for<0, 32>([&](auto I) {
+      constexpr int r = decltype(I)::value;
+      const int p = stageA_pos<NB>(g, r);
+      v[r] = given ? given[p] : encode_at<NB>(sm, L, p, ch);
+    });
+    stageA_forward(v, tw);
+    static_for<0, 32>([&](auto I) { Y[pad(32 * g + decltype(I)::value)] = v[decltype(I)::value]; });
+  });
+  ex.sync();
+  fwd_rounds<NB, 0>(ex, Y, tw);
+}
+
+// ---------------------------------------------------------------- target spectra
+
+// One workgroup per chunk: the four channel spectra (FFTReal layout) to global memory.
+template <int NB, class Ex>
+SXC_HD inline void spectra_body(Ex& ex, char* sm, const SpecArgs& a, int k) {
+  using SM = Smem<NB>;
+  using G = Geo<NB>;
+  const sxc_chunk ck = a.chunks[a.which[k]];
+  const Twiddle tw{(const float*)(sm + SM::LUT), a.osc};
+  if (!a.signals) {
+    phase_prepare<NB>(ex, sm, a.tab, a.ent_lut, a.seq + ck.off, ck.len, ck.flags & 1, 0);
+  } else {
+    float* lut = (float*)(sm + SM::LUT);
+    ex.each([&](int tid) {
+      for (int i = tid; i < 2048; i += G::NT) lut[i] = a.tab->lut12[i];
+    });
+    ex.sync();
+  }
+  float* Y = (float*)(sm + SM::Y);
+  float* out = a.spectra + (size_t)a.dst[k] * 4 * G::N;
+  for (int ch = 0; ch < 4; ++ch) {
+    const float* given = a.signals ? a.signals + ((size_t)k * 4 + ch) * G::N : nullptr;
+    forward_channel<NB>(ex, sm, a.tab, tw, ck.len, ch, given);
+    ex.each([&](int tid) {
+      for (int p = tid; p < G::N; p += G::NT) out[(size_t)ch * G::N + p] = Y[pad(p)];
+    });
+    ex.sync();
+  }
+}
+
+// ---------------------------------------------------------------- correlate
+
+// CrossCorrelation::DoOne product on bins 0 .. H-2; bins H-1 and H keep the target
+// spectrum (analysis/CrossCorr.cc:480-492).  Y holds the query spectrum, T the target's.
+template <int NB, class Ex>
+SXC_HD inline void product(Ex& ex, float* Y, const float* T) {
+  using G = Geo<NB>;
+  constexpr int H = G::H;
+  ex.each([&](int tid) {
+    for (int i = tid; i < H; i += G::NT) {
+      if (i == 0) {
+        Y[pad(0)] = T[0] * Y[pad(0)];
+        Y[pad(H)] = T[H];
+      } else if (i < H - 1) {
+        const float r1 = T[i], i1 = T[i + H];
+        const float r2 = Y[pad(i)], i2 = -Y[pad(i + H)];
+        const float fr = r1 * r2 - i1 * i2;  // ComplexMult (analysis/CrossCorr.cc:20-26)
+        const float fi = i1 * r2 + i2 * r1;
+        Y[pad(i)] = fr;
+        Y[pad(i + H)] = -fi;
+      } else {
+        Y[pad(H - 1)] = T[H - 1];
+        Y[pad(2 * H - 1)] = T[2 * H - 1];
+      }
+    }
+  });
+  ex.sync();
+}
+
+// xc = (((0 + A) + C) + G) + T of rescaled, fftshifted channel correlations, left in
+// natural order in Y.  Per-thread accumulators stay in registers across channels.
+template <int NB, class Ex>
+SXC_HD inline void correlate(Ex& ex, char* sm, const sxc_tables* tab, const Twiddle& tw, int L, const float* T4,
+                             const float* qsig4) {
+  using SM = Smem<NB>;
+  using G = Geo<NB>;
+  float* Y = (float*)(sm + SM::Y);
+  ex.each([&](int g) {
+    Regs<NB>& r = ex.template reg<Regs<NB>>(g);
+    static_for<0, 32>([&](auto I) { r.acc[decltype(I)::value] = 0.f; });
+  });
+  const float mul = (float)(1.0 / (double)G::N);  // FFTReal::rescale (FFTReal.hpp:264)
+  for (int ch = 0; ch < 4; ++ch) {
+    forward_channel<NB>(ex, sm, tab, tw, L, ch, qsig4 ? qsig4 + (size_t)ch * G::N : nullptr);
+    product<NB>(ex, Y, T4 + (size_t)ch * G::N);
+    inv_rounds<NB, Schedule<NB>::NR - 1>(ex, Y, tw);
+    ex.each([&](int g) {
+      if (g >= G::GROUPS) return;
+      Regs<NB>& r = ex.template reg<Regs<NB>>(g);
+      float v[32], o[32];
+      static_for<0, 32>([&](auto I) { v[decltype(I)::value] = Y[pad(32 * g + decltype(I)::value)]; });
+      stageA_inverse(v, o, tw);
+      static_for<0, 32>([&](auto I) {
+        constexpr int k = decltype(I)::value;
+        const float x = o[k] * mul;
+        r.acc[k] = r.acc[k] + x;
+      });
+    });
+    // the next channel's stage A writes the same positions from the same thread
+  }
+  ex.sync();
+  ex.each([&](int g) {
+    if (g >= G::GROUPS) return;
+    Regs<NB>& r = ex.template reg<Regs<NB>>(g);
+    static_for<0, 32>([&](auto I) {
+      constexpr int k = decltype(I)::value;
+      const int p = stageA_pos<NB>(g, k);
+      Y[pad(p ^ G::H)] = r.acc[k];  // fftshift by N/2 (analysis/CrossCorr.cc:500-505)
+    });
+  });
+  ex.sync();
+}
+
+// ---------------------------------------------------------------- peaks
+
+template <int NB, class Ex>
+SXC_HD inline void peaks(Ex& ex, char* sm, double cutoff) {
+  using SM = Smem<NB>;
+  using G = Geo<NB>;
+  constexpr int N = G::N;
+  constexpr int NP = SM::NENV;
+  constexpr int CS = N / G::NT;  // elements per thread, <= 32
+  static_assert(CS <= 32, "flag mask is 32 bits");
+  float* Y = (float*)(sm + SM::Y);
+  double* env = (double*)(sm + SM::ENV);
+  int* cnt = (int*)(sm + SM::CNT);
+  int* misc = (int*)(sm + SM::MISC);
+  ex.each([&](int tid) {
+    if (tid >= NP) return;
+    double e = 0.;
+    if (NP > 8) {
+      for (int i = tid * SXC_ENV_SIZE; i < (tid + 1) * SXC_ENV_SIZE; ++i) {
+        const float x = Y[pad(i)];
+        e += (double)(x * x);
+      }
+    }
+    env[tid] = sqrt(e / (double)SXC_ENV_SIZE);
+  });
+  ex.sync();
+  unsigned mask_reg[1];
+  (void)mask_reg;
+  ex.each([&](int tid) {
+    unsigned m = 0;
+    int c = 0;
+    for (int k = 0; k < CS; ++k) {
+      const int i = tid * CS + k;
+      if ((double)Y[pad(i)] > env[i / SXC_ENV_SIZE] * cutoff + 1.) {
+        m |= 1u << k;
+        ++c;
+      }
+    }
+    cnt[tid] = c;
+    ex.template reg<Regs<NB>>(tid).acc[0] = __builtin_bit_cast(float, m);
+  });
+  ex.sync();
+  ex.each([&](int tid) {
+    if (tid == 0) {
+      int s = 0;
+      for (int t = 0; t < G::NT; ++t) {
+        const int c = cnt[t];
+        cnt[t] = s;
+        s += c;
+      }
+      misc[MISC_NCAND] = s;
+    }
+  });
+  ex.sync();
+  uint16_t* cand = (uint16_t*)(sm + SM::CAND);
+  ex.each([&](int tid) {
+    unsigned m = __builtin_bit_cast(unsigned, ex.template reg<Regs<NB>>(tid).acc[0]);
+    int o = cnt[tid];
+    for (int k = 0; k < CS; ++k)
+      if (m & (1u << k)) cand[o++] = (uint16_t)(tid * CS + k);
+  });
+  ex.sync();
+}
+
+// ---------------------------------------------------------------- scan + tail
+
+struct RunCtx {
+  int shift, i0, npos, cand_rank;
+};
+
+// GetMatchProbabilityEx (analysis/AlignProbability.cc:62-127) on one match; `ident` is the
+// identity both it and PrintMatch return (DNA_Equal summed in the same order).
+SXC_HD inline double match_tail(const unsigned char* ts, const unsigned char* qs, const unsigned char* cls,
+                                const double* equal, const double* gcv, int start_t, int start_q, int len,
+                                double target_size, double& ident) {
+  double matches = 0, gct = 0, gcq = 0;
+  for (int i = 0; i < len; ++i) {
+    const int ca = cls[ts[start_t + i]], cb = cls[qs[start_q + i]];
+    matches += equal[ca * SXC_NCLASS + cb];
+    gct += gcv[ca];
+    gcq += gcv[cb];
+  }
+  ident = matches / (double)len;
+  const double gc_target = gct / (double)len;
+  // GCAdjustExpect / Sigma / CDF (analysis/AlignProbability.cc:11-40)
+  const double at_target = 1. - gc_target;
+  double r = gcq * gc_target;
+  r += ((double)len - gcq) * at_target;
+  const double p = r / (double)len / 2.;
+  const double s = sqrt(p * (1. - p) * (double)len);
+  const double m = p * (double)len;
+  const double x = (double)len * ident;
+  const double cdf = 0.5 * (1. + erf((m - x) / s / 1.414213562));
+  const double expect = cdf * target_size;
+  return exp(-expect);
+}
+
+SXC_HD inline void run_tail_direct(const ProbArgs& a, int k) {
+  const int32_t* r = a.runs + 3 * k;
+  double ident;
+  const double prob = match_tail(a.t, a.q, a.tab->cls, &a.tab->equal[0][0], a.tab->gc, r[0], r[1], r[2],
+                                 a.target_size, ident);
+  a.prob[k] = prob;
+  a.ident[k] = ident;
+}
+
+// FP64 tail of one run; appends a kept t_result (FilterMatches, analysis/HomologyByXCorrSlave.cc:168-219).
+template <int NB, class Ex>
+SXC_HD inline void tail_run(Ex& ex, char* sm, const PairArgs& a, const sxc_pair_dev& pr, int ps, int strand,
+                            const RunCtx& rc, int ns, int ne) {
+  using SM = Smem<NB>;
+  const int start_t = rc.i0 + ns - SXC_MIN_LEN;
+  const int len = ne - ns + SXC_MIN_LEN;
+  const int start_q = start_t + rc.shift;
+  int* misc = (int*)(sm + SM::MISC);
+  ex.atomic_add_shared(&misc[MISC_RUNS], 1);
+  if (len < a.min_len) return;
+  double ident;
+  const double prob = match_tail((const unsigned char*)(sm + SM::TSEQ), (const unsigned char*)(sm + SM::QSEQ),
+                                 (const unsigned char*)(sm + SM::CLS), (const double*)(sm + SM::EQUAL),
+                                 (const double*)(sm + SM::GC), start_t, start_q, len, a.target_total, ident);
+  if (prob < a.min_prob) return;
+  const int k = ex.atomic_add_shared(&misc[MISC_KEPT], 1);
+  if (k >= a.cap) return;
+  const int tstart = pr.t_chunk_start + start_t;
+  // RCQuery (analysis/HomologyByXCorrSlave.cc:56-60) for the reverse strand
+  const int qstart = strand == 0 ? pr.q_chunk_start + start_q : start_q + pr.q_record_len - pr.q_chunk_start - a.q_chunk;
+  sxc_result_dev res;
+  res.query_id = (uint64_t)(int64_t)pr.q_record;
+  res.target_id = (uint64_t)(int64_t)pr.t_record;
+  res.query_size = (uint64_t)(int64_t)pr.q_record_len;
+  res.qstart = (uint64_t)(int64_t)qstart;
+  res.tstart = (uint64_t)(int64_t)tstart;
+  res.len = (uint64_t)(int64_t)len;
+  res.reverse = (uint8_t)(strand == 1);
+  for (int z = 0; z < 7; ++z) res.pad_[z] = 0;
+  res.prob = prob;
+  res.ident = ident;
+  const size_t at = (size_t)ps * a.cap + k;
+  a.out[at] = res;
+  a.keys[at] = ((uint64_t)(uint32_t)rc.cand_rank << 32) | (uint32_t)ns;
+}
+
+// Diagonal scan of every accepted lag.  Lags are taken GSZ threads at a time; each
+// thread slides the 46-wide window over one contiguous segment of the diagonal.
+template <int NB, class Ex, class Emit>
+SXC_HD inline void scan(Ex& ex, char* sm, int Lq, int Lt, Emit&& emit) {
+  using SM = Smem<NB>;
+  using G = Geo<NB>;
+  constexpr int GSZ = (G::N <= 8192) ? 64 : 64 * (G::N / 8192);
+  constexpr int CE = G::NT / GSZ;  // lags per epoch
+  static_assert(CE >= 1, "workgroup smaller than a lag group");
+  constexpr int INF = 0x7fffffff;
+  const int* misc = (const int*)(sm + SM::MISC);
+  const int ncand = misc[MISC_NCAND];
+  const uint16_t* cand = (const uint16_t*)(sm + SM::CAND);
+  const unsigned char* ts = (const unsigned char*)(sm + SM::TSEQ);
+  const unsigned char* qs = (const unsigned char*)(sm + SM::QSEQ);
+  const unsigned char* cls = (const unsigned char*)(sm + SM::CLS);
+  const unsigned char* score = (const unsigned char*)(sm + SM::SCORE);
+  int* summ = (int*)(sm + SM::SUMM);  // [tid][2] = {first_end, open_start}
+  for (int e0 = 0; e0 < ncand; e0 += CE) {
+    ex.each([&](int tid) {
+      const int c = e0 + tid / GSZ;
+      const int l = tid % GSZ;
+      int first_end = INF, open = -1;
+      if (c < ncand) {
+        RunCtx rc;
+        rc.cand_rank = c;
+        rc.shift = (int)cand[c] - G::H;
+        rc.i0 = rc.shift < 0 ? -rc.shift : 0;
+        int iend = Lq - rc.shift;
+        if (Lt - 1 < iend) iend = Lt - 1;
+        rc.npos = (rc.i0 < Lt && iend > rc.i0) ? iend - rc.i0 : 0;
+        const int S = (rc.npos + GSZ - 1) / GSZ;
+        const int na = l * S;
+        const int nb = (na + S < rc.npos) ? na + S : rc.npos;
+        if (na < nb) {
+          const int base_t = rc.i0, base_q = rc.i0 + rc.shift;
+          int W = 0;
+          for (int m = (na - 46 > 0 ? na - 46 : 0); m < na; ++m) W += score_of(cls, score, ts[base_t + m], qs[base_q + m]);
+          bool prev = (na - 1 >= 46) && W > SXC_WIN_THRESHOLD;
+          for (int n = na; n < nb; ++n) {
+            W += score_of(cls, score, ts[base_t + n], qs[base_q + n]);
+            if (n >= 46) W -= score_of(cls, score, ts[base_t + n - 46], qs[base_q + n - 46]);
+            const bool on = (n >= 46) && W > SXC_WIN_THRESHOLD;
+            if (on && !prev) {
+              open = n;
+            } else if (!on && prev) {
+              if (open >= 0) {
+                emit(rc, open, n);
+                open = -1;
+              } else if (first_end == INF) {
+                first_end = n;
+              }
+            }
+            prev = on;
+          }
+        }
+      }
+      summ[2 * tid] = first_end;
+      summ[2 * tid + 1] = open;
+    });
+    ex.sync();
+    ex.each([&](int tid) {
+      const int c = e0 + tid / GSZ;
+      const int l = tid % GSZ;
+      const int open = summ[2 * tid + 1];
+      if (c >= ncand || open < 0) return;
+      int end = INF;
+      const int g0 = tid - l;
+      for (int k = l + 1; k < GSZ; ++k) {
+        const int fe = summ[2 * (g0 + k)];
+        if (fe < end) end = fe;
+      }
+      RunCtx rc;
+      rc.cand_rank = c;
+      rc.shift = (int)cand[c] - G::H;
+      rc.i0 = rc.shift < 0 ? -rc.shift : 0;
+      int iend = Lq - rc.shift;
+      if (Lt - 1 < iend) iend = Lt - 1;
+      rc.npos = iend - rc.i0;
+      if (end == INF) end = rc.npos;  // run still open where the reference loop breaks
+      emit(rc, open, end);
+    });
+    ex.sync();
+  }
+}
+
+// ---------------------------------------------------------------- fused pair body
+
+// One workgroup per (pair, strand): encode -> FFT -> product -> iFFT -> peaks -> scan -> tail.
+template <int NB, class Ex>
+SXC_HD inline void pair_body(Ex& ex, char* sm, const PairArgs& a, int ps) {
+  using SM = Smem<NB>;
+  using G = Geo<NB>;
+  const int pair = ps / a.strands;
+  const int strand = ps % a.strands;
+  const sxc_pair_dev pr = a.pairs[pair];
+  const sxc_chunk tc = a.tchunks[pr.t_chunk];
+  const sxc_chunk qc = a.qchunks[pr.q_chunk];
+  const Twiddle tw{(const float*)(sm + SM::LUT), a.osc};
+  phase_prepare<NB>(ex, sm, a.tab, a.ent_lut, a.seq + qc.off, qc.len, qc.flags & 1, strand);
+  correlate<NB>(ex, sm, a.tab, tw, qc.len, a.spectra + (size_t)pr.t_chunk * 4 * G::N, nullptr);
+  if (a.xc_out) {
+    const float* Y = (const float*)(sm + SM::Y);
+    ex.each([&](int tid) {
+      for (int p = tid; p < G::N; p += G::NT) a.xc_out[(size_t)ps * G::N + p] = Y[pad(p)];
+    });
+  }
+  peaks<NB>(ex, sm, pr.fast ? a.cutoff_fast : a.cutoff);
+  unsigned char* ts = (unsigned char*)(sm + SM::TSEQ);
+  const unsigned char* tsrc = a.seq + tc.off;
+  ex.each([&](int tid) {
+    for (int i = tid; i < tc.len; i += G::NT) ts[i] = tsrc[i];
+  });
+  ex.sync();
+  scan<NB>(ex, sm, qc.len, tc.len,
+           [&](const RunCtx& rc, int ns, int ne) { tail_run<NB>(ex, sm, a, pr, ps, strand, rc, ns, ne); });
+  ex.each([&](int tid) {
+    if (tid == 0) {
+      const int* misc = (const int*)(sm + SM::MISC);
+      a.counts[ps] = misc[MISC_KEPT];
+      if (a.stats) {
+        a.stats[2 * ps] = misc[MISC_NCAND];
+        a.stats[2 * ps + 1] = misc[MISC_RUNS];
+      }
+    }
+  });
+}
+
+// SeqAnalyzer::MatchUp on a given xc (analysis/CrossCorr.cc:583-605): runs, not filtered.
+template <int NB, class Ex>
+SXC_HD inline void matchup_body(Ex& ex, char* sm, const MatchupArgs& a, int item) {
+  using SM = Smem<NB>;
+  using G = Geo<NB>;
+  const sxc_chunk tc = a.tchunks[item];
+  const sxc_chunk qc = a.qchunks[item];
+  phase_prepare<NB>(ex, sm, a.tab, a.ent_lut, a.seq + qc.off, qc.len, qc.flags & 1, 0);
+  float* Y = (float*)(sm + SM::Y);
+  const float* xc = a.xc + (size_t)item * G::N;
+  ex.each([&](int tid) {
+    for (int p = tid; p < G::N; p += G::NT) Y[pad(p)] = xc[p];
+  });
+  ex.sync();
+  peaks<NB>(ex, sm, a.cutoff);
+  unsigned char* ts = (unsigned char*)(sm + SM::TSEQ);
+  const unsigned char* tsrc = a.seq + tc.off;
+  ex.each([&](int tid) {
+    for (int i = tid; i < tc.len; i += G::NT) ts[i] = tsrc[i];
+  });
+  ex.sync();
+  int* misc = (int*)(sm + SM::MISC);
+  scan<NB>(ex, sm, qc.len, tc.len, [&](const RunCtx& rc, int ns, int ne) {
+    const int k = ex.atomic_add_shared(&misc[MISC_KEPT], 1);
+    if (k >= a.cap) return;
+    const int start_t = rc.i0 + ns - SXC_MIN_LEN;
+    const size_t at = (size_t)item * a.cap + k;
+    a.runs[3 * at] = start_t;
+    a.runs[3 * at + 1] = start_t + rc.shift;
+    a.runs[3 * at + 2] = ne - ns + SXC_MIN_LEN;
+    a.keys[at] = ((uint64_t)(uint32_t)rc.cand_rank << 32) | (uint32_t)ns;
+  });
+  ex.each([&](int tid) {
+    if (tid == 0) a.counts[item] = misc[MISC_KEPT];
+  });
+}
+
+}  // namespace sxc

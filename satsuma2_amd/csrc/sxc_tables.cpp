@@ -1,0 +1,131 @@
+// sxc_tables.cpp -- host construction of the constant tables the kernels read.
+//
+// Every value is computed with the same double expression the reference evaluates
+// (glibc libm on the host), so the device only ever looks values up:
+//   * FFTReal twiddles: (float)cos(i * (PI / 4096)) for the level-12 LUT
+//     (extern/RealFFT/FFTReal.hpp:352-374) and the float oscillator sequence for levels
+//     13, 14 (OscSinCos.hpp:61-104, reset per block, FFTReal.hpp:626, 639-641);
+//   * DNACodec rows, reverse complement and DNA_Equal (analysis/DNAVector.cc:13-57, 390-403);
+//   * LookupMatch scores (int)(100 * DNA_EqualAmb + 0.5) (analysis/CrossCorr.cc:547-553);
+//   * Ent(a) / window entropies for windows whose codec sums are multiples of 1/4
+//     (analysis/CrossCorr.cc:28-33, 74-84).
+#include "sxc_tables.h"
+
+#include <math.h>
+#include <string.h>
+
+namespace sxc {
+
+static const char kLetters[] = "ACGTKMRYSWBVHDNX-";  // class ids 0..16; 17 = zero row
+
+static void codec_row(char l, double* v, char* rc) {
+  const double third = 1. / 3.;
+  struct Row {
+    char l;
+    double a, c, g, t;
+    char rc;
+  };
+  static const Row rows[] = {
+      {'A', 1., 0., 0., 0., 'T'},       {'C', 0., 1., 0., 0., 'G'},       {'G', 0., 0., 1., 0., 'C'},
+      {'T', 0., 0., 0., 1., 'A'},       {'K', 0., 0., 0.5, 0.5, 'M'},     {'M', 0.5, 0.5, 0., 0., 'K'},
+      {'R', 0.5, 0., 0.5, 0., 'Y'},     {'Y', 0., 0.5, 0., 0.5, 'R'},     {'S', 0., 0.5, 0.5, 0., 'S'},
+      {'W', 0.5, 0., 0., 0.5, 'W'},     {'B', 0., third, third, third, 'V'}, {'V', third, third, third, 0., 'B'},
+      {'H', third, third, 0., third, 'D'}, {'D', third, 0., third, third, 'H'}, {'-', 0., 0., 0., 0., '-'},
+      {'N', 0.25, 0.25, 0.25, 0.25, 'N'}, {'X', 0.25, 0.25, 0.25, 0.25, 'X'},
+  };
+  v[0] = v[1] = v[2] = v[3] = 0.;
+  *rc = 0;
+  for (const Row& r : rows)
+    if (r.l == l) {
+      v[0] = r.a;
+      v[1] = r.c;
+      v[2] = r.g;
+      v[3] = r.t;
+      *rc = r.rc;
+    }
+}
+
+int class_of(int byte) {
+  for (int c = 0; c < 17; ++c)
+    if (kLetters[c] == (char)byte) return c;
+  return SXC_CLASS_ZERO;
+}
+
+bool class_has_thirds(int c) { return c == 10 || c == 11 || c == 12 || c == 13; }
+
+void build_tables(sxc_tables* t) {
+  memset(t, 0, sizeof *t);
+  const double PI = 3.1415926535897932384626433832795;  // extern/RealFFT/def.h:42
+  const double mul = PI / (double)(2048 << 1);
+  for (int i = 0; i < 2048; ++i) t->lut12[i] = (float)cos((double)i * mul);
+  double v[4];
+  char rc;
+  for (int c = 0; c < SXC_NCLASS; ++c) {
+    if (c < 17) {
+      codec_row(kLetters[c], v, &rc);
+    } else {
+      v[0] = v[1] = v[2] = v[3] = 0.;
+    }
+    for (int k = 0; k < 4; ++k) {
+      t->codec[c][k] = v[k];
+      const double q = v[k] * 4.0;
+      t->quarters[c][k] = (q == (double)(int)q) ? (unsigned char)(int)q : 0;
+    }
+    t->gc[c] = v[1] + v[2];
+  }
+  for (int a = 0; a < SXC_NCLASS; ++a)
+    for (int b = 0; b < SXC_NCLASS; ++b) {
+      // DNA_Equal: ((a+c)+g)+t, + 0.0
+      const double s = t->codec[a][0] * t->codec[b][0] + t->codec[a][1] * t->codec[b][1] +
+                       t->codec[a][2] * t->codec[b][2] + t->codec[a][3] * t->codec[b][3];
+      t->equal[a][b] = s + 0.;
+      t->score[a][b] = (unsigned char)(int)(t->equal[a][b] * (double)100 + 0.5);
+    }
+  for (int byte = 0; byte < 128; ++byte) {
+    t->cls[byte] = (unsigned char)class_of(byte);
+    codec_row((char)byte, v, &rc);
+    t->rc[byte] = (unsigned char)rc;
+  }
+}
+
+void build_osc(float* osc) {
+  // level 13 at [0, 8192): 4096 {cos, sin}; level 14 at [8192, 24576): 8192 pairs
+  const double PI = 3.1415926535897932384626433832795;
+  for (int lv = 13; lv <= 14; ++lv) {
+    const int len = 1 << (lv - 1);  // 1 << (TRIGO_BD_LIMIT + osc index)
+    const double mul = (0.5 * PI) / (double)len;
+    const float sc = (float)cos(mul), ss = (float)sin(mul);
+    float* out = osc + (lv == 13 ? 0 : 8192);
+    float pc = 1.f, ps = 0.f;
+    out[0] = pc;
+    out[1] = ps;
+    for (int i = 1; i < len; ++i) {
+      const float oc = pc, os = ps;
+      pc = oc * sc - os * ss;
+      ps = oc * ss + os * sc;
+      out[2 * i] = pc;
+      out[2 * i + 1] = ps;
+    }
+  }
+}
+
+static double ent_ref(double p) {
+  if (p < 0.001) return 0;
+  return p * log(p) / 0.69314718056;
+}
+
+int ent_lut_len(int nb) {
+  const int win = (1 << nb) / 512;
+  return win < 1 ? 0 : win * (4 * win + 1);
+}
+
+void build_ent_lut(int nb, double* out) {
+  const int win = (1 << nb) / 512;
+  for (int k = 1; k <= win; ++k)
+    for (int q = 0; q <= 4 * win; ++q) {
+      const double sum = (double)q * 0.25;
+      out[(size_t)(k - 1) * (4 * win + 1) + q] = ent_ref(sum / (double)k);
+    }
+}
+
+}  // namespace sxc

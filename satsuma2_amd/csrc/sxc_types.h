@@ -1,0 +1,61 @@
+// sxc_types.h -- plain data shared by the host library, the kernels and the CPU
+// workgroup emulator.  No HIP or C++ runtime types: every struct here crosses the
+// host/device boundary by memcpy.
+#pragma once
+#include <stdint.h>
+
+#define SXC_MIN_NB 9   // N = 512
+#define SXC_MAX_NB 15  // N = 32768
+#define SXC_ENV_SIZE 256        // SeqAnalyzer::m_envSize (analysis/CrossCorr.cc:562)
+#define SXC_MIN_LEN 45          // SeqAnalyzer::m_minLen (analysis/CrossCorr.cc:560)
+#define SXC_WIN_THRESHOLD 1889  // (int)(45 * 0.42 * 100) (analysis/CrossCorr.cc:675)
+#define SXC_NCLASS 18           // 16 IUPAC letters + '-' + "zero row" class
+#define SXC_CLASS_N 14          // class id of 'N'
+#define SXC_CLASS_ZERO 17       // bytes with an all-zero codec row
+
+// One chunk: bytes live at seq_base + off, `len` bases (uppercase ASCII < 128).
+typedef struct {
+  int64_t off;
+  int32_t len;
+  int32_t flags;  // bit 0: contains a codec class with 1/3 entries (B, V, H, D)
+} sxc_chunk;
+
+// One chunk pair of a batch (HomologyByXCorr::Align inputs plus the chunk metadata
+// FilterMatches uses, analysis/HomologyByXCorrSlave.cc:168-253).
+typedef struct {
+  int32_t t_chunk;        // index into the target chunk table (bytes + resident spectra)
+  int32_t q_chunk;        // index into the query chunk table
+  int32_t t_chunk_start;  // SeqChunk::GetStart of the target chunk
+  int32_t q_chunk_start;  // SeqChunk::GetStart of the query chunk
+  int32_t t_record;       // target record id (t_result.target_id)
+  int32_t q_record;       // query record id (t_result.query_id)
+  int32_t q_record_len;   // ChunkManager::GetSize(query record) (t_result.query_size, RCQuery)
+  int32_t fast;           // t_pair.fast: use cutoff_fast
+} sxc_pair_dev;
+
+// Reference t_result (analysis/WorkQueue.h:23-33), 72 bytes on LP64.
+typedef struct {
+  uint64_t query_id;
+  uint64_t target_id;
+  uint64_t query_size;
+  uint64_t qstart;
+  uint64_t tstart;
+  uint64_t len;
+  uint8_t reverse;
+  uint8_t pad_[7];
+  double prob;
+  double ident;
+} sxc_result_dev;
+
+// Constant tables built by the host library at context creation.
+typedef struct {
+  float lut12[2048];            // (float)cos(i * PI / 4096): FFTReal's level-12 LUT, every lower
+                                // level is a stride of it (FFTReal.hpp:352-374)
+  unsigned char cls[128];       // byte -> codec class
+  unsigned char rc[128];        // DNACodec::m_rc (DNAVector.cc:13-57)
+  unsigned char score[SXC_NCLASS][SXC_NCLASS];  // (int)(100 * DNA_Equal + 0.5) per class pair
+  unsigned char quarters[SXC_NCLASS][4];        // 4 * codec value when a multiple of 1/4
+  double codec[SXC_NCLASS][4];  // DNA_A/C/G/T per class
+  double equal[SXC_NCLASS][SXC_NCLASS];  // DNA_Equal per class pair (DNAVector.cc:390-403)
+  double gc[SXC_NCLASS];        // DNA_C + DNA_G per class
+} sxc_tables;

@@ -1,0 +1,198 @@
+// CPU workgroup emulator of the HIP kernels (test infrastructure).
+//
+// Runs the exact kernel bodies of satsuma2_amd/csrc/sxc_pipeline.h with a sequential
+// executor: every "thread" of a workgroup runs each phase in turn, barriers are phase
+// boundaries, LDS is a heap buffer.  Built with AddressSanitizer by tests/test_emu.py so
+// every LDS and global index the kernels compute is bounds-checked on the CPU before the
+// same code runs on a GPU.  Not part of the product.
+#include <stdlib.h>
+#include <string.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "../../satsuma2_amd/csrc/sxc_batch_host.h"
+#include "../../satsuma2_amd/csrc/sxc_pipeline.h"
+
+namespace {
+
+template <class R>
+struct EmuExec {
+  int nt;
+  std::vector<R> regs;
+  explicit EmuExec(int n) : nt(n), regs(n) {}
+  template <class F>
+  void each(F&& f) {
+    for (int t = 0; t < nt; ++t) f(t);
+  }
+  void sync() {}
+  template <class T>
+  T& reg(int tid) {
+    return regs[tid];
+  }
+  int atomic_add_shared(int* p, int v) {
+    const int o = *p;
+    *p += v;
+    return o;
+  }
+};
+
+struct Tables {
+  sxc_tables tab;
+  std::vector<float> osc;
+  std::vector<double> ent[16];
+  Tables() : osc(24576) {
+    sxc::build_tables(&tab);
+    sxc::build_osc(osc.data());
+    for (int nb = 9; nb <= 15; ++nb) {
+      ent[nb].resize(std::max(1, sxc::ent_lut_len(nb)));
+      sxc::build_ent_lut(nb, ent[nb].data());
+    }
+  }
+};
+
+const Tables& tables() {
+  static Tables t;
+  return t;
+}
+
+template <int NB>
+int run_batch(const sxc::HostBatch& hb, const sxc_params& prm, std::vector<sxc_result_dev>& res, int cap,
+              float* xc_out, int32_t* stats) {
+  using G = sxc::Geo<NB>;
+  using SM = sxc::Smem<NB>;
+  const Tables& T = tables();
+  std::vector<float> spectra((size_t)hb.tchunks.size() * 4 * G::N);
+  std::vector<int32_t> which(hb.tchunks.size()), dst(hb.tchunks.size());
+  for (size_t i = 0; i < which.size(); ++i) which[i] = dst[i] = (int)i;
+  std::vector<char> sm(SM::BYTES);
+  sxc::SpecArgs sa{&T.tab, T.osc.data(), T.ent[NB].data(), hb.seq.data(), hb.tchunks.data(), which.data(),
+                   (int)which.size(), spectra.data(), dst.data(), nullptr};
+  for (int k = 0; k < sa.n; ++k) {
+    EmuExec<sxc::Regs<NB>> ex(G::NT);
+    memset(sm.data(), 0xCD, sm.size());
+    sxc::spectra_body<NB>(ex, sm.data(), sa, k);
+  }
+  const int total = (int)hb.pairs.size() * prm.strands;
+  std::vector<sxc_result_dev> slots((size_t)total * cap);
+  std::vector<uint64_t> keys((size_t)total * cap);
+  std::vector<int32_t> counts(total);
+  sxc::PairArgs pa{&T.tab, T.osc.data(), T.ent[NB].data(), hb.seq.data(), hb.tchunks.data(), hb.qchunks.data(),
+                   hb.pairs.data(), spectra.data(), (int)hb.pairs.size(), prm.strands, prm.min_len, prm.q_chunk,
+                   prm.cutoff, prm.cutoff_fast, prm.min_prob, prm.target_total, slots.data(), keys.data(),
+                   counts.data(), cap, xc_out, stats};
+  for (int ps = 0; ps < total; ++ps) {
+    EmuExec<sxc::Regs<NB>> ex(G::NT);
+    memset(sm.data(), 0xCD, sm.size());
+    sxc::pair_body<NB>(ex, sm.data(), pa, ps);
+  }
+  res.clear();
+  for (int ps = 0; ps < total; ++ps) {
+    const int n = std::min(counts[ps], cap);
+    std::vector<int> idx(n);
+    for (int i = 0; i < n; ++i) idx[i] = i;
+    std::sort(idx.begin(), idx.end(), [&](int a, int b) { return keys[(size_t)ps * cap + a] < keys[(size_t)ps * cap + b]; });
+    for (int i : idx) res.push_back(slots[(size_t)ps * cap + i]);
+    if (counts[ps] > cap) return -3;
+  }
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+// Mirrors sxc_align_batch.  xc_out (optional) receives N floats per (pair, strand);
+// stats (optional) receives {candidates, runs} per (pair, strand).
+int emu_align_batch(const sxc_pair* pairs, int n, const sxc_params* prm, sxc_result* out, int64_t cap,
+                    int64_t* n_out, float* xc_out, int32_t* stats) {
+  sxc::HostBatch hb;
+  if (!sxc::pack_batch(pairs, n, prm->N, hb).empty()) return -1;
+  std::vector<sxc_result_dev> res;
+  const int slot_cap = 4096;
+  int rc;
+  switch (sxc::nb_of(prm->N)) {
+    case 9: rc = run_batch<9>(hb, *prm, res, slot_cap, xc_out, stats); break;
+    case 10: rc = run_batch<10>(hb, *prm, res, slot_cap, xc_out, stats); break;
+    case 11: rc = run_batch<11>(hb, *prm, res, slot_cap, xc_out, stats); break;
+    case 12: rc = run_batch<12>(hb, *prm, res, slot_cap, xc_out, stats); break;
+    case 13: rc = run_batch<13>(hb, *prm, res, slot_cap, xc_out, stats); break;
+    case 14: rc = run_batch<14>(hb, *prm, res, slot_cap, xc_out, stats); break;
+    default: return -1;
+  }
+  if (rc) return rc;
+  *n_out = (int64_t)res.size();
+  if ((int64_t)res.size() > cap) return -3;
+  memcpy(out, res.data(), res.size() * sizeof(sxc_result));
+  return 0;
+}
+
+// FFT of one signal through stage A + rounds (spectra_body with a given signal).
+int emu_fft_forward(const float* in, float* out, int N) {
+  const Tables& T = tables();
+  sxc_chunk ck{0, 0, 0};
+  int32_t which = 0, dst = 0;
+  std::vector<float> sig(4 * (size_t)N, 0.f), spec(4 * (size_t)N);
+  memcpy(sig.data(), in, sizeof(float) * N);
+  auto go = [&](auto nbc) {
+    constexpr int NB = decltype(nbc)::value;
+    sxc::SpecArgs sa{&T.tab, T.osc.data(), T.ent[NB].data(), nullptr, &ck, &which, 1, spec.data(), &dst, sig.data()};
+    std::vector<char> sm(sxc::Smem<NB>::BYTES);
+    EmuExec<sxc::Regs<NB>> ex(sxc::Geo<NB>::NT);
+    sxc::spectra_body<NB>(ex, sm.data(), sa, 0);
+  };
+  switch (sxc::nb_of(N)) {
+    case 9: go(std::integral_constant<int, 9>{}); break;
+    case 10: go(std::integral_constant<int, 10>{}); break;
+    case 11: go(std::integral_constant<int, 11>{}); break;
+    case 12: go(std::integral_constant<int, 12>{}); break;
+    case 13: go(std::integral_constant<int, 13>{}); break;
+    case 14: go(std::integral_constant<int, 14>{}); break;
+    default: return -1;
+  }
+  memcpy(out, spec.data(), sizeof(float) * N);
+  return 0;
+}
+
+}  // extern "C"
+
+#ifdef SXC_EMU_MAIN
+// Standalone checker (built with AddressSanitizer): reads pairs from a file written by
+// tests/test_emu.py, runs the emulated kernels, writes the t_result records.
+// input: int32 N, strands, n; double cutoff, cutoff_fast, min_prob, target_total;
+//        per pair: int32 tlen, qlen, t_chunk_start, q_chunk_start, t_record, q_record,
+//        q_record_len, fast; then tlen + qlen bytes.
+#include <stdio.h>
+int main(int argc, char** argv) {
+  if (argc < 3) return 2;
+  FILE* f = fopen(argv[1], "rb");
+  if (!f) return 2;
+  int32_t hdr[3];
+  double d[4];
+  if (fread(hdr, 4, 3, f) != 3 || fread(d, 8, 4, f) != 4) return 2;
+  std::vector<sxc_pair> pairs(hdr[2]);
+  std::vector<std::vector<char>> store;
+  for (int k = 0; k < hdr[2]; ++k) {
+    int32_t m[8];
+    if (fread(m, 4, 8, f) != 8) return 2;
+    store.emplace_back(m[0] + 1);
+    char* t = store.back().data();
+    if (fread(t, 1, m[0], f) != (size_t)m[0]) return 2;
+    store.emplace_back(m[1] + 1);
+    char* q = store.back().data();
+    if (fread(q, 1, m[1], f) != (size_t)m[1]) return 2;
+    pairs[k] = sxc_pair{t, m[0], q, m[1], m[2], m[3], m[4], m[5], m[6], m[7]};
+  }
+  fclose(f);
+  sxc_params prm{hdr[0], 4096, 0, hdr[1], d[0], d[1], d[2], d[3]};
+  std::vector<sxc_result> out(1 << 20);
+  int64_t n = 0;
+  const int rc = emu_align_batch(pairs.data(), hdr[2], &prm, out.data(), (int64_t)out.size(), &n, nullptr, nullptr);
+  if (rc) return 3;
+  FILE* o = fopen(argv[2], "wb");
+  fwrite(&n, 8, 1, o);
+  fwrite(out.data(), sizeof(sxc_result), (size_t)n, o);
+  fclose(o);
+  return 0;
+}
+#endif
