@@ -1,0 +1,171 @@
+"""Chunk-pair throughput of the MI355X cross-correlation path on one GPU.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+
+Workload (uniform_pairs): 4096 chunk pairs of 4096 bp, i.i.d. bases at GC 0.41, half of
+them carrying one planted homology (50-2000 bp, 70-95% identity, 1% indels), FFT length
+8192, both strands, slave parameters (cutoff 1.8, keep threshold 0.99).  Inputs are
+uploaded to HBM before the timed region (sxc_batch_create); one step = one
+sxc_batch_run over all 4096 pairs (target spectra, fused pair kernel, result ordering).
+
+Printed: one JSON line with the batched throughput (`value`), the plain one-pair-per-call
+throughput beside it, the dominant kernel's roofline figures from HIP events, and the
+reference CPU path timed on this host's cores on a bounded sample.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import subprocess
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "chunk-pairs/s at 4096bp (batched vs plain, 1 GPU); serve p50/p99 ms"
+N_PAIRS = 4096
+HBM_PEAK_GBS = 8000.0
+# Algorithmic HBM bytes per chunk pair (both strands, N = 8192, 4 channels, FP32 spectra
+# materialised, target spectra resident, peak search fused): forward
+# 2*4*8192*4 + 1024 = 263,168 B; inverse 262,144 + 131,072 + 65,536 = 458,752 B.
+B_PAIR = 263_168 + 458_752
+
+
+def make_workload(n_pairs: int, seed: int = 1):
+    from satsuma2_amd import synth
+    from satsuma2_amd.api import ChunkPair
+
+    T, Q = synth.uniform_pairs(n_pairs, seed=seed)
+    pairs = [ChunkPair(T[r], Q[r], t_record=r, q_record=r, q_record_len=4096) for r in range(n_pairs)]
+    return T, Q, pairs
+
+
+def cpu_baseline(T, Q, sample: int, threads: int):
+    """The reference slave path (compiled from source into oracle/_ref) on `sample` pairs,
+    `threads` worker threads; the oracle restatement when that binary is absent."""
+    drv = os.path.join(ROOT, "oracle", "_ref", "ref_driver")
+    from satsuma2_amd import synth
+
+    if os.path.exists(drv):
+        with tempfile.TemporaryDirectory() as tmp:
+            qf, tf, pf, of = (os.path.join(tmp, x) for x in ("q.fa", "t.fa", "p.txt", "r.bin"))
+            synth.write_fasta(qf, Q[:sample])
+            synth.write_fasta(tf, T[:sample])
+            with open(pf, "w") as f:
+                for r in range(sample):
+                    f.write(f"{2 * r} {2 * r} {2 * r} {2 * r} 0\n")
+            out = subprocess.run([drv, "pairs", qf, tf, pf, of, str(threads)], check=True, capture_output=True,
+                                 text=True).stdout.strip().splitlines()[-1]
+            kv = dict(x.split("=") for x in out.split())
+            secs = float(kv["seconds"])
+            return {"value": sample / secs, "unit": "chunk-pairs/s", "cores": threads, "kind": "reference",
+                    "sample": f"{sample} pairs of the same workload, both strands, HomologyByXCorr::align_target"}
+    import oracle
+
+    prm = oracle.make_params(target_total=float(4096 * N_PAIRS))
+    pairs = [dict(tseq=T[r], qseq=Q[r], t_record=r, q_record=r, q_record_len=4096) for r in range(sample)]
+    t0 = time.perf_counter()
+    oracle.align_batch(pairs, prm, threads=threads)
+    secs = time.perf_counter() - t0
+    return {"value": sample / secs, "unit": "chunk-pairs/s", "cores": threads, "kind": "port",
+            "sample": f"{sample} pairs of the same workload, both strands, oracle restatement"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--pairs", type=int, default=N_PAIRS)
+    ap.add_argument("--plain-pairs", type=int, default=256)
+    ap.add_argument("--cpu-sample", type=int, default=2048)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (chunk pairs shard by target chunk "
+              f"across GPUs with no collective, see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    from satsuma2_amd.api import Context, HomologyByXCorr, ResidentBatch, SlaveParams
+
+    print(f"[bench] generating {args.pairs} pairs", file=sys.stderr, flush=True)
+    T, Q, pairs = make_workload(args.pairs)
+    params = SlaveParams(target_total=float(4096 * args.pairs))
+    ctx = Context(0)
+    rb = ResidentBatch(ctx, pairs, params)
+
+    print("[bench] warmup", file=sys.stderr, flush=True)
+    for _ in range(args.warmup):
+        rb.run()
+    pair_ms, spec_ms, fin_ms, cor_ms, scan_ms = [], [], [], [], []
+    print(f"[bench] timing {args.steps} steps", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        rb.run()  # returns after the step's last kernel has completed
+        tm = ctx.timing()
+        pair_ms.append(tm.pair_ms)
+        spec_ms.append(tm.spectra_ms)
+        fin_ms.append(tm.finalize_ms)
+        cor_ms.append(tm.correlate_ms)
+        scan_ms.append(tm.scan_ms)
+    wall = time.perf_counter() - t0
+    ms_per_step = 1e3 * wall / args.steps
+    value = args.pairs / (wall / args.steps)
+    n_res = len(rb.results())
+    cands, runs = rb.stats()
+    rb.close()
+
+    # plain drop-in: one chunk pair per call, host buffers in and out (as the slave calls Align)
+    h = HomologyByXCorr(ctx, params)
+    for p in pairs[:4]:
+        h.align_pair(p)
+    t0 = time.perf_counter()
+    for p in pairs[: args.plain_pairs]:
+        h.align_pair(p)
+    plain = args.plain_pairs / (time.perf_counter() - t0)
+
+    k_ms = float(np.mean(cor_ms))
+    achieved = B_PAIR * args.pairs / (k_ms * 1e-3) / 1e9
+    out = {
+        "metric": METRIC,
+        "value": round(value, 1),
+        "unit": "chunk-pairs/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic",
+        "config": {"workload": "uniform_pairs: 4096-bp target x 4096-bp query chunk pairs, N=8192, both strands",
+                   "pairs": args.pairs, "chunk_bp": 4096, "fft_len": 8192, "strands": 2, "cutoff": 1.8,
+                   "keep_prob": 0.99},
+        "plain_pairs_per_s": round(plain, 1),
+        "batched_over_plain": round(value / plain, 2),
+        "phase_ms": {"target_spectra": round(float(np.mean(spec_ms)), 3),
+                     "correlate": round(float(np.mean(cor_ms)), 3), "scan": round(float(np.mean(scan_ms)), 3),
+                     "finalize": round(float(np.mean(fin_ms)), 3)},
+        "results": n_res,
+        "candidate_lags": cands,
+        "diagonal_runs": runs,
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "k_correlate<13>", "bytes_per_pair": B_PAIR},
+    }
+    if not args.no_cpu:
+        threads = max(1, min(16, os.cpu_count() or 1))
+        print(f"[bench] cpu baseline on {threads} threads", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = cpu_baseline(T, Q, min(args.cpu_sample, args.pairs), threads)
+    ctx.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -138,6 +138,8 @@ typedef struct {
   double download_ms;
   int64_t candidates;
   int64_t runs;
+  double correlate_ms;  /* encode + FFT + product + iFFT + peak search (part of pair_ms) */
+  double scan_ms;       /* diagonal scan + probability tail (part of pair_ms) */
 } sxc_timing;
 int sxc_last_timing(sxc_ctx* ctx, sxc_timing* out);
 

@@ -51,7 +51,8 @@ class Pair(C.Structure):
 class Timing(C.Structure):
     _fields_ = [("total_ms", C.c_double), ("upload_ms", C.c_double), ("spectra_ms", C.c_double),
                 ("pair_ms", C.c_double), ("finalize_ms", C.c_double), ("download_ms", C.c_double),
-                ("candidates", C.c_int64), ("runs", C.c_int64)]
+                ("candidates", C.c_int64), ("runs", C.c_int64), ("correlate_ms", C.c_double),
+                ("scan_ms", C.c_double)]
 
 
 RESULT_DTYPE = np.dtype([("query_id", "<u8"), ("target_id", "<u8"), ("query_size", "<u8"), ("qstart", "<u8"),

@@ -62,7 +62,7 @@ struct DevBuf {
 }  // namespace
 
 struct sxc_batch_dev {
-  DevBuf seq, tch, qch, pairs, spectra, which, slots, keys, counts, offsets, total, maxc, final_, stats;
+  DevBuf seq, tch, qch, pairs, spectra, which, slots, keys, counts, offsets, total, maxc, final_, stats, cands, ncand;
   sxc::HostBatch hb;
   sxc_params prm{};
   int nb = 0;
@@ -70,7 +70,7 @@ struct sxc_batch_dev {
   int64_t n_results = 0;
   void release() {
     for (DevBuf* b : {&seq, &tch, &qch, &pairs, &spectra, &which, &slots, &keys, &counts, &offsets, &total, &maxc,
-                      &final_, &stats})
+                      &final_, &stats, &cands, &ncand})
       b->release();
   }
 };
@@ -132,12 +132,10 @@ int batch_upload(sxc_ctx* ctx, sxc_batch_dev& b) {
   return SXC_OK;
 }
 
-int launch_pairs(sxc_ctx* ctx, sxc_batch_dev& b) {
+int launch_correlate(sxc_ctx* ctx, sxc_batch_dev& b) {
   const int total = (int)b.hb.pairs.size() * b.prm.strands;
-  HIPCHK(b.slots.ensure((size_t)total * b.slot_cap * sizeof(sxc_result_dev)));
-  HIPCHK(b.keys.ensure((size_t)total * b.slot_cap * sizeof(uint64_t)));
-  HIPCHK(b.counts.ensure((size_t)total * sizeof(int32_t)));
-  HIPCHK(b.stats.ensure((size_t)total * 2 * sizeof(int32_t)));
+  HIPCHK(b.cands.ensure((size_t)total * b.prm.N * sizeof(uint16_t)));
+  HIPCHK(b.ncand.ensure((size_t)total * sizeof(int32_t)));
   sxc::PairArgs a{};
   a.tab = ctx->d_tab;
   a.osc = ctx->d_osc;
@@ -149,19 +147,43 @@ int launch_pairs(sxc_ctx* ctx, sxc_batch_dev& b) {
   a.spectra = b.spectra.as<float>();
   a.npairs = (int)b.hb.pairs.size();
   a.strands = b.prm.strands;
-  a.min_len = b.prm.min_len;
-  a.q_chunk = b.prm.q_chunk;
   a.cutoff = b.prm.cutoff;
   a.cutoff_fast = b.prm.cutoff_fast;
+  a.cands = b.cands.as<uint16_t>();
+  a.ncand = b.ncand.as<int32_t>();
+  a.xc_out = nullptr;
+  HIPCHK(sxc::launch_correlate(b.nb, a, total, ctx->stream));
+  return SXC_OK;
+}
+
+int launch_pairs(sxc_ctx* ctx, sxc_batch_dev& b) {
+  const int total = (int)b.hb.pairs.size() * b.prm.strands;
+  HIPCHK(b.slots.ensure((size_t)total * b.slot_cap * sizeof(sxc_result_dev)));
+  HIPCHK(b.keys.ensure((size_t)total * b.slot_cap * sizeof(uint64_t)));
+  HIPCHK(b.counts.ensure((size_t)total * sizeof(int32_t)));
+  HIPCHK(b.stats.ensure((size_t)total * 2 * sizeof(int32_t)));
+  sxc::ScanArgs a{};
+  a.tab = ctx->d_tab;
+  a.seq = b.seq.as<unsigned char>();
+  a.tchunks = b.tch.as<sxc_chunk>();
+  a.qchunks = b.qch.as<sxc_chunk>();
+  a.pairs = b.pairs.as<sxc_pair_dev>();
+  a.cands = b.cands.as<uint16_t>();
+  a.ncand = b.ncand.as<int32_t>();
+  a.n_fft = b.prm.N;
+  a.strands = b.prm.strands;
+  a.min_len = b.prm.min_len;
+  a.q_chunk = b.prm.q_chunk;
   a.min_prob = b.prm.min_prob;
   a.target_total = b.prm.target_total;
+  a.mode = 0;
   a.out = b.slots.as<sxc_result_dev>();
+  a.runs = nullptr;
   a.keys = b.keys.as<uint64_t>();
   a.counts = b.counts.as<int32_t>();
   a.cap = b.slot_cap;
-  a.xc_out = nullptr;
   a.stats = b.stats.as<int32_t>();
-  HIPCHK(sxc::launch_pair(b.nb, a, total, ctx->stream));
+  HIPCHK(sxc::launch_scan(b.nb, a, total, ctx->stream));
   return SXC_OK;
 }
 
@@ -187,12 +209,16 @@ int batch_run(sxc_ctx* ctx, sxc_batch_dev& b) {
   HIPCHK(hipEventRecord(ctx->ev[2], s));
   if (total == 0) {
     b.n_results = 0;
+    HIPCHK(hipEventRecord(ctx->ev[6], s));
     HIPCHK(hipEventRecord(ctx->ev[3], s));
     HIPCHK(hipEventRecord(ctx->ev[4], s));
     HIPCHK(hipEventSynchronize(ctx->ev[4]));
     return SXC_OK;
   }
-  int rc = launch_pairs(ctx, b);
+  int rc = launch_correlate(ctx, b);
+  if (rc) return rc;
+  HIPCHK(hipEventRecord(ctx->ev[6], s));
+  rc = launch_pairs(ctx, b);
   if (rc) return rc;
   HIPCHK(hipEventRecord(ctx->ev[3], s));
   HIPCHK(b.offsets.ensure((size_t)total * sizeof(int64_t)));
@@ -321,6 +347,8 @@ int sxc_align_batch(sxc_ctx* ctx, const sxc_pair* pairs, int32_t n, const sxc_pa
   ctx->last.upload_ms = elapsed(ctx->ev[0], ctx->ev[1]);
   ctx->last.spectra_ms = elapsed(ctx->ev[1], ctx->ev[2]);
   ctx->last.pair_ms = elapsed(ctx->ev[2], ctx->ev[3]);
+  ctx->last.correlate_ms = elapsed(ctx->ev[2], ctx->ev[6]);
+  ctx->last.scan_ms = elapsed(ctx->ev[6], ctx->ev[3]);
   ctx->last.finalize_ms = elapsed(ctx->ev[3], ctx->ev[4]);
   ctx->last.download_ms = elapsed(ctx->ev[4], ctx->ev[5]);
   return rc;
@@ -379,6 +407,8 @@ int sxc_batch_run(sxc_batch* bt) {
   c->last.upload_ms = 0;
   c->last.spectra_ms = elapsed(c->ev[1], c->ev[2]);
   c->last.pair_ms = elapsed(c->ev[2], c->ev[3]);
+  c->last.correlate_ms = elapsed(c->ev[2], c->ev[6]);
+  c->last.scan_ms = elapsed(c->ev[6], c->ev[3]);
   c->last.finalize_ms = elapsed(c->ev[3], c->ev[4]);
   c->last.download_ms = 0;
   c->last.total_ms = elapsed(c->ev[1], c->ev[4]);
@@ -551,38 +581,50 @@ int sxc_matchup(sxc_ctx* ctx, const char* q, int32_t lq, const char* t, int32_t 
   rc = set_device(ctx);
   if (rc) return rc;
   hipStream_t s = ctx->stream;
-  // runs per lag are bounded by the diagonal length / 2 + 1
-  const int slot = std::max(16, (int)std::min<int64_t>((int64_t)N * (N / 4 + 1), 1 << 22));
+  // every accepted lag yields at most N/4 + 1 runs
+  const int slot = std::max(64, (int)std::min<int64_t>((int64_t)N * (N / 4 + 1), 1 << 24));
   HIPCHK(ctx->p0.ensure(hb.seq.size()));
-  HIPCHK(ctx->p1.ensure(2 * sizeof(sxc_chunk)));
-  HIPCHK(ctx->p2.ensure((size_t)N * sizeof(float)));
+  HIPCHK(ctx->p1.ensure(2 * sizeof(sxc_chunk) + sizeof(sxc_pair_dev)));
+  HIPCHK(ctx->p2.ensure((size_t)N * sizeof(float) + (size_t)N * sizeof(uint16_t) + 2 * sizeof(int32_t)));
   HIPCHK(ctx->p3.ensure((size_t)slot * 3 * sizeof(int32_t)));
   HIPCHK(ctx->p4.ensure((size_t)slot * sizeof(uint64_t)));
-  HIPCHK(ctx->p5.ensure(sizeof(int32_t)));
   HIPCHK(hipMemcpyAsync(ctx->p0.p, hb.seq.data(), hb.seq.size(), hipMemcpyHostToDevice, s));
-  sxc_chunk cks[2] = {hb.tchunks[0], hb.qchunks[0]};
-  HIPCHK(hipMemcpyAsync(ctx->p1.p, cks, sizeof cks, hipMemcpyHostToDevice, s));
-  HIPCHK(hipMemcpyAsync(ctx->p2.p, xc, (size_t)N * sizeof(float), hipMemcpyHostToDevice, s));
-  sxc::MatchupArgs a{};
+  char stage[2 * sizeof(sxc_chunk) + sizeof(sxc_pair_dev)];
+  memcpy(stage, &hb.tchunks[0], sizeof(sxc_chunk));
+  memcpy(stage + sizeof(sxc_chunk), &hb.qchunks[0], sizeof(sxc_chunk));
+  sxc_pair_dev pd = hb.pairs[0];
+  pd.t_chunk = 0;
+  pd.q_chunk = 0;
+  memcpy(stage + 2 * sizeof(sxc_chunk), &pd, sizeof pd);
+  HIPCHK(hipMemcpyAsync(ctx->p1.p, stage, sizeof stage, hipMemcpyHostToDevice, s));
+  float* d_xc = ctx->p2.as<float>();
+  uint16_t* d_cands = (uint16_t*)(d_xc + N);
+  int32_t* d_misc = (int32_t*)(d_cands + N);  // {ncand, count}
+  HIPCHK(hipMemcpyAsync(d_xc, xc, (size_t)N * sizeof(float), hipMemcpyHostToDevice, s));
+  HIPCHK(sxc::launch_peaks(nb, d_xc, cutoff, d_cands, d_misc, s));
+  sxc::ScanArgs a{};
   a.tab = ctx->d_tab;
-  a.ent_lut = ctx->d_ent[nb];
   a.seq = ctx->p0.as<unsigned char>();
-  a.tchunks = ctx->p1.as<sxc_chunk>();
-  a.qchunks = ctx->p1.as<sxc_chunk>() + 1;
-  a.xc = ctx->p2.as<float>();
-  a.cutoff = cutoff;
+  a.tchunks = (const sxc_chunk*)ctx->p1.p;
+  a.qchunks = (const sxc_chunk*)ctx->p1.p + 1;
+  a.pairs = (const sxc_pair_dev*)((const char*)ctx->p1.p + 2 * sizeof(sxc_chunk));
+  a.cands = d_cands;
+  a.ncand = d_misc;
+  a.n_fft = N;
+  a.strands = 1;
+  a.mode = 1;
   a.runs = ctx->p3.as<int32_t>();
   a.keys = ctx->p4.as<uint64_t>();
-  a.counts = ctx->p5.as<int32_t>();
+  a.counts = d_misc + 1;
   a.cap = slot;
-  HIPCHK(sxc::launch_matchup(nb, a, 1, s));
+  HIPCHK(sxc::launch_scan(nb, a, 1, s));
   int32_t cnt = 0;
-  HIPCHK(hipMemcpyAsync(&cnt, ctx->p5.p, sizeof cnt, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipMemcpyAsync(&cnt, d_misc + 1, sizeof cnt, hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));
   *n_out = cnt;
-  if (cnt > cap) return fail(SXC_ERR_CAPACITY, "output capacity too small");
   if (cnt > slot) return fail(SXC_ERR_CAPACITY, "run buffer overflow");
-  // runs come back unordered across lanes; order them by (lag rank, start) on the GPU
+  if (cnt > cap) return fail(SXC_ERR_CAPACITY, "output capacity too small");
+  // runs come back unordered; order them by (lag rank, start) on the GPU
   HIPCHK(ctx->p6.ensure((size_t)std::max(cnt, 1) * 3 * sizeof(int32_t)));
   HIPCHK(sxc::launch_order_runs(ctx->p3.as<int32_t>(), ctx->p4.as<uint64_t>(), cnt, ctx->p6.as<int32_t>(), s));
   if (cnt) HIPCHK(hipMemcpyAsync(out, ctx->p6.p, (size_t)cnt * 3 * sizeof(int32_t), hipMemcpyDeviceToHost, s));

@@ -46,11 +46,13 @@ inline std::string pack_batch(const sxc_pair* in, int n, int N, HostBatch& hb) {
     sxc_chunk c;
     c.off = (int64_t)hb.seq.size();
     c.len = len;
-    c.flags = 0;
+    c.flags = 2;  // bit 1: only A, C, G, T (scores are 100 or 0)
     for (int i = 0; i < len; ++i) {
       const unsigned char b = (unsigned char)s[i];
       if (b >= 128) return "sequence byte >= 128";
-      if (class_has_thirds(class_of(b))) c.flags |= 1;
+      const int cl = class_of(b);
+      if (class_has_thirds(cl)) c.flags |= 1;
+      if (cl > 3) c.flags &= ~2;
       hb.seq.push_back(b);
     }
     idx = (int)tab.size();
@@ -63,6 +65,9 @@ inline std::string pack_batch(const sxc_pair* in, int n, int N, HostBatch& hb) {
   for (int k = 0; k < n; ++k) {
     const sxc_pair& p = in[k];
     sxc_pair_dev& d = hb.pairs[k];
+    // lags span +-N/2: the shorter chunk must fit in half the transform (the slave uses
+    // N = 2 * t_chunk, so every chunk does)
+    if (p.tlen > N / 2 && p.qlen > N / 2) return "both chunks longer than N/2";
     std::string e = add(p.tseq, p.tlen, hb.tchunks, tmap, d.t_chunk);
     if (!e.empty()) return e;
     e = add(p.qseq, p.qlen, hb.qchunks, qmap, d.q_chunk);

@@ -6,6 +6,7 @@
 
 #include "sxc_kernels.h"
 #include "sxc_pipeline.h"
+#include "sxc_scan.h"
 
 namespace sxc {
 
@@ -36,15 +37,33 @@ __device__ __forceinline__ int xcd_pair_strand(int b, int strands, int total) {
 }
 
 template <int NB>
-__global__ __launch_bounds__(Geo<NB>::NT) void k_pair(PairArgs a, int total, int grid) {
+__global__ __launch_bounds__(Geo<NB>::NT) void k_correlate(PairArgs a, int total) {
   __shared__ __align__(16) char sm[Smem<NB>::BYTES];
   Regs<NB> r;
   DevExec<Regs<NB>> ex{r};
-  // grid is rounded up to a multiple of 16 for the strand interleave
+  // the grid is rounded up to a multiple of 16 for the strand interleave
   const int ps = xcd_pair_strand((int)blockIdx.x, a.strands, total);
   if (ps < 0) return;
-  (void)grid;
-  pair_body<NB>(ex, sm, a, ps);
+  correlate_body<NB>(ex, sm, a, ps);
+}
+
+struct NoRegs {};
+
+template <int NB>
+__global__ __launch_bounds__(SCAN_NT) void k_scan(ScanArgs a) {
+  __shared__ __align__(16) char sm[ScanSmem<NB>::BYTES];
+  NoRegs r;
+  DevExec<NoRegs> ex{r};
+  scan_body<NB>(ex, sm, a, (int)blockIdx.x);
+}
+
+template <int NB>
+__global__ __launch_bounds__(Geo<NB>::NT) void k_peaks(const float* xc, double cutoff, uint16_t* cands,
+                                                     int32_t* ncand) {
+  __shared__ __align__(16) char sm[Smem<NB>::BYTES];
+  Regs<NB> r;
+  DevExec<Regs<NB>> ex{r};
+  peaks_body<NB>(ex, sm, xc, cutoff, cands, ncand);
 }
 
 template <int NB>
@@ -87,20 +106,32 @@ __global__ __launch_bounds__(Geo<NB>::NT) void k_xcorr(XcorrArgs a) {
   for (int p = threadIdx.x; p < Geo<NB>::N; p += Geo<NB>::NT) a.xc[(size_t)k * Geo<NB>::N + p] = Y[pad(p)];
 }
 
-// SeqAnalyzer::MatchUp on a given xc: peaks + scan, runs written instead of filtered.
-template <int NB>
-__global__ __launch_bounds__(Geo<NB>::NT) void k_matchup(MatchupArgs a) {
-  __shared__ __align__(16) char sm[Smem<NB>::BYTES];
-  Regs<NB> r;
-  DevExec<Regs<NB>> ex{r};
-  matchup_body<NB>(ex, sm, a, (int)blockIdx.x);
-}
-
 // GetMatchProbabilityEx + PrintMatch identity for given runs (one thread per run).
 __global__ void k_prob(ProbArgs a) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= a.n) return;
-  run_tail_direct(a, k);
+  const int32_t* r = a.runs + 3 * k;
+  const int st = r[0], sq = r[1], len = r[2];
+  const sxc_tables* tab = a.tab;
+  double matches = 0, gct = 0, gcq = 0;
+  for (int i = 0; i < len; ++i) {
+    const int ca = tab->cls[a.t[st + i] & 127], cb = tab->cls[a.q[sq + i] & 127];
+    matches += tab->equal[ca][cb];
+    gct += tab->gc[ca];
+    gcq += tab->gc[cb];
+  }
+  const double ident = matches / (double)len;
+  const double gc_target = gct / (double)len;
+  const double at_target = 1. - gc_target;
+  double rr = gcq * gc_target;
+  rr += ((double)len - gcq) * at_target;
+  const double p = rr / (double)len / 2.;
+  const double s = sqrt(p * (1. - p) * (double)len);
+  const double m = p * (double)len;
+  const double x = (double)len * ident;
+  const double cdf = 0.5 * (1. + erf((m - x) / s / 1.414213562));
+  a.prob[k] = exp(-(cdf * a.target_size));
+  a.ident[k] = ident;
 }
 
 // Exclusive scan of per-(pair, strand) kept counts (clipped to cap) -> offsets, total.
@@ -189,9 +220,23 @@ hipError_t launch_order_runs(const int32_t* runs, const uint64_t* keys, int n, i
     default: return hipErrorInvalidValue; \
   }
 
-hipError_t launch_pair(int nb, const PairArgs& a, int total, hipStream_t s) {
+hipError_t launch_correlate(int nb, const PairArgs& a, int total, hipStream_t s) {
   const int grid = (a.strands == 2) ? ((total + 15) / 16) * 16 : total;
-#define CALL(X) hipLaunchKernelGGL(k_pair<X>, dim3(grid), dim3(Geo<X>::NT), 0, s, a, total, grid)
+#define CALL(X) hipLaunchKernelGGL(k_correlate<X>, dim3(grid), dim3(Geo<X>::NT), 0, s, a, total)
+  SXC_NB_SWITCH(nb, CALL)
+#undef CALL
+  return hipGetLastError();
+}
+
+hipError_t launch_scan(int nb, const ScanArgs& a, int total, hipStream_t s) {
+#define CALL(X) hipLaunchKernelGGL(k_scan<X>, dim3(total), dim3(SCAN_NT), 0, s, a)
+  SXC_NB_SWITCH(nb, CALL)
+#undef CALL
+  return hipGetLastError();
+}
+
+hipError_t launch_peaks(int nb, const float* xc, double cutoff, uint16_t* cands, int32_t* ncand, hipStream_t s) {
+#define CALL(X) hipLaunchKernelGGL(k_peaks<X>, dim3(1), dim3(Geo<X>::NT), 0, s, xc, cutoff, cands, ncand)
   SXC_NB_SWITCH(nb, CALL)
 #undef CALL
   return hipGetLastError();
@@ -213,13 +258,6 @@ hipError_t launch_encode(int nb, const EncodeArgs& a, int n, hipStream_t s) {
 
 hipError_t launch_xcorr(int nb, const XcorrArgs& a, int n, hipStream_t s) {
 #define CALL(X) hipLaunchKernelGGL(k_xcorr<X>, dim3(n), dim3(Geo<X>::NT), 0, s, a)
-  SXC_NB_SWITCH(nb, CALL)
-#undef CALL
-  return hipGetLastError();
-}
-
-hipError_t launch_matchup(int nb, const MatchupArgs& a, int n, hipStream_t s) {
-#define CALL(X) hipLaunchKernelGGL(k_matchup<X>, dim3(n), dim3(Geo<X>::NT), 0, s, a)
   SXC_NB_SWITCH(nb, CALL)
 #undef CALL
   return hipGetLastError();

@@ -14,10 +14,7 @@
 //   correlate   DoOne product (:477-492) + do_ifft + rescale + fftshift (:493-505), summed
 //               over A, C, G, T in that order (:395-403)
 //   peaks       SeqAnalyzer::FindTop (:878-944)
-//   scan        SeqAnalyzer::DoOne, OPTIM_DOONE (:667-724) over every accepted lag
-//   tail        GetMatchProbabilityEx (analysis/AlignProbability.cc:62-127), PrintMatch
-//               identity (analysis/CrossCorr.cc:967-1037), FilterMatches
-//               (analysis/HomologyByXCorrSlave.cc:168-219)
+// The diagonal scan and the FP64 tail live in sxc_scan.h.
 #pragma once
 #include <math.h>
 
@@ -36,18 +33,11 @@ struct PairArgs {
   const float* spectra;      // resident target spectra, 4*N floats per target chunk
   int32_t npairs;
   int32_t strands;           // 1 = forward only, 2 = forward then reverse complement
-  int32_t min_len;           // -l
-  int32_t q_chunk;           // -q_chunk (RCQuery chunk size)
   double cutoff;
   double cutoff_fast;
-  double min_prob;
-  double target_total;
-  sxc_result_dev* out;       // cap records per (pair, strand)
-  uint64_t* keys;            // ordering key per record
-  int32_t* counts;           // kept records per (pair, strand) (may exceed cap)
-  int32_t cap;
+  uint16_t* cands;           // accepted lags, N slots per (pair, strand)
+  int32_t* ncand;            // accepted lags per (pair, strand)
   float* xc_out;             // optional: N floats per (pair, strand)
-  int32_t* stats;            // optional: [candidates, runs] per (pair, strand)
 };
 
 struct SpecArgs {
@@ -78,20 +68,6 @@ struct XcorrArgs {
   const float* spectra;      // 4*N target spectra per item
   const float* qsig;         // 4*N query signals per item
   float* xc;                 // N floats per item
-};
-
-struct MatchupArgs {
-  const sxc_tables* tab;
-  const double* ent_lut;
-  const unsigned char* seq;
-  const sxc_chunk* tchunks;  // one target chunk per item
-  const sxc_chunk* qchunks;  // one query chunk per item
-  const float* xc;           // N floats per item
-  double cutoff;
-  int32_t* runs;             // cap x {start_target, start_query, len} per item
-  uint64_t* keys;
-  int32_t* counts;
-  int32_t cap;
 };
 
 struct ProbArgs {
@@ -131,11 +107,8 @@ struct Smem {
   static constexpr int QUART = SCORE + SXC_NCLASS * SXC_NCLASS;  // uchar[18][4]
   static constexpr int QSEQ = (QUART + 4 * SXC_NCLASS + 15) & ~15;  // uchar[N] strand query
   static constexpr int BYTES = (QSEQ + N + 15) & ~15;
-  // scan-phase carve-up of Y (after the peaks phase)
+  // after the peaks phase Y holds the accepted lags
   static constexpr int CAND = Y;                               // uint16[N]
-  static constexpr int TSEQ = Y + 2 * N;                       // uchar[N]
-  static constexpr int SUMM = TSEQ + N;                        // int2[NT]
-  static_assert(SUMM + 8 * NT <= Y + 4 * G::YLEN, "scan carve-up exceeds the FFT area");
 };
 
 enum { MISC_L = 0, MISC_THIRDS = 1, MISC_NCAND = 2, MISC_KEPT = 3, MISC_Q4 = 4 /* 4 ints */, MISC_RUNS = 8 };
@@ -496,182 +469,17 @@ SXC_HD inline void peaks(Ex& ex, char* sm, double cutoff) {
   ex.sync();
 }
 
-// ---------------------------------------------------------------- scan + tail
+// ---------------------------------------------------------------- fused correlate body
 
-struct RunCtx {
-  int shift, i0, npos, cand_rank;
-};
-
-// GetMatchProbabilityEx (analysis/AlignProbability.cc:62-127) on one match; `ident` is the
-// identity both it and PrintMatch return (DNA_Equal summed in the same order).
-SXC_HD inline double match_tail(const unsigned char* ts, const unsigned char* qs, const unsigned char* cls,
-                                const double* equal, const double* gcv, int start_t, int start_q, int len,
-                                double target_size, double& ident) {
-  double matches = 0, gct = 0, gcq = 0;
-  for (int i = 0; i < len; ++i) {
-    const int ca = cls[ts[start_t + i]], cb = cls[qs[start_q + i]];
-    matches += equal[ca * SXC_NCLASS + cb];
-    gct += gcv[ca];
-    gcq += gcv[cb];
-  }
-  ident = matches / (double)len;
-  const double gc_target = gct / (double)len;
-  // GCAdjustExpect / Sigma / CDF (analysis/AlignProbability.cc:11-40)
-  const double at_target = 1. - gc_target;
-  double r = gcq * gc_target;
-  r += ((double)len - gcq) * at_target;
-  const double p = r / (double)len / 2.;
-  const double s = sqrt(p * (1. - p) * (double)len);
-  const double m = p * (double)len;
-  const double x = (double)len * ident;
-  const double cdf = 0.5 * (1. + erf((m - x) / s / 1.414213562));
-  const double expect = cdf * target_size;
-  return exp(-expect);
-}
-
-SXC_HD inline void run_tail_direct(const ProbArgs& a, int k) {
-  const int32_t* r = a.runs + 3 * k;
-  double ident;
-  const double prob = match_tail(a.t, a.q, a.tab->cls, &a.tab->equal[0][0], a.tab->gc, r[0], r[1], r[2],
-                                 a.target_size, ident);
-  a.prob[k] = prob;
-  a.ident[k] = ident;
-}
-
-// FP64 tail of one run; appends a kept t_result (FilterMatches, analysis/HomologyByXCorrSlave.cc:168-219).
+// One workgroup per (pair, strand): encode -> FFT -> product -> iFFT -> channel sum ->
+// FindTop; the accepted lags (ascending) go to global memory for the scan kernel.
 template <int NB, class Ex>
-SXC_HD inline void tail_run(Ex& ex, char* sm, const PairArgs& a, const sxc_pair_dev& pr, int ps, int strand,
-                            const RunCtx& rc, int ns, int ne) {
-  using SM = Smem<NB>;
-  const int start_t = rc.i0 + ns - SXC_MIN_LEN;
-  const int len = ne - ns + SXC_MIN_LEN;
-  const int start_q = start_t + rc.shift;
-  int* misc = (int*)(sm + SM::MISC);
-  ex.atomic_add_shared(&misc[MISC_RUNS], 1);
-  if (len < a.min_len) return;
-  double ident;
-  const double prob = match_tail((const unsigned char*)(sm + SM::TSEQ), (const unsigned char*)(sm + SM::QSEQ),
-                                 (const unsigned char*)(sm + SM::CLS), (const double*)(sm + SM::EQUAL),
-                                 (const double*)(sm + SM::GC), start_t, start_q, len, a.target_total, ident);
-  if (prob < a.min_prob) return;
-  const int k = ex.atomic_add_shared(&misc[MISC_KEPT], 1);
-  if (k >= a.cap) return;
-  const int tstart = pr.t_chunk_start + start_t;
-  // RCQuery (analysis/HomologyByXCorrSlave.cc:56-60) for the reverse strand
-  const int qstart = strand == 0 ? pr.q_chunk_start + start_q : start_q + pr.q_record_len - pr.q_chunk_start - a.q_chunk;
-  sxc_result_dev res;
-  res.query_id = (uint64_t)(int64_t)pr.q_record;
-  res.target_id = (uint64_t)(int64_t)pr.t_record;
-  res.query_size = (uint64_t)(int64_t)pr.q_record_len;
-  res.qstart = (uint64_t)(int64_t)qstart;
-  res.tstart = (uint64_t)(int64_t)tstart;
-  res.len = (uint64_t)(int64_t)len;
-  res.reverse = (uint8_t)(strand == 1);
-  for (int z = 0; z < 7; ++z) res.pad_[z] = 0;
-  res.prob = prob;
-  res.ident = ident;
-  const size_t at = (size_t)ps * a.cap + k;
-  a.out[at] = res;
-  a.keys[at] = ((uint64_t)(uint32_t)rc.cand_rank << 32) | (uint32_t)ns;
-}
-
-// Diagonal scan of every accepted lag.  Lags are taken GSZ threads at a time; each
-// thread slides the 46-wide window over one contiguous segment of the diagonal.
-template <int NB, class Ex, class Emit>
-SXC_HD inline void scan(Ex& ex, char* sm, int Lq, int Lt, Emit&& emit) {
-  using SM = Smem<NB>;
-  using G = Geo<NB>;
-  constexpr int GSZ = (G::N <= 8192) ? 64 : 64 * (G::N / 8192);
-  constexpr int CE = G::NT / GSZ;  // lags per epoch
-  static_assert(CE >= 1, "workgroup smaller than a lag group");
-  constexpr int INF = 0x7fffffff;
-  const int* misc = (const int*)(sm + SM::MISC);
-  const int ncand = misc[MISC_NCAND];
-  const uint16_t* cand = (const uint16_t*)(sm + SM::CAND);
-  const unsigned char* ts = (const unsigned char*)(sm + SM::TSEQ);
-  const unsigned char* qs = (const unsigned char*)(sm + SM::QSEQ);
-  const unsigned char* cls = (const unsigned char*)(sm + SM::CLS);
-  const unsigned char* score = (const unsigned char*)(sm + SM::SCORE);
-  int* summ = (int*)(sm + SM::SUMM);  // [tid][2] = {first_end, open_start}
-  for (int e0 = 0; e0 < ncand; e0 += CE) {
-    ex.each([&](int tid) {
-      const int c = e0 + tid / GSZ;
-      const int l = tid % GSZ;
-      int first_end = INF, open = -1;
-      if (c < ncand) {
-        RunCtx rc;
-        rc.cand_rank = c;
-        rc.shift = (int)cand[c] - G::H;
-        rc.i0 = rc.shift < 0 ? -rc.shift : 0;
-        int iend = Lq - rc.shift;
-        if (Lt - 1 < iend) iend = Lt - 1;
-        rc.npos = (rc.i0 < Lt && iend > rc.i0) ? iend - rc.i0 : 0;
-        const int S = (rc.npos + GSZ - 1) / GSZ;
-        const int na = l * S;
-        const int nb = (na + S < rc.npos) ? na + S : rc.npos;
-        if (na < nb) {
-          const int base_t = rc.i0, base_q = rc.i0 + rc.shift;
-          int W = 0;
-          for (int m = (na - 46 > 0 ? na - 46 : 0); m < na; ++m) W += score_of(cls, score, ts[base_t + m], qs[base_q + m]);
-          bool prev = (na - 1 >= 46) && W > SXC_WIN_THRESHOLD;
-          for (int n = na; n < nb; ++n) {
-            W += score_of(cls, score, ts[base_t + n], qs[base_q + n]);
-            if (n >= 46) W -= score_of(cls, score, ts[base_t + n - 46], qs[base_q + n - 46]);
-            const bool on = (n >= 46) && W > SXC_WIN_THRESHOLD;
-            if (on && !prev) {
-              open = n;
-            } else if (!on && prev) {
-              if (open >= 0) {
-                emit(rc, open, n);
-                open = -1;
-              } else if (first_end == INF) {
-                first_end = n;
-              }
-            }
-            prev = on;
-          }
-        }
-      }
-      summ[2 * tid] = first_end;
-      summ[2 * tid + 1] = open;
-    });
-    ex.sync();
-    ex.each([&](int tid) {
-      const int c = e0 + tid / GSZ;
-      const int l = tid % GSZ;
-      const int open = summ[2 * tid + 1];
-      if (c >= ncand || open < 0) return;
-      int end = INF;
-      const int g0 = tid - l;
-      for (int k = l + 1; k < GSZ; ++k) {
-        const int fe = summ[2 * (g0 + k)];
-        if (fe < end) end = fe;
-      }
-      RunCtx rc;
-      rc.cand_rank = c;
-      rc.shift = (int)cand[c] - G::H;
-      rc.i0 = rc.shift < 0 ? -rc.shift : 0;
-      int iend = Lq - rc.shift;
-      if (Lt - 1 < iend) iend = Lt - 1;
-      rc.npos = iend - rc.i0;
-      if (end == INF) end = rc.npos;  // run still open where the reference loop breaks
-      emit(rc, open, end);
-    });
-    ex.sync();
-  }
-}
-
-// ---------------------------------------------------------------- fused pair body
-
-// One workgroup per (pair, strand): encode -> FFT -> product -> iFFT -> peaks -> scan -> tail.
-template <int NB, class Ex>
-SXC_HD inline void pair_body(Ex& ex, char* sm, const PairArgs& a, int ps) {
+SXC_HD inline void correlate_body(Ex& ex, char* sm, const PairArgs& a, int ps) {
   using SM = Smem<NB>;
   using G = Geo<NB>;
   const int pair = ps / a.strands;
   const int strand = ps % a.strands;
   const sxc_pair_dev pr = a.pairs[pair];
-  const sxc_chunk tc = a.tchunks[pr.t_chunk];
   const sxc_chunk qc = a.qchunks[pr.q_chunk];
   const Twiddle tw{(const float*)(sm + SM::LUT), a.osc};
   phase_prepare<NB>(ex, sm, a.tab, a.ent_lut, a.seq + qc.off, qc.len, qc.flags & 1, strand);
@@ -683,60 +491,35 @@ SXC_HD inline void pair_body(Ex& ex, char* sm, const PairArgs& a, int ps) {
     });
   }
   peaks<NB>(ex, sm, pr.fast ? a.cutoff_fast : a.cutoff);
-  unsigned char* ts = (unsigned char*)(sm + SM::TSEQ);
-  const unsigned char* tsrc = a.seq + tc.off;
+  const int* misc = (const int*)(sm + SM::MISC);
+  const uint16_t* cand = (const uint16_t*)(sm + SM::CAND);
+  const int n = misc[MISC_NCAND];
+  uint16_t* dst = a.cands + (size_t)ps * G::N;
   ex.each([&](int tid) {
-    for (int i = tid; i < tc.len; i += G::NT) ts[i] = tsrc[i];
-  });
-  ex.sync();
-  scan<NB>(ex, sm, qc.len, tc.len,
-           [&](const RunCtx& rc, int ns, int ne) { tail_run<NB>(ex, sm, a, pr, ps, strand, rc, ns, ne); });
-  ex.each([&](int tid) {
-    if (tid == 0) {
-      const int* misc = (const int*)(sm + SM::MISC);
-      a.counts[ps] = misc[MISC_KEPT];
-      if (a.stats) {
-        a.stats[2 * ps] = misc[MISC_NCAND];
-        a.stats[2 * ps + 1] = misc[MISC_RUNS];
-      }
-    }
+    for (int i = tid; i < n; i += G::NT) dst[i] = cand[i];
+    if (tid == 0) a.ncand[ps] = n;
   });
 }
 
-// SeqAnalyzer::MatchUp on a given xc (analysis/CrossCorr.cc:583-605): runs, not filtered.
+// FindTop on a given xc (the first half of SeqAnalyzer::MatchUp, analysis/CrossCorr.cc:583-592).
 template <int NB, class Ex>
-SXC_HD inline void matchup_body(Ex& ex, char* sm, const MatchupArgs& a, int item) {
+SXC_HD inline void peaks_body(Ex& ex, char* sm, const float* xc, double cutoff, uint16_t* cands, int32_t* ncand) {
   using SM = Smem<NB>;
   using G = Geo<NB>;
-  const sxc_chunk tc = a.tchunks[item];
-  const sxc_chunk qc = a.qchunks[item];
-  phase_prepare<NB>(ex, sm, a.tab, a.ent_lut, a.seq + qc.off, qc.len, qc.flags & 1, 0);
   float* Y = (float*)(sm + SM::Y);
-  const float* xc = a.xc + (size_t)item * G::N;
+  int* misc0 = (int*)(sm + SM::MISC);
   ex.each([&](int tid) {
     for (int p = tid; p < G::N; p += G::NT) Y[pad(p)] = xc[p];
+    if (tid < 16) misc0[tid] = 0;
   });
   ex.sync();
-  peaks<NB>(ex, sm, a.cutoff);
-  unsigned char* ts = (unsigned char*)(sm + SM::TSEQ);
-  const unsigned char* tsrc = a.seq + tc.off;
+  peaks<NB>(ex, sm, cutoff);
+  const int* misc = (const int*)(sm + SM::MISC);
+  const uint16_t* cand = (const uint16_t*)(sm + SM::CAND);
+  const int n = misc[MISC_NCAND];
   ex.each([&](int tid) {
-    for (int i = tid; i < tc.len; i += G::NT) ts[i] = tsrc[i];
-  });
-  ex.sync();
-  int* misc = (int*)(sm + SM::MISC);
-  scan<NB>(ex, sm, qc.len, tc.len, [&](const RunCtx& rc, int ns, int ne) {
-    const int k = ex.atomic_add_shared(&misc[MISC_KEPT], 1);
-    if (k >= a.cap) return;
-    const int start_t = rc.i0 + ns - SXC_MIN_LEN;
-    const size_t at = (size_t)item * a.cap + k;
-    a.runs[3 * at] = start_t;
-    a.runs[3 * at + 1] = start_t + rc.shift;
-    a.runs[3 * at + 2] = ne - ns + SXC_MIN_LEN;
-    a.keys[at] = ((uint64_t)(uint32_t)rc.cand_rank << 32) | (uint32_t)ns;
-  });
-  ex.each([&](int tid) {
-    if (tid == 0) a.counts[item] = misc[MISC_KEPT];
+    for (int i = tid; i < n; i += G::NT) cands[i] = cand[i];
+    if (tid == 0) *ncand = n;
   });
 }
 

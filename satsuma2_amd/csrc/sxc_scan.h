@@ -1,0 +1,355 @@
+// sxc_scan.h -- diagonal scan of every accepted lag and the FP64 match tail.
+//
+// One workgroup per (pair, strand).  SeqAnalyzer::DoOne (OPTIM_DOONE,
+// analysis/CrossCorr.cc:667-724) slides a 46-wide window of integer scores
+// (int)(100 * DNA_EqualAmb + 0.5) along the diagonal of each lag FindTop accepted and
+// opens a run while the window sum exceeds 1889.  Here a lag's diagonal is split into
+// contiguous segments, one per thread; each thread recomputes the window at its segment
+// start, records the runs that open and close inside its segment, and a second phase
+// closes runs that cross segment boundaries.  Runs go to an LDS queue; the queue is
+// drained by all threads at once, one run per thread, through GetMatchProbabilityEx
+// (analysis/AlignProbability.cc:62-127) and FilterMatches
+// (analysis/HomologyByXCorrSlave.cc:168-219).  Kept records carry the key
+// (lag rank, run start) so the finalize kernel restores the reference's order.
+//
+// Queue capacity holds every run one lag can produce (runs need a below-threshold gap,
+// so at most npos/2 + 1 per lag, npos <= N/2); an epoch of several lags that overflows
+// is discarded and redone one lag at a time.
+#pragma once
+#include <math.h>
+
+#include "sxc_fft.h"
+
+namespace sxc {
+
+struct ScanArgs {
+  const sxc_tables* tab;
+  const unsigned char* seq;
+  const sxc_chunk* tchunks;
+  const sxc_chunk* qchunks;
+  const sxc_pair_dev* pairs;
+  const uint16_t* cands;     // accepted lags, N slots per (pair, strand)
+  const int32_t* ncand;      // accepted lags per (pair, strand)
+  int32_t n_fft;             // N
+  int32_t strands;
+  int32_t min_len;
+  int32_t q_chunk;
+  double min_prob;
+  double target_total;
+  int32_t mode;              // 0: filter runs into t_results; 1: write runs (MatchUp)
+  sxc_result_dev* out;       // mode 0: cap records per (pair, strand)
+  int32_t* runs;             // mode 1: cap x {start_target, start_query, len} per (pair, strand)
+  uint64_t* keys;
+  int32_t* counts;
+  int32_t cap;
+  int32_t* stats;            // optional {candidates, runs} per (pair, strand)
+};
+
+constexpr int SCAN_NT = 256;  // threads per scan workgroup
+constexpr int SCAN_GSZ = 64;  // threads per lag in the batched phase
+
+template <int NB>
+struct ScanSmem {
+  static constexpr int N = 1 << NB;
+  static constexpr int QCAP = N / 4 + 64;  // runs of one lag: npos / 2 + 1 with npos <= N / 2
+  static constexpr int TCODE = 0;                       // uchar[N]
+  static constexpr int QCODE = TCODE + N;               // uchar[N]
+  static constexpr int QUEUE = QCODE + N;               // int2[QCAP]: {lag rank, ns | ne << 16}
+  static constexpr int SUMM = QUEUE + 8 * QCAP;         // int2[SCAN_NT]
+  static constexpr int EQUAL = SUMM + 8 * SCAN_NT;      // double[18][18]
+  static constexpr int GC = EQUAL + 8 * SXC_NCLASS * SXC_NCLASS;  // double[18]
+  static constexpr int SCORE = GC + 8 * SXC_NCLASS;     // uchar[18][18]
+  static constexpr int MISC = (SCORE + SXC_NCLASS * SXC_NCLASS + 15) & ~15;  // int[16]
+  static constexpr int BYTES = MISC + 64;
+};
+
+enum { SM_QN = 0, SM_KEPT = 1, SM_RUNS = 2, SM_NCAND = 3, SM_LT = 4, SM_LQ = 5, SM_FAST = 6 };
+
+// Base code: class id (0..16) of a codec letter, or 128 | byte for a byte whose codec
+// row is zero.  Two bases are identical bytes iff their codes are equal.
+SXC_HD inline unsigned char base_code(const sxc_tables* tab, unsigned char b) {
+  const unsigned char c = tab->cls[b & 127];
+  return c < SXC_CLASS_ZERO ? c : (unsigned char)(128 | b);
+}
+
+// (int)(100 * DNA_EqualAmb(a, b) + 0.5): identical letters other than 'N' score 100
+// (analysis/DNAVector.cc:372-387), 'N':'N' scores 25, a zero-row byte scores 0 against
+// any different byte.
+SXC_HD inline int code_score(const unsigned char* score, unsigned ca, unsigned cb) {
+  if (ca == cb) return ca == SXC_CLASS_N ? 25 : 100;
+  if ((ca | cb) & 128) return 0;
+  return score[ca * SXC_NCLASS + cb];
+}
+
+SXC_HD inline int code_class(unsigned c) { return c < 128 ? (int)c : SXC_CLASS_ZERO; }
+
+// GetMatchProbabilityEx on base codes; `ident` is also PrintMatch's identity (the same
+// DNA_Equal sum in the same order).
+SXC_HD inline double code_tail(const unsigned char* tc, const unsigned char* qc, const double* equal,
+                               const double* gcv, int start_t, int start_q, int len, double target_size,
+                               double& ident) {
+  double matches = 0, gct = 0, gcq = 0;
+  for (int i = 0; i < len; ++i) {
+    const int ca = code_class(tc[start_t + i]), cb = code_class(qc[start_q + i]);
+    matches += equal[ca * SXC_NCLASS + cb];
+    gct += gcv[ca];
+    gcq += gcv[cb];
+  }
+  ident = matches / (double)len;
+  const double gc_target = gct / (double)len;
+  // GCAdjustExpect, Sigma, CDF (analysis/AlignProbability.cc:11-40)
+  const double at_target = 1. - gc_target;
+  double r = gcq * gc_target;
+  r += ((double)len - gcq) * at_target;
+  const double p = r / (double)len / 2.;
+  const double s = sqrt(p * (1. - p) * (double)len);
+  const double m = p * (double)len;
+  const double x = (double)len * ident;
+  const double cdf = 0.5 * (1. + erf((m - x) / s / 1.414213562));
+  const double expect = cdf * target_size;
+  return exp(-expect);
+}
+
+struct Lag {
+  int shift, i0, npos;
+};
+
+SXC_HD inline Lag lag_geometry(int idx, int H, int Lq, int Lt) {
+  // SeqAnalyzer::DoOne loop bounds: i from max(0, -shift) while j = i + shift < Lq and
+  // i + 1 < Lt (analysis/CrossCorr.cc:685-696)
+  Lag g;
+  g.shift = idx - H;
+  g.i0 = g.shift < 0 ? -g.shift : 0;
+  int iend = Lq - g.shift;
+  if (Lt - 1 < iend) iend = Lt - 1;
+  g.npos = (g.i0 < Lt && iend > g.i0) ? iend - g.i0 : 0;
+  return g;
+}
+
+// Scans lags [c0, c1) with gsz threads per lag; runs are pushed to the LDS queue.
+template <int NB, class Ex>
+SXC_HD inline void scan_epoch(Ex& ex, char* sm, const uint16_t* cands, int c0, int c1, int gsz) {
+  using SS = ScanSmem<NB>;
+  constexpr int INF = 0x7fffffff;
+  constexpr int H = (1 << NB) / 2;
+  const unsigned char* tc = (const unsigned char*)(sm + SS::TCODE);
+  const unsigned char* qc = (const unsigned char*)(sm + SS::QCODE);
+  const unsigned char* score = (const unsigned char*)(sm + SS::SCORE);
+  int* misc = (int*)(sm + SS::MISC);
+  int* queue = (int*)(sm + SS::QUEUE);
+  int* summ = (int*)(sm + SS::SUMM);
+  const int Lt = misc[SM_LT], Lq = misc[SM_LQ];
+  const bool fast = misc[SM_FAST] != 0;
+  auto push = [&](int rank, int ns, int ne) {
+    const int k = ex.atomic_add_shared(&misc[SM_QN], 1);
+    if (k < SS::QCAP) {
+      queue[2 * k] = rank;
+      queue[2 * k + 1] = ns | (ne << 16);
+    }
+  };
+  ex.each([&](int tid) {
+    const int c = c0 + tid / gsz;
+    const int l = tid % gsz;
+    int first_end = INF, open = -1;
+    if (c < c1) {
+      const Lag g = lag_geometry(cands[c], H, Lq, Lt);
+      const int S = (g.npos + gsz - 1) / gsz;
+      const int na = l * S;
+      const int nb = (na + S < g.npos) ? na + S : g.npos;
+      if (na < nb) {
+        const unsigned char* tp = tc + g.i0;
+        const unsigned char* qp = qc + g.i0 + g.shift;
+        auto sc = [&](int n) -> int {
+          const unsigned a = tp[n], b = qp[n];
+          return fast ? (a == b ? 100 : 0) : code_score(score, a, b);
+        };
+        int W = 0;
+        for (int m = (na - 46 > 0 ? na - 46 : 0); m < na; ++m) W += sc(m);
+        bool prev = (na - 1 >= 46) && W > SXC_WIN_THRESHOLD;
+        for (int n = na; n < nb; ++n) {
+          W += sc(n);
+          if (n >= 46) W -= sc(n - 46);
+          const bool on = (n >= 46) && W > SXC_WIN_THRESHOLD;
+          if (on && !prev) {
+            open = n;
+          } else if (!on && prev) {
+            if (open >= 0) {
+              push(c, open, n);
+              open = -1;
+            } else if (first_end == INF) {
+              first_end = n;
+            }
+          }
+          prev = on;
+        }
+      }
+    }
+    summ[2 * tid] = first_end;
+    summ[2 * tid + 1] = open;
+  });
+  ex.sync();
+  ex.each([&](int tid) {
+    const int c = c0 + tid / gsz;
+    const int l = tid % gsz;
+    const int open = summ[2 * tid + 1];
+    if (c >= c1 || open < 0) return;
+    int end = INF;
+    const int g0 = tid - l;
+    for (int k = l + 1; k < gsz; ++k) {
+      const int fe = summ[2 * (g0 + k)];
+      if (fe < end) end = fe;
+    }
+    if (end == INF) end = lag_geometry(cands[c], H, Lq, Lt).npos;  // open where the loop breaks
+    push(c, open, end);
+  });
+  ex.sync();
+}
+
+// Drains queue entries [0, n): mode 0 filters into t_results, mode 1 writes runs.
+template <int NB, class Ex>
+SXC_HD inline void drain(Ex& ex, char* sm, const ScanArgs& a, const uint16_t* cands, int ps, int n) {
+  using SS = ScanSmem<NB>;
+  constexpr int H = (1 << NB) / 2;
+  const unsigned char* tc = (const unsigned char*)(sm + SS::TCODE);
+  const unsigned char* qc = (const unsigned char*)(sm + SS::QCODE);
+  const double* equal = (const double*)(sm + SS::EQUAL);
+  const double* gcv = (const double*)(sm + SS::GC);
+  int* misc = (int*)(sm + SS::MISC);
+  const int* queue = (const int*)(sm + SS::QUEUE);
+  const int Lt = misc[SM_LT], Lq = misc[SM_LQ];
+  const int strand = ps % a.strands;
+  ex.each([&](int tid) {
+    for (int k = tid; k < n; k += SCAN_NT) {
+      const int rank = queue[2 * k];
+      const int ns = queue[2 * k + 1] & 0xffff, ne = (queue[2 * k + 1] >> 16) & 0xffff;
+      const Lag g = lag_geometry(cands[rank], H, Lq, Lt);
+      const int start_t = g.i0 + ns - SXC_MIN_LEN;
+      const int len = ne - ns + SXC_MIN_LEN;
+      const int start_q = start_t + g.shift;
+      const uint64_t key = ((uint64_t)(uint32_t)rank << 32) | (uint32_t)ns;
+      if (a.mode == 1) {
+        const int slot = ex.atomic_add_shared(&misc[SM_KEPT], 1);
+        if (slot >= a.cap) continue;
+        const size_t at = (size_t)ps * a.cap + slot;
+        a.runs[3 * at] = start_t;
+        a.runs[3 * at + 1] = start_q;
+        a.runs[3 * at + 2] = len;
+        a.keys[at] = key;
+        continue;
+      }
+      if (len < a.min_len) continue;  // FilterMatches: GetLength() < m_minLen
+      double ident;
+      const double prob = code_tail(tc, qc, equal, gcv, start_t, start_q, len, a.target_total, ident);
+      if (prob < a.min_prob) continue;
+      const int slot = ex.atomic_add_shared(&misc[SM_KEPT], 1);
+      if (slot >= a.cap) continue;
+      const sxc_pair_dev pr = a.pairs[ps / a.strands];
+      const int tstart = pr.t_chunk_start + start_t;
+      // RCQuery (analysis/HomologyByXCorrSlave.cc:56-60) on the reverse strand
+      const int qstart =
+          strand == 0 ? pr.q_chunk_start + start_q : start_q + pr.q_record_len - pr.q_chunk_start - a.q_chunk;
+      sxc_result_dev r;
+      r.query_id = (uint64_t)(int64_t)pr.q_record;
+      r.target_id = (uint64_t)(int64_t)pr.t_record;
+      r.query_size = (uint64_t)(int64_t)pr.q_record_len;
+      r.qstart = (uint64_t)(int64_t)qstart;
+      r.tstart = (uint64_t)(int64_t)tstart;
+      r.len = (uint64_t)(int64_t)len;
+      r.reverse = (uint8_t)(strand == 1);
+      for (int z = 0; z < 7; ++z) r.pad_[z] = 0;
+      r.prob = prob;
+      r.ident = ident;
+      const size_t at = (size_t)ps * a.cap + slot;
+      a.out[at] = r;
+      a.keys[at] = key;
+    }
+  });
+  ex.sync();
+}
+
+template <int NB, class Ex>
+SXC_HD inline void scan_body(Ex& ex, char* sm, const ScanArgs& a, int ps) {
+  using SS = ScanSmem<NB>;
+  constexpr int N = 1 << NB;
+  const sxc_pair_dev pr = a.pairs[ps / a.strands];
+  const int strand = ps % a.strands;
+  const sxc_chunk tch = a.tchunks[pr.t_chunk];
+  const sxc_chunk qch = a.qchunks[pr.q_chunk];
+  const sxc_tables* tab = a.tab;
+  unsigned char* tc = (unsigned char*)(sm + SS::TCODE);
+  unsigned char* qc = (unsigned char*)(sm + SS::QCODE);
+  double* equal = (double*)(sm + SS::EQUAL);
+  double* gcv = (double*)(sm + SS::GC);
+  unsigned char* score = (unsigned char*)(sm + SS::SCORE);
+  int* misc = (int*)(sm + SS::MISC);
+  const unsigned char* ts = a.seq + tch.off;
+  const unsigned char* qs = a.seq + qch.off;
+  const uint16_t* cands = a.cands + (size_t)ps * N;
+  ex.each([&](int tid) {
+    for (int i = tid; i < tch.len; i += SCAN_NT) tc[i] = base_code(tab, ts[i]);
+    // DNAVector::ReverseComplement for strand 1 (analysis/DNAVector.cc:482-521)
+    for (int i = tid; i < qch.len; i += SCAN_NT)
+      qc[i] = base_code(tab, strand ? tab->rc[qs[qch.len - 1 - i] & 127] : qs[i]);
+    for (int i = tid; i < SXC_NCLASS * SXC_NCLASS; i += SCAN_NT) {
+      equal[i] = (&tab->equal[0][0])[i];
+      score[i] = (&tab->score[0][0])[i];
+    }
+    for (int i = tid; i < SXC_NCLASS; i += SCAN_NT) gcv[i] = tab->gc[i];
+    if (tid == 0) {
+      misc[SM_QN] = 0;
+      misc[SM_KEPT] = 0;
+      misc[SM_RUNS] = 0;
+      misc[SM_NCAND] = a.ncand[ps];
+      misc[SM_LT] = tch.len;
+      misc[SM_LQ] = qch.len;
+      misc[SM_FAST] = ((tch.flags & 2) && (qch.flags & 2)) ? 1 : 0;  // both pure A/C/G/T
+    }
+  });
+  ex.sync();
+  const int ncand = misc[SM_NCAND];
+  constexpr int CE = SCAN_NT / SCAN_GSZ;
+  int runs = 0;
+  for (int e0 = 0; e0 < ncand; e0 += CE) {
+    const int qstart = misc[SM_QN];
+    const int e1 = (e0 + CE < ncand) ? e0 + CE : ncand;
+    scan_epoch<NB>(ex, sm, cands, e0, e1, SCAN_GSZ);
+    const int qn = misc[SM_QN];
+    if (qn > SS::QCAP) {
+      // the epoch overflowed the queue: keep what came before it, redo its lags one by one
+      drain<NB>(ex, sm, a, cands, ps, qstart);
+      runs += qstart;
+      for (int c = e0; c < e1; ++c) {
+        ex.each([&](int tid) {
+          if (tid == 0) misc[SM_QN] = 0;
+        });
+        ex.sync();
+        scan_epoch<NB>(ex, sm, cands, c, c + 1, SCAN_NT);
+        const int n1 = misc[SM_QN];
+        drain<NB>(ex, sm, a, cands, ps, n1);
+        runs += n1;
+      }
+      ex.each([&](int tid) {
+        if (tid == 0) misc[SM_QN] = 0;
+      });
+      ex.sync();
+    } else if (qn > SS::QCAP / 2 || e1 == ncand) {
+      drain<NB>(ex, sm, a, cands, ps, qn);
+      runs += qn;
+      ex.each([&](int tid) {
+        if (tid == 0) misc[SM_QN] = 0;
+      });
+      ex.sync();
+    }
+  }
+  ex.each([&](int tid) {
+    if (tid == 0) {
+      a.counts[ps] = misc[SM_KEPT];
+      if (a.stats) {
+        a.stats[2 * ps] = ncand;
+        a.stats[2 * ps + 1] = runs;
+      }
+    }
+  });
+}
+
+}  // namespace sxc

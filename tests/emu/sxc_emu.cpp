@@ -13,6 +13,7 @@
 
 #include "../../satsuma2_amd/csrc/sxc_batch_host.h"
 #include "../../satsuma2_amd/csrc/sxc_pipeline.h"
+#include "../../satsuma2_amd/csrc/sxc_scan.h"
 
 namespace {
 
@@ -76,15 +77,54 @@ int run_batch(const sxc::HostBatch& hb, const sxc_params& prm, std::vector<sxc_r
   const int total = (int)hb.pairs.size() * prm.strands;
   std::vector<sxc_result_dev> slots((size_t)total * cap);
   std::vector<uint64_t> keys((size_t)total * cap);
-  std::vector<int32_t> counts(total);
-  sxc::PairArgs pa{&T.tab, T.osc.data(), T.ent[NB].data(), hb.seq.data(), hb.tchunks.data(), hb.qchunks.data(),
-                   hb.pairs.data(), spectra.data(), (int)hb.pairs.size(), prm.strands, prm.min_len, prm.q_chunk,
-                   prm.cutoff, prm.cutoff_fast, prm.min_prob, prm.target_total, slots.data(), keys.data(),
-                   counts.data(), cap, xc_out, stats};
+  std::vector<int32_t> counts(total), ncand(total);
+  std::vector<uint16_t> cands((size_t)total * G::N);
+  sxc::PairArgs pa{};
+  pa.tab = &T.tab;
+  pa.osc = T.osc.data();
+  pa.ent_lut = T.ent[NB].data();
+  pa.seq = hb.seq.data();
+  pa.tchunks = hb.tchunks.data();
+  pa.qchunks = hb.qchunks.data();
+  pa.pairs = hb.pairs.data();
+  pa.spectra = spectra.data();
+  pa.npairs = (int)hb.pairs.size();
+  pa.strands = prm.strands;
+  pa.cutoff = prm.cutoff;
+  pa.cutoff_fast = prm.cutoff_fast;
+  pa.cands = cands.data();
+  pa.ncand = ncand.data();
+  pa.xc_out = xc_out;
   for (int ps = 0; ps < total; ++ps) {
     EmuExec<sxc::Regs<NB>> ex(G::NT);
     memset(sm.data(), 0xCD, sm.size());
-    sxc::pair_body<NB>(ex, sm.data(), pa, ps);
+    sxc::correlate_body<NB>(ex, sm.data(), pa, ps);
+  }
+  sxc::ScanArgs sc{};
+  sc.tab = &T.tab;
+  sc.seq = hb.seq.data();
+  sc.tchunks = hb.tchunks.data();
+  sc.qchunks = hb.qchunks.data();
+  sc.pairs = hb.pairs.data();
+  sc.cands = cands.data();
+  sc.ncand = ncand.data();
+  sc.n_fft = G::N;
+  sc.strands = prm.strands;
+  sc.min_len = prm.min_len;
+  sc.q_chunk = prm.q_chunk;
+  sc.min_prob = prm.min_prob;
+  sc.target_total = prm.target_total;
+  sc.mode = 0;
+  sc.out = slots.data();
+  sc.keys = keys.data();
+  sc.counts = counts.data();
+  sc.cap = cap;
+  sc.stats = stats;
+  std::vector<char> ssm(sxc::ScanSmem<NB>::BYTES);
+  for (int ps = 0; ps < total; ++ps) {
+    EmuExec<int> ex(sxc::SCAN_NT);
+    memset(ssm.data(), 0xCD, ssm.size());
+    sxc::scan_body<NB>(ex, ssm.data(), sc, ps);
   }
   res.clear();
   for (int ps = 0; ps < total; ++ps) {
