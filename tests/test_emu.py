@@ -1,0 +1,136 @@
+"""The HIP kernel bodies run on the CPU workgroup emulator (tests/emu/sxc_emu.cpp) against
+the oracle.  The emulator executes the exact code of satsuma2_amd/csrc/sxc_*.h with a
+sequential executor; the AddressSanitizer build bounds-checks every LDS and global
+index the kernels compute.  This is how kernel indexing is checked before a GPU runs it.
+"""
+import ctypes as C
+import os
+import struct
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle
+from conftest import ROOT, lcg_noise, unpack2
+
+EMU = os.path.join(ROOT, "tests", "emu")
+BUILD = os.path.join(EMU, "_build")
+SRCS = [os.path.join(EMU, "sxc_emu.cpp"), os.path.join(ROOT, "satsuma2_amd", "csrc", "sxc_tables.cpp")]
+DEPS = SRCS + [os.path.join(ROOT, "satsuma2_amd", "csrc", f) for f in
+               ("sxc_fft.h", "sxc_pipeline.h", "sxc_scan.h", "sxc_types.h", "sxc_batch_host.h", "sxc_tables.h")]
+FLAGS = ["-std=c++17", "-ffp-contract=off", "-Wno-unknown-pragmas"]
+COLS = ["query_id", "target_id", "query_size", "qstart", "tstart", "len", "reverse", "prob", "ident"]
+
+
+def _stale(target):
+    return not os.path.exists(target) or os.path.getmtime(target) < max(os.path.getmtime(d) for d in DEPS)
+
+
+@pytest.fixture(scope="module")
+def emu_lib():
+    os.makedirs(BUILD, exist_ok=True)
+    so = os.path.join(BUILD, "libsxc_emu.so")
+    if _stale(so):
+        subprocess.run(["g++", "-O2", "-fPIC", "-shared", *FLAGS, "-o", so, *SRCS], check=True)
+    return C.CDLL(so)
+
+
+@pytest.fixture(scope="module")
+def emu_check():
+    os.makedirs(BUILD, exist_ok=True)
+    exe = os.path.join(BUILD, "emu_check")
+    if _stale(exe):
+        subprocess.run(["g++", "-O1", "-g", "-DSXC_EMU_MAIN", "-fsanitize=address", "-fno-omit-frame-pointer", *FLAGS,
+                        "-o", exe, *SRCS], check=True)
+    return exe
+
+
+class Pair(C.Structure):
+    _fields_ = [("tseq", C.c_char_p), ("tlen", C.c_int32), ("qseq", C.c_char_p), ("qlen", C.c_int32),
+                ("t_chunk_start", C.c_int32), ("q_chunk_start", C.c_int32), ("t_record", C.c_int32),
+                ("q_record", C.c_int32), ("q_record_len", C.c_int32), ("fast", C.c_int32)]
+
+
+def emu_batch(lib, pairs, N=8192, strands=2, target_total=4096.0 * 64, want_xc=False):
+    arr = (Pair * len(pairs))(*[Pair(p["tseq"], len(p["tseq"]), p["qseq"], len(p["qseq"]), 0, 0, p.get("t_record", 0),
+                                     p.get("q_record", 0), len(p["qseq"]), p.get("fast", 0)) for p in pairs])
+    prm = oracle.make_params(N=N, strands=strands, target_total=target_total)
+    out = np.zeros(1 << 18, oracle.RESULT_DTYPE)
+    n = C.c_int64(0)
+    total = len(pairs) * strands
+    xc = np.zeros((total, N), np.float32) if want_xc else None
+    st = np.zeros((total, 2), np.int32)
+    rc = lib.emu_align_batch(arr, len(pairs), C.byref(prm), out.ctypes.data_as(C.c_void_p), C.c_int64(len(out)),
+                             C.byref(n), xc.ctypes.data_as(C.c_void_p) if want_xc else None,
+                             st.ctypes.data_as(C.c_void_p))
+    assert rc == 0
+    return out[: n.value].copy(), xc, st
+
+
+@pytest.mark.parametrize("nb", range(9, 15))
+def test_emulated_fft(emu_lib, nb):
+    N = 1 << nb
+    x = lcg_noise(N, 3 * N + 7)
+    out = np.zeros(N, np.float32)
+    assert emu_lib.emu_fft_forward(x.ctypes.data_as(C.c_void_p), out.ctypes.data_as(C.c_void_p), N) == 0
+    assert np.array_equal(out.view(np.uint32), oracle.fft_forward(x).view(np.uint32))
+
+
+def test_emulated_batch_uniform(emu_lib, golden):
+    g = golden("uniform_pairs.npz")
+    P = 6
+    pairs = [dict(tseq=unpack2(g["t"][r], 4096), qseq=unpack2(g["q"][r], 4096), t_record=r, q_record=r,
+                  fast=int(g["fast"][r])) for r in range(P)]
+    got, xc, st = emu_batch(emu_lib, pairs, want_xc=True)
+    ref = np.concatenate(oracle.align_batch(
+        [dict(p, q_record_len=4096) for p in pairs], oracle.make_params(target_total=4096.0 * 64)))
+    for c in COLS:
+        assert np.array_equal(got[c], ref[c]), c
+    for ps in range(2 * P):
+        p = pairs[ps // 2]
+        qs = p["qseq"] if ps % 2 == 0 else oracle.reverse_complement(p["qseq"])
+        oxc = oracle.xcorr(oracle.encode(p["tseq"], 8192), oracle.encode(qs, 8192))
+        assert np.array_equal(xc[ps].view(np.uint32), oxc.view(np.uint32))
+        assert st[ps, 0] == len(oracle.findtop(oxc, 2.9 if p["fast"] else 1.8))
+
+
+def _write_input(path, pairs, N, target_total=1e6):
+    with open(path, "wb") as f:
+        f.write(struct.pack("<3i", N, 2, len(pairs)))
+        f.write(struct.pack("<4d", 1.8, 2.9, 0.99, target_total))
+        for t, q, fast in pairs:
+            f.write(struct.pack("<8i", len(t), len(q), 0, 0, 0, 0, len(q), fast))
+            f.write(t)
+            f.write(q)
+
+
+def _run_check(exe, tmp_path, pairs, N):
+    inp, outp = tmp_path / "in.bin", tmp_path / "out.bin"
+    _write_input(inp, pairs, N)
+    r = subprocess.run([exe, str(inp), str(outp)], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr[-2000:]
+    b = outp.read_bytes()
+    n = int(np.frombuffer(b, np.int64, 1)[0])
+    got = np.frombuffer(b, oracle.RESULT_DTYPE, n, 8)
+    ref = np.concatenate(oracle.align_batch([dict(tseq=t, qseq=q, fast=f) for t, q, f in pairs],
+                                            oracle.make_params(N=N, target_total=1e6)))
+    assert len(got) == len(ref)
+    for c in COLS:
+        assert np.array_equal(got[c], ref[c]), c
+
+
+def test_asan_edge_pairs(emu_check, golden, tmp_path):
+    g = golden("edge_pairs.npz")
+    pairs = [(g[f"{n}_t"].tobytes(), g[f"{n}_q"].tobytes(), fast) for n in g["names"] for fast in (0, 1)]
+    _run_check(emu_check, tmp_path, pairs, 8192)
+
+
+def test_asan_uneven_pairs(emu_check, golden, tmp_path):
+    g = golden("uneven_pairs.npz")
+    for k in range(0, int(g["n"][0]), 3):
+        N = int(g[f"N{k}"][0])
+        if N > 16384:
+            continue
+        lt, lq = g[f"len{k}"]
+        _run_check(emu_check, tmp_path, [(unpack2(g[f"t{k}"], lt), unpack2(g[f"q{k}"], lq), 0)], N)
