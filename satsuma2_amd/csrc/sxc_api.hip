@@ -176,6 +176,7 @@ int launch_pairs(sxc_ctx* ctx, sxc_batch_dev& b) {
   a.q_chunk = b.prm.q_chunk;
   a.min_prob = b.prm.min_prob;
   a.target_total = b.prm.target_total;
+  a.z_skip = sxc::keep_z_bound(b.prm.min_prob, b.prm.target_total);
   a.mode = 0;
   a.out = b.slots.as<sxc_result_dev>();
   a.runs = nullptr;
@@ -612,6 +613,7 @@ int sxc_matchup(sxc_ctx* ctx, const char* q, int32_t lq, const char* t, int32_t 
   a.ncand = d_misc;
   a.n_fft = N;
   a.strands = 1;
+  a.z_skip = HUGE_VAL;
   a.mode = 1;
   a.runs = ctx->p3.as<int32_t>();
   a.keys = ctx->p4.as<uint64_t>();

@@ -55,6 +55,7 @@ inline std::string pack_batch(const sxc_pair* in, int n, int N, HostBatch& hb) {
       if (cl > 3) c.flags &= ~2;
       hb.seq.push_back(b);
     }
+    while (hb.seq.size() % 16) hb.seq.push_back(0);  // next chunk 16-byte aligned
     idx = (int)tab.size();
     tab.push_back(c);
     m[key] = idx;
@@ -79,7 +80,7 @@ inline std::string pack_batch(const sxc_pair* in, int n, int N, HostBatch& hb) {
     d.q_record_len = p.q_record_len;
     d.fast = p.fast;
   }
-  while (hb.seq.size() % 16) hb.seq.push_back(0);
+  hb.seq.resize(hb.seq.size() + 16, 0);  // read-ahead of aligned word copies
   return "";
 }
 

@@ -36,6 +36,7 @@ struct ScanArgs {
   int32_t q_chunk;
   double min_prob;
   double target_total;
+  double z_skip;             // runs with (m - x) / s / 1.414213562 >= z_skip have prob < min_prob
   int32_t mode;              // 0: filter runs into t_results; 1: write runs (MatchUp)
   sxc_result_dev* out;       // mode 0: cap records per (pair, strand)
   int32_t* runs;             // mode 1: cap x {start_target, start_query, len} per (pair, strand)
@@ -46,20 +47,30 @@ struct ScanArgs {
 };
 
 constexpr int SCAN_NT = 256;  // threads per scan workgroup
-constexpr int SCAN_GSZ = 64;  // threads per lag in the batched phase
+constexpr int SCAN_GSZ = 16;  // threads per lag in the batched phase
+// smallest count c of identical A/C/G/T bases with 100 * c > 1889 (19 of 46)
+constexpr int SXC_MIN_MATCHES = SXC_WIN_THRESHOLD / 100 + 1;
+static_assert(100 * SXC_MIN_MATCHES > SXC_WIN_THRESHOLD && 100 * (SXC_MIN_MATCHES - 1) <= SXC_WIN_THRESHOLD, "");
 
 template <int NB>
 struct ScanSmem {
   static constexpr int N = 1 << NB;
   static constexpr int QCAP = N / 4 + 64;  // runs of one lag: npos / 2 + 1 with npos <= N / 2
+  static constexpr int PKW = N / 16 + 4;                // 2-bit packed words (+3 words of read-ahead)
   static constexpr int TCODE = 0;                       // uchar[N]
   static constexpr int QCODE = TCODE + N;               // uchar[N]
-  static constexpr int QUEUE = QCODE + N;               // int2[QCAP]: {lag rank, ns | ne << 16}
-  static constexpr int SUMM = QUEUE + 8 * QCAP;         // int2[SCAN_NT]
+  static constexpr int TPK = QCODE + N;                 // uint32[PKW]: 16 bases per word (pure A/C/G/T only)
+  static constexpr int QPK = TPK + 4 * PKW;             // uint32[PKW]
+  static constexpr int QUEUE = QPK + 4 * PKW;           // int3[QCAP]: {lag rank, ns | ne << 16, shift}
+  static constexpr int SUMM = QUEUE + 12 * QCAP;        // int2[SCAN_NT]
   static constexpr int EQUAL = SUMM + 8 * SCAN_NT;      // double[18][18]
   static constexpr int GC = EQUAL + 8 * SXC_NCLASS * SXC_NCLASS;  // double[18]
   static constexpr int SCORE = GC + 8 * SXC_NCLASS;     // uchar[18][18]
-  static constexpr int MISC = (SCORE + SXC_NCLASS * SXC_NCLASS + 15) & ~15;  // int[16]
+  static constexpr int CLS = SCORE + SXC_NCLASS * SXC_NCLASS;  // uchar[128]
+  static constexpr int RC = CLS + 128;                  // uchar[128]
+  static constexpr int MISC = (RC + 128 + 15) & ~15;    // int[16]
+  // pure A/C/G/T pairs: after packing, the code area holds the accepted lags (uint16[N])
+  static constexpr int CAND = TCODE;
   static constexpr int BYTES = MISC + 64;
 };
 
@@ -110,6 +121,78 @@ SXC_HD inline double code_tail(const unsigned char* tc, const unsigned char* qc,
   return exp(-expect);
 }
 
+// ---- bit-parallel window counts for pure A/C/G/T diagonals
+//
+// Bases are packed 2 bits each, 16 per 32-bit word.  32 bases of the target and of the
+// query diagonal XOR to 32 2-bit groups; a group is zero exactly where the bases match.
+// Block k of a diagonal holds the match bits of positions [32k, 32k + 32) at even bit
+// positions; the 46-wide window count at position n = 32k + r reads blocks k, k-1, k-2.
+
+SXC_HD inline uint64_t load32(const uint32_t* pk, int p) {  // 32 bases starting at base p >= 0
+  const int w = p >> 4, b = (p & 15) * 2;
+  const uint64_t lo = (uint64_t)pk[w] | ((uint64_t)pk[w + 1] << 32);
+  if (!b) return lo;
+  return (lo >> b) | ((uint64_t)pk[w + 2] << (64 - b));
+}
+
+SXC_HD inline uint64_t match_block(const uint32_t* tpk, const uint32_t* qpk, int ti, int qj, int k) {
+  if (k < 0) return 0;
+  const uint64_t x = load32(tpk, ti + 32 * k) ^ load32(qpk, qj + 32 * k);
+  return ~(x | (x >> 1)) & 0x5555555555555555ull;
+}
+
+SXC_HD inline int popc64(uint64_t v) { return __builtin_popcountll(v); }
+
+// matches in the window [n - 45, n], n = 32k + r
+SXC_HD inline int window_count(uint64_t e0, uint64_t e1, uint64_t e2, int r) {
+  const uint64_t low = (r == 31) ? ~0ull : ((1ull << (2 * r + 2)) - 1);
+  int c = popc64(e0 & low);
+  if (r >= 13) {
+    c += popc64(e1 & ~((1ull << (2 * (r - 13))) - 1));
+  } else {
+    c += popc64(e1) + popc64(e2 & ~((1ull << (2 * (r + 19))) - 1));
+  }
+  return c;
+}
+
+// Counts over a run of pure A/C/G/T bases: identical positions and C/G bases of each
+// side.  The reference's double sums of 0/1 terms equal these counts exactly.
+SXC_HD inline void packed_counts(const uint32_t* tpk, const uint32_t* qpk, int st, int sq, int len, int& match,
+                                 int& gct, int& gcq) {
+  match = gct = gcq = 0;
+  for (int off = 0; off < len; off += 32) {
+    const int m = (len - off < 32) ? len - off : 32;
+    const uint64_t keep = (m == 32) ? ~0ull : ((1ull << (2 * m)) - 1);
+    const uint64_t tv = load32(tpk, st + off), qv = load32(qpk, sq + off);
+    const uint64_t x = tv ^ qv;
+    match += popc64(~(x | (x >> 1)) & 0x5555555555555555ull & keep);
+    gct += popc64((tv ^ (tv >> 1)) & 0x5555555555555555ull & keep);  // codes 1 (C), 2 (G)
+    gcq += popc64((qv ^ (qv >> 1)) & 0x5555555555555555ull & keep);
+  }
+}
+
+// The tail after the sums (GCAdjustExpect, Sigma, CDF; analysis/AlignProbability.cc:11-40,
+// 107-126).  Returns false, without evaluating erf/exp, when z >= z_skip guarantees the
+// probability is below the keep threshold.
+SXC_HD inline bool tail_from_sums(double matches, double gct, double gcq, int len, double target_size, double z_skip,
+                                  double& prob, double& ident) {
+  ident = matches / (double)len;
+  const double gc_target = gct / (double)len;
+  const double at_target = 1. - gc_target;
+  double r = gcq * gc_target;
+  r += ((double)len - gcq) * at_target;
+  const double p = r / (double)len / 2.;
+  const double s = sqrt(p * (1. - p) * (double)len);
+  const double m = p * (double)len;
+  const double x = (double)len * ident;
+  const double z = (m - x) / s / 1.414213562;
+  if (z >= z_skip) return false;
+  const double cdf = 0.5 * (1. + erf(z));
+  const double expect = cdf * target_size;
+  prob = exp(-expect);
+  return true;
+}
+
 struct Lag {
   int shift, i0, npos;
 };
@@ -143,8 +226,9 @@ SXC_HD inline void scan_epoch(Ex& ex, char* sm, const uint16_t* cands, int c0, i
   auto push = [&](int rank, int ns, int ne) {
     const int k = ex.atomic_add_shared(&misc[SM_QN], 1);
     if (k < SS::QCAP) {
-      queue[2 * k] = rank;
-      queue[2 * k + 1] = ns | (ne << 16);
+      queue[3 * k] = rank;
+      queue[3 * k + 1] = ns | (ne << 16);
+      queue[3 * k + 2] = (int)cands[rank] - H;
     }
   };
   ex.each([&](int tid) {
@@ -156,7 +240,57 @@ SXC_HD inline void scan_epoch(Ex& ex, char* sm, const uint16_t* cands, int c0, i
       const int S = (g.npos + gsz - 1) / gsz;
       const int na = l * S;
       const int nb = (na + S < g.npos) ? na + S : g.npos;
-      if (na < nb) {
+      if (na < nb && fast) {
+        // Pure A/C/G/T: score = 100 * [match] and a window sum > 1889 <=> >= 19 matches.
+        // C(n) = C(n - 1) + m(n) - m(n - 46) with m(< 0) = 0 equals the reference's running
+        // `match` (analysis/CrossCorr.cc:700-705).  Every position of a 32-block is updated
+        // branch-free; flags become a 32-bit mask and run boundaries are its transitions.
+        const uint32_t* tpk = (const uint32_t*)(sm + SS::TPK);
+        const uint32_t* qpk = (const uint32_t*)(sm + SS::QPK);
+        const int ti = g.i0, qj = g.i0 + g.shift;
+        const int k0 = na >> 5, klast = (nb - 1) >> 5;
+        uint64_t e3 = match_block(tpk, qpk, ti, qj, k0 - 3);
+        uint64_t e2 = match_block(tpk, qpk, ti, qj, k0 - 2);
+        uint64_t e1 = match_block(tpk, qpk, ti, qj, k0 - 1);
+        int C = window_count(e1, e2, e3, 31);  // C at position 32 k0 - 1
+        bool prev = (na - 1 >= 46) && (na & 31) == 0 && C >= SXC_MIN_MATCHES;
+        for (int k = k0; k <= klast; ++k) {
+          const uint64_t e0 = match_block(tpk, qpk, ti, qj, k);
+          // m(n - 46) for the block: offsets r < 14 come from block k-2, r >= 14 from block k-1
+          const uint64_t minus = (e2 >> 36) | (e1 << 28);
+          uint32_t f = 0;
+          static_for<0, 32>([&](auto R) {
+            constexpr int r = decltype(R)::value;
+            C += (int)((e0 >> (2 * r)) & 1) - (int)((minus >> (2 * r)) & 1);
+            f |= (C >= SXC_MIN_MATCHES ? 1u : 0u) << r;
+          });
+          if (32 * k < 46) f &= (46 - 32 * k >= 32) ? 0u : ~((1u << (46 - 32 * k)) - 1);  // flags from n = 46
+          const int r0 = (k == k0) ? na - 32 * k : 0;
+          const int r1 = (k == klast) ? nb - 32 * k : 32;
+          if (k == k0 && r0 > 0) prev = ((f >> (r0 - 1)) & 1) != 0;
+          const uint32_t range = (r1 == 32 ? ~0u : ((1u << r1) - 1)) & ~((1u << r0) - 1);
+          f &= range;
+          uint32_t trans = (f ^ ((f << 1) | ((prev ? 1u : 0u) << r0))) & range;
+          while (trans) {
+            const int r = __builtin_ctz(trans);
+            trans &= trans - 1;
+            const int n = 32 * k + r;
+            if ((f >> r) & 1) {
+              open = n;
+            } else if (open >= 0) {
+              push(c, open, n);
+              open = -1;
+            } else if (first_end == INF) {
+              first_end = n;
+            }
+          }
+          prev = ((f >> (r1 - 1)) & 1) != 0;
+          e3 = e2;
+          e2 = e1;
+          e1 = e0;
+        }
+        (void)e3;
+      } else if (na < nb) {
         const unsigned char* tp = tc + g.i0;
         const unsigned char* qp = qc + g.i0 + g.shift;
         auto sc = [&](int n) -> int {
@@ -207,7 +341,7 @@ SXC_HD inline void scan_epoch(Ex& ex, char* sm, const uint16_t* cands, int c0, i
 
 // Drains queue entries [0, n): mode 0 filters into t_results, mode 1 writes runs.
 template <int NB, class Ex>
-SXC_HD inline void drain(Ex& ex, char* sm, const ScanArgs& a, const uint16_t* cands, int ps, int n) {
+SXC_HD inline void drain(Ex& ex, char* sm, const ScanArgs& a, const sxc_pair_dev& pr, int ps, int n) {
   using SS = ScanSmem<NB>;
   constexpr int H = (1 << NB) / 2;
   const unsigned char* tc = (const unsigned char*)(sm + SS::TCODE);
@@ -216,16 +350,17 @@ SXC_HD inline void drain(Ex& ex, char* sm, const ScanArgs& a, const uint16_t* ca
   const double* gcv = (const double*)(sm + SS::GC);
   int* misc = (int*)(sm + SS::MISC);
   const int* queue = (const int*)(sm + SS::QUEUE);
-  const int Lt = misc[SM_LT], Lq = misc[SM_LQ];
+  const bool fast = misc[SM_FAST] != 0;
   const int strand = ps % a.strands;
   ex.each([&](int tid) {
     for (int k = tid; k < n; k += SCAN_NT) {
-      const int rank = queue[2 * k];
-      const int ns = queue[2 * k + 1] & 0xffff, ne = (queue[2 * k + 1] >> 16) & 0xffff;
-      const Lag g = lag_geometry(cands[rank], H, Lq, Lt);
-      const int start_t = g.i0 + ns - SXC_MIN_LEN;
+      const int rank = queue[3 * k];
+      const int ns = queue[3 * k + 1] & 0xffff, ne = (queue[3 * k + 1] >> 16) & 0xffff;
+      const int shift = queue[3 * k + 2];
+      const int i0 = shift < 0 ? -shift : 0;
+      const int start_t = i0 + ns - SXC_MIN_LEN;
       const int len = ne - ns + SXC_MIN_LEN;
-      const int start_q = start_t + g.shift;
+      const int start_q = start_t + shift;
       const uint64_t key = ((uint64_t)(uint32_t)rank << 32) | (uint32_t)ns;
       if (a.mode == 1) {
         const int slot = ex.atomic_add_shared(&misc[SM_KEPT], 1);
@@ -238,12 +373,25 @@ SXC_HD inline void drain(Ex& ex, char* sm, const ScanArgs& a, const uint16_t* ca
         continue;
       }
       if (len < a.min_len) continue;  // FilterMatches: GetLength() < m_minLen
-      double ident;
-      const double prob = code_tail(tc, qc, equal, gcv, start_t, start_q, len, a.target_total, ident);
+      double ident, prob = 0.;
+      if (fast) {
+        int cm, ct, cq;
+        packed_counts((const uint32_t*)(sm + SS::TPK), (const uint32_t*)(sm + SS::QPK), start_t, start_q, len, cm,
+                      ct, cq);
+        if (!tail_from_sums((double)cm, (double)ct, (double)cq, len, a.target_total, a.z_skip, prob, ident)) continue;
+      } else {
+        double matches = 0, gct = 0, gcq = 0;
+        for (int i = 0; i < len; ++i) {
+          const int ca = code_class(tc[start_t + i]), cb = code_class(qc[start_q + i]);
+          matches += equal[ca * SXC_NCLASS + cb];
+          gct += gcv[ca];
+          gcq += gcv[cb];
+        }
+        if (!tail_from_sums(matches, gct, gcq, len, a.target_total, a.z_skip, prob, ident)) continue;
+      }
       if (prob < a.min_prob) continue;
       const int slot = ex.atomic_add_shared(&misc[SM_KEPT], 1);
       if (slot >= a.cap) continue;
-      const sxc_pair_dev pr = a.pairs[ps / a.strands];
       const int tstart = pr.t_chunk_start + start_t;
       // RCQuery (analysis/HomologyByXCorrSlave.cc:56-60) on the reverse strand
       const int qstart =
@@ -284,17 +432,27 @@ SXC_HD inline void scan_body(Ex& ex, char* sm, const ScanArgs& a, int ps) {
   int* misc = (int*)(sm + SS::MISC);
   const unsigned char* ts = a.seq + tch.off;
   const unsigned char* qs = a.seq + qch.off;
-  const uint16_t* cands = a.cands + (size_t)ps * N;
+  const uint16_t* gcands = a.cands + (size_t)ps * N;
+  unsigned char* cls = (unsigned char*)(sm + SS::CLS);
+  unsigned char* rc = (unsigned char*)(sm + SS::RC);
+  const bool fast = (tch.flags & 2) && (qch.flags & 2);  // both pure A/C/G/T
+  // stage the chunk bytes (16-byte aligned in HBM) and the tables in LDS
   ex.each([&](int tid) {
-    for (int i = tid; i < tch.len; i += SCAN_NT) tc[i] = base_code(tab, ts[i]);
-    // DNAVector::ReverseComplement for strand 1 (analysis/DNAVector.cc:482-521)
-    for (int i = tid; i < qch.len; i += SCAN_NT)
-      qc[i] = base_code(tab, strand ? tab->rc[qs[qch.len - 1 - i] & 127] : qs[i]);
+    const uint32_t* ts4 = (const uint32_t*)ts;
+    const uint32_t* qs4 = (const uint32_t*)qs;
+    uint32_t* tc4 = (uint32_t*)tc;
+    uint32_t* qc4 = (uint32_t*)qc;
+    for (int i = tid; i < (tch.len + 3) / 4; i += SCAN_NT) tc4[i] = ts4[i];
+    for (int i = tid; i < (qch.len + 3) / 4; i += SCAN_NT) qc4[i] = qs4[i];
     for (int i = tid; i < SXC_NCLASS * SXC_NCLASS; i += SCAN_NT) {
       equal[i] = (&tab->equal[0][0])[i];
       score[i] = (&tab->score[0][0])[i];
     }
     for (int i = tid; i < SXC_NCLASS; i += SCAN_NT) gcv[i] = tab->gc[i];
+    for (int i = tid; i < 128; i += SCAN_NT) {
+      cls[i] = tab->cls[i];
+      rc[i] = tab->rc[i];
+    }
     if (tid == 0) {
       misc[SM_QN] = 0;
       misc[SM_KEPT] = 0;
@@ -302,10 +460,66 @@ SXC_HD inline void scan_body(Ex& ex, char* sm, const ScanArgs& a, int ps) {
       misc[SM_NCAND] = a.ncand[ps];
       misc[SM_LT] = tch.len;
       misc[SM_LQ] = qch.len;
-      misc[SM_FAST] = ((tch.flags & 2) && (qch.flags & 2)) ? 1 : 0;  // both pure A/C/G/T
+      misc[SM_FAST] = fast ? 1 : 0;
     }
   });
   ex.sync();
+  const int Lt = tch.len, Lq = qch.len;
+  if (fast) {
+    // 2-bit codes A=0, C=1, G=2, T=3 from the ASCII letters; complement = code ^ 3
+    uint32_t* tpk = (uint32_t*)(sm + SS::TPK);
+    uint32_t* qpk = (uint32_t*)(sm + SS::QPK);
+    ex.each([&](int tid) {
+      for (int w = tid; w < SS::PKW; w += SCAN_NT) {
+        uint32_t vt = 0, vq = 0;
+        for (int b = 0; b < 16; ++b) {
+          const int i = 16 * w + b;
+          if (i < Lt) {
+            const unsigned x = tc[i];
+            vt |= (((x >> 1) ^ (x >> 2)) & 3u) << (2 * b);
+          }
+          if (i < Lq) {
+            // DNAVector::ReverseComplement for strand 1 (analysis/DNAVector.cc:482-521)
+            const unsigned x = strand ? qc[Lq - 1 - i] : qc[i];
+            vq |= ((((x >> 1) ^ (x >> 2)) & 3u) ^ (strand ? 3u : 0u)) << (2 * b);
+          }
+        }
+        tpk[w] = vt;
+        qpk[w] = vq;
+      }
+    });
+    ex.sync();
+    uint16_t* lc = (uint16_t*)(sm + SS::CAND);
+    const int nc = misc[SM_NCAND];
+    ex.each([&](int tid) {
+      for (int i = tid; i < nc; i += SCAN_NT) lc[i] = gcands[i];
+    });
+    ex.sync();
+  } else {
+    // base codes in place; the reverse complement swaps symmetric positions
+    ex.each([&](int tid) {
+      for (int i = tid; i < Lt; i += SCAN_NT) {
+        const unsigned char c = cls[tc[i] & 127];
+        tc[i] = c < SXC_CLASS_ZERO ? c : (unsigned char)(128 | tc[i]);
+      }
+      if (strand) {
+        for (int i = tid; i < (Lq + 1) / 2; i += SCAN_NT) {
+          const unsigned char x = qc[i], y = qc[Lq - 1 - i];
+          const unsigned char rx = rc[x & 127], ry = rc[y & 127];
+          const unsigned char cx = cls[rx & 127], cy = cls[ry & 127];
+          qc[i] = cy < SXC_CLASS_ZERO ? cy : (unsigned char)(128 | ry);
+          qc[Lq - 1 - i] = cx < SXC_CLASS_ZERO ? cx : (unsigned char)(128 | rx);
+        }
+      } else {
+        for (int i = tid; i < Lq; i += SCAN_NT) {
+          const unsigned char c = cls[qc[i] & 127];
+          qc[i] = c < SXC_CLASS_ZERO ? c : (unsigned char)(128 | qc[i]);
+        }
+      }
+    });
+    ex.sync();
+  }
+  const uint16_t* cands = fast ? (const uint16_t*)(sm + SS::CAND) : gcands;
   const int ncand = misc[SM_NCAND];
   constexpr int CE = SCAN_NT / SCAN_GSZ;
   int runs = 0;
@@ -316,7 +530,7 @@ SXC_HD inline void scan_body(Ex& ex, char* sm, const ScanArgs& a, int ps) {
     const int qn = misc[SM_QN];
     if (qn > SS::QCAP) {
       // the epoch overflowed the queue: keep what came before it, redo its lags one by one
-      drain<NB>(ex, sm, a, cands, ps, qstart);
+      drain<NB>(ex, sm, a, pr, ps, qstart);
       runs += qstart;
       for (int c = e0; c < e1; ++c) {
         ex.each([&](int tid) {
@@ -325,7 +539,7 @@ SXC_HD inline void scan_body(Ex& ex, char* sm, const ScanArgs& a, int ps) {
         ex.sync();
         scan_epoch<NB>(ex, sm, cands, c, c + 1, SCAN_NT);
         const int n1 = misc[SM_QN];
-        drain<NB>(ex, sm, a, cands, ps, n1);
+        drain<NB>(ex, sm, a, pr, ps, n1);
         runs += n1;
       }
       ex.each([&](int tid) {
@@ -333,7 +547,7 @@ SXC_HD inline void scan_body(Ex& ex, char* sm, const ScanArgs& a, int ps) {
       });
       ex.sync();
     } else if (qn > SS::QCAP / 2 || e1 == ncand) {
-      drain<NB>(ex, sm, a, cands, ps, qn);
+      drain<NB>(ex, sm, a, pr, ps, qn);
       runs += qn;
       ex.each([&](int tid) {
         if (tid == 0) misc[SM_QN] = 0;

@@ -128,4 +128,18 @@ void build_ent_lut(int nb, double* out) {
     }
 }
 
+double keep_z_bound(double min_prob, double target_size) {
+  // keep <=> exp(-0.5 * (1 + erf(z)) * T) >= min_prob <=> cdf(z) <= -ln(min_prob) / T.
+  // Return z_skip with cdf(z_skip) >= 4 * that bound: every z >= z_skip is dropped.
+  if (!(min_prob > 0.) || !(target_size > 0.)) return HUGE_VAL;
+  const double cmax = (-log(min_prob) + 1e-15) / target_size;
+  if (4. * cmax >= 1.) return HUGE_VAL;
+  double lo = -40., hi = 40.;  // cdf(lo) < 4 cmax <= cdf(hi)
+  for (int it = 0; it < 200; ++it) {
+    const double mid = 0.5 * (lo + hi);
+    if (0.5 * (1. + erf(mid)) >= 4. * cmax) hi = mid; else lo = mid;
+  }
+  return hi;
+}
+
 }  // namespace sxc

@@ -10,5 +10,7 @@ int ent_lut_len(int nb);     // doubles in the entropy table for N = 2^nb
 void build_ent_lut(int nb, double* out);
 int class_of(int byte);
 bool class_has_thirds(int c);
+// z above which a run's probability is certainly below min_prob (see sxc_scan.h tail).
+double keep_z_bound(double min_prob, double target_size);
 
 }  // namespace sxc

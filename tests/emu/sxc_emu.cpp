@@ -114,6 +114,7 @@ int run_batch(const sxc::HostBatch& hb, const sxc_params& prm, std::vector<sxc_r
   sc.q_chunk = prm.q_chunk;
   sc.min_prob = prm.min_prob;
   sc.target_total = prm.target_total;
+  sc.z_skip = sxc::keep_z_bound(prm.min_prob, prm.target_total);
   sc.mode = 0;
   sc.out = slots.data();
   sc.keys = keys.data();
