@@ -55,7 +55,9 @@ typedef struct {
 /* Slave parameters (analysis/HomologyByXCorrSlave.cc:334-384 flags). */
 typedef struct {
   int32_t N;            /* FFT length = 2 * t_chunk */
-  int32_t q_chunk;      /* -q_chunk (RCQuery) */
+  int32_t q_chunk;      /* RCQuery chunk size: -q_chunk on the slave path; 0 = each query chunk's own
+                           length, as the standalone aligner computes it
+                           (tools/analysis/HomologyByXCorr.cc:226-228, 793-798) */
   int32_t min_len;      /* -l */
   int32_t strands;      /* 2 = forward then reverse complement (Align), 1 = forward only */
   double cutoff;        /* -cutoff */

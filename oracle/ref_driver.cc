@@ -17,6 +17,9 @@
 //   pairs   the slave path: FASTA load, chunking, align_target per t_pair,
 //           FilterMatches -> t_result list (analysis/HomologyByXCorrSlave.cc:
 //           270-300, 168-219, 386-409), optionally on T worker threads.
+//   chunks  vecDNAVector::Read + ChunkManager::ChunkIt on FASTA files, printed
+//           record by record and chunk by chunk (analysis/DNAVector.cc:1176-1293,
+//           analysis/SeqChunk.cc:72-165)
 //
 // The reference's main() is renamed so this driver can provide its own, and
 // SeqAnalyzer::FindTop (private in analysis/CrossCorr.h:275) is opened up so
@@ -266,6 +269,28 @@ int mode_pairs(int argc, char** argv) {
   return 0;
 }
 
+// chunks FILES SIZE OVERLAP: "R index length name" per record, "C record start len hex"
+// per chunk, "T total" (sum of ChunkManager::GetSize).
+int mode_chunks(int argc, char** argv) {
+  if (argc < 5) die("usage: chunks files size overlap");
+  vecDNAVector raw, out;
+  std::vector<string> names;
+  raw.Read(argv[2], names);
+  for (int i = 0; i < (int)raw.size(); ++i) printf("R %d %d %s\n", i, raw[i].size(), names[i].c_str());
+  ChunkManager cm(atoi(argv[3]), atoi(argv[4]));
+  std::vector<SeqChunk> info;
+  cm.ChunkIt(out, info, raw, names, 0, 0);
+  for (int k = 0; k < (int)out.size(); ++k) {
+    printf("C %d %d %d ", info[k].GetID(), info[k].GetStart(), out[k].size());
+    for (int i = 0; i < out[k].size(); ++i) printf("%02x", (unsigned char)out[k][i]);
+    printf("\n");
+  }
+  double total = 0;
+  for (int i = 0; i < cm.GetCount(); i++) total += (double)cm.GetSize(i);
+  printf("T %.1f\n", total);
+  return 0;
+}
+
 // layout probe: prints sizeof/offsetof of the wire structs (analysis/WorkQueue.h:17-33)
 int mode_layout() {
   printf("t_pair %zu tF %zu tT %zu qF %zu qT %zu fast %zu sid %zu status %zu\n", sizeof(t_pair),
@@ -289,6 +314,7 @@ int main(int argc, char** argv) {
   if (m == "detail") return mode_detail(argc, argv);
   if (m == "pairs") return mode_pairs(argc, argv);
   if (m == "layout") return mode_layout();
+  if (m == "chunks") return mode_chunks(argc, argv);
   die("unknown mode");
   return 2;
 }

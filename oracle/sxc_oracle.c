@@ -716,7 +716,8 @@ static int32_t align_pair_ws(align_ws* ws, const sxo_pair* p, const sxo_params* 
       int32_t len = m->len;
       int32_t tstart = p->t_chunk_start + m->start_target;
       int32_t qstart = strand == 0 ? p->q_chunk_start + m->start_query
-                                   : rc_query(p->q_chunk_start, m->start_query, prm->q_chunk, p->q_record_len);
+                                   : rc_query(p->q_chunk_start, m->start_query,
+                                              prm->q_chunk > 0 ? prm->q_chunk : p->qlen, p->q_record_len);
       double prob = sxo_match_probability(p->tseq, qs, m->start_target, m->start_query, len, prm->target_total);
       if (prob < prm->min_prob) continue;
       double ident = sxo_print_match_ident(qs, p->tseq, m->start_target, m->start_query, len);

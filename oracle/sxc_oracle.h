@@ -55,7 +55,7 @@ typedef struct {
 
 typedef struct {
   int32_t N;            /* FFT length (2 * t_chunk in the slave) */
-  int32_t q_chunk;      /* -q_chunk, used by RCQuery */
+  int32_t q_chunk;      /* -q_chunk, used by RCQuery; 0 = the query chunk's own length */
   int32_t min_len;      /* -l */
   int32_t strands;      /* 1 = forward only, 2 = forward then reverse complement */
   double cutoff;        /* -cutoff */

@@ -25,6 +25,12 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall", "-Wno-unus
           "-Wno-unknown-pragmas"]
 DEVICE = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 SOURCES = ["sxc_kernels.hip", "sxc_api.hip", "sxc_tables.cpp"]
+# Host programs linked against the library (plain C++, no device code).
+PROGRAMS = {
+    "HomologyByXCorrSlave": ["host/slave_main.cpp", "host/sxc_seqio.cpp"],
+    "HomologyByXCorr": ["host/xcorr_main.cpp", "host/sxc_seqio.cpp"],
+}
+CXX = os.environ.get("CXX", "g++")
 
 
 def _compile(src: str) -> str:
@@ -55,6 +61,16 @@ def build(verbose: bool = True) -> str:
     os.replace(tmp, LIB)
     if verbose:
         print(f"built {LIB}")
+    for name, srcs in PROGRAMS.items():
+        exe = os.path.join(LIBDIR, name)
+        cmd = [CXX, "-O2", "-std=c++17", "-Wall", "-pthread", "-o", exe + ".tmp",
+               *[os.path.join(CSRC, s) for s in srcs], f"-L{LIBDIR}", "-lsxc_hip", "-Wl,-rpath,$ORIGIN"]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+        os.replace(exe + ".tmp", exe)
+        if verbose:
+            print(f"built {exe}")
     return LIB
 
 

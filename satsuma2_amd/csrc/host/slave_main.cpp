@@ -1,0 +1,347 @@
+// HomologyByXCorrSlave -- the Satsuma2 worker process with its chunk-pair alignment on the GPU.
+//
+// A drop-in for analysis/HomologyByXCorrSlave.cc: same flags (:334-384), same input
+// loading and chunking (:386-409), same master exchange (:440-523) and the same t_result
+// records per block (align_target / Align / FilterMatches, :168-300).  What changes is
+// how the work runs: the reference gives each of its -p threads one t_pair block at a
+// time; here one submitter thread drains every queued block into a single batch of
+// chunk pairs and runs it through libsxc_hip.so, so the GPU sees thousands of chunk
+// pairs per launch.  The master exchange stays as it is: ask for blocks while fewer than
+// 2 * p are queued, and hand back every finished result on the next connection.
+//
+// Results of a batch are appended block by block in arrival order, each block in the
+// reference's order (query chunks outer, target chunks inner, forward strand then
+// reverse), i.e. the order the reference slave produces with -p 1.
+#include <arpa/inet.h>
+#include <errno.h>
+#include <netdb.h>
+#include <netinet/in.h>
+#include <stdio.h>
+#include <string.h>
+#include <sys/socket.h>
+#include <time.h>
+#include <unistd.h>
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../../../include/sxc.h"
+#include "sxc_flags.h"
+#include "sxc_seqio.h"
+#include "sxc_wire.h"
+
+using sxc_host::ChunkSet;
+using sxc_host::Records;
+using sxc_host::WirePair;
+
+namespace {
+
+// Chunk pairs per GPU launch: large enough to fill the device (256 CUs, one workgroup
+// per pair and strand), small enough to keep the batch's device buffers modest.
+constexpr int kMaxPairsPerLaunch = 16384;
+
+struct State {
+  std::mutex mu;
+  std::condition_variable work_cv;   // blocks were queued or the run is over
+  std::condition_variable space_cv;  // the submitter took blocks off the queue
+  std::deque<WirePair> blocks;
+  std::vector<sxc_result> results;
+  std::atomic<bool> finished{false};
+};
+
+bool write_all(int fd, const void* p, size_t n) {
+  const char* c = (const char*)p;
+  while (n > 0) {
+    const ssize_t w = write(fd, c, n);
+    if (w < 0 && errno == EINTR) continue;
+    if (w <= 0) return false;
+    c += w;
+    n -= (size_t)w;
+  }
+  return true;
+}
+
+bool read_all(int fd, void* p, size_t n) {
+  char* c = (char*)p;
+  while (n > 0) {
+    const ssize_t r = read(fd, c, n);
+    if (r < 0 && errno == EINTR) continue;
+    if (r <= 0) return false;
+    c += r;
+    n -= (size_t)r;
+  }
+  return true;
+}
+
+struct Job {
+  const ChunkSet* query;
+  const ChunkSet* target;
+  sxc_params prm;
+  sxc_ctx* ctx;
+};
+
+// Appends the chunk pairs of one block (align_target, :270-300) in reference order.
+bool append_block(const Job& job, const WirePair& p, std::vector<sxc_pair>* out) {
+  const int nt = (int)job.target->chunks.size(), nq = (int)job.query->chunks.size();
+  if (p.target_from < 0 || p.target_to >= nt || p.query_from < 0 || p.query_to >= nq ||
+      p.target_from > p.target_to || p.query_from > p.query_to) {
+    printf("ERROR block t:%d-%d q:%d-%d is outside the %d target / %d query chunks; skipped\n",
+           p.target_from, p.target_to, p.query_from, p.query_to, nt, nq);
+    return false;
+  }
+  for (int qi = p.query_from; qi <= p.query_to; ++qi) {
+    const sxc_host::Chunk& q = job.query->chunks[(size_t)qi];
+    for (int ti = p.target_from; ti <= p.target_to; ++ti) {
+      const sxc_host::Chunk& t = job.target->chunks[(size_t)ti];
+      sxc_pair s;
+      s.tseq = t.data;
+      s.tlen = t.len;
+      s.qseq = q.data;
+      s.qlen = q.len;
+      s.t_chunk_start = t.start;
+      s.q_chunk_start = q.start;
+      s.t_record = t.record;
+      s.q_record = q.record;
+      s.q_record_len = job.query->lengths[(size_t)q.record];
+      s.fast = p.fast ? 1 : 0;
+      out->push_back(s);
+    }
+  }
+  return true;
+}
+
+// Runs chunk pairs through the library in launches of at most kMaxPairsPerLaunch pairs
+// and appends their records to *out.
+bool align_pairs(const Job& job, const std::vector<sxc_pair>& pairs, std::vector<sxc_result>* out) {
+  for (size_t at = 0; at < pairs.size(); at += kMaxPairsPerLaunch) {
+    const int n = (int)std::min(pairs.size() - at, (size_t)kMaxPairsPerLaunch);
+    sxc_batch* b = nullptr;
+    int rc = sxc_batch_create(job.ctx, pairs.data() + at, n, &job.prm, &b);
+    if (rc == SXC_OK) rc = sxc_batch_run(b);
+    std::vector<sxc_result> r;
+    int64_t got = 0;
+    if (rc == SXC_OK) {
+      r.resize((size_t)n * 8 + 64);
+      rc = sxc_batch_results(b, r.data(), (int64_t)r.size(), &got);
+      if (rc == SXC_ERR_CAPACITY) {
+        r.resize((size_t)got);
+        rc = sxc_batch_results(b, r.data(), (int64_t)r.size(), &got);
+      }
+    }
+    sxc_batch_destroy(b);
+    if (rc != SXC_OK) {
+      printf("ERROR aligning %d chunk pairs: %s\n", n, sxc_last_error());
+      return false;
+    }
+    out->insert(out->end(), r.begin(), r.begin() + got);
+  }
+  return true;
+}
+
+void submitter(const Job& job, State* st) {
+  for (;;) {
+    std::vector<WirePair> take;
+    {
+      std::unique_lock<std::mutex> lk(st->mu);
+      st->work_cv.wait(lk, [&] { return st->finished.load() || !st->blocks.empty(); });
+      if (st->finished.load()) return;
+      take.assign(st->blocks.begin(), st->blocks.end());
+      st->blocks.clear();
+    }
+    st->space_cv.notify_all();
+    std::vector<sxc_pair> pairs;
+    for (const WirePair& p : take) append_block(job, p, &pairs);
+    std::vector<sxc_result> res;
+    if (!align_pairs(job, pairs, &res)) {
+      fflush(stdout);
+      _exit(1);  // a device failure leaves the master's blocks unanswered; fail loudly
+    }
+    std::lock_guard<std::mutex> g(st->mu);
+    st->results.insert(st->results.end(), res.begin(), res.end());
+  }
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+  sxc_host::Flags P(argc, argv, "Compares two sequences via cross-correlation");
+  P.add("-master", "string", "name of the submit host", nullptr);
+  P.add("-port", "int", "port of the master to connect", "3491");
+  P.add("-p", "int", "number of working processes (threads)", "1");
+  P.add("-sid", "int", "slave_id", nullptr);
+  P.add("-q", "string", "query fasta sequence", nullptr);
+  P.add("-t", "string", "target fasta sequence", nullptr);
+  P.add("-l", "int", "minimum alignment length", "0");
+  P.add("-q_chunk", "int", "query chunk size", "4096");
+  P.add("-t_chunk", "int", "target chunk size", "4096");
+  P.add("-min_prob", "double", "minimum probability to keep match", "0.9999");
+  P.add("-cutoff", "double", "signal selection cutoff", "1.8");
+  P.add("-cutoff_fast", "double", "signal selection cutoff (fast)", "2.9");
+  P.add("-prob_table", "bool", "lookup probability in table (faster, less accurate)", "0");
+  P.add("-debug_targets", "int",
+        "number of targets to receive before exiting (for debug purposes mostly, don't use)", "0");
+  P.add("-device", "int", "HIP device to run on", "0");
+  P.parse();
+
+  const std::string master = P.str("-master");
+  const int port = P.i32("-port");
+  int32_t slave_id = P.i32("-sid");
+  const int threads = P.i32("-p") > 0 ? P.i32("-p") : 1;
+  const std::string qfile = P.str("-q"), tfile = P.str("-t");
+  const int min_len = P.i32("-l");
+  const int q_chunk = P.i32("-q_chunk"), t_chunk = P.i32("-t_chunk");
+  const double cutoff = P.f64("-cutoff"), cutoff_fast = P.f64("-cutoff_fast");
+  const bool prob_table = P.boolean("-prob_table");
+  const unsigned debug_max = (unsigned)P.i32("-debug_targets");
+  const int device = P.i32("-device");
+  (void)P.f64("-min_prob");  // read but unused: FilterMatches keeps prob >= 0.99 (:76)
+  if (prob_table) {
+    printf("ERROR -prob_table is not supported by this build\n");
+    return 1;
+  }
+
+  time_t t0 = time(nullptr);
+  Records qraw, traw;
+  std::string err;
+  printf("Loading query sequence:  %s\n", qfile.c_str());
+  if (!sxc_host::read_fasta(qfile, &qraw, &err)) {
+    printf("ERROR %s\n", err.c_str());
+    return 1;
+  }
+  ChunkSet query, target;
+  sxc_host::chunk_records(qraw, q_chunk, 0, &query);
+  printf("Loading target sequence:  %s\n", tfile.c_str());
+  if (!sxc_host::read_fasta(tfile, &traw, &err)) {
+    printf("ERROR %s\n", err.c_str());
+    return 1;
+  }
+  sxc_host::chunk_records(traw, t_chunk, t_chunk / 4, &target);
+  printf("query chunks: %zu  target chunks: %zu\n", query.chunks.size(), target.chunks.size());
+  printf("TIME SPENT ON LOADING: %ld\n", (long)(time(nullptr) - t0));
+
+  Job job;
+  job.query = &query;
+  job.target = &target;
+  job.prm.N = 2 * t_chunk;
+  job.prm.q_chunk = q_chunk;
+  job.prm.min_len = min_len;
+  job.prm.strands = 2;
+  job.prm.cutoff = cutoff;
+  job.prm.cutoff_fast = cutoff_fast;
+  job.prm.min_prob = 0.99;
+  job.prm.target_total = target.total();
+  if (sxc_create(&job.ctx, device) != SXC_OK) {
+    printf("ERROR %s\n", sxc_last_error());
+    return 1;
+  }
+
+  State st;
+  std::thread worker(submitter, std::cref(job), &st);
+  printf("== Entering communication loop ==\ncomm loop for %s %d\n", master.c_str(), port);
+  fflush(stdout);
+
+  struct hostent* server = gethostbyname(master.c_str());
+  if (!server) {
+    printf("ERROR resolving master hostname\n");
+    st.finished = true;
+    st.work_cv.notify_all();
+    worker.join();
+    sxc_destroy(job.ctx);
+    return -1;
+  }
+  sockaddr_in addr;
+  memset(&addr, 0, sizeof addr);
+  addr.sin_family = AF_INET;
+  memcpy(&addr.sin_addr.s_addr, server->h_addr, (size_t)server->h_length);
+  addr.sin_port = htons((uint16_t)port);
+
+  unsigned total_blocks = 0;
+  while (!st.finished.load()) {
+    {
+      std::unique_lock<std::mutex> lk(st.mu);
+      if (st.blocks.size() >= (size_t)(2 * threads)) {
+        st.space_cv.wait_for(lk, std::chrono::seconds(10));
+        continue;
+      }
+    }
+    int fd = -1;
+    while (fd < 0) {
+      fd = socket(AF_INET, SOCK_STREAM, 0);
+      if (fd < 0) {
+        printf("ERROR opening socket\n");
+        sleep(1);
+        continue;
+      }
+      if (connect(fd, (sockaddr*)&addr, sizeof addr) < 0) {
+        printf("ERROR connecting to master: %s\n", strerror(errno));
+        close(fd);
+        fd = -1;
+        sleep(1);
+        continue;
+      }
+      if (!write_all(fd, &slave_id, sizeof slave_id)) {
+        printf("ERROR sending id to master\n");
+        close(fd);
+        fd = -1;
+        sleep(1);
+      }
+    }
+    std::vector<sxc_result> out;
+    {
+      std::lock_guard<std::mutex> g(st.mu);
+      out.swap(st.results);
+    }
+    const uint32_t count = (uint32_t)out.size();
+    bool ok = write_all(fd, &count, sizeof count) && write_all(fd, out.data(), out.size() * sizeof(sxc_result));
+    int32_t m = 0;
+    ok = ok && read_all(fd, &m, sizeof m);
+    if (!ok) {
+      // The master closed early: keep the results for the next exchange.
+      std::lock_guard<std::mutex> g(st.mu);
+      st.results.insert(st.results.begin(), out.begin(), out.end());
+      close(fd);
+      sleep(1);
+      continue;
+    }
+    if (m == -1) {
+      close(fd);
+      std::lock_guard<std::mutex> g(st.mu);
+      st.finished = true;
+      break;
+    }
+    if (m > 0) {
+      std::vector<WirePair> tv((size_t)m);
+      if (!read_all(fd, tv.data(), tv.size() * sizeof(WirePair))) {
+        printf("ERROR reading %d blocks from master\n", m);
+        close(fd);
+        continue;
+      }
+      close(fd);
+      {
+        std::lock_guard<std::mutex> g(st.mu);
+        for (const WirePair& p : tv) st.blocks.push_back(p);
+      }
+      st.work_cv.notify_all();
+      total_blocks += (unsigned)m;
+      if (debug_max && total_blocks >= debug_max) st.finished = true;
+    } else {
+      close(fd);
+      // Nothing to do: poll the master again shortly.
+      std::this_thread::sleep_for(std::chrono::milliseconds(50));
+    }
+  }
+  st.work_cv.notify_all();
+  printf("== Processing finished, waiting for the slaves to die ==\n");
+  printf("TIME SPENT WORKING: %ld\n", (long)(time(nullptr) - t0));
+  fflush(stdout);
+  worker.join();
+  sxc_destroy(job.ctx);
+  return 0;
+}

@@ -264,6 +264,7 @@ int batch_download(sxc_ctx* ctx, sxc_batch_dev& b, sxc_result* out, int64_t cap,
 int batch_prepare(sxc_batch_dev& b, const sxc_pair* pairs, int32_t n, const sxc_params* prm) {
   if (!prm || n < 0 || (n > 0 && !pairs)) return fail(SXC_ERR_ARG, "null argument");
   if (prm->strands != 1 && prm->strands != 2) return fail(SXC_ERR_ARG, "strands must be 1 or 2");
+  if (prm->q_chunk < 0) return fail(SXC_ERR_ARG, "q_chunk must be >= 0");
   int nb;
   int rc = check_N(prm->N, &nb);
   if (rc) return rc;

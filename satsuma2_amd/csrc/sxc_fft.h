@@ -7,9 +7,9 @@
 // the same ops to the same operands gives the same bits.  This file regroups the
 // butterflies so one thread owns a closed set of positions for several levels:
 //
-//   stage A : levels pass1+2, pass3, pass_3, pass_4 on 32 contiguous positions
-//             (the bit-reversed gather of pass 1+2 is done while loading);
-//   rounds  : levels pass_m .. pass_{m+K-1} (K <= 3) on "units" of 2^(K+2) positions.
+//   stage A : levels pass1+2, pass3, pass_3 (and pass_4 when SA = 32) on SA contiguous
+//             positions (the bit-reversed gather of pass 1+2 is done while loading);
+//   rounds  : levels pass_m .. pass_{m+K-1} (K <= KMAX) on "units" of 2^(K+2) positions.
 //
 // A unit at level m is butterfly index j and h_m - j (h_m = 2^(m-1)) in 2^(K-1)
 // neighbouring blocks; level m+1 butterflies j' and 2h_m - j' read exactly what level-m
@@ -50,13 +50,24 @@ SXC_HD inline void static_for(F&& f) {
 // LDS index of FFT position p: one pad word per 32 floats keeps the 32-float rows of
 // stage A on distinct banks.
 SXC_HD constexpr int pad(int p) { return p + (p >> 5); }
+// Stage-A group size (positions per thread in the first levels) and the largest number
+// of levels one LDS round carries.
+#ifndef SXC_STAGE_A
+#define SXC_STAGE_A 16
+#endif
+#ifndef SXC_KMAX
+#define SXC_KMAX 2
+#endif
+
 template <int NB>
 struct Geo {
   static constexpr int N = 1 << NB;
   static constexpr int H = N / 2;
-  static constexpr int NT = (N / 32 < 64) ? 64 : N / 32;  // threads per workgroup
+  static constexpr int SA = SXC_STAGE_A;                   // 16 or 32
+  static constexpr int LSA = SA == 32 ? 5 : 4;             // log2(SA)
+  static constexpr int NT = (N / SA < 64) ? 64 : N / SA;   // threads per workgroup
   static constexpr int YLEN = N + N / 32;                  // padded LDS floats
-  static constexpr int GROUPS = N / 32;                    // stage-A units
+  static constexpr int GROUPS = N / SA;                    // stage-A units
 };
 
 SXC_HD inline unsigned bitrev(unsigned x, int bits) {
@@ -144,14 +155,17 @@ SXC_HD inline void level_contig(float* v, const Twiddle& tw) {
 // Time position read by slot r of stage-A group g (pass 1+2 reads x[br[4k+m]]).
 template <int NB>
 SXC_HD inline int stageA_pos(int g, int r) {
-  return (int)((bitrev((unsigned)r, 5) << (NB - 5)) | bitrev((unsigned)g, NB - 5));
+  constexpr int L = Geo<NB>::LSA;
+  return (int)((bitrev((unsigned)r, L) << (NB - L)) | bitrev((unsigned)g, NB - L));
 }
 
-// Forward levels pass1+2, pass3, pass_3, pass_4 (FFTReal.hpp:466-528, 553-601).
-// v[r] holds the time sample at stageA_pos(g, r) on entry, spectrum positions 32g+r on exit.
+// Forward levels pass1+2, pass3, pass_3 (and pass_4 for SA = 32) (FFTReal.hpp:466-528,
+// 553-601).  v[r] holds the time sample at stageA_pos(g, r) on entry, spectrum position
+// SA*g + r on exit.
+template <int SA>
 SXC_HD inline void stageA_forward(float* v, const Twiddle& tw) {
-  float d[32];
-  static_for<0, 8>([&](auto K) {
+  float d[SA];
+  static_for<0, SA / 4>([&](auto K) {
     constexpr int b = 4 * decltype(K)::value;
     d[b + 1] = v[b + 0] - v[b + 1];
     d[b + 3] = v[b + 2] - v[b + 3];
@@ -161,7 +175,7 @@ SXC_HD inline void stageA_forward(float* v, const Twiddle& tw) {
     d[b + 2] = s0 - s2;
   });
   const float sq = SXC_SQRT2_2;
-  static_for<0, 4>([&](auto K) {
+  static_for<0, SA / 8>([&](auto K) {
     constexpr int c = 8 * decltype(K)::value;
     float t;
     v[c + 0] = d[c + 0] + d[c + 4];
@@ -175,19 +189,20 @@ SXC_HD inline void stageA_forward(float* v, const Twiddle& tw) {
     v[c + 5] = t + d[c + 3];
     v[c + 7] = t - d[c + 3];
   });
-  level_contig<3, 2, false>(v, tw);
-  level_contig<4, 1, false>(v, tw);
+  level_contig<3, SA / 16, false>(v, tw);
+  if constexpr (SA == 32) level_contig<4, 1, false>(v, tw);
 }
 
-// Inverse of stage A: pass_4, pass_3 inverse (FFTReal.hpp:729-776), inverse pass 3
+// Inverse of stage A: (pass_4), pass_3 inverse (FFTReal.hpp:729-776), inverse pass 3
 // (:834-862), inverse pass 1+2 (:866-906).  On exit out[r] is the time sample for
 // position stageA_pos(g, r) (the bit-reversed store of pass 1+2).
+template <int SA>
 SXC_HD inline void stageA_inverse(float* v, float* out, const Twiddle& tw) {
-  level_contig<4, 1, true>(v, tw);
-  level_contig<3, 2, true>(v, tw);
+  if constexpr (SA == 32) level_contig<4, 1, true>(v, tw);
+  level_contig<3, SA / 16, true>(v, tw);
   const float sq = SXC_SQRT2_2;
-  float d[32];
-  static_for<0, 4>([&](auto K) {
+  float d[SA];
+  static_for<0, SA / 8>([&](auto K) {
     constexpr int c = 8 * decltype(K)::value;
     d[c + 0] = v[c + 0] + v[c + 4];
     d[c + 4] = v[c + 0] - v[c + 4];
@@ -200,7 +215,7 @@ SXC_HD inline void stageA_inverse(float* v, float* out, const Twiddle& tw) {
     d[c + 5] = (vr + vi) * sq;
     d[c + 7] = (vi - vr) * sq;
   });
-  static_for<0, 8>([&](auto K) {
+  static_for<0, SA / 4>([&](auto K) {
     constexpr int b = 4 * decltype(K)::value;
     const float b0 = d[b + 0] + d[b + 2];
     const float b2 = d[b + 0] - d[b + 2];
@@ -348,13 +363,15 @@ SXC_HD inline void round_unit(float* Y, int u, const Twiddle& tw) {
   }
 }
 
-// Level schedule above stage A: levels 5 .. NB-1 in rounds of 3, the remainder last.
+// Level schedule above stage A: levels LSA .. NB-1 in rounds of KMAX, the remainder last.
 template <int NB>
 struct Schedule {
-  static constexpr int R = NB - 5;
-  static constexpr int NR = (R + 2) / 3;
-  static constexpr int k(int r) { return (r < R / 3) ? 3 : (R - 3 * (R / 3)); }
-  static constexpr int m(int r) { return 5 + 3 * r; }
+  static constexpr int M0 = Geo<NB>::LSA;
+  static constexpr int KM = SXC_KMAX;
+  static constexpr int R = NB - M0;
+  static constexpr int NR = (R + KM - 1) / KM;
+  static constexpr int k(int r) { return (r < R / KM) ? KM : (R - KM * (R / KM)); }
+  static constexpr int m(int r) { return M0 + KM * r; }
 };
 
 }  // namespace sxc

@@ -11,12 +11,17 @@
 namespace sxc {
 
 // Device executor: one HIP thread per emulated thread; per-thread state in registers.
+// Each phase receives an opaque copy of the thread index, so the compiler recomputes the
+// phase's (cheap) LDS addresses instead of hoisting every round's addresses out of the
+// channel loop and keeping them all live.
 template <class R>
 struct DevExec {
   R& r;
   template <class F>
   __device__ __forceinline__ void each(F&& f) {
-    f((int)threadIdx.x);
+    int t = (int)threadIdx.x;
+    asm volatile("" : "+v"(t));
+    f(t);
   }
   __device__ __forceinline__ void sync() { __syncthreads(); }
   template <class T>

@@ -115,7 +115,7 @@ enum { MISC_L = 0, MISC_THIRDS = 1, MISC_NCAND = 2, MISC_KEPT = 3, MISC_Q4 = 4 /
 
 template <int NB>
 struct Regs {
-  float acc[32];  // correlation accumulated over channels, stage-A output order
+  float acc[Geo<NB>::SA];  // correlation accumulated over channels, stage-A output order
 };
 
 // ---------------------------------------------------------------- helpers
@@ -289,14 +289,14 @@ SXC_HD inline void forward_channel(Ex& ex, char* sm, const sxc_tables* tab, cons
   float* Y = (float*)(sm + SM::Y);
   ex.each([&](int g) {
     if (g >= G::GROUPS) return;
-    float v[32];
-    static_for<0, 32>([&](auto I) {
+    float v[G::SA];
+    static_for<0, G::SA>([&](auto I) {
       constexpr int r = decltype(I)::value;
       const int p = stageA_pos<NB>(g, r);
       v[r] = given ? given[p] : encode_at<NB>(sm, L, p, ch);
     });
-    stageA_forward(v, tw);
-    static_for<0, 32>([&](auto I) { Y[pad(32 * g + decltype(I)::value)] = v[decltype(I)::value]; });
+    stageA_forward<G::SA>(v, tw);
+    static_for<0, G::SA>([&](auto I) { Y[pad(G::SA * g + decltype(I)::value)] = v[decltype(I)::value]; });
   });
   ex.sync();
   fwd_rounds<NB, 0>(ex, Y, tw);
@@ -371,7 +371,7 @@ SXC_HD inline void correlate(Ex& ex, char* sm, const sxc_tables* tab, const Twid
   float* Y = (float*)(sm + SM::Y);
   ex.each([&](int g) {
     Regs<NB>& r = ex.template reg<Regs<NB>>(g);
-    static_for<0, 32>([&](auto I) { r.acc[decltype(I)::value] = 0.f; });
+    static_for<0, G::SA>([&](auto I) { r.acc[decltype(I)::value] = 0.f; });
   });
   const float mul = (float)(1.0 / (double)G::N);  // FFTReal::rescale (FFTReal.hpp:264)
   for (int ch = 0; ch < 4; ++ch) {
@@ -381,10 +381,10 @@ SXC_HD inline void correlate(Ex& ex, char* sm, const sxc_tables* tab, const Twid
     ex.each([&](int g) {
       if (g >= G::GROUPS) return;
       Regs<NB>& r = ex.template reg<Regs<NB>>(g);
-      float v[32], o[32];
-      static_for<0, 32>([&](auto I) { v[decltype(I)::value] = Y[pad(32 * g + decltype(I)::value)]; });
-      stageA_inverse(v, o, tw);
-      static_for<0, 32>([&](auto I) {
+      float v[G::SA], o[G::SA];
+      static_for<0, G::SA>([&](auto I) { v[decltype(I)::value] = Y[pad(G::SA * g + decltype(I)::value)]; });
+      stageA_inverse<G::SA>(v, o, tw);
+      static_for<0, G::SA>([&](auto I) {
         constexpr int k = decltype(I)::value;
         const float x = o[k] * mul;
         r.acc[k] = r.acc[k] + x;
@@ -396,7 +396,7 @@ SXC_HD inline void correlate(Ex& ex, char* sm, const sxc_tables* tab, const Twid
   ex.each([&](int g) {
     if (g >= G::GROUPS) return;
     Regs<NB>& r = ex.template reg<Regs<NB>>(g);
-    static_for<0, 32>([&](auto I) {
+    static_for<0, G::SA>([&](auto I) {
       constexpr int k = decltype(I)::value;
       const int p = stageA_pos<NB>(g, k);
       Y[pad(p ^ G::H)] = r.acc[k];  // fftshift by N/2 (analysis/CrossCorr.cc:500-505)

@@ -393,9 +393,12 @@ SXC_HD inline void drain(Ex& ex, char* sm, const ScanArgs& a, const sxc_pair_dev
       const int slot = ex.atomic_add_shared(&misc[SM_KEPT], 1);
       if (slot >= a.cap) continue;
       const int tstart = pr.t_chunk_start + start_t;
-      // RCQuery (analysis/HomologyByXCorrSlave.cc:56-60) on the reverse strand
+      // RCQuery (analysis/HomologyByXCorrSlave.cc:56-60) on the reverse strand, with the
+      // -q_chunk size (slave) or, for q_chunk == 0, the query chunk's own length (the
+      // standalone aligner, tools/analysis/HomologyByXCorr.cc:226-228, 793-798)
+      const int rc_chunk = a.q_chunk > 0 ? a.q_chunk : a.qchunks[pr.q_chunk].len;
       const int qstart =
-          strand == 0 ? pr.q_chunk_start + start_q : start_q + pr.q_record_len - pr.q_chunk_start - a.q_chunk;
+          strand == 0 ? pr.q_chunk_start + start_q : start_q + pr.q_record_len - pr.q_chunk_start - rc_chunk;
       sxc_result_dev r;
       r.query_id = (uint64_t)(int64_t)pr.q_record;
       r.target_id = (uint64_t)(int64_t)pr.t_record;

@@ -237,3 +237,63 @@ int main(int argc, char** argv) {
   return 0;
 }
 #endif
+
+#ifdef SXC_EMU_ABI
+// The subset of the libsxc_hip.so C ABI that the host programs (slave, standalone
+// aligner) call, served by the emulator, so their protocol and file handling can be
+// tested on a machine without a GPU.  Test infrastructure only.
+#include "../../include/sxc.h"
+
+struct sxc_ctx {
+  int device;
+};
+struct sxc_batch {
+  std::vector<sxc_pair> pairs;
+  sxc_params prm;
+  std::vector<sxc_result> res;
+};
+
+extern "C" {
+static thread_local const char* g_emu_err = "";
+int sxc_create(sxc_ctx** out, int device) {
+  *out = new sxc_ctx{device};
+  return SXC_OK;
+}
+void sxc_destroy(sxc_ctx* ctx) { delete ctx; }
+const char* sxc_last_error(void) { return g_emu_err; }
+const char* sxc_version(void) { return "emulator"; }
+int sxc_batch_create(sxc_ctx*, const sxc_pair* pairs, int32_t n, const sxc_params* prm, sxc_batch** out) {
+  *out = new sxc_batch{std::vector<sxc_pair>(pairs, pairs + n), *prm, {}};
+  return SXC_OK;
+}
+int sxc_batch_run(sxc_batch* b) {
+  std::vector<sxc_result> r((size_t)b->pairs.size() * 64 + 1024);
+  int64_t n = 0;
+  int rc = emu_align_batch(b->pairs.data(), (int)b->pairs.size(), &b->prm, r.data(), (int64_t)r.size(), &n,
+                           nullptr, nullptr);
+  if (rc == SXC_ERR_CAPACITY) {
+    r.resize((size_t)n);
+    rc = emu_align_batch(b->pairs.data(), (int)b->pairs.size(), &b->prm, r.data(), (int64_t)r.size(), &n, nullptr,
+                         nullptr);
+  }
+  if (rc) {
+    g_emu_err = "emulated batch failed";
+    return SXC_ERR_ARG;
+  }
+  r.resize((size_t)n);
+  b->res.swap(r);
+  return SXC_OK;
+}
+int sxc_batch_results(sxc_batch* b, sxc_result* out, int64_t cap, int64_t* n_out) {
+  *n_out = (int64_t)b->res.size();
+  if (*n_out > cap) return SXC_ERR_CAPACITY;
+  memcpy(out, b->res.data(), b->res.size() * sizeof(sxc_result));
+  return SXC_OK;
+}
+void sxc_batch_destroy(sxc_batch* b) { delete b; }
+int sxc_align_batch(sxc_ctx*, const sxc_pair* pairs, int32_t n, const sxc_params* prm, sxc_result* out, int64_t cap,
+                    int64_t* n_out) {
+  return emu_align_batch(pairs, n, prm, out, cap, n_out, nullptr, nullptr);
+}
+}  // extern "C"
+#endif
