@@ -8,9 +8,15 @@ them carrying one planted homology (50-2000 bp, 70-95% identity, 1% indels), FFT
 uploaded to HBM before the timed region (sxc_batch_create); one step = one
 sxc_batch_run over all 4096 pairs (target spectra, fused pair kernel, result ordering).
 
+Serving (config of the serving loop): two synthetic 50-Mbp genomes sharing 2,000 syntenic
+blocks, chunked as the slave chunks them (16,277 target chunks of 4096 bp every 3072 bp,
+12,208 query chunks), target spectra resident in HBM; battleship-shaped requests (seed
+lines of 64 chunk pairs, then the 24 x 24 matrix cells of hits) arrive as Poisson streams
+at swept rates; each rate reports request latency p50/p99 and chunk pairs per second.
+
 Printed: one JSON line with the batched throughput (`value`), the plain one-pair-per-call
-throughput beside it, the dominant kernel's roofline figures from HIP events, and the
-reference CPU path timed on this host's cores on a bounded sample.
+throughput beside it, the serving latencies, the dominant kernel's roofline figures from
+HIP events, and the reference CPU path timed on this host's cores on a bounded sample.
 """
 from __future__ import annotations
 
@@ -34,6 +40,11 @@ HBM_PEAK_GBS = 8000.0
 # materialised, target spectra resident, peak search fused): forward
 # 2*4*8192*4 + 1024 = 263,168 B; inverse 262,144 + 131,072 + 65,536 = 458,752 B.
 B_PAIR = 263_168 + 458_752
+# HBM bytes k_correlate<13> moves per chunk pair, from rocprofv3 FETCH_SIZE (x2.0) and
+# WRITE_SIZE (x1.0) on the 4096-pair launch (profiles/uniform_pairs_4096_kernels.md):
+# (271,555 * 2 + 4,916) KiB / 4096 pairs.
+TRAFFIC_PER_PAIR = (271_555 * 2 + 4_916) * 1024 / 4096
+SERVE_RATES = (250.0, 1000.0, 2000.0)  # requests/s; 1000/s is the headline point
 
 
 def make_workload(n_pairs: int, seed: int = 1):
@@ -76,6 +87,35 @@ def cpu_baseline(T, Q, sample: int, threads: int):
             "sample": f"{sample} pairs of the same workload, both strands, oracle restatement"}
 
 
+def serve_bench(ctx, genome_bp: int, duration: float, rates, seed: int = 4):
+    from satsuma2_amd import synth
+    from satsuma2_amd.api import Server, SlaveParams
+    from satsuma2_amd.serve_load import Battleship, genome_chunks, run_stream
+
+    print(f"[bench] serving: generating two {genome_bp / 1e6:.0f}-Mbp genomes", file=sys.stderr, flush=True)
+    T, Q, _ = synth.syntenic_genomes(length=genome_bp, n_blocks=max(1, genome_bp * 2000 // 50_000_000), seed=seed)
+    tch = genome_chunks(T, 4096, 3072)
+    qch = genome_chunks(Q, 4096, 4096)
+    t0 = time.perf_counter()
+    srv = Server(ctx, SlaveParams(target_total=float(len(T))), tch, qch)
+    create_s = time.perf_counter() - t0
+    info = srv.stats()
+    runs = []
+    for k, rate in enumerate(rates):
+        print(f"[bench] serving: {rate:.0f} requests/s for {duration:.0f} s", file=sys.stderr, flush=True)
+        prod = Battleship(len(tch), len(qch), np.random.default_rng(seed + k))
+        runs.append(run_stream(srv, prod, rate, duration, seed=seed + k))
+    info = srv.stats()
+    srv.close()
+    head = runs[min(1, len(runs) - 1)]
+    return {"workload": f"two {genome_bp / 1e6:.0f}-Mbp synthetic genomes, {len(tch)} target x {len(qch)} query "
+                        f"chunks, battleship requests (seed lines of 64 pairs, 24x24 cells of hits)",
+            "p50_ms": head["p50_ms"], "p99_ms": head["p99_ms"], "at_requests_per_s": head["offered_requests_per_s"],
+            "rates": runs, "create_s": round(create_s, 2), "resident_gb": round(info["resident_bytes"] / 1e9, 3),
+            "slots": info["slots"], "max_pairs_per_launch": info["max_pairs_per_launch"],
+            "launches": info["launches"]}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -85,6 +125,9 @@ def main():
     ap.add_argument("--plain-pairs", type=int, default=256)
     ap.add_argument("--cpu-sample", type=int, default=2048)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--serve-bp", type=int, default=50_000_000)
+    ap.add_argument("--serve-seconds", type=float, default=4.0)
+    ap.add_argument("--no-serve", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (chunk pairs shard by target chunk "
@@ -156,9 +199,14 @@ def main():
         "candidate_lags": cands,
         "diagonal_runs": runs,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": int(TRAFFIC_PER_PAIR * args.pairs),
                      "kernel": "k_correlate<13>", "bytes_per_pair": B_PAIR},
     }
+    if not args.no_serve:
+        sv = serve_bench(ctx, args.serve_bp, args.serve_seconds, SERVE_RATES)
+        out["serve_p50_ms"] = sv["p50_ms"]
+        out["serve_p99_ms"] = sv["p99_ms"]
+        out["serve"] = sv
     if not args.no_cpu:
         threads = max(1, min(16, os.cpu_count() or 1))
         print(f"[bench] cpu baseline on {threads} threads", file=sys.stderr, flush=True)

@@ -9,7 +9,7 @@
  *
  * Sequences are byte strings of uppercase ASCII letters (< 128), as the reference holds
  * them after vecDNAVector::Read (analysis/DNAVector.cc:1176-1293, ToUpper at :586).
- * FFT lengths N are powers of two in [512, 16384] (2 * t_chunk in the slave,
+ * FFT lengths N are powers of two in [512, 32768] (2 * t_chunk in the slave,
  * analysis/HomologyByXCorrSlave.cc:259); sequence lengths must not exceed N.
  */
 #ifndef SXC_H
@@ -26,7 +26,8 @@ enum {
   SXC_ERR_ARG = -1,       /* bad argument (size, N, byte >= 128, null pointer) */
   SXC_ERR_HIP = -2,       /* a HIP runtime call failed */
   SXC_ERR_CAPACITY = -3,  /* output capacity too small; *n_out holds the required count */
-  SXC_ERR_NOMEM = -4
+  SXC_ERR_NOMEM = -4,
+  SXC_PENDING = 1         /* a server request is not finished yet */
 };
 
 typedef struct sxc_ctx sxc_ctx;
@@ -54,7 +55,8 @@ typedef struct {
 
 /* Slave parameters (analysis/HomologyByXCorrSlave.cc:334-384 flags). */
 typedef struct {
-  int32_t N;            /* FFT length = 2 * t_chunk */
+  int32_t N;            /* FFT length = 2 * t_chunk; 0 = per pair, the next power of two >= tlen + qlen
+                           (at least 512), pairs of each length run as one class */
   int32_t q_chunk;      /* RCQuery chunk size: -q_chunk on the slave path; 0 = each query chunk's own
                            length, as the standalone aligner computes it
                            (tools/analysis/HomologyByXCorr.cc:226-228, 793-798) */
@@ -119,16 +121,97 @@ int sxc_align_pair(sxc_ctx* ctx, const sxc_pair* pair, const sxc_params* prm, sx
 int sxc_align_batch(sxc_ctx* ctx, const sxc_pair* pairs, int32_t n, const sxc_params* prm, sxc_result* out,
                     int64_t cap, int64_t* n_out);
 
+/* Signal tails of the standalone aligner (tools/analysis/HomologyByXCorr.cc:706-716, 780-798).
+ * That program reuses one CCSignal per side, and CCSignal::SetSequence writes only positions
+ * [0, L) of its N-long channels (analysis/CrossCorr.cc:118-131, 182-206): positions from the
+ * chunk length on keep what an earlier, longer sequence left there.  A tail lists those
+ * earlier sequences as consecutive segments: the first covers [L, end[0]), the next
+ * [end[0], end[1]), ...; each holds the signal of `seq` (reverse-complemented when
+ * strand = 1) at the same positions, so end <= len.  Positions past the last end are 0. */
+typedef struct {
+  int32_t end;
+  int32_t len;
+  const char* seq;
+  int32_t strand;
+  int32_t pad_;
+} sxc_tail_seg;
+
+typedef struct {
+  const sxc_tail_seg* segs;
+  int32_t nseg;
+  int32_t pad_;
+} sxc_signal_tail;
+
 /* Resident batch: the inputs of sxc_align_batch uploaded once to HBM, so the device work
  * can be run (and timed) repeatedly without host transfers.  Same results as
  * sxc_align_batch. */
 typedef struct sxc_batch sxc_batch;
 int sxc_batch_create(sxc_ctx* ctx, const sxc_pair* pairs, int32_t n, const sxc_params* prm, sxc_batch** out);
+/* sxc_batch_create with signal tails (see sxc_tail_seg): t_tail[k] extends pair k's target
+ * signal, q_tail[k] its query signal on both strands (the reverse-complement call of the
+ * standalone aligner overwrites exactly the positions the forward call wrote).  Either
+ * array may be NULL; nseg = 0 means no tail. */
+int sxc_batch_create_tails(sxc_ctx* ctx, const sxc_pair* pairs, int32_t n, const sxc_params* prm,
+                           const sxc_signal_tail* t_tail, const sxc_signal_tail* q_tail, sxc_batch** out);
 int sxc_batch_run(sxc_batch* batch);
 int sxc_batch_results(sxc_batch* batch, sxc_result* out, int64_t cap, int64_t* n_out);
 /* Accepted lags (FindTop) and diagonal runs (DoOne) of the last run, summed over the batch. */
 int sxc_batch_stats(sxc_batch* batch, int64_t* candidates, int64_t* runs);
 void sxc_batch_destroy(sxc_batch* batch);
+
+/* ---------------------------------------------------------------- serving loop
+ *
+ * A server keeps a target and a query genome's chunks resident in HBM, with the four
+ * spectra of every target chunk computed once, and serves requests made of t_pair blocks
+ * (analysis/WorkQueue.h:17-22) -- the battleship search's seed lines and hit
+ * neighbourhoods (analysis/GridSearch.cc:188-411).  Requests are taken as they arrive:
+ * a dispatcher thread packs the oldest pending chunk pairs, across requests, into launch
+ * slots that run concurrently on their own streams, and admits more as slots free up.
+ * The number of slots is what the HBM budget leaves after the resident data.  A request's
+ * records are those align_target produces for its blocks, in the same order, whatever
+ * else shares its launches. */
+typedef struct sxc_server sxc_server;
+
+/* One chunk of a resident genome (a ChunkManager chunk, analysis/SeqChunk.cc:72-165). */
+typedef struct {
+  const char* seq;
+  int32_t len;          /* targets at most N/2, queries at most N */
+  int32_t start;        /* SeqChunk::GetStart */
+  int32_t record;       /* record id (t_result.target_id / query_id) */
+  int32_t record_len;   /* ChunkManager::GetSize(record) (t_result.query_size, RCQuery) */
+} sxc_chunk_desc;
+
+/* t_pair: chunk index ranges, both inclusive, and the fast flag (cutoff_fast). */
+typedef struct {
+  int32_t target_from, target_to, query_from, query_to;
+  int32_t fast;
+} sxc_block;
+
+typedef struct {
+  int64_t hbm_budget;            /* device bytes for resident data + slots; 0 = 3/4 of free memory */
+  int32_t max_pairs_per_launch;  /* 0 = 2048 */
+  int32_t max_slots;             /* launches in flight; 0 = 3 */
+} sxc_server_config;
+
+typedef struct {
+  int64_t requests_done, pairs_done, launches, results;
+  int64_t resident_bytes;        /* genome bytes, chunk tables and target spectra */
+  int64_t slot_bytes;            /* device bytes of one launch slot */
+  int32_t slots, max_pairs_per_launch;
+  double device_ms;              /* summed device time of all launches */
+} sxc_server_info;
+
+/* prm->strands is ignored (both strands, as Align). */
+int sxc_server_create(sxc_ctx* ctx, const sxc_params* prm, const sxc_chunk_desc* targets, int32_t nt,
+                      const sxc_chunk_desc* queries, int32_t nq, const sxc_server_config* cfg, sxc_server** out);
+int sxc_server_submit(sxc_server* srv, const sxc_block* blocks, int32_t nblocks, uint64_t* ticket);
+/* Waits up to timeout_ms (< 0: no limit) for the request; SXC_PENDING if it is not done.
+ * latency_ms: submission to completion. */
+int sxc_server_wait(sxc_server* srv, uint64_t ticket, int32_t timeout_ms, int64_t* n_results, double* latency_ms);
+/* Copies a finished request's records and forgets the request (SXC_PENDING if not finished). */
+int sxc_server_results(sxc_server* srv, uint64_t ticket, sxc_result* out, int64_t cap, int64_t* n_out);
+int sxc_server_stats(sxc_server* srv, sxc_server_info* out);
+void sxc_server_destroy(sxc_server* srv);
 
 /* Device time of the last sxc_align_batch / sxc_align_pair / sxc_batch_run, by phase (milliseconds). */
 typedef struct {

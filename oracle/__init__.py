@@ -123,6 +123,9 @@ def lib():
         L.sxo_align_batch.argtypes = [C.POINTER(Pair), C.c_int32, C.POINTER(Params), C.c_int32,
                                       C.POINTER(Result), C.c_int32, C.POINTER(C.c_int32)]
         L.sxo_align_batch.restype = C.c_int64
+        L.sxo_align_batch_tails.argtypes = [C.POINTER(Pair), C.c_int32, C.POINTER(Params), C.c_void_p, C.c_void_p,
+                                            C.c_int32, C.POINTER(Result), C.c_int32, C.POINTER(C.c_int32)]
+        L.sxo_align_batch_tails.restype = C.c_int64
         _lib = L
     return _lib
 
@@ -211,9 +214,32 @@ def make_params(N=8192, q_chunk=4096, min_len=0, strands=2, cutoff=1.8, cutoff_f
     return Params(N, q_chunk, min_len, strands, cutoff, cutoff_fast, min_prob, target_total)
 
 
-def align_batch(pairs, params: Params, threads: int = 1, cap_per_pair: int = 1 << 14):
+class TailSeg(C.Structure):
+    _fields_ = [("end", C.c_int32), ("len", C.c_int32), ("seq", C.c_char_p), ("strand", C.c_int32),
+                ("pad_", C.c_int32)]
+
+
+class Tail(C.Structure):
+    _fields_ = [("segs", C.POINTER(TailSeg)), ("nseg", C.c_int32), ("pad_", C.c_int32)]
+
+
+def _tails(lst, keep):
+    """lst: per pair a list of (end, seq bytes, strand) segments (sxo_tail_seg)."""
+    arr = (Tail * len(lst))()
+    for k, segs in enumerate(lst):
+        if not segs:
+            continue
+        sa = (TailSeg * len(segs))(*[TailSeg(e, len(q), q, st, 0) for e, q, st in segs])
+        keep += [sa] + [q for _, q, _ in segs]
+        arr[k] = Tail(C.cast(sa, C.POINTER(TailSeg)), len(segs), 0)
+    return arr
+
+
+def align_batch(pairs, params: Params, threads: int = 1, cap_per_pair: int = 1 << 14, t_tails=None, q_tails=None):
     """pairs: list of dicts with keys tseq, qseq, t_chunk_start, q_chunk_start, t_record,
-    q_record, q_record_len, fast.  Returns a list of numpy structured arrays (RESULT_DTYPE)."""
+    q_record, q_record_len, fast.  t_tails / q_tails (optional): per pair the signal tail
+    segments (end, sequence, strand) of sxo_align_batch_tails.  Returns a list of numpy
+    structured arrays (RESULT_DTYPE)."""
     n = len(pairs)
     arr = (Pair * n)()
     keep = []
@@ -224,6 +250,10 @@ def align_batch(pairs, params: Params, threads: int = 1, cap_per_pair: int = 1 <
                       p.get("t_record", 0), p.get("q_record", 0), p.get("q_record_len", len(q)), p.get("fast", 0))
     out = np.zeros(n * cap_per_pair, RESULT_DTYPE)
     counts = np.zeros(n, np.int32)
-    lib().sxo_align_batch(arr, n, C.byref(params), threads, out.ctypes.data_as(C.POINTER(Result)), cap_per_pair,
-                          counts.ctypes.data_as(C.POINTER(C.c_int32)))
+    tt = _tails(t_tails, keep) if t_tails is not None else None
+    qt = _tails(q_tails, keep) if q_tails is not None else None
+    lib().sxo_align_batch_tails(arr, n, C.byref(params), C.cast(tt, C.c_void_p) if tt is not None else None,
+                                C.cast(qt, C.c_void_p) if qt is not None else None, threads,
+                                out.ctypes.data_as(C.POINTER(Result)), cap_per_pair,
+                                counts.ctypes.data_as(C.POINTER(C.c_int32)))
     return [out[k * cap_per_pair: k * cap_per_pair + counts[k]].copy() for k in range(n)]

@@ -661,6 +661,33 @@ typedef struct {
   int32_t mcap;
 } align_ws;
 
+int sxo_apply_tail(float* sig, int32_t L, int32_t N, const sxo_tail* t) {
+  if (!t || t->nseg <= 0) return 0;
+  float* tmp = (float*)malloc(sizeof(float) * 4 * (size_t)N);
+  char* rc = (char*)malloc((size_t)N + 1);
+  int32_t start = L;
+  int err = 0;
+  for (int32_t s = 0; s < t->nseg && !err; ++s) {
+    const sxo_tail_seg* g = &t->segs[s];
+    if (g->end <= start || g->end > g->len || g->len > N) {
+      err = -1;
+      break;
+    }
+    const char* src = g->seq;
+    if (g->strand) {
+      sxo_reverse_complement(g->seq, g->len, rc);
+      src = rc;
+    }
+    sxo_encode(src, g->len, N, tmp);
+    for (int ch = 0; ch < 4; ++ch)
+      memcpy(sig + (size_t)ch * N + start, tmp + (size_t)ch * N + start, sizeof(float) * (size_t)(g->end - start));
+    start = g->end;
+  }
+  free(tmp);
+  free(rc);
+  return err;
+}
+
 static int ws_init(align_ws* ws, int32_t N) {
   memset(ws, 0, sizeof *ws);
   if (plan_init(&ws->plan, N)) return -1;
@@ -686,14 +713,15 @@ static void ws_free(align_ws* ws) {
 
 /* HomologyByXCorr::Align (analysis/HomologyByXCorrSlave.cc:221-253) and
  * FilterMatches (:168-219) with prob_table off. */
-static int32_t align_pair_ws(align_ws* ws, const sxo_pair* p, const sxo_params* prm, sxo_result* out,
-                             int32_t cap) {
+static int32_t align_pair_ws(align_ws* ws, const sxo_pair* p, const sxo_params* prm, const sxo_tail* t_tail,
+                             const sxo_tail* q_tail, sxo_result* out, int32_t cap) {
   pthread_once(&g_score_once, score_init);
   const int32_t N = prm->N;
   const double cutoff = p->fast ? prm->cutoff_fast : prm->cutoff;
   int32_t nres = 0;
   if (p->tlen > N || p->qlen > N) return -1;
   sxo_encode(p->tseq, p->tlen, N, ws->sigT);
+  if (sxo_apply_tail(ws->sigT, p->tlen, N, t_tail)) return -1;
   for (int strand = 0; strand < prm->strands; ++strand) {
     const char* qs = p->qseq;
     if (strand == 1) {
@@ -701,6 +729,7 @@ static int32_t align_pair_ws(align_ws* ws, const sxo_pair* p, const sxo_params* 
       qs = ws->qrc;
     }
     sxo_encode(qs, p->qlen, N, ws->sigQ);
+    if (sxo_apply_tail(ws->sigQ, p->qlen, N, q_tail)) return -1;
     xcorr_with_plan(&ws->plan, ws->sigT, ws->sigQ, ws->xc, ws->w);
     int32_t nm;
     for (;;) {
@@ -743,13 +772,15 @@ static int32_t align_pair_ws(align_ws* ws, const sxo_pair* p, const sxo_params* 
 int32_t sxo_align_pair(const sxo_pair* p, const sxo_params* prm, sxo_result* out, int32_t cap) {
   align_ws ws;
   if (ws_init(&ws, prm->N)) return -1;
-  int32_t n = align_pair_ws(&ws, p, prm, out, cap);
+  int32_t n = align_pair_ws(&ws, p, prm, NULL, NULL, out, cap);
   ws_free(&ws);
   return n;
 }
 
 typedef struct {
   const sxo_pair* pairs;
+  const sxo_tail* t_tails;
+  const sxo_tail* q_tails;
   int32_t n;
   const sxo_params* prm;
   sxo_result* out;
@@ -769,7 +800,8 @@ static void* batch_worker(void* arg) {
     int32_t k = b->next++;
     pthread_mutex_unlock(&b->mu);
     if (k >= b->n) break;
-    int32_t c = align_pair_ws(&ws, &b->pairs[k], b->prm, b->out + (size_t)k * b->cap, b->cap);
+    int32_t c = align_pair_ws(&ws, &b->pairs[k], b->prm, b->t_tails ? &b->t_tails[k] : NULL,
+                              b->q_tails ? &b->q_tails[k] : NULL, b->out + (size_t)k * b->cap, b->cap);
     b->counts[k] = c < b->cap ? c : b->cap;
     pthread_mutex_lock(&b->mu);
     b->total += c;
@@ -781,8 +813,16 @@ static void* batch_worker(void* arg) {
 
 int64_t sxo_align_batch(const sxo_pair* pairs, int32_t n, const sxo_params* prm, int32_t threads,
                         sxo_result* out, int32_t cap_per_pair, int32_t* counts) {
+  return sxo_align_batch_tails(pairs, n, prm, NULL, NULL, threads, out, cap_per_pair, counts);
+}
+
+int64_t sxo_align_batch_tails(const sxo_pair* pairs, int32_t n, const sxo_params* prm, const sxo_tail* t_tails,
+                              const sxo_tail* q_tails, int32_t threads, sxo_result* out, int32_t cap_per_pair,
+                              int32_t* counts) {
   batch_ctx b;
   b.pairs = pairs;
+  b.t_tails = t_tails;
+  b.q_tails = q_tails;
   b.n = n;
   b.prm = prm;
   b.out = out;

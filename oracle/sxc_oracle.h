@@ -99,10 +99,37 @@ double sxo_print_match_ident(const char* q, const char* t, int32_t st, int32_t s
  * 168-253).  Returns number of results (may exceed cap). */
 int32_t sxo_align_pair(const sxo_pair* p, const sxo_params* prm, sxo_result* out, int32_t cap);
 
+/* Signal tail of a reused CCSignal (tools/analysis/HomologyByXCorr.cc:706-716, 780-798):
+ * CCSignal::SetSequence writes only [0, L) (analysis/CrossCorr.cc:118-131, 182-206), so the
+ * positions from L on keep an earlier, longer sequence's signal.  Segment s covers
+ * [L or the previous end, end) with the signal of seq (reverse complement if strand = 1). */
+typedef struct {
+  int32_t end;
+  int32_t len;
+  const char* seq;
+  int32_t strand;
+  int32_t pad_;
+} sxo_tail_seg;
+
+typedef struct {
+  const sxo_tail_seg* segs;
+  int32_t nseg;
+  int32_t pad_;
+} sxo_tail;
+
+/* Overwrites sig (4*N floats, channels A, C, G, T) at [L, last end) from the tail. */
+int sxo_apply_tail(float* sig, int32_t L, int32_t N, const sxo_tail* t);
+
 /* Many pairs on `threads` POSIX threads; results of pair k go to out + offsets[k],
  * counts[k] receives the number produced (clipped to cap_per_pair). Returns total produced. */
 int64_t sxo_align_batch(const sxo_pair* pairs, int32_t n, const sxo_params* prm, int32_t threads,
                         sxo_result* out, int32_t cap_per_pair, int32_t* counts);
+
+/* sxo_align_batch with signal tails: t_tails[k] / q_tails[k] (either may be NULL) extend
+ * pair k's target signal and its query signal on both strands. */
+int64_t sxo_align_batch_tails(const sxo_pair* pairs, int32_t n, const sxo_params* prm, const sxo_tail* t_tails,
+                              const sxo_tail* q_tails, int32_t threads, sxo_result* out, int32_t cap_per_pair,
+                              int32_t* counts);
 
 #ifdef __cplusplus
 }

@@ -18,6 +18,8 @@ SXC_OK = 0
 SXC_ERR_ARG = -1
 SXC_ERR_HIP = -2
 SXC_ERR_CAPACITY = -3
+SXC_ERR_NOMEM = -4
+SXC_PENDING = 1
 
 
 class SxcError(RuntimeError):
@@ -55,6 +57,35 @@ class Timing(C.Structure):
                 ("scan_ms", C.c_double)]
 
 
+class TailSeg(C.Structure):
+    _fields_ = [("end", C.c_int32), ("len", C.c_int32), ("seq", C.c_char_p), ("strand", C.c_int32),
+                ("pad_", C.c_int32)]
+
+
+class SignalTail(C.Structure):
+    _fields_ = [("segs", C.POINTER(TailSeg)), ("nseg", C.c_int32), ("pad_", C.c_int32)]
+
+
+class ChunkDesc(C.Structure):
+    _fields_ = [("seq", C.c_char_p), ("len", C.c_int32), ("start", C.c_int32), ("record", C.c_int32),
+                ("record_len", C.c_int32)]
+
+
+class Block(C.Structure):
+    _fields_ = [("target_from", C.c_int32), ("target_to", C.c_int32), ("query_from", C.c_int32),
+                ("query_to", C.c_int32), ("fast", C.c_int32)]
+
+
+class ServerConfig(C.Structure):
+    _fields_ = [("hbm_budget", C.c_int64), ("max_pairs_per_launch", C.c_int32), ("max_slots", C.c_int32)]
+
+
+class ServerInfo(C.Structure):
+    _fields_ = [("requests_done", C.c_int64), ("pairs_done", C.c_int64), ("launches", C.c_int64),
+                ("results", C.c_int64), ("resident_bytes", C.c_int64), ("slot_bytes", C.c_int64),
+                ("slots", C.c_int32), ("max_pairs_per_launch", C.c_int32), ("device_ms", C.c_double)]
+
+
 RESULT_DTYPE = np.dtype([("query_id", "<u8"), ("target_id", "<u8"), ("query_size", "<u8"), ("qstart", "<u8"),
                          ("tstart", "<u8"), ("len", "<u8"), ("reverse", "u1"), ("pad_", "u1", (7,)),
                          ("prob", "<f8"), ("ident", "<f8")])
@@ -62,8 +93,9 @@ assert RESULT_DTYPE.itemsize == C.sizeof(Result) == 72
 
 EXPORTS = ["sxc_create", "sxc_destroy", "sxc_last_error", "sxc_version", "sxc_encode_signal", "sxc_fft_forward",
            "sxc_xcorr", "sxc_matchup", "sxc_match_probability", "sxc_align_pair", "sxc_align_batch",
-           "sxc_batch_create", "sxc_batch_run", "sxc_batch_results", "sxc_batch_stats", "sxc_batch_destroy",
-           "sxc_last_timing"]
+           "sxc_batch_create", "sxc_batch_create_tails", "sxc_batch_run", "sxc_batch_results", "sxc_batch_stats",
+           "sxc_batch_destroy", "sxc_last_timing", "sxc_server_create", "sxc_server_submit", "sxc_server_wait",
+           "sxc_server_results", "sxc_server_stats", "sxc_server_destroy"]
 
 _lib = None
 _lock = threading.Lock()
@@ -103,6 +135,16 @@ def load() -> C.CDLL:
         L.sxc_batch_destroy.argtypes = [vp]
         L.sxc_batch_destroy.restype = None
         L.sxc_last_timing.argtypes = [vp, C.POINTER(Timing)]
+        L.sxc_batch_create_tails.argtypes = [vp, C.POINTER(Pair), i32, C.POINTER(Params), C.POINTER(SignalTail),
+                                             C.POINTER(SignalTail), C.POINTER(vp)]
+        L.sxc_server_create.argtypes = [vp, C.POINTER(Params), C.POINTER(ChunkDesc), i32, C.POINTER(ChunkDesc), i32,
+                                        C.POINTER(ServerConfig), C.POINTER(vp)]
+        L.sxc_server_submit.argtypes = [vp, C.POINTER(Block), i32, C.POINTER(C.c_uint64)]
+        L.sxc_server_wait.argtypes = [vp, C.c_uint64, i32, C.POINTER(i64), dp]
+        L.sxc_server_results.argtypes = [vp, C.c_uint64, C.POINTER(Result), i64, C.POINTER(i64)]
+        L.sxc_server_stats.argtypes = [vp, C.POINTER(ServerInfo)]
+        L.sxc_server_destroy.argtypes = [vp]
+        L.sxc_server_destroy.restype = None
         _lib = L
         return L
 

@@ -229,6 +229,79 @@ class ResidentBatch:
             pass
 
 
+@dataclass
+class Chunk:
+    """One chunk of a resident genome (ChunkManager, analysis/SeqChunk.cc:72-165)."""
+
+    seq: bytes
+    start: int = 0
+    record: int = 0
+    record_len: int = -1
+
+
+class Server:
+    """The serving loop (sxc_server_*): genomes resident in HBM, requests of t_pair blocks
+    (target_from, target_to, query_from, query_to, fast) taken as they arrive."""
+
+    def __init__(self, ctx: Context, params: SlaveParams, targets: Sequence[Chunk], queries: Sequence[Chunk],
+                 hbm_budget: int = 0, max_pairs_per_launch: int = 0, max_slots: int = 0):
+        self.ctx = ctx
+        self._keep = [c.seq for c in targets] + [c.seq for c in queries]
+
+        def descs(chunks):
+            arr = (_lib.ChunkDesc * max(1, len(chunks)))()
+            for k, c in enumerate(chunks):
+                arr[k] = _lib.ChunkDesc(c.seq, len(c.seq), c.start, c.record,
+                                        c.record_len if c.record_len >= 0 else len(c.seq))
+            return arr
+
+        self._t, self._q = descs(targets), descs(queries)
+        self._prm = params.to_c()
+        cfg = _lib.ServerConfig(hbm_budget, max_pairs_per_launch, max_slots)
+        self._h = C.c_void_p()
+        check(ctx.L.sxc_server_create(ctx.handle, C.byref(self._prm), self._t, len(targets), self._q, len(queries),
+                                      C.byref(cfg), C.byref(self._h)))
+
+    def submit(self, blocks) -> int:
+        arr = (_lib.Block * max(1, len(blocks)))(*[_lib.Block(*[int(x) for x in b]) for b in blocks])
+        t = C.c_uint64(0)
+        check(self.ctx.L.sxc_server_submit(self._h, arr, len(blocks), C.byref(t)))
+        return t.value
+
+    def wait(self, ticket: int, timeout_ms: int = -1):
+        """(number of records, latency in ms), or None if not done within timeout_ms."""
+        n, lat = C.c_int64(0), C.c_double(0)
+        rc = self.ctx.L.sxc_server_wait(self._h, ticket, timeout_ms, C.byref(n), C.byref(lat))
+        if rc == _lib.SXC_PENDING:
+            return None
+        check(rc)
+        return n.value, lat.value
+
+    def results(self, ticket: int) -> np.ndarray:
+        n, _ = self.wait(ticket)
+        out = np.zeros(max(1, n), RESULT_DTYPE)
+        got = C.c_int64(0)
+        check(self.ctx.L.sxc_server_results(self._h, ticket, out.ctypes.data_as(C.POINTER(Result)), len(out),
+                                            C.byref(got)))
+        return out[: got.value].copy()
+
+    def stats(self) -> dict:
+        i = _lib.ServerInfo()
+        check(self.ctx.L.sxc_server_stats(self._h, C.byref(i)))
+        return {f: getattr(i, f) for f, _ in _lib.ServerInfo._fields_}
+
+    def close(self):
+        if self._h:
+            self.ctx.L.sxc_server_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 # ---------------------------------------------------------------- reference-named mirror
 
 _default_ctx = None

@@ -8,6 +8,14 @@
 // the same output file (MultiMatches::Write, version 3, analysis/SequenceMatch.cc:320-360).
 // Every chunk pair runs in batched launches through libsxc_hip.so.
 //
+// The reference reuses one CCSignal object for every target chunk and one for every query
+// chunk (:706-716, 780-798), and CCSignal::SetSequence writes only the first L positions
+// of its N-long channels (analysis/CrossCorr.cc:118-131, 182-206).  A chunk shorter than
+// an earlier one therefore correlates with that earlier chunk's signal in its tail.  This
+// program replays the order of SetSequence calls on a stack per object to name, for every
+// signal, the earlier sequences whose values sit in its tail, and hands them to the library
+// as signal tails (sxc_batch_create_tails), which encodes them on the GPU.
+//
 // The reference's optional modes that change what is aligned or how results are post-
 // processed (-guide, -proteins, -chain, -select, -pairs, -line) are not part of this
 // program; naming one is an error.
@@ -15,7 +23,10 @@
 #include <string.h>
 
 #include <algorithm>
+#include <deque>
+#include <map>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "../../../include/sxc.h"
@@ -45,6 +56,41 @@ void restrict_blocks(ChunkSet* cs, int n_blocks, int my_block) {
   for (int k = 0; k < n; ++k)
     if (k < first || k >= last) cs->chunks[(size_t)k].len = 0;
 }
+
+// Replays SetSequence calls on one reused CCSignal.  The stack holds the sequences whose
+// values are still visible, with strictly decreasing lengths from bottom to top: entry e
+// owns positions [length of the entry above it (or of the newest call), e.len).  A call of
+// length L hides every entry of length <= L; the rest form its tail, top to bottom.
+class SignalHistory {
+ public:
+  struct Src {
+    int32_t len;
+    const char* seq;
+    int32_t strand;
+    bool operator==(const Src& o) const { return len == o.len && seq == o.seq && strand == o.strand; }
+    bool operator<(const Src& o) const { return std::tie(len, seq, strand) < std::tie(o.len, o.seq, o.strand); }
+  };
+  // Records the call and returns its tail (segments interned in `pool`).
+  sxc_signal_tail call(const Src& s, std::map<std::vector<Src>, std::vector<sxc_tail_seg>*>& pool,
+                       std::deque<std::vector<sxc_tail_seg>>& store) {
+    while (!stack_.empty() && stack_.back().len <= s.len) stack_.pop_back();
+    std::vector<Src> below(stack_.rbegin(), stack_.rend());
+    stack_.push_back(s);
+    if (below.empty()) return sxc_signal_tail{nullptr, 0, 0};
+    auto it = pool.find(below);
+    if (it == pool.end()) {
+      store.emplace_back();
+      for (const Src& b : below) store.back().push_back(sxc_tail_seg{b.len, b.len, b.seq, b.strand, 0});
+      it = pool.emplace(below, &store.back()).first;
+    }
+    return sxc_signal_tail{it->second->data(), (int32_t)it->second->size(), 0};
+  }
+  const std::vector<Src>& state() const { return stack_; }
+  void set_state(const std::vector<Src>& s) { stack_ = s; }
+
+ private:
+  std::vector<Src> stack_;
+};
 
 // MultiMatches::Write, version 3 (analysis/SequenceMatch.cc:320-360, SingleMatch::Write
 // :49-64; CMWriteFileStream writes raw host-order values and strings as int64 length
@@ -158,12 +204,44 @@ int main(int argc, char** argv) {
   printf("Done chunking.\nKeeping alignments more like real than %g\n", min_prob);
 
   // Chunk pairs in the reference's loop order (:712-800); empty chunks are skipped there.
+  // The query object's inner loop is the same for every target chunk of one record name,
+  // so its tails are computed once per (starting state, name) and reused.
   std::vector<sxc_pair> pairs;
+  std::vector<sxc_signal_tail> t_tails, q_tails;
+  std::map<std::vector<SignalHistory::Src>, std::vector<sxc_tail_seg>*> pool;
+  std::deque<std::vector<sxc_tail_seg>> store;
+  SignalHistory thist, qhist;
+  struct Loop {
+    std::vector<sxc_signal_tail> tails;  // per query chunk index (nseg = 0 where skipped)
+    std::vector<SignalHistory::Src> end_state;
+  };
+  std::map<std::pair<std::vector<SignalHistory::Src>, std::string>, Loop> loops;
   for (const sxc_host::Chunk& t : target.chunks) {
     if (t.len == 0) continue;
-    for (const sxc_host::Chunk& q : query.chunks) {
+    const sxc_signal_tail ttail = thist.call({t.len, t.data, 0}, pool, store);
+    const std::string& tname = traw.names[(size_t)t.record];
+    const auto key = std::make_pair(qhist.state(), same_only ? tname : std::string());
+    auto li = loops.find(key);
+    if (li == loops.end()) {
+      Loop lp;
+      lp.tails.assign(query.chunks.size(), sxc_signal_tail{nullptr, 0, 0});
+      for (size_t qi = 0; qi < query.chunks.size(); ++qi) {
+        const sxc_host::Chunk& q = query.chunks[qi];
+        if (q.len == 0) continue;
+        if (same_only && tname != qraw.names[(size_t)q.record]) continue;
+        lp.tails[qi] = qhist.call({q.len, q.data, 0}, pool, store);  // forward (:780-782)
+        qhist.call({q.len, q.data, 1}, pool, store);                  // reverse complement (:786-792)
+      }
+      lp.end_state = qhist.state();
+      li = loops.emplace(key, std::move(lp)).first;
+    }
+    qhist.set_state(li->second.end_state);
+    for (size_t qi = 0; qi < query.chunks.size(); ++qi) {
+      const sxc_host::Chunk& q = query.chunks[qi];
       if (q.len == 0) continue;
-      if (same_only && traw.names[(size_t)t.record] != qraw.names[(size_t)q.record]) continue;
+      if (same_only && tname != qraw.names[(size_t)q.record]) continue;
+      t_tails.push_back(ttail);
+      q_tails.push_back(li->second.tails[qi]);
       sxc_pair s;
       s.tseq = t.data;
       s.tlen = t.len;
@@ -198,7 +276,7 @@ int main(int argc, char** argv) {
   for (size_t at = 0; at < pairs.size(); at += kMaxPairsPerLaunch) {
     const int n = (int)std::min(pairs.size() - at, (size_t)kMaxPairsPerLaunch);
     sxc_batch* b = nullptr;
-    int rc = sxc_batch_create(ctx, pairs.data() + at, n, &prm, &b);
+    int rc = sxc_batch_create_tails(ctx, pairs.data() + at, n, &prm, t_tails.data() + at, q_tails.data() + at, &b);
     if (rc == SXC_OK) rc = sxc_batch_run(b);
     int64_t got = 0;
     std::vector<sxc_result> r;

@@ -7,92 +7,34 @@
 #include <string.h>
 
 #include <algorithm>
+#include <chrono>
+#include <map>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <vector>
 
 #include "../../include/sxc.h"
+#include "sxc_api_internal.h"
 #include "sxc_batch_host.h"
 #include "sxc_kernels.h"
 #include "sxc_tables.h"
 
-namespace {
-
+namespace sxc {
 thread_local std::string g_err;
-
 int fail(int code, const std::string& msg) {
   g_err = msg;
   return code;
 }
+}  // namespace sxc
 
-#define HIPCHK(expr)                                                                          \
-  do {                                                                                        \
-    hipError_t e_ = (expr);                                                                   \
-    if (e_ != hipSuccess) return fail(SXC_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
-  } while (0)
-
-struct DevBuf {
-  void* p = nullptr;
-  size_t cap = 0;
-  hipError_t ensure(size_t bytes) {
-    if (bytes <= cap && p) return hipSuccess;
-    if (p) {
-      hipError_t e = hipFree(p);
-      if (e != hipSuccess) return e;
-      p = nullptr;
-      cap = 0;
-    }
-    size_t want = std::max<size_t>(bytes, 256);
-    want = (want + 4095) & ~(size_t)4095;
-    hipError_t e = hipMalloc(&p, want);
-    if (e == hipSuccess) cap = want;
-    return e;
-  }
-  void release() {
-    if (p) (void)hipFree(p);
-    p = nullptr;
-    cap = 0;
-  }
-  template <class T>
-  T* as() const {
-    return (T*)p;
-  }
-};
-
-}  // namespace
-
-struct sxc_batch_dev {
-  DevBuf seq, tch, qch, pairs, spectra, which, slots, keys, counts, offsets, total, maxc, final_, stats, cands, ncand;
-  sxc::HostBatch hb;
-  sxc_params prm{};
-  int nb = 0;
-  int slot_cap = 64;
-  int64_t n_results = 0;
-  void release() {
-    for (DevBuf* b : {&seq, &tch, &qch, &pairs, &spectra, &which, &slots, &keys, &counts, &offsets, &total, &maxc,
-                      &final_, &stats, &cands, &ncand})
-      b->release();
-  }
-};
-
-struct sxc_ctx {
-  int device = 0;
-  hipStream_t stream = nullptr;
-  sxc_tables* d_tab = nullptr;
-  float* d_osc = nullptr;
-  double* d_ent[16] = {};
-  hipEvent_t ev[8] = {};
-  sxc_timing last{};
-  std::mutex mu;
-  sxc_batch_dev scratch;  // buffers reused by the one-shot entry points
-  DevBuf p0, p1, p2, p3, p4, p5, p6;  // plain-API staging
-};
+using sxc::fail;
 
 namespace {
 
 int check_N(int N, int* nb) {
   *nb = sxc::nb_of(N);
-  if (*nb < SXC_MIN_NB || *nb > 14) return fail(SXC_ERR_ARG, "N must be a power of two in [512, 16384]");
+  if (*nb < SXC_MIN_NB || *nb > SXC_MAX_NB) return fail(SXC_ERR_ARG, "N must be a power of two in [512, 32768]");
   return SXC_OK;
 }
 
@@ -128,6 +70,19 @@ int batch_upload(sxc_ctx* ctx, sxc_batch_dev& b) {
   std::vector<int32_t> idx(hb.tchunks.size());
   for (size_t i = 0; i < idx.size(); ++i) idx[i] = (int32_t)i;
   if (!idx.empty()) HIPCHK(hipMemcpyAsync(b.which.p, idx.data(), idx.size() * sizeof(int32_t), hipMemcpyHostToDevice, s));
+  if (!hb.tails.empty()) {
+    HIPCHK(b.sch.ensure(hb.schunks.size() * sizeof(sxc_chunk)));
+    HIPCHK(b.sstr.ensure(hb.sstrand.size() * sizeof(int32_t)));
+    HIPCHK(b.tails.ensure(hb.tails.size() * sizeof(sxc_tail_dev)));
+    HIPCHK(b.segs.ensure(hb.segs.size() * sizeof(sxc_tail_seg_dev)));
+    HIPCHK(b.ttail.ensure(hb.ttail.size() * sizeof(int32_t)));
+    HIPCHK(b.src_sig.ensure(hb.schunks.size() * 4 * (size_t)N * sizeof(float)));
+    HIPCHK(hipMemcpyAsync(b.sch.p, hb.schunks.data(), hb.schunks.size() * sizeof(sxc_chunk), hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(b.sstr.p, hb.sstrand.data(), hb.sstrand.size() * sizeof(int32_t), hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(b.tails.p, hb.tails.data(), hb.tails.size() * sizeof(sxc_tail_dev), hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(b.segs.p, hb.segs.data(), hb.segs.size() * sizeof(sxc_tail_seg_dev), hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(b.ttail.p, hb.ttail.data(), hb.ttail.size() * sizeof(int32_t), hipMemcpyHostToDevice, s));
+  }
   HIPCHK(hipStreamSynchronize(s));  // idx is a host temporary
   return SXC_OK;
 }
@@ -152,6 +107,11 @@ int launch_correlate(sxc_ctx* ctx, sxc_batch_dev& b) {
   a.cands = b.cands.as<uint16_t>();
   a.ncand = b.ncand.as<int32_t>();
   a.xc_out = nullptr;
+  if (!b.hb.tails.empty()) {
+    a.tails = b.tails.as<sxc_tail_dev>();
+    a.segs = b.segs.as<sxc_tail_seg_dev>();
+    a.src_sig = b.src_sig.as<float>();
+  }
   HIPCHK(sxc::launch_correlate(b.nb, a, total, ctx->stream));
   return SXC_OK;
 }
@@ -193,6 +153,12 @@ int batch_run(sxc_ctx* ctx, sxc_batch_dev& b) {
   hipStream_t s = ctx->stream;
   const int total = (int)b.hb.pairs.size() * b.prm.strands;
   HIPCHK(hipEventRecord(ctx->ev[1], s));
+  if (!b.hb.schunks.empty()) {
+    // signals of the sequences named by signal tails
+    sxc::EncodeArgs ea{ctx->d_tab, ctx->d_ent[b.nb], b.seq.as<unsigned char>(), b.sch.as<sxc_chunk>(), 0,
+                       b.sstr.as<int32_t>(), b.src_sig.as<float>()};
+    HIPCHK(sxc::launch_encode(b.nb, ea, (int)b.hb.schunks.size(), s));
+  }
   if (!b.hb.tchunks.empty()) {
     sxc::SpecArgs sa{};
     sa.tab = ctx->d_tab;
@@ -205,6 +171,12 @@ int batch_run(sxc_ctx* ctx, sxc_batch_dev& b) {
     sa.spectra = b.spectra.as<float>();
     sa.dst = b.which.as<int32_t>();
     sa.signals = nullptr;
+    if (!b.hb.tails.empty()) {
+      sa.chunk_tail = b.ttail.as<int32_t>();
+      sa.tails = b.tails.as<sxc_tail_dev>();
+      sa.segs = b.segs.as<sxc_tail_seg_dev>();
+      sa.src_sig = b.src_sig.as<float>();
+    }
     HIPCHK(sxc::launch_spectra(b.nb, sa, s));
   }
   HIPCHK(hipEventRecord(ctx->ev[2], s));
@@ -214,6 +186,8 @@ int batch_run(sxc_ctx* ctx, sxc_batch_dev& b) {
     HIPCHK(hipEventRecord(ctx->ev[3], s));
     HIPCHK(hipEventRecord(ctx->ev[4], s));
     HIPCHK(hipEventSynchronize(ctx->ev[4]));
+    b.spectra_ms = elapsed(ctx->ev[1], ctx->ev[2]);
+    b.correlate_ms = b.scan_ms = b.finalize_ms = 0;
     return SXC_OK;
   }
   int rc = launch_correlate(ctx, b);
@@ -249,6 +223,10 @@ int batch_run(sxc_ctx* ctx, sxc_batch_dev& b) {
   HIPCHK(hipEventRecord(ctx->ev[4], s));
   HIPCHK(hipEventSynchronize(ctx->ev[4]));
   b.n_results = tot;
+  b.spectra_ms = elapsed(ctx->ev[1], ctx->ev[2]);
+  b.correlate_ms = elapsed(ctx->ev[2], ctx->ev[6]);
+  b.scan_ms = elapsed(ctx->ev[6], ctx->ev[3]);
+  b.finalize_ms = elapsed(ctx->ev[3], ctx->ev[4]);
   return SXC_OK;
 }
 
@@ -261,17 +239,152 @@ int batch_download(sxc_ctx* ctx, sxc_batch_dev& b, sxc_result* out, int64_t cap,
   return SXC_OK;
 }
 
-int batch_prepare(sxc_batch_dev& b, const sxc_pair* pairs, int32_t n, const sxc_params* prm) {
+int batch_prepare(sxc_batch_dev& b, const sxc_pair* pairs, int32_t n, const sxc_params* prm,
+                  const sxc_signal_tail* t_tail = nullptr, const sxc_signal_tail* q_tail = nullptr) {
   if (!prm || n < 0 || (n > 0 && !pairs)) return fail(SXC_ERR_ARG, "null argument");
   if (prm->strands != 1 && prm->strands != 2) return fail(SXC_ERR_ARG, "strands must be 1 or 2");
   if (prm->q_chunk < 0) return fail(SXC_ERR_ARG, "q_chunk must be >= 0");
   int nb;
   int rc = check_N(prm->N, &nb);
   if (rc) return rc;
-  std::string e = sxc::pack_batch(pairs, n, prm->N, b.hb);
+  std::string e = sxc::pack_batch(pairs, n, prm->N, b.hb, t_tail, q_tail);
   if (!e.empty()) return fail(SXC_ERR_ARG, e);
   b.prm = *prm;
   b.nb = nb;
+  b.slot_cap = 64;  // records per (pair, strand) a run first makes room for; grows on overflow
+  return SXC_OK;
+}
+
+// ---------------------------------------------------------------- length classes
+//
+// With prm->N == 0 every pair gets its own FFT length, the next power of two >= tlen + qlen
+// (at least 512), and a batch is one sub-batch per length class, run class after class on
+// the context's stream; records are merged back into input order.  With prm->N > 0 there is
+// one class and no merge.
+
+struct Group {
+  int nb = 0;
+  std::vector<int32_t> idx;  // input pair index of each entry (empty: the identity)
+  sxc_batch_dev* b = nullptr;
+};
+
+int auto_nb(const sxc_pair& p) {
+  int nb = SXC_MIN_NB;
+  while ((1 << nb) < p.tlen + p.qlen && nb < 30) ++nb;
+  return nb;
+}
+
+int group_pairs(const sxc_pair* pairs, int32_t n, const sxc_params* prm, std::vector<Group>& gs) {
+  gs.clear();
+  if (!prm || n < 0 || (n > 0 && !pairs)) return fail(SXC_ERR_ARG, "null argument");
+  if (prm->N != 0) {
+    gs.resize(1);
+    gs[0].nb = sxc::nb_of(prm->N);
+    return SXC_OK;
+  }
+  std::map<int, std::vector<int32_t>> by;
+  for (int32_t k = 0; k < n; ++k) {
+    if (pairs[k].tlen < 0 || pairs[k].qlen < 0) return fail(SXC_ERR_ARG, "negative length");
+    const int nb = auto_nb(pairs[k]);
+    if (nb > SXC_MAX_NB)
+      return fail(SXC_ERR_ARG, "pair " + std::to_string(k) + " needs N = 2^" + std::to_string(nb) +
+                                   "; FFT lengths above 32768 are not supported");
+    by[nb].push_back(k);
+  }
+  for (auto& kv : by) {
+    Group g;
+    g.nb = kv.first;
+    g.idx = std::move(kv.second);
+    gs.push_back(std::move(g));
+  }
+  return SXC_OK;
+}
+
+// Prepares every group's sub-batch (its pairs and tails gathered in input order).
+int groups_prepare(std::vector<Group>& gs, const sxc_pair* pairs, int32_t n, const sxc_params* prm,
+                   const sxc_signal_tail* t_tail, const sxc_signal_tail* q_tail) {
+  for (Group& g : gs) {
+    if (g.idx.empty() && gs.size() == 1 && prm->N != 0) {
+      const int rc = batch_prepare(*g.b, pairs, n, prm, t_tail, q_tail);
+      if (rc) return rc;
+      continue;
+    }
+    std::vector<sxc_pair> sub(g.idx.size());
+    std::vector<sxc_signal_tail> st, sq;
+    for (size_t i = 0; i < g.idx.size(); ++i) sub[i] = pairs[g.idx[i]];
+    if (t_tail) {
+      st.resize(g.idx.size());
+      for (size_t i = 0; i < g.idx.size(); ++i) st[i] = t_tail[g.idx[i]];
+    }
+    if (q_tail) {
+      sq.resize(g.idx.size());
+      for (size_t i = 0; i < g.idx.size(); ++i) sq[i] = q_tail[g.idx[i]];
+    }
+    sxc_params p = *prm;
+    p.N = 1 << g.nb;
+    const int rc = batch_prepare(*g.b, sub.data(), (int32_t)sub.size(), &p, t_tail ? st.data() : nullptr,
+                                 q_tail ? sq.data() : nullptr);
+    if (rc) return rc;
+  }
+  return SXC_OK;
+}
+
+int groups_run(sxc_ctx* ctx, std::vector<Group>& gs, sxc_timing& t) {
+  t.spectra_ms = t.correlate_ms = t.scan_ms = t.finalize_ms = 0;
+  for (Group& g : gs) {
+    const int rc = batch_run(ctx, *g.b);
+    if (rc) return rc;
+    t.spectra_ms += g.b->spectra_ms;
+    t.correlate_ms += g.b->correlate_ms;
+    t.scan_ms += g.b->scan_ms;
+    t.finalize_ms += g.b->finalize_ms;
+  }
+  t.pair_ms = t.correlate_ms + t.scan_ms;
+  return SXC_OK;
+}
+
+int64_t groups_results(const std::vector<Group>& gs) {
+  int64_t n = 0;
+  for (const Group& g : gs) n += g.b->n_results;
+  return n;
+}
+
+// Records of all groups in input pair order.
+int groups_download(sxc_ctx* ctx, std::vector<Group>& gs, int32_t n, sxc_result* out, int64_t cap, int64_t* n_out) {
+  if (gs.size() == 1 && gs[0].idx.empty()) return batch_download(ctx, *gs[0].b, out, cap, n_out);
+  *n_out = groups_results(gs);
+  if (*n_out > cap) return fail(SXC_ERR_CAPACITY, "output capacity too small");
+  hipStream_t s = ctx->stream;
+  std::vector<std::vector<sxc_result>> recs(gs.size());
+  std::vector<std::vector<int64_t>> offs(gs.size());
+  std::vector<int32_t> where((size_t)n * 2);  // (group, entry) of every input pair
+  for (size_t gi = 0; gi < gs.size(); ++gi) {
+    sxc_batch_dev& b = *gs[gi].b;
+    const int total = (int)b.hb.pairs.size() * b.prm.strands;
+    recs[gi].resize((size_t)b.n_results);
+    offs[gi].resize((size_t)total + 1);
+    if (b.n_results > 0)
+      HIPCHK(hipMemcpyAsync(recs[gi].data(), b.final_.p, (size_t)b.n_results * sizeof(sxc_result),
+                            hipMemcpyDeviceToHost, s));
+    if (total > 0 && b.n_results > 0)
+      HIPCHK(hipMemcpyAsync(offs[gi].data(), b.offsets.p, (size_t)total * sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    else
+      std::fill(offs[gi].begin(), offs[gi].end(), 0);
+    offs[gi][(size_t)total] = b.n_results;
+    for (size_t i = 0; i < gs[gi].idx.size(); ++i) {
+      where[2 * (size_t)gs[gi].idx[i]] = (int32_t)gi;
+      where[2 * (size_t)gs[gi].idx[i] + 1] = (int32_t)i;
+    }
+  }
+  HIPCHK(hipStreamSynchronize(s));
+  int64_t at = 0;
+  for (int32_t k = 0; k < n; ++k) {
+    const int gi = where[2 * (size_t)k], i = where[2 * (size_t)k + 1];
+    const int strands = gs[(size_t)gi].b->prm.strands;
+    const int64_t b0 = offs[(size_t)gi][(size_t)i * strands], b1 = offs[(size_t)gi][(size_t)(i + 1) * strands];
+    if (b1 > b0) memcpy(out + at, recs[(size_t)gi].data() + b0, (size_t)(b1 - b0) * sizeof(sxc_result));
+    at += b1 - b0;
+  }
   return SXC_OK;
 }
 
@@ -279,7 +392,7 @@ int batch_prepare(sxc_batch_dev& b, const sxc_pair* pairs, int32_t n, const sxc_
 
 extern "C" {
 
-const char* sxc_last_error(void) { return g_err.c_str(); }
+const char* sxc_last_error(void) { return sxc::g_err.c_str(); }
 const char* sxc_version(void) { return "sxc 0.1 gfx950"; }
 
 int sxc_create(sxc_ctx** out, int device) {
@@ -317,6 +430,7 @@ void sxc_destroy(sxc_ctx* c) {
   (void)hipSetDevice(c->device);
   (void)hipStreamSynchronize(c->stream);
   c->scratch.release();
+  for (sxc_batch_dev& b : c->scratch_nb) b.release();
   for (DevBuf* b : {&c->p0, &c->p1, &c->p2, &c->p3, &c->p4, &c->p5, &c->p6}) b->release();
   (void)hipFree(c->d_tab);
   (void)hipFree(c->d_osc);
@@ -334,25 +448,31 @@ int sxc_align_batch(sxc_ctx* ctx, const sxc_pair* pairs, int32_t n, const sxc_pa
   std::lock_guard<std::mutex> g(ctx->mu);
   int rc = set_device(ctx);
   if (rc) return rc;
-  sxc_batch_dev& b = ctx->scratch;
-  rc = batch_prepare(b, pairs, n, prm);
+  std::vector<Group> gs;
+  rc = group_pairs(pairs, n, prm, gs);
   if (rc) return rc;
-  HIPCHK(hipEventRecord(ctx->ev[0], ctx->stream));
-  rc = batch_upload(ctx, b);
+  for (Group& gr : gs) gr.b = (prm->N != 0) ? &ctx->scratch : &ctx->scratch_nb[gr.nb];
+  rc = groups_prepare(gs, pairs, n, prm, nullptr, nullptr);
   if (rc) return rc;
-  rc = batch_run(ctx, b);
+  const auto t0 = std::chrono::steady_clock::now();
+  for (Group& gr : gs) {
+    rc = batch_upload(ctx, *gr.b);
+    if (rc) return rc;
+  }
+  const auto t1 = std::chrono::steady_clock::now();
+  sxc_timing t{};
+  rc = groups_run(ctx, gs, t);
   if (rc) return rc;
-  rc = batch_download(ctx, b, out, cap, n_out);
-  HIPCHK(hipEventRecord(ctx->ev[5], ctx->stream));
-  HIPCHK(hipEventSynchronize(ctx->ev[5]));
-  ctx->last.total_ms = elapsed(ctx->ev[0], ctx->ev[5]);
-  ctx->last.upload_ms = elapsed(ctx->ev[0], ctx->ev[1]);
-  ctx->last.spectra_ms = elapsed(ctx->ev[1], ctx->ev[2]);
-  ctx->last.pair_ms = elapsed(ctx->ev[2], ctx->ev[3]);
-  ctx->last.correlate_ms = elapsed(ctx->ev[2], ctx->ev[6]);
-  ctx->last.scan_ms = elapsed(ctx->ev[6], ctx->ev[3]);
-  ctx->last.finalize_ms = elapsed(ctx->ev[3], ctx->ev[4]);
-  ctx->last.download_ms = elapsed(ctx->ev[4], ctx->ev[5]);
+  const auto t2 = std::chrono::steady_clock::now();
+  rc = groups_download(ctx, gs, n, out, cap, n_out);
+  const auto t3 = std::chrono::steady_clock::now();
+  auto ms = [](std::chrono::steady_clock::time_point a, std::chrono::steady_clock::time_point b) {
+    return std::chrono::duration<double, std::milli>(b - a).count();
+  };
+  t.upload_ms = ms(t0, t1);
+  t.download_ms = ms(t2, t3);
+  t.total_ms = ms(t0, t3);
+  ctx->last = t;
   return rc;
 }
 
@@ -375,10 +495,20 @@ int sxc_last_timing(sxc_ctx* ctx, sxc_timing* out) {
 
 typedef struct sxc_batch {
   sxc_ctx* ctx;
-  sxc_batch_dev b;
+  int32_t n = 0;
+  std::vector<Group> groups;
+  std::vector<std::unique_ptr<sxc_batch_dev>> owned;
+  void release() {
+    for (auto& b : owned) b->release();
+  }
 } sxc_batch;
 
 int sxc_batch_create(sxc_ctx* ctx, const sxc_pair* pairs, int32_t n, const sxc_params* prm, sxc_batch** out) {
+  return sxc_batch_create_tails(ctx, pairs, n, prm, nullptr, nullptr, out);
+}
+
+int sxc_batch_create_tails(sxc_ctx* ctx, const sxc_pair* pairs, int32_t n, const sxc_params* prm,
+                           const sxc_signal_tail* t_tail, const sxc_signal_tail* q_tail, sxc_batch** out) {
   if (!ctx || !out) return fail(SXC_ERR_ARG, "null argument");
   *out = nullptr;
   std::lock_guard<std::mutex> g(ctx->mu);
@@ -386,10 +516,18 @@ int sxc_batch_create(sxc_ctx* ctx, const sxc_pair* pairs, int32_t n, const sxc_p
   if (rc) return rc;
   sxc_batch* bt = new sxc_batch();
   bt->ctx = ctx;
-  rc = batch_prepare(bt->b, pairs, n, prm);
-  if (!rc) rc = batch_upload(ctx, bt->b);
+  bt->n = n;
+  rc = group_pairs(pairs, n, prm, bt->groups);
+  if (!rc) {
+    for (Group& gr : bt->groups) {
+      bt->owned.push_back(std::make_unique<sxc_batch_dev>());
+      gr.b = bt->owned.back().get();
+    }
+    rc = groups_prepare(bt->groups, pairs, n, prm, t_tail, q_tail);
+  }
+  for (size_t i = 0; !rc && i < bt->groups.size(); ++i) rc = batch_upload(ctx, *bt->groups[i].b);
   if (rc) {
-    bt->b.release();
+    bt->release();
     delete bt;
     return rc;
   }
@@ -402,18 +540,10 @@ int sxc_batch_run(sxc_batch* bt) {
   std::lock_guard<std::mutex> g(bt->ctx->mu);
   int rc = set_device(bt->ctx);
   if (rc) return rc;
-  HIPCHK(hipEventRecord(bt->ctx->ev[0], bt->ctx->stream));
-  HIPCHK(hipEventRecord(bt->ctx->ev[1], bt->ctx->stream));
-  rc = batch_run(bt->ctx, bt->b);
-  sxc_ctx* c = bt->ctx;
-  c->last.upload_ms = 0;
-  c->last.spectra_ms = elapsed(c->ev[1], c->ev[2]);
-  c->last.pair_ms = elapsed(c->ev[2], c->ev[3]);
-  c->last.correlate_ms = elapsed(c->ev[2], c->ev[6]);
-  c->last.scan_ms = elapsed(c->ev[6], c->ev[3]);
-  c->last.finalize_ms = elapsed(c->ev[3], c->ev[4]);
-  c->last.download_ms = 0;
-  c->last.total_ms = elapsed(c->ev[1], c->ev[4]);
+  sxc_timing t{};
+  rc = groups_run(bt->ctx, bt->groups, t);
+  t.total_ms = t.spectra_ms + t.pair_ms + t.finalize_ms;
+  bt->ctx->last = t;
   return rc;
 }
 
@@ -422,22 +552,25 @@ int sxc_batch_results(sxc_batch* bt, sxc_result* out, int64_t cap, int64_t* n_ou
   std::lock_guard<std::mutex> g(bt->ctx->mu);
   int rc = set_device(bt->ctx);
   if (rc) return rc;
-  return batch_download(bt->ctx, bt->b, out, cap, n_out);
+  return groups_download(bt->ctx, bt->groups, bt->n, out, cap, n_out);
 }
 
 int sxc_batch_stats(sxc_batch* bt, int64_t* candidates, int64_t* runs) {
   if (!bt || !candidates || !runs) return fail(SXC_ERR_ARG, "null argument");
   std::lock_guard<std::mutex> g(bt->ctx->mu);
-  const int total = (int)bt->b.hb.pairs.size() * bt->b.prm.strands;
-  std::vector<int32_t> st((size_t)total * 2);
-  if (total > 0) {
-    HIPCHK(hipMemcpyAsync(st.data(), bt->b.stats.p, st.size() * sizeof(int32_t), hipMemcpyDeviceToHost, bt->ctx->stream));
-    HIPCHK(hipStreamSynchronize(bt->ctx->stream));
-  }
   int64_t c = 0, r = 0;
-  for (int i = 0; i < total; ++i) {
-    c += st[2 * i];
-    r += st[2 * i + 1];
+  for (const Group& gr : bt->groups) {
+    const int total = (int)gr.b->hb.pairs.size() * gr.b->prm.strands;
+    std::vector<int32_t> st((size_t)total * 2);
+    if (total > 0) {
+      HIPCHK(hipMemcpyAsync(st.data(), gr.b->stats.p, st.size() * sizeof(int32_t), hipMemcpyDeviceToHost,
+                            bt->ctx->stream));
+      HIPCHK(hipStreamSynchronize(bt->ctx->stream));
+    }
+    for (int i = 0; i < total; ++i) {
+      c += st[2 * i];
+      r += st[2 * i + 1];
+    }
   }
   *candidates = c;
   *runs = r;
@@ -450,7 +583,7 @@ void sxc_batch_destroy(sxc_batch* bt) {
     std::lock_guard<std::mutex> g(bt->ctx->mu);
     (void)hipSetDevice(bt->ctx->device);
     (void)hipStreamSynchronize(bt->ctx->stream);
-    bt->b.release();
+    bt->release();
   }
   delete bt;
 }
@@ -478,7 +611,7 @@ int sxc_encode_signal(sxc_ctx* ctx, const char* seq, int32_t len, int32_t N, flo
   HIPCHK(ctx->p2.ensure((size_t)4 * N * sizeof(float)));
   if (len) HIPCHK(hipMemcpyAsync(ctx->p0.p, seq, len, hipMemcpyHostToDevice, s));
   HIPCHK(hipMemcpyAsync(ctx->p1.p, &ck, sizeof ck, hipMemcpyHostToDevice, s));
-  sxc::EncodeArgs a{ctx->d_tab, ctx->d_ent[nb], ctx->p0.as<unsigned char>(), ctx->p1.as<sxc_chunk>(), 0,
+  sxc::EncodeArgs a{ctx->d_tab, ctx->d_ent[nb], ctx->p0.as<unsigned char>(), ctx->p1.as<sxc_chunk>(), 0, nullptr,
                     ctx->p2.as<float>()};
   HIPCHK(sxc::launch_encode(nb, a, 1, s));
   HIPCHK(hipMemcpyAsync(out, ctx->p2.p, (size_t)4 * N * sizeof(float), hipMemcpyDeviceToHost, s));

@@ -1,0 +1,86 @@
+// sxc_api_internal.h -- state shared by the C ABI translation units (sxc_api.hip,
+// sxc_server.hip): error reporting, device buffers, the context.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <mutex>
+#include <string>
+
+#include "../../include/sxc.h"
+#include "sxc_batch_host.h"
+#include "sxc_types.h"
+
+namespace sxc {
+
+extern thread_local std::string g_err;
+int fail(int code, const std::string& msg);
+
+#define HIPCHK(expr)                                                                                      \
+  do {                                                                                                    \
+    hipError_t e_ = (expr);                                                                               \
+    if (e_ != hipSuccess) return ::sxc::fail(SXC_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+// A device allocation that grows on demand (contents are not kept when it grows).
+struct DevBuf {
+  void* p = nullptr;
+  size_t cap = 0;
+  hipError_t ensure(size_t bytes) {
+    if (bytes <= cap && p) return hipSuccess;
+    if (p) {
+      hipError_t e = hipFree(p);
+      if (e != hipSuccess) return e;
+      p = nullptr;
+      cap = 0;
+    }
+    size_t want = std::max<size_t>(bytes, 256);
+    want = (want + 4095) & ~(size_t)4095;
+    hipError_t e = hipMalloc(&p, want);
+    if (e == hipSuccess) cap = want;
+    return e;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+  }
+  template <class T>
+  T* as() const {
+    return (T*)p;
+  }
+};
+
+}  // namespace sxc
+
+using sxc::DevBuf;
+
+struct sxc_batch_dev {
+  DevBuf seq, tch, qch, pairs, spectra, which, slots, keys, counts, offsets, total, maxc, final_, stats, cands, ncand;
+  DevBuf sch, sstr, tails, segs, ttail, src_sig;  // signal tails
+  sxc::HostBatch hb;
+  sxc_params prm{};
+  int nb = 0;
+  int slot_cap = 64;
+  int64_t n_results = 0;
+  double spectra_ms = 0, correlate_ms = 0, scan_ms = 0, finalize_ms = 0;  // last run
+  void release() {
+    for (DevBuf* b : {&seq, &tch, &qch, &pairs, &spectra, &which, &slots, &keys, &counts, &offsets, &total, &maxc,
+                      &final_, &stats, &cands, &ncand, &sch, &sstr, &tails, &segs, &ttail, &src_sig})
+      b->release();
+  }
+};
+
+struct sxc_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  sxc_tables* d_tab = nullptr;
+  float* d_osc = nullptr;
+  double* d_ent[16] = {};
+  hipEvent_t ev[8] = {};
+  sxc_timing last{};
+  std::mutex mu;
+  sxc_batch_dev scratch;          // buffers reused by the one-shot entry points
+  sxc_batch_dev scratch_nb[16];  // ... per length class when N is chosen per pair
+  DevBuf p0, p1, p2, p3, p4, p5, p6;  // plain-API staging
+};

@@ -13,6 +13,7 @@
 #pragma once
 #include <map>
 #include <string>
+#include <tuple>
 #include <utility>
 #include <vector>
 
@@ -27,23 +28,26 @@ struct HostBatch {
   std::vector<sxc_chunk> tchunks, qchunks;
   std::vector<sxc_pair_dev> pairs;
   int max_len = 0;
+  // signal tails (sxc_batch_create_tails): source sequences, their strands, the tail and
+  // segment tables, and the tail of each target chunk (-1: none)
+  std::vector<sxc_chunk> schunks;
+  std::vector<int32_t> sstrand;
+  std::vector<sxc_tail_dev> tails;
+  std::vector<sxc_tail_seg_dev> segs;
+  std::vector<int32_t> ttail;
 };
 
 // Returns an empty string on success, else the reason the input is rejected.
-inline std::string pack_batch(const sxc_pair* in, int n, int N, HostBatch& hb) {
+// t_tail / q_tail (optional, n entries each) are the signal tails of include/sxc.h.
+inline std::string pack_batch(const sxc_pair* in, int n, int N, HostBatch& hb,
+                              const sxc_signal_tail* t_tail = nullptr, const sxc_signal_tail* q_tail = nullptr) {
   hb = HostBatch();
-  std::map<std::pair<const char*, int>, int> tmap, qmap;
-  auto add = [&](const char* s, int len, std::vector<sxc_chunk>& tab,
-                 std::map<std::pair<const char*, int>, int>& m, int& idx) -> std::string {
+  using Key = std::tuple<const char*, int, int>;
+  std::map<Key, int> tmap, qmap, smap;
+  std::map<std::pair<const sxc_tail_seg*, int>, int> tailmap;
+  auto bytes = [&](const char* s, int len, sxc_chunk& c) -> std::string {
     if (len < 0 || (len > 0 && !s)) return "negative length or null sequence";
     if (len > N) return "sequence longer than the FFT length N";
-    auto key = std::make_pair(s, len);
-    auto it = m.find(key);
-    if (it != m.end()) {
-      idx = it->second;
-      return "";
-    }
-    sxc_chunk c;
     c.off = (int64_t)hb.seq.size();
     c.len = len;
     c.flags = 2;  // bit 1: only A, C, G, T (scores are 100 or 0)
@@ -56,10 +60,66 @@ inline std::string pack_batch(const sxc_pair* in, int n, int N, HostBatch& hb) {
       hb.seq.push_back(b);
     }
     while (hb.seq.size() % 16) hb.seq.push_back(0);  // next chunk 16-byte aligned
+    if (len > hb.max_len) hb.max_len = len;
+    return "";
+  };
+  auto add = [&](const char* s, int len, int tail, std::vector<sxc_chunk>& tab, std::map<Key, int>& m,
+                 int& idx) -> std::string {
+    const Key key(s, len, tail);
+    auto it = m.find(key);
+    if (it != m.end()) {
+      idx = it->second;
+      return "";
+    }
+    sxc_chunk c;
+    std::string e = bytes(s, len, c);
+    if (!e.empty()) return e;
     idx = (int)tab.size();
     tab.push_back(c);
     m[key] = idx;
-    if (len > hb.max_len) hb.max_len = len;
+    return "";
+  };
+  // A tail for a chunk of length L: segments must tile [L, end) upwards within each
+  // source's length and N.  Returns the tail index (-1 for none) through idx.
+  auto add_tail = [&](const sxc_signal_tail* t, int L, int& idx) -> std::string {
+    idx = -1;
+    if (!t || t->nseg == 0) return "";
+    if (t->nseg < 0 || !t->segs) return "bad signal tail";
+    int start = L;
+    for (int k = 0; k < t->nseg; ++k) {
+      const sxc_tail_seg& g = t->segs[k];
+      if (g.end <= start || g.end > g.len || g.end > N || (g.strand != 0 && g.strand != 1))
+        return "signal tail segments must tile [chunk length, end) within their sources";
+      start = g.end;
+    }
+    const auto key = std::make_pair(t->segs, t->nseg);
+    auto it = tailmap.find(key);
+    if (it != tailmap.end()) {
+      idx = it->second;
+      return "";
+    }
+    sxc_tail_dev td{(int32_t)hb.segs.size(), t->nseg};
+    for (int k = 0; k < t->nseg; ++k) {
+      const sxc_tail_seg& g = t->segs[k];
+      const Key sk(g.seq, g.len, g.strand);
+      int src;
+      auto si = smap.find(sk);
+      if (si != smap.end()) {
+        src = si->second;
+      } else {
+        sxc_chunk c;
+        std::string e = bytes(g.seq, g.len, c);
+        if (!e.empty()) return e;
+        src = (int)hb.schunks.size();
+        hb.schunks.push_back(c);
+        hb.sstrand.push_back(g.strand);
+        smap[sk] = src;
+      }
+      hb.segs.push_back(sxc_tail_seg_dev{g.end, src});
+    }
+    idx = (int)hb.tails.size();
+    hb.tails.push_back(td);
+    tailmap[key] = idx;
     return "";
   };
   hb.pairs.resize(n);
@@ -69,9 +129,16 @@ inline std::string pack_batch(const sxc_pair* in, int n, int N, HostBatch& hb) {
     // lags span +-N/2: the shorter chunk must fit in half the transform (the slave uses
     // N = 2 * t_chunk, so every chunk does)
     if (p.tlen > N / 2 && p.qlen > N / 2) return "both chunks longer than N/2";
-    std::string e = add(p.tseq, p.tlen, hb.tchunks, tmap, d.t_chunk);
+    int tt, qt;
+    std::string e = add_tail(t_tail ? &t_tail[k] : nullptr, p.tlen, tt);
     if (!e.empty()) return e;
-    e = add(p.qseq, p.qlen, hb.qchunks, qmap, d.q_chunk);
+    e = add_tail(q_tail ? &q_tail[k] : nullptr, p.qlen, qt);
+    if (!e.empty()) return e;
+    const size_t before = hb.tchunks.size();
+    e = add(p.tseq, p.tlen, tt, hb.tchunks, tmap, d.t_chunk);
+    if (!e.empty()) return e;
+    if (hb.tchunks.size() > before) hb.ttail.push_back(tt);
+    e = add(p.qseq, p.qlen, 0, hb.qchunks, qmap, d.q_chunk);
     if (!e.empty()) return e;
     d.t_chunk_start = p.t_chunk_start;
     d.q_chunk_start = p.q_chunk_start;
@@ -79,7 +146,9 @@ inline std::string pack_batch(const sxc_pair* in, int n, int N, HostBatch& hb) {
     d.q_record = p.q_record;
     d.q_record_len = p.q_record_len;
     d.fast = p.fast;
+    d.q_tail = qt;
   }
+  if (hb.tails.empty()) hb.ttail.clear();  // no tails at all: kernels skip the lookups
   hb.seq.resize(hb.seq.size() + 16, 0);  // read-ahead of aligned word copies
   return "";
 }

@@ -56,18 +56,20 @@ SXC_HD constexpr int pad(int p) { return p + (p >> 5); }
 #define SXC_STAGE_A 16
 #endif
 #ifndef SXC_KMAX
-#define SXC_KMAX 2
+#define SXC_KMAX 3
 #endif
 
 template <int NB>
 struct Geo {
   static constexpr int N = 1 << NB;
   static constexpr int H = N / 2;
-  static constexpr int SA = SXC_STAGE_A;                   // 16 or 32
+  // 16 or 32 positions per thread in stage A; N = 32768 needs 32 to stay at 1024 threads
+  static constexpr int SA = (NB >= 15) ? 32 : SXC_STAGE_A;
   static constexpr int LSA = SA == 32 ? 5 : 4;             // log2(SA)
   static constexpr int NT = (N / SA < 64) ? 64 : N / SA;   // threads per workgroup
   static constexpr int YLEN = N + N / 32;                  // padded LDS floats
   static constexpr int GROUPS = N / SA;                    // stage-A units
+  static_assert(NT <= 1024, "a workgroup has at most 1024 threads");
 };
 
 SXC_HD inline unsigned bitrev(unsigned x, int bits) {

@@ -87,11 +87,7 @@ __global__ __launch_bounds__(Geo<NB>::NT) void k_encode(EncodeArgs a) {
   __shared__ __align__(16) char sm[Smem<NB>::BYTES];
   Regs<NB> r;
   DevExec<Regs<NB>> ex{r};
-  const sxc_chunk ck = a.chunks[blockIdx.x];
-  phase_prepare<NB>(ex, sm, a.tab, a.ent_lut, a.seq + ck.off, ck.len, ck.flags & 1, a.strand);
-  for (int ch = 0; ch < 4; ++ch)
-    for (int p = threadIdx.x; p < Geo<NB>::N; p += Geo<NB>::NT)
-      a.out[((size_t)blockIdx.x * 4 + ch) * Geo<NB>::N + p] = encode_at<NB>(sm, ck.len, p, ch);
+  encode_body<NB>(ex, sm, a, (int)blockIdx.x);
 }
 
 // CrossCorrelation::CrossCorrelate of given signals against resident target spectra.
@@ -106,7 +102,8 @@ __global__ __launch_bounds__(Geo<NB>::NT) void k_xcorr(XcorrArgs a) {
   __syncthreads();
   const Twiddle tw{lut, a.osc};
   const int k = blockIdx.x;
-  correlate<NB>(ex, sm, a.tab, tw, 0, a.spectra + (size_t)k * 4 * Geo<NB>::N, a.qsig + (size_t)k * 4 * Geo<NB>::N);
+  correlate<NB>(ex, sm, a.tab, tw, SeqView{nullptr, nullptr, 0}, a.spectra + (size_t)k * 4 * Geo<NB>::N,
+                a.qsig + (size_t)k * 4 * Geo<NB>::N, TailRef{nullptr, 0, nullptr});
   const float* Y = (const float*)(sm + SM::Y);
   for (int p = threadIdx.x; p < Geo<NB>::N; p += Geo<NB>::NT) a.xc[(size_t)k * Geo<NB>::N + p] = Y[pad(p)];
 }
@@ -222,6 +219,7 @@ hipError_t launch_order_runs(const int32_t* runs, const uint64_t* keys, int n, i
     case 12: CALL(12); break;        \
     case 13: CALL(13); break;        \
     case 14: CALL(14); break;        \
+    case 15: CALL(15); break;        \
     default: return hipErrorInvalidValue; \
   }
 

@@ -38,6 +38,9 @@ struct PairArgs {
   uint16_t* cands;           // accepted lags, N slots per (pair, strand)
   int32_t* ncand;            // accepted lags per (pair, strand)
   float* xc_out;             // optional: N floats per (pair, strand)
+  const sxc_tail_dev* tails;     // signal tails (pairs[].q_tail indexes them), may be null
+  const sxc_tail_seg_dev* segs;
+  const float* src_sig;          // tail source signals, 4*N floats each
 };
 
 struct SpecArgs {
@@ -51,6 +54,10 @@ struct SpecArgs {
   float* spectra;            // output: 4*N floats per entry of `which`, at spectra + dst[k]*4N
   const int32_t* dst;        // destination slot per entry
   const float* signals;      // optional: given 4*N signals instead of encoding chunk bytes
+  const int32_t* chunk_tail;     // optional: signal tail per chunk (-1: none)
+  const sxc_tail_dev* tails;
+  const sxc_tail_seg_dev* segs;
+  const float* src_sig;
 };
 
 struct EncodeArgs {
@@ -59,6 +66,7 @@ struct EncodeArgs {
   const unsigned char* seq;
   const sxc_chunk* chunks;
   int32_t strand;
+  const int32_t* strands;    // optional: strand per chunk (overrides `strand`)
   float* out;                // 4*N floats per chunk
 };
 
@@ -105,8 +113,12 @@ struct Smem {
   static constexpr int CLS = GC + 8 * SXC_NCLASS;              // uchar[128] byte -> class
   static constexpr int SCORE = CLS + 128;                      // uchar[18*18]
   static constexpr int QUART = SCORE + SXC_NCLASS * SXC_NCLASS;  // uchar[18][4]
-  static constexpr int QSEQ = (QUART + 4 * SXC_NCLASS + 15) & ~15;  // uchar[N] strand query
-  static constexpr int BYTES = (QSEQ + N + 15) & ~15;
+  static constexpr int RCT = QUART + 4 * SXC_NCLASS;           // uchar[128] reverse complement
+  // strand-applied chunk bytes: an LDS copy up to N = 16384; at 32768 the plan would not fit
+  // the 160 KB of a workgroup, and the bytes are read from global memory instead
+  static constexpr bool SEQ_IN_LDS = NB <= 14;
+  static constexpr int QSEQ = (RCT + 128 + 15) & ~15;          // uchar[N] when SEQ_IN_LDS
+  static constexpr int BYTES = (QSEQ + (SEQ_IN_LDS ? N : 0) + 15) & ~15;
   // after the peaks phase Y holds the accepted lags
   static constexpr int CAND = Y;                               // uint16[N]
 };
@@ -133,16 +145,26 @@ SXC_HD inline int score_of(const unsigned char* cls, const unsigned char* score,
   return score[cls[tb] * SXC_NCLASS + cls[qb]];
 }
 
-// Loads one chunk's bytes (reverse-complemented for strand 1) to LDS and computes the
-// per-window entropy weights and the four channel means.
+// The bytes being encoded with the strand applied: the LDS copy, or (rc != null) the chunk's
+// bytes in global memory read reverse-complemented (DNAVector::ReverseComplement,
+// analysis/DNAVector.cc:482-521).
+struct SeqView {
+  const unsigned char* p;
+  const unsigned char* rc;
+  int L;
+  SXC_HD unsigned char operator[](int i) const { return rc ? rc[p[L - 1 - i] & 127] : p[i]; }
+};
+
+// Loads one chunk's bytes (reverse-complemented for strand 1) and computes the per-window
+// entropy weights and the four channel means.
 template <int NB, class Ex>
-SXC_HD inline void phase_prepare(Ex& ex, char* sm, const sxc_tables* tab, const double* ent_lut,
-                                 const unsigned char* src, int L, int thirds, int strand) {
+SXC_HD inline SeqView phase_prepare(Ex& ex, char* sm, const sxc_tables* tab, const double* ent_lut,
+                                    const unsigned char* src, int L, int thirds, int strand) {
   using SM = Smem<NB>;
   using G = Geo<NB>;
   float* lut = (float*)(sm + SM::LUT);
-  unsigned char* qs = (unsigned char*)(sm + SM::QSEQ);
   unsigned char* cls = (unsigned char*)(sm + SM::CLS);
+  unsigned char* rct = (unsigned char*)(sm + SM::RCT);
   unsigned char* score = (unsigned char*)(sm + SM::SCORE);
   unsigned char* quart = (unsigned char*)(sm + SM::QUART);
   double* codec = (double*)(sm + SM::CODEC);
@@ -161,11 +183,21 @@ SXC_HD inline void phase_prepare(Ex& ex, char* sm, const sxc_tables* tab, const 
       codec[i] = (&tab->codec[0][0])[i];
     }
     for (int i = tid; i < SXC_NCLASS; i += G::NT) gcv[i] = tab->gc[i];
-    // DNAVector::ReverseComplement (analysis/DNAVector.cc:482-521)
-    for (int i = tid; i < L; i += G::NT) qs[i] = strand ? tab->rc[src[L - 1 - i] & 127] : src[i];
+    for (int i = tid; i < 128; i += G::NT) rct[i] = tab->rc[i];
+    if constexpr (SM::SEQ_IN_LDS) {
+      // DNAVector::ReverseComplement (analysis/DNAVector.cc:482-521)
+      unsigned char* q = (unsigned char*)(sm + SM::QSEQ);
+      for (int i = tid; i < L; i += G::NT) q[i] = strand ? tab->rc[src[L - 1 - i] & 127] : src[i];
+    }
     if (tid < 16) misc[tid] = 0;
   });
   ex.sync();
+  SeqView qs;
+  if constexpr (SM::SEQ_IN_LDS) {
+    qs = SeqView{(const unsigned char*)(sm + SM::QSEQ), nullptr, L};
+  } else {
+    qs = SeqView{src, strand ? rct : nullptr, L};
+  }
   float* ent = (float*)(sm + SM::ENT);
   double* dsum = (double*)(sm + SM::DSUM);
   const int win = G::N / 512;
@@ -236,14 +268,42 @@ SXC_HD inline void phase_prepare(Ex& ex, char* sm, const sxc_tables* tab, const 
     }
   });
   ex.sync();
+  return qs;
+}
+
+// Positions at or past a chunk's length: 0, or the signal an earlier sequence left there
+// when the caller reuses one CCSignal object (include/sxc.h, sxc_tail_seg).  Segments are
+// ordered by end; a binary search finds the one holding p.
+struct TailRef {
+  const sxc_tail_seg_dev* seg;
+  int n;
+  const float* sig;  // source signals, 4*N floats each
+};
+
+template <int NB>
+SXC_HD inline TailRef tail_ref(const sxc_tail_dev* tails, const sxc_tail_seg_dev* segs, const float* sig, int idx) {
+  if (!tails || idx < 0) return TailRef{nullptr, 0, nullptr};
+  const sxc_tail_dev t = tails[idx];
+  return TailRef{segs + t.first, t.n, sig};
+}
+
+template <int NB>
+SXC_HD inline float tail_at(const TailRef& t, int p, int ch) {
+  int lo = 0, hi = t.n;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (p < t.seg[mid].end) hi = mid; else lo = mid + 1;
+  }
+  if (lo >= t.n) return 0.f;
+  return t.sig[((size_t)t.seg[lo].src * 4 + ch) * (size_t)(1 << NB) + p];
 }
 
 // Signal value of channel ch at time position p (CCSignal::SeqToPCM, :118-131).
 template <int NB>
-SXC_HD inline float encode_at(const char* sm, int L, int p, int ch) {
+SXC_HD inline float encode_at(const char* sm, const SeqView& qs, int p, int ch, const TailRef& tail) {
   using SM = Smem<NB>;
-  if (p >= L) return 0.f;
-  const unsigned char* qs = (const unsigned char*)(sm + SM::QSEQ);
+  const int L = qs.L;
+  if (p >= L) return tail.n ? tail_at<NB>(tail, p, ch) : 0.f;
   const unsigned char* cls = (const unsigned char*)(sm + SM::CLS);
   const double* codec = (const double*)(sm + SM::CODEC);
   const float e = (L < 1024) ? 1.f : ((const float*)(sm + SM::ENT))[p / (Geo<NB>::N / 512)];
@@ -282,8 +342,8 @@ SXC_HD inline void inv_rounds(Ex& ex, float* Y, const Twiddle& tw) {
 }
 
 template <int NB, class Ex>
-SXC_HD inline void forward_channel(Ex& ex, char* sm, const sxc_tables* tab, const Twiddle& tw, int L, int ch,
-                                   const float* given) {
+SXC_HD inline void forward_channel(Ex& ex, char* sm, const sxc_tables* tab, const Twiddle& tw, const SeqView& qs,
+                                   int ch, const float* given, const TailRef& tail) {
   using SM = Smem<NB>;
   using G = Geo<NB>;
   float* Y = (float*)(sm + SM::Y);
@@ -293,13 +353,28 @@ SXC_HD inline void forward_channel(Ex& ex, char* sm, const sxc_tables* tab, cons
     static_for<0, G::SA>([&](auto I) {
       constexpr int r = decltype(I)::value;
       const int p = stageA_pos<NB>(g, r);
-      v[r] = given ? given[p] : encode_at<NB>(sm, L, p, ch);
+      v[r] = given ? given[p] : encode_at<NB>(sm, qs, p, ch, tail);
     });
     stageA_forward<G::SA>(v, tw);
     static_for<0, G::SA>([&](auto I) { Y[pad(G::SA * g + decltype(I)::value)] = v[decltype(I)::value]; });
   });
   ex.sync();
   fwd_rounds<NB, 0>(ex, Y, tw);
+}
+
+// One workgroup per chunk: CCSignal::SetSequence(chunk, N) (reverse-complemented for
+// strand 1) into 4*N floats of global memory.
+template <int NB, class Ex>
+SXC_HD inline void encode_body(Ex& ex, char* sm, const EncodeArgs& a, int k) {
+  using G = Geo<NB>;
+  const sxc_chunk ck = a.chunks[k];
+  const int strand = a.strands ? a.strands[k] : a.strand;
+  const SeqView qs = phase_prepare<NB>(ex, sm, a.tab, a.ent_lut, a.seq + ck.off, ck.len, ck.flags & 1, strand);
+  const TailRef none{nullptr, 0, nullptr};
+  ex.each([&](int tid) {
+    for (int ch = 0; ch < 4; ++ch)
+      for (int p = tid; p < G::N; p += G::NT) a.out[((size_t)k * 4 + ch) * G::N + p] = encode_at<NB>(sm, qs, p, ch, none);
+  });
 }
 
 // ---------------------------------------------------------------- target spectra
@@ -311,8 +386,9 @@ SXC_HD inline void spectra_body(Ex& ex, char* sm, const SpecArgs& a, int k) {
   using G = Geo<NB>;
   const sxc_chunk ck = a.chunks[a.which[k]];
   const Twiddle tw{(const float*)(sm + SM::LUT), a.osc};
+  SeqView qs{nullptr, nullptr, ck.len};
   if (!a.signals) {
-    phase_prepare<NB>(ex, sm, a.tab, a.ent_lut, a.seq + ck.off, ck.len, ck.flags & 1, 0);
+    qs = phase_prepare<NB>(ex, sm, a.tab, a.ent_lut, a.seq + ck.off, ck.len, ck.flags & 1, 0);
   } else {
     float* lut = (float*)(sm + SM::LUT);
     ex.each([&](int tid) {
@@ -322,9 +398,11 @@ SXC_HD inline void spectra_body(Ex& ex, char* sm, const SpecArgs& a, int k) {
   }
   float* Y = (float*)(sm + SM::Y);
   float* out = a.spectra + (size_t)a.dst[k] * 4 * G::N;
+  const TailRef tail =
+      tail_ref<NB>(a.tails, a.segs, a.src_sig, a.chunk_tail ? a.chunk_tail[a.which[k]] : -1);
   for (int ch = 0; ch < 4; ++ch) {
     const float* given = a.signals ? a.signals + ((size_t)k * 4 + ch) * G::N : nullptr;
-    forward_channel<NB>(ex, sm, a.tab, tw, ck.len, ch, given);
+    forward_channel<NB>(ex, sm, a.tab, tw, qs, ch, given, tail);
     ex.each([&](int tid) {
       for (int p = tid; p < G::N; p += G::NT) out[(size_t)ch * G::N + p] = Y[pad(p)];
     });
@@ -364,8 +442,8 @@ SXC_HD inline void product(Ex& ex, float* Y, const float* T) {
 // xc = (((0 + A) + C) + G) + T of rescaled, fftshifted channel correlations, left in
 // natural order in Y.  Per-thread accumulators stay in registers across channels.
 template <int NB, class Ex>
-SXC_HD inline void correlate(Ex& ex, char* sm, const sxc_tables* tab, const Twiddle& tw, int L, const float* T4,
-                             const float* qsig4) {
+SXC_HD inline void correlate(Ex& ex, char* sm, const sxc_tables* tab, const Twiddle& tw, const SeqView& qs,
+                             const float* T4, const float* qsig4, const TailRef& tail) {
   using SM = Smem<NB>;
   using G = Geo<NB>;
   float* Y = (float*)(sm + SM::Y);
@@ -375,7 +453,7 @@ SXC_HD inline void correlate(Ex& ex, char* sm, const sxc_tables* tab, const Twid
   });
   const float mul = (float)(1.0 / (double)G::N);  // FFTReal::rescale (FFTReal.hpp:264)
   for (int ch = 0; ch < 4; ++ch) {
-    forward_channel<NB>(ex, sm, tab, tw, L, ch, qsig4 ? qsig4 + (size_t)ch * G::N : nullptr);
+    forward_channel<NB>(ex, sm, tab, tw, qs, ch, qsig4 ? qsig4 + (size_t)ch * G::N : nullptr, tail);
     product<NB>(ex, Y, T4 + (size_t)ch * G::N);
     inv_rounds<NB, Schedule<NB>::NR - 1>(ex, Y, tw);
     ex.each([&](int g) {
@@ -482,8 +560,9 @@ SXC_HD inline void correlate_body(Ex& ex, char* sm, const PairArgs& a, int ps) {
   const sxc_pair_dev pr = a.pairs[pair];
   const sxc_chunk qc = a.qchunks[pr.q_chunk];
   const Twiddle tw{(const float*)(sm + SM::LUT), a.osc};
-  phase_prepare<NB>(ex, sm, a.tab, a.ent_lut, a.seq + qc.off, qc.len, qc.flags & 1, strand);
-  correlate<NB>(ex, sm, a.tab, tw, qc.len, a.spectra + (size_t)pr.t_chunk * 4 * G::N, nullptr);
+  const SeqView qs = phase_prepare<NB>(ex, sm, a.tab, a.ent_lut, a.seq + qc.off, qc.len, qc.flags & 1, strand);
+  correlate<NB>(ex, sm, a.tab, tw, qs, a.spectra + (size_t)pr.t_chunk * 4 * G::N, nullptr,
+                tail_ref<NB>(a.tails, a.segs, a.src_sig, pr.q_tail));
   if (a.xc_out) {
     const float* Y = (const float*)(sm + SM::Y);
     ex.each([&](int tid) {

@@ -61,8 +61,8 @@ struct ScanSmem {
   static constexpr int QCODE = TCODE + N;               // uchar[N]
   static constexpr int TPK = QCODE + N;                 // uint32[PKW]: 16 bases per word (pure A/C/G/T only)
   static constexpr int QPK = TPK + 4 * PKW;             // uint32[PKW]
-  static constexpr int QUEUE = QPK + 4 * PKW;           // int3[QCAP]: {lag rank, ns | ne << 16, shift}
-  static constexpr int SUMM = QUEUE + 12 * QCAP;        // int2[SCAN_NT]
+  static constexpr int QUEUE = QPK + 4 * PKW;           // int2[QCAP]: {lag rank, ns | ne << 16}
+  static constexpr int SUMM = QUEUE + 8 * QCAP;         // int2[SCAN_NT]
   static constexpr int EQUAL = SUMM + 8 * SCAN_NT;      // double[18][18]
   static constexpr int GC = EQUAL + 8 * SXC_NCLASS * SXC_NCLASS;  // double[18]
   static constexpr int SCORE = GC + 8 * SXC_NCLASS;     // uchar[18][18]
@@ -226,9 +226,8 @@ SXC_HD inline void scan_epoch(Ex& ex, char* sm, const uint16_t* cands, int c0, i
   auto push = [&](int rank, int ns, int ne) {
     const int k = ex.atomic_add_shared(&misc[SM_QN], 1);
     if (k < SS::QCAP) {
-      queue[3 * k] = rank;
-      queue[3 * k + 1] = ns | (ne << 16);
-      queue[3 * k + 2] = (int)cands[rank] - H;
+      queue[2 * k] = rank;
+      queue[2 * k + 1] = ns | (ne << 16);
     }
   };
   ex.each([&](int tid) {
@@ -341,9 +340,10 @@ SXC_HD inline void scan_epoch(Ex& ex, char* sm, const uint16_t* cands, int c0, i
 
 // Drains queue entries [0, n): mode 0 filters into t_results, mode 1 writes runs.
 template <int NB, class Ex>
-SXC_HD inline void drain(Ex& ex, char* sm, const ScanArgs& a, const sxc_pair_dev& pr, int ps, int n) {
-  using SS = ScanSmem<NB>;
+SXC_HD inline void drain(Ex& ex, char* sm, const ScanArgs& a, const sxc_pair_dev& pr, const uint16_t* cands, int ps,
+                         int n) {
   constexpr int H = (1 << NB) / 2;
+  using SS = ScanSmem<NB>;
   const unsigned char* tc = (const unsigned char*)(sm + SS::TCODE);
   const unsigned char* qc = (const unsigned char*)(sm + SS::QCODE);
   const double* equal = (const double*)(sm + SS::EQUAL);
@@ -354,9 +354,9 @@ SXC_HD inline void drain(Ex& ex, char* sm, const ScanArgs& a, const sxc_pair_dev
   const int strand = ps % a.strands;
   ex.each([&](int tid) {
     for (int k = tid; k < n; k += SCAN_NT) {
-      const int rank = queue[3 * k];
-      const int ns = queue[3 * k + 1] & 0xffff, ne = (queue[3 * k + 1] >> 16) & 0xffff;
-      const int shift = queue[3 * k + 2];
+      const int rank = queue[2 * k];
+      const int ns = queue[2 * k + 1] & 0xffff, ne = (queue[2 * k + 1] >> 16) & 0xffff;
+      const int shift = (int)cands[rank] - H;
       const int i0 = shift < 0 ? -shift : 0;
       const int start_t = i0 + ns - SXC_MIN_LEN;
       const int len = ne - ns + SXC_MIN_LEN;
@@ -533,7 +533,7 @@ SXC_HD inline void scan_body(Ex& ex, char* sm, const ScanArgs& a, int ps) {
     const int qn = misc[SM_QN];
     if (qn > SS::QCAP) {
       // the epoch overflowed the queue: keep what came before it, redo its lags one by one
-      drain<NB>(ex, sm, a, pr, ps, qstart);
+      drain<NB>(ex, sm, a, pr, cands, ps, qstart);
       runs += qstart;
       for (int c = e0; c < e1; ++c) {
         ex.each([&](int tid) {
@@ -542,7 +542,7 @@ SXC_HD inline void scan_body(Ex& ex, char* sm, const ScanArgs& a, int ps) {
         ex.sync();
         scan_epoch<NB>(ex, sm, cands, c, c + 1, SCAN_NT);
         const int n1 = misc[SM_QN];
-        drain<NB>(ex, sm, a, pr, ps, n1);
+        drain<NB>(ex, sm, a, pr, cands, ps, n1);
         runs += n1;
       }
       ex.each([&](int tid) {
@@ -550,7 +550,7 @@ SXC_HD inline void scan_body(Ex& ex, char* sm, const ScanArgs& a, int ps) {
       });
       ex.sync();
     } else if (qn > SS::QCAP / 2 || e1 == ncand) {
-      drain<NB>(ex, sm, a, pr, ps, qn);
+      drain<NB>(ex, sm, a, pr, cands, ps, qn);
       runs += qn;
       ex.each([&](int tid) {
         if (tid == 0) misc[SM_QN] = 0;

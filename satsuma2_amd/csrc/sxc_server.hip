@@ -1,0 +1,599 @@
+// sxc_server.hip -- the serving loop: chunk-pair requests against device-resident genomes.
+//
+// What it serves is the slave's unit of work, a block of target chunks x query chunks
+// (t_pair, analysis/WorkQueue.h:17-22; align_target, analysis/HomologyByXCorrSlave.cc:
+// 270-300), as the master's battleship search produces them (seed lines and the blocks
+// around hits, analysis/GridSearch.cc:188-411).  Records per block are the slave's, in
+// the slave's order.
+//
+// Layout in HBM, built once at creation:
+//   seq       every target and query chunk's bytes, 16-byte aligned
+//   tch, qch  the chunk tables (offset, length, flags)
+//   spectra   the four FFTReal spectra of every target chunk (4*N floats each), so a pair
+//             costs only its query transforms, product, inverse, peaks and scan
+// Work runs in launch slots: each slot owns a stream and the buffers of one launch of up
+// to `max_pairs_per_launch` chunk pairs.  The number of slots is what the HBM budget
+// leaves after the resident data.  A dispatcher thread fills every free slot with the
+// oldest pending pairs -- across requests, so a launch mixes requests and a request may
+// span launches (continuous batching) -- and completes slots as their events fire.
+// Results depend only on the chunk pair, never on what shares its launch.
+#include <hip/hip_runtime.h>
+#include <string.h>
+
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <map>
+#include <memory>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "sxc_api_internal.h"
+#include "sxc_kernels.h"
+#include "sxc_tables.h"
+
+using sxc::fail;
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+double ms_since(Clock::time_point a, Clock::time_point b) {
+  return std::chrono::duration<double, std::milli>(b - a).count();
+}
+
+struct ChunkMeta {
+  int32_t start, record, record_len;
+};
+
+struct Request {
+  uint64_t id = 0;
+  std::vector<sxc_block> blocks;
+  int64_t total_pairs = 0;
+  // expansion cursor: block, query, target of the next pair to dispatch
+  size_t cb = 0;
+  int32_t cq = 0, ct = 0;
+  int64_t dispatched = 0, done_pairs = 0;
+  std::map<int64_t, std::vector<sxc_result>> parts;  // first pair index -> records
+  std::vector<sxc_result> results;
+  Clock::time_point t_submit, t_done;
+  bool complete = false;
+  int error = 0;
+};
+
+// A run of consecutive pairs of one request inside a launch.
+struct Piece {
+  Request* req;
+  int64_t first;  // pair index in the request
+  int32_t count;
+  int32_t at;     // pair index in the launch
+};
+
+enum SlotState { FREE = 0, COUNTING = 1, PACKING = 2 };
+
+constexpr int kDefaultCap = 64;  // records per (pair, strand) a launch first makes room for
+
+struct Slot {
+  hipStream_t s = nullptr;
+  hipEvent_t ev_start = nullptr, ev_count = nullptr, ev_pack = nullptr;
+  DevBuf pairs, cands, ncand, outs, keys, counts, offsets, total, maxc, final_;
+  sxc_pair_dev* h_pairs = nullptr;  // pinned
+  int64_t* h_offsets = nullptr;     // pinned, 2 * P + 1 (the last entry is the total)
+  int32_t* h_maxc = nullptr;        // pinned
+  sxc_result* h_final = nullptr;    // pinned, h_final_cap records
+  int64_t h_final_cap = 0;
+  int cap = kDefaultCap;            // records per (pair, strand) of the current launch
+  int npairs = 0;
+  std::vector<Piece> pieces;
+  SlotState state = FREE;
+};
+
+}  // namespace
+
+struct sxc_server {
+  sxc_ctx* ctx = nullptr;
+  sxc_params prm{};
+  int nb = 0;
+  int P = 2048;
+  DevBuf seq, tch, qch, spectra;
+  std::vector<ChunkMeta> tmeta, qmeta;
+  int64_t resident_bytes = 0, slot_bytes = 0;
+  std::vector<Slot> slots;
+  std::mutex mu;
+  std::condition_variable work_cv, done_cv;
+  std::deque<Request*> pending;  // requests with pairs not yet dispatched
+  std::unordered_map<uint64_t, std::unique_ptr<Request>> reqs;
+  uint64_t next_id = 1;
+  bool stop = false;
+  std::thread dispatcher;
+  int fatal = 0;
+  std::string fatal_msg;
+  // statistics
+  int64_t st_requests = 0, st_pairs = 0, st_launches = 0, st_results = 0;
+  double st_device_ms = 0;
+};
+
+namespace {
+
+int64_t block_pairs(const sxc_block& b) {
+  return (int64_t)(b.target_to - b.target_from + 1) * (int64_t)(b.query_to - b.query_from + 1);
+}
+
+int slot_alloc(sxc_server* sv, Slot& sl) {
+  const int P = sv->P, N = sv->prm.N;
+  const size_t total = (size_t)2 * P;
+  HIPCHK(hipStreamCreateWithFlags(&sl.s, hipStreamNonBlocking));
+  HIPCHK(hipEventCreate(&sl.ev_start));
+  HIPCHK(hipEventCreate(&sl.ev_count));
+  HIPCHK(hipEventCreate(&sl.ev_pack));
+  HIPCHK(sl.pairs.ensure((size_t)P * sizeof(sxc_pair_dev)));
+  HIPCHK(sl.cands.ensure(total * N * sizeof(uint16_t)));
+  HIPCHK(sl.ncand.ensure(total * sizeof(int32_t)));
+  HIPCHK(sl.outs.ensure(total * kDefaultCap * sizeof(sxc_result_dev)));
+  HIPCHK(sl.keys.ensure(total * kDefaultCap * sizeof(uint64_t)));
+  HIPCHK(sl.counts.ensure(total * sizeof(int32_t)));
+  HIPCHK(sl.offsets.ensure(total * sizeof(int64_t)));
+  HIPCHK(sl.total.ensure(sizeof(int64_t)));
+  HIPCHK(sl.maxc.ensure(sizeof(int32_t)));
+  HIPCHK(sl.final_.ensure(total * kDefaultCap * sizeof(sxc_result_dev)));
+  HIPCHK(hipHostMalloc((void**)&sl.h_pairs, (size_t)P * sizeof(sxc_pair_dev), hipHostMallocDefault));
+  HIPCHK(hipHostMalloc((void**)&sl.h_offsets, (total + 1) * sizeof(int64_t), hipHostMallocDefault));
+  HIPCHK(hipHostMalloc((void**)&sl.h_maxc, sizeof(int32_t), hipHostMallocDefault));
+  sl.h_final_cap = (int64_t)total * kDefaultCap;
+  HIPCHK(hipHostMalloc((void**)&sl.h_final, (size_t)sl.h_final_cap * sizeof(sxc_result), hipHostMallocDefault));
+  return SXC_OK;
+}
+
+void slot_free(Slot& sl) {
+  for (DevBuf* b : {&sl.pairs, &sl.cands, &sl.ncand, &sl.outs, &sl.keys, &sl.counts, &sl.offsets, &sl.total,
+                    &sl.maxc, &sl.final_})
+    b->release();
+  if (sl.h_pairs) (void)hipHostFree(sl.h_pairs);
+  if (sl.h_offsets) (void)hipHostFree(sl.h_offsets);
+  if (sl.h_maxc) (void)hipHostFree(sl.h_maxc);
+  if (sl.h_final) (void)hipHostFree(sl.h_final);
+  for (hipEvent_t e : {sl.ev_start, sl.ev_count, sl.ev_pack})
+    if (e) (void)hipEventDestroy(e);
+  if (sl.s) (void)hipStreamDestroy(sl.s);
+  sl = Slot();
+}
+
+// Device bytes one slot holds at the default record capacity.
+int64_t slot_bytes_for(int P, int N, int cap) {
+  const int64_t total = 2 * (int64_t)P;
+  return (int64_t)P * sizeof(sxc_pair_dev) + total * N * 2 + total * 4 + total * cap * (72 + 8 + 72) + total * 12 + 64;
+}
+
+int launch_scan_slot(sxc_server* sv, Slot& sl) {
+  // Room for cap records of every (pair, strand) of this launch.  The buffers only grow;
+  // a launch may have more pairs than the one that last grew them.
+  const size_t nrec = (size_t)2 * sl.npairs * sl.cap;
+  HIPCHK(sl.outs.ensure(nrec * sizeof(sxc_result_dev)));
+  HIPCHK(sl.keys.ensure(nrec * sizeof(uint64_t)));
+  sxc::ScanArgs a{};
+  a.tab = sv->ctx->d_tab;
+  a.seq = sv->seq.as<unsigned char>();
+  a.tchunks = sv->tch.as<sxc_chunk>();
+  a.qchunks = sv->qch.as<sxc_chunk>();
+  a.pairs = sl.pairs.as<sxc_pair_dev>();
+  a.cands = sl.cands.as<uint16_t>();
+  a.ncand = sl.ncand.as<int32_t>();
+  a.n_fft = sv->prm.N;
+  a.strands = 2;
+  a.min_len = sv->prm.min_len;
+  a.q_chunk = sv->prm.q_chunk;
+  a.min_prob = sv->prm.min_prob;
+  a.target_total = sv->prm.target_total;
+  a.z_skip = sxc::keep_z_bound(sv->prm.min_prob, sv->prm.target_total);
+  a.mode = 0;
+  a.out = sl.outs.as<sxc_result_dev>();
+  a.runs = nullptr;
+  a.keys = sl.keys.as<uint64_t>();
+  a.counts = sl.counts.as<int32_t>();
+  a.cap = sl.cap;
+  a.stats = nullptr;
+  HIPCHK(sxc::launch_scan(sv->nb, a, 2 * sl.npairs, sl.s));
+  const int total = 2 * sl.npairs;
+  HIPCHK(sxc::launch_finalize(sl.counts.as<int32_t>(), total, sl.cap, sl.offsets.as<int64_t>(), sl.total.as<int64_t>(),
+                              sl.maxc.as<int32_t>(), nullptr, nullptr, nullptr, sl.s, true));
+  HIPCHK(hipMemcpyAsync(sl.h_offsets, sl.offsets.p, (size_t)total * sizeof(int64_t), hipMemcpyDeviceToHost, sl.s));
+  HIPCHK(hipMemcpyAsync(sl.h_offsets + total, sl.total.p, sizeof(int64_t), hipMemcpyDeviceToHost, sl.s));
+  HIPCHK(hipMemcpyAsync(sl.h_maxc, sl.maxc.p, sizeof(int32_t), hipMemcpyDeviceToHost, sl.s));
+  HIPCHK(hipEventRecord(sl.ev_count, sl.s));
+  return SXC_OK;
+}
+
+// Takes up to P pending pairs (oldest requests first) into the slot and launches
+// correlate -> scan -> counts.  Called with sv->mu held.
+int slot_fill(sxc_server* sv, Slot& sl) {
+  sl.pieces.clear();
+  sl.npairs = 0;
+  sl.cap = kDefaultCap;  // an earlier launch's overflow does not carry over
+  while (sl.npairs < sv->P && !sv->pending.empty()) {
+    Request* r = sv->pending.front();
+    Piece pc{r, r->dispatched, 0, sl.npairs};
+    while (sl.npairs < sv->P && r->cb < r->blocks.size()) {
+      const sxc_block& b = r->blocks[r->cb];
+      const int qi = b.query_from + r->cq, ti = b.target_from + r->ct;
+      sxc_pair_dev& d = sl.h_pairs[sl.npairs++];
+      const ChunkMeta& tm = sv->tmeta[(size_t)ti];
+      const ChunkMeta& qm = sv->qmeta[(size_t)qi];
+      d.t_chunk = ti;
+      d.q_chunk = qi;
+      d.t_chunk_start = tm.start;
+      d.q_chunk_start = qm.start;
+      d.t_record = tm.record;
+      d.q_record = qm.record;
+      d.q_record_len = qm.record_len;
+      d.fast = b.fast ? 1 : 0;
+      d.q_tail = -1;
+      ++pc.count;
+      ++r->dispatched;
+      // align_target order: query chunks outer, target chunks inner
+      if (++r->ct > b.target_to - b.target_from) {
+        r->ct = 0;
+        if (++r->cq > b.query_to - b.query_from) {
+          r->cq = 0;
+          ++r->cb;
+        }
+      }
+    }
+    sl.pieces.push_back(pc);
+    if (r->cb >= r->blocks.size()) sv->pending.pop_front();
+  }
+  if (sl.npairs == 0) return SXC_OK;
+  HIPCHK(hipEventRecord(sl.ev_start, sl.s));
+  HIPCHK(hipMemcpyAsync(sl.pairs.p, sl.h_pairs, (size_t)sl.npairs * sizeof(sxc_pair_dev), hipMemcpyHostToDevice, sl.s));
+  sxc::PairArgs a{};
+  a.tab = sv->ctx->d_tab;
+  a.osc = sv->ctx->d_osc;
+  a.ent_lut = sv->ctx->d_ent[sv->nb];
+  a.seq = sv->seq.as<unsigned char>();
+  a.tchunks = sv->tch.as<sxc_chunk>();
+  a.qchunks = sv->qch.as<sxc_chunk>();
+  a.pairs = sl.pairs.as<sxc_pair_dev>();
+  a.spectra = sv->spectra.as<float>();
+  a.npairs = sl.npairs;
+  a.strands = 2;
+  a.cutoff = sv->prm.cutoff;
+  a.cutoff_fast = sv->prm.cutoff_fast;
+  a.cands = sl.cands.as<uint16_t>();
+  a.ncand = sl.ncand.as<int32_t>();
+  HIPCHK(sxc::launch_correlate(sv->nb, a, 2 * sl.npairs, sl.s));
+  int rc = launch_scan_slot(sv, sl);
+  if (rc) return rc;
+  sl.state = COUNTING;
+  ++sv->st_launches;
+  return SXC_OK;
+}
+
+// Counts are back: redo the scan with room for every record if a (pair, strand) kept
+// more than the slot holds, then order the records and copy them back.
+int slot_pack(sxc_server* sv, Slot& sl) {
+  const int total = 2 * sl.npairs;
+  if (*sl.h_maxc > sl.cap) {
+    sl.cap = *sl.h_maxc;
+    int rc = launch_scan_slot(sv, sl);  // grows the record buffers to total x cap
+    if (rc) return rc;
+    HIPCHK(hipEventSynchronize(sl.ev_count));
+  }
+  const int64_t n = sl.h_offsets[total];
+  HIPCHK(sl.final_.ensure((size_t)std::max<int64_t>(n, 1) * sizeof(sxc_result_dev)));
+  if (n > sl.h_final_cap) {
+    if (sl.h_final) HIPCHK(hipHostFree(sl.h_final));
+    sl.h_final = nullptr;
+    sl.h_final_cap = n;
+    HIPCHK(hipHostMalloc((void**)&sl.h_final, (size_t)n * sizeof(sxc_result), hipHostMallocDefault));
+  }
+  HIPCHK(sxc::launch_finalize(sl.counts.as<int32_t>(), total, sl.cap, sl.offsets.as<int64_t>(), nullptr, nullptr,
+                              sl.outs.as<sxc_result_dev>(), sl.keys.as<uint64_t>(), sl.final_.as<sxc_result_dev>(),
+                              sl.s, false));
+  if (n > 0) HIPCHK(hipMemcpyAsync(sl.h_final, sl.final_.p, (size_t)n * sizeof(sxc_result), hipMemcpyDeviceToHost, sl.s));
+  HIPCHK(hipEventRecord(sl.ev_pack, sl.s));
+  sl.state = PACKING;
+  return SXC_OK;
+}
+
+// Records are on the host: hand each request its pairs' records.  Called with sv->mu held.
+void slot_complete(sxc_server* sv, Slot& sl) {
+  float ms = 0.f;
+  if (hipEventElapsedTime(&ms, sl.ev_start, sl.ev_pack) == hipSuccess) sv->st_device_ms += ms;
+  const int total = 2 * sl.npairs;
+  const auto now = Clock::now();
+  for (const Piece& pc : sl.pieces) {
+    const int64_t b = sl.h_offsets[2 * pc.at];
+    const int64_t e = (2 * (pc.at + pc.count) < total) ? sl.h_offsets[2 * (pc.at + pc.count)] : sl.h_offsets[total];
+    Request* r = pc.req;
+    r->parts[pc.first].assign(sl.h_final + b, sl.h_final + e);
+    r->done_pairs += pc.count;
+    sv->st_results += e - b;
+    if (r->done_pairs == r->total_pairs) {
+      for (auto& kv : r->parts) r->results.insert(r->results.end(), kv.second.begin(), kv.second.end());
+      r->parts.clear();
+      r->t_done = now;
+      r->complete = true;
+      ++sv->st_requests;
+    }
+  }
+  sv->st_pairs += sl.npairs;
+  sl.pieces.clear();
+  sl.npairs = 0;
+  sl.state = FREE;
+}
+
+void fail_all(sxc_server* sv, int rc) {
+  sv->fatal = rc;
+  sv->fatal_msg = sxc::g_err;
+  for (auto& kv : sv->reqs)
+    if (!kv.second->complete) {
+      kv.second->complete = true;
+      kv.second->error = rc;
+    }
+  sv->pending.clear();
+}
+
+void dispatch_loop(sxc_server* sv) {
+  (void)hipSetDevice(sv->ctx->device);
+  std::unique_lock<std::mutex> lk(sv->mu);
+  while (!sv->stop) {
+    bool progressed = false, busy = false;
+    for (Slot& sl : sv->slots) {
+      if (sl.state == COUNTING && hipEventQuery(sl.ev_count) == hipSuccess) {
+        const int rc = slot_pack(sv, sl);
+        if (rc) fail_all(sv, rc);
+        progressed = true;
+      }
+      if (sl.state == PACKING && hipEventQuery(sl.ev_pack) == hipSuccess) {
+        slot_complete(sv, sl);
+        progressed = true;
+      }
+    }
+    for (Slot& sl : sv->slots) {
+      if (sl.state == FREE && !sv->pending.empty() && !sv->fatal) {
+        const int rc = slot_fill(sv, sl);
+        if (rc) fail_all(sv, rc);
+        progressed = true;
+      }
+      busy |= sl.state != FREE;
+    }
+    if (progressed) sv->done_cv.notify_all();
+    if (sv->fatal) {
+      sv->done_cv.notify_all();
+      sv->work_cv.wait(lk, [&] { return sv->stop; });
+      break;
+    }
+    if (progressed) continue;
+    if (busy) {
+      lk.unlock();
+      std::this_thread::sleep_for(std::chrono::microseconds(20));
+      lk.lock();
+    } else {
+      sv->work_cv.wait(lk, [&] { return sv->stop || !sv->pending.empty(); });
+    }
+  }
+}
+
+int upload_chunks(std::vector<unsigned char>& bytes, std::vector<sxc_chunk>& table, std::vector<ChunkMeta>& meta,
+                  const sxc_chunk_desc* in, int32_t n, int max_len) {
+  table.resize((size_t)n);
+  meta.resize((size_t)n);
+  for (int32_t k = 0; k < n; ++k) {
+    const sxc_chunk_desc& c = in[k];
+    if (c.len < 0 || c.len > max_len || (c.len > 0 && !c.seq))
+      return fail(SXC_ERR_ARG, "target chunks must be at most N/2 long, query chunks at most N");
+    sxc_chunk& t = table[(size_t)k];
+    t.off = (int64_t)bytes.size();
+    t.len = c.len;
+    t.flags = 2;
+    for (int i = 0; i < c.len; ++i) {
+      const unsigned char b = (unsigned char)c.seq[i];
+      if (b >= 128) return fail(SXC_ERR_ARG, "sequence byte >= 128");
+      const int cl = sxc::class_of(b);
+      if (sxc::class_has_thirds(cl)) t.flags |= 1;
+      if (cl > 3) t.flags &= ~2;
+      bytes.push_back(b);
+    }
+    while (bytes.size() % 16) bytes.push_back(0);
+    meta[(size_t)k] = ChunkMeta{c.start, c.record, c.record_len};
+  }
+  return SXC_OK;
+}
+
+int server_init(sxc_server* sv, const sxc_chunk_desc* targets, int32_t nt, const sxc_chunk_desc* queries,
+                int32_t nq, const sxc_server_config* cfg) {
+  const int N = sv->prm.N;
+  std::vector<unsigned char> bytes;
+  std::vector<sxc_chunk> ttab, qtab;
+  // targets at most N/2 and queries at most N: every lag of every pair fits the transform
+  int rc = upload_chunks(bytes, ttab, sv->tmeta, targets, nt, N / 2);
+  if (!rc) rc = upload_chunks(bytes, qtab, sv->qmeta, queries, nq, N);
+  if (rc) return rc;
+  bytes.resize(bytes.size() + 16, 0);
+  sv->P = cfg && cfg->max_pairs_per_launch > 0 ? cfg->max_pairs_per_launch : 2048;
+  const int max_slots = cfg && cfg->max_slots > 0 ? cfg->max_slots : 3;
+  size_t free_b = 0, total_b = 0;
+  HIPCHK(hipMemGetInfo(&free_b, &total_b));
+  const int64_t budget = cfg && cfg->hbm_budget > 0 ? cfg->hbm_budget : (int64_t)(free_b / 4 * 3);
+  sv->resident_bytes = (int64_t)bytes.size() + (int64_t)(ttab.size() + qtab.size()) * (int64_t)sizeof(sxc_chunk) +
+                       (int64_t)nt * 4 * N * (int64_t)sizeof(float);
+  sv->slot_bytes = slot_bytes_for(sv->P, N, 64);
+  const int64_t room = budget - sv->resident_bytes;
+  const int nslots = (int)std::min<int64_t>(max_slots, room > 0 ? room / sv->slot_bytes : 0);
+  if (nslots < 1)
+    return fail(SXC_ERR_NOMEM, "HBM budget too small: resident genome and spectra need " +
+                                   std::to_string(sv->resident_bytes) + " bytes plus " +
+                                   std::to_string(sv->slot_bytes) + " per launch slot");
+  hipStream_t s = sv->ctx->stream;
+  HIPCHK(sv->seq.ensure(bytes.size()));
+  HIPCHK(sv->tch.ensure(std::max<size_t>(1, ttab.size()) * sizeof(sxc_chunk)));
+  HIPCHK(sv->qch.ensure(std::max<size_t>(1, qtab.size()) * sizeof(sxc_chunk)));
+  HIPCHK(sv->spectra.ensure(std::max<size_t>(1, ttab.size()) * 4 * (size_t)N * sizeof(float)));
+  HIPCHK(hipMemcpyAsync(sv->seq.p, bytes.data(), bytes.size(), hipMemcpyHostToDevice, s));
+  if (!ttab.empty())
+    HIPCHK(hipMemcpyAsync(sv->tch.p, ttab.data(), ttab.size() * sizeof(sxc_chunk), hipMemcpyHostToDevice, s));
+  if (!qtab.empty())
+    HIPCHK(hipMemcpyAsync(sv->qch.p, qtab.data(), qtab.size() * sizeof(sxc_chunk), hipMemcpyHostToDevice, s));
+  // resident target spectra, computed once in launches of up to 65536 chunks
+  DevBuf which;
+  const int step = 65536;
+  HIPCHK(which.ensure((size_t)std::min<int32_t>(nt, step) * sizeof(int32_t) + 4));
+  std::vector<int32_t> idx;
+  for (int32_t at = 0; at < nt; at += step) {
+    const int n = std::min<int32_t>(step, nt - at);
+    idx.resize((size_t)n);
+    for (int i = 0; i < n; ++i) idx[(size_t)i] = at + i;
+    HIPCHK(hipMemcpyAsync(which.p, idx.data(), (size_t)n * sizeof(int32_t), hipMemcpyHostToDevice, s));
+    sxc::SpecArgs sa{};
+    sa.tab = sv->ctx->d_tab;
+    sa.osc = sv->ctx->d_osc;
+    sa.ent_lut = sv->ctx->d_ent[sv->nb];
+    sa.seq = sv->seq.as<unsigned char>();
+    sa.chunks = sv->tch.as<sxc_chunk>();
+    sa.which = which.as<int32_t>();
+    sa.n = n;
+    sa.spectra = sv->spectra.as<float>();
+    sa.dst = which.as<int32_t>();
+    HIPCHK(sxc::launch_spectra(sv->nb, sa, s));
+    HIPCHK(hipStreamSynchronize(s));  // idx is reused
+  }
+  which.release();
+  sv->slots.resize((size_t)nslots);
+  for (Slot& sl : sv->slots) {
+    rc = slot_alloc(sv, sl);
+    if (rc) return rc;
+  }
+  return SXC_OK;
+}
+
+void server_release(sxc_server* sv) {
+  for (Slot& sl : sv->slots) {
+    if (sl.s) (void)hipStreamSynchronize(sl.s);
+    slot_free(sl);
+  }
+  for (DevBuf* b : {&sv->seq, &sv->tch, &sv->qch, &sv->spectra}) b->release();
+}
+
+}  // namespace
+
+extern "C" {
+
+int sxc_server_create(sxc_ctx* ctx, const sxc_params* prm, const sxc_chunk_desc* targets, int32_t nt,
+                      const sxc_chunk_desc* queries, int32_t nq, const sxc_server_config* cfg, sxc_server** out) {
+  if (!ctx || !prm || !out || nt < 0 || nq < 0 || (nt > 0 && !targets) || (nq > 0 && !queries))
+    return fail(SXC_ERR_ARG, "null argument");
+  *out = nullptr;
+  const int nb = sxc::nb_of(prm->N);
+  if (nb < SXC_MIN_NB || nb > SXC_MAX_NB) return fail(SXC_ERR_ARG, "N must be a power of two in [512, 32768]");
+  if (prm->q_chunk < 0) return fail(SXC_ERR_ARG, "q_chunk must be >= 0");
+  std::lock_guard<std::mutex> g(ctx->mu);
+  HIPCHK(hipSetDevice(ctx->device));
+  sxc_server* sv = new sxc_server();
+  sv->ctx = ctx;
+  sv->prm = *prm;
+  sv->prm.strands = 2;
+  sv->nb = nb;
+  const int rc = server_init(sv, targets, nt, queries, nq, cfg);
+  if (rc) {
+    server_release(sv);
+    delete sv;
+    return rc;
+  }
+  sv->dispatcher = std::thread(dispatch_loop, sv);
+  *out = sv;
+  return SXC_OK;
+}
+
+int sxc_server_submit(sxc_server* sv, const sxc_block* blocks, int32_t nblocks, uint64_t* ticket) {
+  if (!sv || !ticket || nblocks < 0 || (nblocks > 0 && !blocks)) return fail(SXC_ERR_ARG, "null argument");
+  auto r = std::make_unique<Request>();
+  const int32_t nt = (int32_t)sv->tmeta.size(), nq = (int32_t)sv->qmeta.size();
+  for (int32_t k = 0; k < nblocks; ++k) {
+    const sxc_block& b = blocks[k];
+    if (b.target_from < 0 || b.target_to >= nt || b.target_from > b.target_to || b.query_from < 0 ||
+        b.query_to >= nq || b.query_from > b.query_to)
+      return fail(SXC_ERR_ARG, "block outside the server's chunk tables");
+    r->blocks.push_back(b);
+    r->total_pairs += block_pairs(b);
+  }
+  r->t_submit = Clock::now();
+  std::lock_guard<std::mutex> g(sv->mu);
+  if (sv->fatal) return fail(sv->fatal, "server stopped after an error: " + sv->fatal_msg);
+  r->id = sv->next_id++;
+  *ticket = r->id;
+  Request* rp = r.get();
+  sv->reqs[r->id] = std::move(r);
+  if (rp->total_pairs == 0) {
+    rp->complete = true;
+    rp->t_done = Clock::now();
+    ++sv->st_requests;
+  } else {
+    sv->pending.push_back(rp);
+    sv->work_cv.notify_all();
+  }
+  return SXC_OK;
+}
+
+int sxc_server_wait(sxc_server* sv, uint64_t ticket, int32_t timeout_ms, int64_t* n_results, double* latency_ms) {
+  if (!sv) return fail(SXC_ERR_ARG, "null server");
+  std::unique_lock<std::mutex> lk(sv->mu);
+  auto it = sv->reqs.find(ticket);
+  if (it == sv->reqs.end()) return fail(SXC_ERR_ARG, "unknown ticket");
+  Request* r = it->second.get();
+  auto ready = [&] { return r->complete; };
+  if (timeout_ms < 0) {
+    sv->done_cv.wait(lk, ready);
+  } else if (!sv->done_cv.wait_for(lk, std::chrono::milliseconds(timeout_ms), ready)) {
+    return SXC_PENDING;
+  }
+  if (r->error) return fail(r->error, "request failed: " + sv->fatal_msg);
+  if (n_results) *n_results = (int64_t)r->results.size();
+  if (latency_ms) *latency_ms = ms_since(r->t_submit, r->t_done);
+  return SXC_OK;
+}
+
+int sxc_server_results(sxc_server* sv, uint64_t ticket, sxc_result* out, int64_t cap, int64_t* n_out) {
+  if (!sv || !n_out || (cap > 0 && !out)) return fail(SXC_ERR_ARG, "null argument");
+  std::lock_guard<std::mutex> g(sv->mu);
+  auto it = sv->reqs.find(ticket);
+  if (it == sv->reqs.end()) return fail(SXC_ERR_ARG, "unknown ticket");
+  Request* r = it->second.get();
+  if (!r->complete) return SXC_PENDING;
+  if (r->error) return fail(r->error, "request failed: " + sv->fatal_msg);
+  *n_out = (int64_t)r->results.size();
+  if (*n_out > cap) return fail(SXC_ERR_CAPACITY, "output capacity too small");
+  if (*n_out) memcpy(out, r->results.data(), r->results.size() * sizeof(sxc_result));
+  sv->reqs.erase(it);
+  return SXC_OK;
+}
+
+int sxc_server_stats(sxc_server* sv, sxc_server_info* out) {
+  if (!sv || !out) return fail(SXC_ERR_ARG, "null argument");
+  std::lock_guard<std::mutex> g(sv->mu);
+  out->requests_done = sv->st_requests;
+  out->pairs_done = sv->st_pairs;
+  out->launches = sv->st_launches;
+  out->results = sv->st_results;
+  out->resident_bytes = sv->resident_bytes;
+  out->slot_bytes = sv->slot_bytes;
+  out->slots = (int32_t)sv->slots.size();
+  out->max_pairs_per_launch = sv->P;
+  out->device_ms = sv->st_device_ms;
+  return SXC_OK;
+}
+
+void sxc_server_destroy(sxc_server* sv) {
+  if (!sv) return;
+  {
+    std::lock_guard<std::mutex> g(sv->mu);
+    sv->stop = true;
+  }
+  sv->work_cv.notify_all();
+  if (sv->dispatcher.joinable()) sv->dispatcher.join();
+  (void)hipSetDevice(sv->ctx->device);
+  server_release(sv);
+  delete sv;
+}
+
+}  // extern "C"

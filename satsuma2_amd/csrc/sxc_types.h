@@ -31,7 +31,21 @@ typedef struct {
   int32_t q_record;       // query record id (t_result.query_id)
   int32_t q_record_len;   // ChunkManager::GetSize(query record) (t_result.query_size, RCQuery)
   int32_t fast;           // t_pair.fast: use cutoff_fast
+  int32_t q_tail;         // query signal tail (index into the tail table), -1: none
 } sxc_pair_dev;
+
+// Signal tail: segments [first, first + n) of the segment table.
+typedef struct {
+  int32_t first;
+  int32_t n;
+} sxc_tail_dev;
+
+// Tail segment: positions below `end` (and at or above the previous segment's end, or
+// the chunk length) come from source signal `src` (4*N floats per source).
+typedef struct {
+  int32_t end;
+  int32_t src;
+} sxc_tail_seg_dev;
 
 // Reference t_result (analysis/WorkQueue.h:23-33), 72 bytes on LP64.
 typedef struct {

@@ -12,6 +12,12 @@ Workloads, named by what they are:
                          [256, 16384], GC U(0.35, 0.55) per pair, planted as above,
                          10% with a tandem repeat (period 1-6, length 100-1000).
 
+  * ``syntenic_genomes``  two genomes (one record each, default 50 Mbp) at GC 0.41
+                         sharing syntenic blocks: lengths log-uniform 5-200 kbp,
+                         identity U(0.75, 0.95), 1% indels, 10% inverted, placed along
+                         a noisy diagonal (later blocks overwrite earlier ones where
+                         they overlap).
+
 Base probabilities at GC = g: P(C) = P(G) = g/2, P(A) = P(T) = (1 - g)/2.
 """
 from __future__ import annotations
@@ -128,3 +134,61 @@ def write_fasta(path: str, records, names=None, width: int = 60) -> None:
             f.write(b">" + name.encode() + b"\n")
             for k in range(0, len(s), width):
                 f.write(s[k: k + width] + b"\n")
+
+
+def mutate_fast(rng: np.random.Generator, seg: np.ndarray, ident: float, indel_rate: float) -> np.ndarray:
+    """mutate() for long segments: substitutions vectorised, indels (50/50 insertion or
+    deletion, geometric length p = 0.5) applied event by event."""
+    seg = seg.copy()
+    sub = rng.random(seg.size) < (1.0 - ident)
+    if sub.any():
+        cur = np.searchsorted(ALPHA, seg[sub])
+        seg[sub] = ALPHA[(cur + rng.integers(1, 4, size=cur.size)) % 4]
+    ev = np.nonzero(rng.random(seg.size) < indel_rate)[0]
+    if ev.size == 0:
+        return seg
+    lens = rng.geometric(0.5, size=ev.size)
+    ins = rng.random(ev.size) < 0.5
+    pieces, prev = [], 0
+    for pos, ln, is_ins in zip(ev.tolist(), lens.tolist(), ins.tolist()):
+        if pos < prev:
+            continue
+        pieces.append(seg[prev:pos + 1])
+        if is_ins:
+            pieces.append(ALPHA[rng.integers(0, 4, size=ln)])
+            prev = pos + 1
+        else:
+            prev = pos + 1 + ln
+    pieces.append(seg[prev:])
+    return np.concatenate(pieces)
+
+
+def revcomp(a: np.ndarray) -> np.ndarray:
+    return ALPHA[3 - np.searchsorted(ALPHA, a[::-1])]
+
+
+def random_seq_fast(rng: np.random.Generator, n: int, gc: float) -> np.ndarray:
+    """random_seq by inverse CDF on uniform bytes (for genome-sized n)."""
+    cum = np.array([(1 - gc) / 2, 0.5, (1 + gc) / 2]) * 256.0
+    u = rng.integers(0, 256, size=n, dtype=np.uint8)
+    return ALPHA[np.searchsorted(cum, u, side="right")]
+
+
+def syntenic_genomes(length: int = 50_000_000, n_blocks: int = 2000, seed: int = 4, gc: float = 0.41):
+    """(target bytes, query bytes, blocks) with blocks = list of (t_start, q_start, len, inverted)."""
+    rng = np.random.default_rng(seed)
+    t = random_seq_fast(rng, length, gc)
+    q = random_seq_fast(rng, length, gc)
+    lens = np.exp(rng.uniform(np.log(5_000), np.log(200_000), n_blocks)).astype(np.int64)
+    blocks = []
+    for k in range(n_blocks):
+        ln = int(lens[k])
+        ts = int(rng.integers(0, length - ln))
+        qs = int(min(max(ts + rng.normal(0, 2e6), 0), length - ln))
+        seg = mutate_fast(rng, t[ts:ts + ln], float(rng.uniform(0.75, 0.95)), 0.01)[:ln]
+        inv = bool(rng.random() < 0.1)
+        if inv:
+            seg = revcomp(seg)
+        q[qs:qs + seg.size] = seg
+        blocks.append((ts, qs, int(seg.size), inv))
+    return t.tobytes(), q.tobytes(), blocks

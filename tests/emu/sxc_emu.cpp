@@ -67,8 +67,20 @@ int run_batch(const sxc::HostBatch& hb, const sxc_params& prm, std::vector<sxc_r
   std::vector<int32_t> which(hb.tchunks.size()), dst(hb.tchunks.size());
   for (size_t i = 0; i < which.size(); ++i) which[i] = dst[i] = (int)i;
   std::vector<char> sm(SM::BYTES);
+  // signals of the sequences named by signal tails
+  std::vector<float> src_sig((size_t)hb.schunks.size() * 4 * G::N);
+  sxc::EncodeArgs ea{&T.tab, T.ent[NB].data(), hb.seq.data(), hb.schunks.data(), 0, hb.sstrand.data(),
+                     src_sig.data()};
+  for (int k = 0; k < (int)hb.schunks.size(); ++k) {
+    EmuExec<sxc::Regs<NB>> ex(G::NT);
+    memset(sm.data(), 0xCD, sm.size());
+    sxc::encode_body<NB>(ex, sm.data(), ea, k);
+  }
+  const bool tails = !hb.tails.empty();
   sxc::SpecArgs sa{&T.tab, T.osc.data(), T.ent[NB].data(), hb.seq.data(), hb.tchunks.data(), which.data(),
-                   (int)which.size(), spectra.data(), dst.data(), nullptr};
+                   (int)which.size(), spectra.data(), dst.data(), nullptr,
+                   tails ? hb.ttail.data() : nullptr, tails ? hb.tails.data() : nullptr,
+                   tails ? hb.segs.data() : nullptr, tails ? src_sig.data() : nullptr};
   for (int k = 0; k < sa.n; ++k) {
     EmuExec<sxc::Regs<NB>> ex(G::NT);
     memset(sm.data(), 0xCD, sm.size());
@@ -95,6 +107,11 @@ int run_batch(const sxc::HostBatch& hb, const sxc_params& prm, std::vector<sxc_r
   pa.cands = cands.data();
   pa.ncand = ncand.data();
   pa.xc_out = xc_out;
+  if (tails) {
+    pa.tails = hb.tails.data();
+    pa.segs = hb.segs.data();
+    pa.src_sig = src_sig.data();
+  }
   for (int ps = 0; ps < total; ++ps) {
     EmuExec<sxc::Regs<NB>> ex(G::NT);
     memset(sm.data(), 0xCD, sm.size());
@@ -145,10 +162,11 @@ extern "C" {
 
 // Mirrors sxc_align_batch.  xc_out (optional) receives N floats per (pair, strand);
 // stats (optional) receives {candidates, runs} per (pair, strand).
-int emu_align_batch(const sxc_pair* pairs, int n, const sxc_params* prm, sxc_result* out, int64_t cap,
-                    int64_t* n_out, float* xc_out, int32_t* stats) {
+int emu_align_batch_tails(const sxc_pair* pairs, int n, const sxc_params* prm, const sxc_signal_tail* t_tail,
+                          const sxc_signal_tail* q_tail, sxc_result* out, int64_t cap, int64_t* n_out, float* xc_out,
+                          int32_t* stats) {
   sxc::HostBatch hb;
-  if (!sxc::pack_batch(pairs, n, prm->N, hb).empty()) return -1;
+  if (!sxc::pack_batch(pairs, n, prm->N, hb, t_tail, q_tail).empty()) return -1;
   std::vector<sxc_result_dev> res;
   const int slot_cap = 4096;
   int rc;
@@ -159,6 +177,7 @@ int emu_align_batch(const sxc_pair* pairs, int n, const sxc_params* prm, sxc_res
     case 12: rc = run_batch<12>(hb, *prm, res, slot_cap, xc_out, stats); break;
     case 13: rc = run_batch<13>(hb, *prm, res, slot_cap, xc_out, stats); break;
     case 14: rc = run_batch<14>(hb, *prm, res, slot_cap, xc_out, stats); break;
+    case 15: rc = run_batch<15>(hb, *prm, res, slot_cap, xc_out, stats); break;
     default: return -1;
   }
   if (rc) return rc;
@@ -166,6 +185,11 @@ int emu_align_batch(const sxc_pair* pairs, int n, const sxc_params* prm, sxc_res
   if ((int64_t)res.size() > cap) return -3;
   memcpy(out, res.data(), res.size() * sizeof(sxc_result));
   return 0;
+}
+
+int emu_align_batch(const sxc_pair* pairs, int n, const sxc_params* prm, sxc_result* out, int64_t cap,
+                    int64_t* n_out, float* xc_out, int32_t* stats) {
+  return emu_align_batch_tails(pairs, n, prm, nullptr, nullptr, out, cap, n_out, xc_out, stats);
 }
 
 // FFT of one signal through stage A + rounds (spectra_body with a given signal).
@@ -177,7 +201,8 @@ int emu_fft_forward(const float* in, float* out, int N) {
   memcpy(sig.data(), in, sizeof(float) * N);
   auto go = [&](auto nbc) {
     constexpr int NB = decltype(nbc)::value;
-    sxc::SpecArgs sa{&T.tab, T.osc.data(), T.ent[NB].data(), nullptr, &ck, &which, 1, spec.data(), &dst, sig.data()};
+    sxc::SpecArgs sa{&T.tab, T.osc.data(), T.ent[NB].data(), nullptr, &ck, &which, 1, spec.data(), &dst, sig.data(),
+                     nullptr, nullptr, nullptr, nullptr};
     std::vector<char> sm(sxc::Smem<NB>::BYTES);
     EmuExec<sxc::Regs<NB>> ex(sxc::Geo<NB>::NT);
     sxc::spectra_body<NB>(ex, sm.data(), sa, 0);
@@ -189,6 +214,7 @@ int emu_fft_forward(const float* in, float* out, int N) {
     case 12: go(std::integral_constant<int, 12>{}); break;
     case 13: go(std::integral_constant<int, 13>{}); break;
     case 14: go(std::integral_constant<int, 14>{}); break;
+    case 15: go(std::integral_constant<int, 15>{}); break;
     default: return -1;
   }
   memcpy(out, spec.data(), sizeof(float) * N);
@@ -197,52 +223,11 @@ int emu_fft_forward(const float* in, float* out, int N) {
 
 }  // extern "C"
 
-#ifdef SXC_EMU_MAIN
-// Standalone checker (built with AddressSanitizer): reads pairs from a file written by
-// tests/test_emu.py, runs the emulated kernels, writes the t_result records.
-// input: int32 N, strands, n; double cutoff, cutoff_fast, min_prob, target_total;
-//        per pair: int32 tlen, qlen, t_chunk_start, q_chunk_start, t_record, q_record,
-//        q_record_len, fast; then tlen + qlen bytes.
-#include <stdio.h>
-int main(int argc, char** argv) {
-  if (argc < 3) return 2;
-  FILE* f = fopen(argv[1], "rb");
-  if (!f) return 2;
-  int32_t hdr[3];
-  double d[4];
-  if (fread(hdr, 4, 3, f) != 3 || fread(d, 8, 4, f) != 4) return 2;
-  std::vector<sxc_pair> pairs(hdr[2]);
-  std::vector<std::vector<char>> store;
-  for (int k = 0; k < hdr[2]; ++k) {
-    int32_t m[8];
-    if (fread(m, 4, 8, f) != 8) return 2;
-    store.emplace_back(m[0] + 1);
-    char* t = store.back().data();
-    if (fread(t, 1, m[0], f) != (size_t)m[0]) return 2;
-    store.emplace_back(m[1] + 1);
-    char* q = store.back().data();
-    if (fread(q, 1, m[1], f) != (size_t)m[1]) return 2;
-    pairs[k] = sxc_pair{t, m[0], q, m[1], m[2], m[3], m[4], m[5], m[6], m[7]};
-  }
-  fclose(f);
-  sxc_params prm{hdr[0], 4096, 0, hdr[1], d[0], d[1], d[2], d[3]};
-  std::vector<sxc_result> out(1 << 20);
-  int64_t n = 0;
-  const int rc = emu_align_batch(pairs.data(), hdr[2], &prm, out.data(), (int64_t)out.size(), &n, nullptr, nullptr);
-  if (rc) return 3;
-  FILE* o = fopen(argv[2], "wb");
-  fwrite(&n, 8, 1, o);
-  fwrite(out.data(), sizeof(sxc_result), (size_t)n, o);
-  fclose(o);
-  return 0;
-}
-#endif
 
-#ifdef SXC_EMU_ABI
+
 // The subset of the libsxc_hip.so C ABI that the host programs (slave, standalone
 // aligner) call, served by the emulator, so their protocol and file handling can be
 // tested on a machine without a GPU.  Test infrastructure only.
-#include "../../include/sxc.h"
 
 struct sxc_ctx {
   int device;
@@ -251,6 +236,7 @@ struct sxc_batch {
   std::vector<sxc_pair> pairs;
   sxc_params prm;
   std::vector<sxc_result> res;
+  std::vector<sxc_signal_tail> t_tail, q_tail;
 };
 
 extern "C" {
@@ -262,19 +248,27 @@ int sxc_create(sxc_ctx** out, int device) {
 void sxc_destroy(sxc_ctx* ctx) { delete ctx; }
 const char* sxc_last_error(void) { return g_emu_err; }
 const char* sxc_version(void) { return "emulator"; }
-int sxc_batch_create(sxc_ctx*, const sxc_pair* pairs, int32_t n, const sxc_params* prm, sxc_batch** out) {
-  *out = new sxc_batch{std::vector<sxc_pair>(pairs, pairs + n), *prm, {}};
+int sxc_batch_create_tails(sxc_ctx*, const sxc_pair* pairs, int32_t n, const sxc_params* prm,
+                           const sxc_signal_tail* t_tail, const sxc_signal_tail* q_tail, sxc_batch** out) {
+  *out = new sxc_batch{std::vector<sxc_pair>(pairs, pairs + n), *prm, {}, {}, {}};
+  if (t_tail) (*out)->t_tail.assign(t_tail, t_tail + n);
+  if (q_tail) (*out)->q_tail.assign(q_tail, q_tail + n);
   return SXC_OK;
+}
+int sxc_batch_create(sxc_ctx* c, const sxc_pair* pairs, int32_t n, const sxc_params* prm, sxc_batch** out) {
+  return sxc_batch_create_tails(c, pairs, n, prm, nullptr, nullptr, out);
 }
 int sxc_batch_run(sxc_batch* b) {
   std::vector<sxc_result> r((size_t)b->pairs.size() * 64 + 1024);
   int64_t n = 0;
-  int rc = emu_align_batch(b->pairs.data(), (int)b->pairs.size(), &b->prm, r.data(), (int64_t)r.size(), &n,
-                           nullptr, nullptr);
+  const sxc_signal_tail* tt = b->t_tail.empty() ? nullptr : b->t_tail.data();
+  const sxc_signal_tail* qt = b->q_tail.empty() ? nullptr : b->q_tail.data();
+  int rc = emu_align_batch_tails(b->pairs.data(), (int)b->pairs.size(), &b->prm, tt, qt, r.data(), (int64_t)r.size(),
+                                 &n, nullptr, nullptr);
   if (rc == SXC_ERR_CAPACITY) {
     r.resize((size_t)n);
-    rc = emu_align_batch(b->pairs.data(), (int)b->pairs.size(), &b->prm, r.data(), (int64_t)r.size(), &n, nullptr,
-                         nullptr);
+    rc = emu_align_batch_tails(b->pairs.data(), (int)b->pairs.size(), &b->prm, tt, qt, r.data(), (int64_t)r.size(),
+                               &n, nullptr, nullptr);
   }
   if (rc) {
     g_emu_err = "emulated batch failed";
@@ -296,4 +290,3 @@ int sxc_align_batch(sxc_ctx*, const sxc_pair* pairs, int32_t n, const sxc_params
   return emu_align_batch(pairs, n, prm, out, cap, n_out, nullptr, nullptr);
 }
 }  // extern "C"
-#endif

@@ -15,6 +15,8 @@ Fixtures:
   uniform_pairs.npz  the slave path (chunking, both strands, FilterMatches) on 64 pairs
   samples_slice.npz  40 kbp x 40 kbp slice of the reference's samples/ (soft-masked, with N)
                      through the slave path and through the standalone CLI's match file
+  cli_records.npz    12 + 12 records of uneven lengths through the standalone CLI under four
+                     flag sets (match files, MultiMatches v3)
 """
 from __future__ import annotations
 
@@ -287,6 +289,60 @@ def gen_samples(tmp, n=40000):
                         pairs=np.array([[0, 6, 0, 4, 0], [7, 13, 5, 9, 1]], np.int32))
 
 
+CLI_RUNS = [
+    ("default", []),
+    ("small_chunks", ["-t_chunk", "2048", "-q_chunk", "1024", "-l", "60", "-min_prob", "0.999"]),
+    ("same_only", ["-same_only", "true", "-cutoff", "1.6"]),
+    ("blocks", ["-nblocks", "3", "-block", "1", "-nblocks_query", "2", "-block_query", "0"]),
+]
+
+
+def cli_records(seed=5):
+    """Query and target record sets of uneven lengths (many shorter than a chunk, some in
+    descending order, some under 1024 bp) cut from one random genome, the query side
+    mutated (80-95% identity, indels), soft-masked runs and N runs, shared record names."""
+    rng = np.random.default_rng(seed)
+    base = synth.random_seq(rng, 120_000, 0.41)
+    tl = [12000, 4096, 3000, 9000, 2500, 2000, 700, 6000, 1500, 900, 5000, 3500]
+    ql = [7000, 3100, 2900, 8192, 2600, 1800, 640, 5000, 4000, 1200, 950, 4500]
+    T, Q, tn, qn = [], [], [], []
+    pos = 0
+    for k, (a, b) in enumerate(zip(tl, ql)):
+        t = base[pos:pos + a].copy()
+        off = int(rng.integers(0, max(1, a // 3)))
+        q = synth.mutate(rng, base[pos + off:pos + off + b], float(rng.uniform(0.8, 0.95)), 0.01)[:b]
+        if k % 3 == 1:  # inverted homology
+            q = synth.ALPHA[3 - np.searchsorted(synth.ALPHA, q[::-1])]
+        tb, qb = bytearray(t.tobytes()), bytearray(q.tobytes())
+        if k % 4 == 0 and len(tb) > 600:  # soft mask
+            i = int(rng.integers(0, len(tb) - 500))
+            tb[i:i + 400] = tb[i:i + 400].lower()
+        if k % 5 == 2 and len(qb) > 300:  # N run
+            i = int(rng.integers(0, len(qb) - 200))
+            qb[i:i + 150] = b"N" * 150
+        T.append(bytes(tb))
+        Q.append(bytes(qb))
+        tn.append(f"chr{k % 5}")
+        qn.append(f"chr{(k + 1) % 5}")
+        pos += a + 500
+    return T, Q, tn, qn
+
+
+def gen_cli(tmp):
+    T, Q, tn, qn = cli_records()
+    qf, tf = os.path.join(tmp, "cq.fa"), os.path.join(tmp, "ct.fa")
+    synth.write_fasta(qf, Q, qn)
+    synth.write_fasta(tf, T, tn)
+    d = {"q_fasta": np.frombuffer(open(qf, "rb").read(), np.uint8),
+         "t_fasta": np.frombuffer(open(tf, "rb").read(), np.uint8)}
+    for name, flags in CLI_RUNS:
+        out = os.path.join(tmp, f"{name}.out")
+        run([CLI, "-q", qf, "-t", tf, "-o", out, *flags])
+        d[f"out_{name}"] = np.frombuffer(open(out, "rb").read(), np.uint8)
+        d[f"flags_{name}"] = np.array(" ".join(flags))
+    np.savez_compressed(os.path.join(HERE, "cli_records.npz"), **d)
+
+
 def main():
     subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "ref", f"REF={REF}"], check=True)
     with tempfile.TemporaryDirectory() as tmp:
@@ -296,6 +352,7 @@ def main():
         gen_uneven(tmp)
         gen_uniform(tmp)
         gen_samples(tmp)
+        gen_cli(tmp)
     for f in sorted(os.listdir(HERE)):
         if f.endswith(".npz"):
             print(f, os.path.getsize(os.path.join(HERE, f)))

@@ -30,18 +30,52 @@ def _stale(target, deps):
     return not os.path.exists(target) or os.path.getmtime(target) < max(os.path.getmtime(d) for d in deps)
 
 
-def emu_program(name: str, sanitize: bool = False) -> str:
-    """The program `name` linked against the emulator (optionally with AddressSanitizer)."""
+FLAGS = ["-std=c++17", "-ffp-contract=off", "-Wno-unknown-pragmas"]
+OPT = {False: ["-O2", "-fPIC"], True: ["-O1", "-fsanitize=address", "-fno-omit-frame-pointer"]}
+
+
+def emu_objects(sanitize: bool = False) -> list:
+    """The emulator (kernel bodies + its C ABI) and the host tables, compiled once per flavour."""
     os.makedirs(BUILD, exist_ok=True)
-    exe = os.path.join(BUILD, name + ("_asan" if sanitize else "") + "_emu")
-    srcs = [os.path.join(CSRC, s) for s in PROGRAM_SOURCES[name]] + [
-        os.path.join(EMU, "sxc_emu.cpp"), os.path.join(CSRC, "sxc_tables.cpp")]
-    if _stale(exe, srcs + KERNEL_DEPS + HOST_DEPS):
-        opt = ["-O1", "-g", "-fsanitize=address,undefined", "-fno-omit-frame-pointer"] if sanitize else ["-O2"]
-        subprocess.run(["g++", *opt, "-std=c++17", "-ffp-contract=off", "-Wno-unknown-pragmas", "-DSXC_EMU_ABI",
-                        "-pthread", "-o", exe + ".tmp", *srcs], check=True)
+    tag = "asan" if sanitize else "o2"
+    objs = []
+    for src in (os.path.join(EMU, "sxc_emu.cpp"), os.path.join(CSRC, "sxc_tables.cpp")):
+        obj = os.path.join(BUILD, f"{os.path.basename(src)}.{tag}.o")
+        if _stale(obj, [src] + KERNEL_DEPS + [os.path.join(ROOT, "include", "sxc.h")]):
+            subprocess.run(["g++", *OPT[sanitize], *FLAGS, "-c", "-o", obj + ".tmp", src], check=True)
+            os.replace(obj + ".tmp", obj)
+        objs.append(obj)
+    return objs
+
+
+def _link(exe, srcs, objs, sanitize):
+    if _stale(exe, srcs + objs + HOST_DEPS):
+        subprocess.run(["g++", *OPT[sanitize], *FLAGS, "-pthread", "-o", exe + ".tmp", *srcs, *objs], check=True)
         os.replace(exe + ".tmp", exe)
     return exe
+
+
+def emu_program(name: str, sanitize: bool = False) -> str:
+    """The program `name` linked against the emulator (optionally with AddressSanitizer)."""
+    exe = os.path.join(BUILD, name + ("_asan" if sanitize else "") + "_emu")
+    srcs = [os.path.join(CSRC, s) for s in PROGRAM_SOURCES[name]]
+    return _link(exe, srcs, emu_objects(sanitize), sanitize)
+
+
+def emu_library() -> str:
+    """libsxc_emu.so: the emulator's entry points for ctypes."""
+    so = os.path.join(BUILD, "libsxc_emu.so")
+    objs = emu_objects(False)
+    if _stale(so, objs):
+        subprocess.run(["g++", "-shared", "-o", so + ".tmp", *objs], check=True)
+        os.replace(so + ".tmp", so)
+    return so
+
+
+def emu_checker() -> str:
+    """tests/emu/emu_check.cpp against the AddressSanitizer emulator."""
+    exe = os.path.join(BUILD, "emu_check")
+    return _link(exe, [os.path.join(EMU, "emu_check.cpp")], emu_objects(True), True)
 
 
 def seqio_check() -> str:

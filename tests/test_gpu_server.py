@@ -1,0 +1,143 @@
+"""The serving loop (sxc_server_*) on the GPU.
+
+Requests of t_pair blocks against resident genomes must return exactly the records the
+batch path (sxc_align_batch, itself checked against the reference slave) produces for the
+same chunk pairs -- alone, and in a mix where launches carry pairs of many requests and a
+request spans several launches."""
+import threading
+
+import numpy as np
+import pytest
+
+from satsuma2_amd import synth
+from satsuma2_amd._lib import SxcError
+
+pytestmark = pytest.mark.gpu
+
+
+def _chunks(seq: bytes, size: int, step: int, record: int):
+    """ChunkManager::ChunkItSelect (analysis/SeqChunk.cc:72-165) for one pure-ACGT record."""
+    out = []
+    for j in range(1 + len(seq) // step):
+        s = j * step
+        out.append((seq[s:min(s + size, len(seq))], s, record, len(seq)))
+    return out
+
+
+@pytest.fixture(scope="module")
+def genomes():
+    T, Q, _ = synth.syntenic_genomes(length=600_000, n_blocks=12, seed=9)
+    # a record pair carrying 1500 bp of a period-3 tandem repeat: the reverse strand keeps
+    # 151 records (oracle), more than a launch slot holds at first
+    rng = np.random.default_rng(3)
+
+    def embedded_repeat(period=3, n=1500):
+        r = synth.random_seq(rng, 4096, 0.41).tobytes()
+        unit = synth.random_seq(rng, period, 0.5).tobytes()
+        return r[:1000] + (unit * (n // period + 1))[:n] + r[1000 + n:]
+
+    trep, qrep = embedded_repeat(), embedded_repeat()
+    tch = _chunks(T, 4096, 3072, 0) + _chunks(trep, 4096, 3072, 1)
+    qch = _chunks(Q, 4096, 4096, 0) + _chunks(qrep, 4096, 4096, 1)
+    return T, Q, tch, qch
+
+
+@pytest.fixture(scope="module")
+def served(genomes):
+    from satsuma2_amd.api import Chunk, Context, Server, SlaveParams
+    T, Q, tch, qch = genomes
+    ctx = Context(0)
+    params = SlaveParams(target_total=float(len(T) + 4096))
+    srv = Server(ctx, params, [Chunk(s, st, r, rl) for s, st, r, rl in tch],
+                 [Chunk(s, st, r, rl) for s, st, r, rl in qch], max_pairs_per_launch=256, max_slots=3)
+    yield ctx, params, srv
+    srv.close()
+    ctx.close()
+
+
+def _requests(tch, qch, rng, n=24):
+    nt, nq = len(tch), len(qch)
+    reqs = []
+    for k in range(n):
+        if k % 3 == 0:  # seed line: one target chunk against a run of query chunks (fast)
+            x = int(rng.integers(0, nt - 2))
+            y = int(rng.integers(0, nq - 40))
+            reqs.append([(x, x, y, y + 39, 1)])
+        else:  # neighbourhood block around the diagonal
+            x = int(rng.integers(0, nt - 24))
+            y = int(min(max(x * 3 // 4 + rng.integers(-8, 8), 0), nq - 24))
+            reqs.append([(x, x + 23, y, y + 23, 0)])
+    reqs.append([(nt - 2, nt - 1, nq - 2, nq - 1, 0)])  # the repeat records
+    reqs.append([(0, 1, 0, 1, 0), (5, 6, 3, 3, 1)])    # two blocks in one request
+    return reqs
+
+
+def _batch_records(ctx, params, tch, qch, blocks):
+    from satsuma2_amd.api import ChunkPair, HomologyByXCorr
+    h = HomologyByXCorr(ctx, params)
+    pairs = [ChunkPair(tch[ti][0], qch[qi][0], tch[ti][1], qch[qi][1], tch[ti][2], qch[qi][2], qch[qi][3], fast=f)
+             for tf, tt, qf, qt, f in blocks for qi in range(qf, qt + 1) for ti in range(tf, tt + 1)]
+    return h.align_batch(pairs)
+
+
+def test_server_alone_equals_batch_and_mix(genomes, served):
+    T, Q, tch, qch = genomes
+    ctx, params, srv = served
+    reqs = _requests(tch, qch, np.random.default_rng(3))
+    alone = []
+    for blocks in reqs:
+        t = srv.submit(blocks)
+        alone.append(srv.results(t))
+    assert sum(len(a) for a in alone) > 0
+    assert len(alone[-2]) > 64  # the repeat pair overflowed the first slot capacity
+    for blocks, a in zip(reqs, alone):
+        assert a.tobytes() == _batch_records(ctx, params, tch, qch, blocks).tobytes()
+    # the mix: every request at once, in shuffled order, from two submitting threads
+    order = np.random.default_rng(5).permutation(len(reqs))
+    tickets = {}
+    lock = threading.Lock()
+
+    def submit(ids):
+        for i in ids:
+            t = srv.submit(reqs[i])
+            with lock:
+                tickets[i] = t
+
+    th = [threading.Thread(target=submit, args=(order[k::2],)) for k in range(2)]
+    for x in th:
+        x.start()
+    for x in th:
+        x.join()
+    for i in range(len(reqs)):
+        assert srv.results(tickets[i]).tobytes() == alone[i].tobytes(), i
+    st = srv.stats()
+    assert st["slots"] == 3 and st["max_pairs_per_launch"] == 256
+    assert st["pairs_done"] >= 2 * sum((b[1] - b[0] + 1) * (b[3] - b[2] + 1) for r in reqs for b in r)
+
+
+def test_server_latency_and_edge_requests(served, genomes):
+    T, Q, tch, qch = genomes
+    ctx, params, srv = served
+    t = srv.submit([])
+    n, lat = srv.wait(t)
+    assert n == 0 and lat >= 0
+    assert len(srv.results(t)) == 0
+    with pytest.raises(SxcError):
+        srv.submit([(0, len(tch), 0, 0, 0)])
+    with pytest.raises(SxcError):
+        srv.submit([(3, 2, 0, 0, 0)])
+    t = srv.submit([(0, 23, 0, 23, 0)])
+    n, lat = srv.wait(t)
+    assert lat > 0
+    assert len(srv.results(t)) == n
+
+
+def test_server_budget_rejects_small_hbm(genomes):
+    from satsuma2_amd.api import Chunk, Context, Server, SlaveParams
+    T, Q, tch, qch = genomes
+    ctx = Context(0)
+    with pytest.raises(SxcError) as e:
+        Server(ctx, SlaveParams(target_total=1e6), [Chunk(s, st, r, rl) for s, st, r, rl in tch],
+               [Chunk(s, st, r, rl) for s, st, r, rl in qch], hbm_budget=64 << 20)
+    assert e.value.code == -4
+    ctx.close()

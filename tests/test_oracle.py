@@ -332,3 +332,70 @@ def test_samples_slice_slave_golden(golden):
                                   t_record=trec, q_record=qrec, q_record_len=len(q), fast=int(fast)))
     got = np.concatenate(oracle.align_batch(pairs, prm, threads=4))
     assert got.tobytes() == g["results"].tobytes()
+
+
+def _read_fasta(b: bytes):
+    recs, names = [], []
+    for line in b.split(b"\n"):
+        if line.startswith(b">"):
+            names.append(line[1:].decode())
+            recs.append([])
+        elif line:
+            recs[-1].append(line)
+    return [b"".join(r).upper() for r in recs], names
+
+
+def _mm_records(b: bytes):
+    """(ints, matches, prob, ident) of a MultiMatches v3 file (analysis/SequenceMatch.cc:320-360)."""
+    o = 4
+    for _ in range(2):
+        n = int(np.frombuffer(b, "<i4", 1, o)[0])
+        o += 4
+        for _ in range(n):
+            o += 8 + int(np.frombuffer(b, "<i8", 1, o)[0])
+    n = int(np.frombuffer(b, "<i4", 1, o)[0])
+    dt = np.dtype([("ints", "<i4", (7,)), ("matches", "<f8"), ("prob", "<f8"), ("ident", "<f8")])
+    return np.frombuffer(b, dt, n, o + 4)
+
+
+def test_cli_signal_tails_golden(golden):
+    """The standalone CLI reuses one CCSignal per side, so a chunk shorter than an earlier
+    one correlates with that chunk's signal in its tail (tools/analysis/HomologyByXCorr.cc:
+    706-716, 780-798; analysis/CrossCorr.cc:118-131).  The oracle with tails built by
+    replaying the CLI's SetSequence calls must reproduce the reference's match file; without
+    the tails it must not (the record set has chunks shorter than their predecessors)."""
+    g = golden("cli_records.npz")
+    T, _ = _read_fasta(g["t_fasta"].tobytes())
+    Q, _ = _read_fasta(g["q_fasta"].tobytes())
+    tch = [c for c in chunk_records(T, 4096, 2048) if c[2]]
+    qch = [c for c in chunk_records(Q, 4096, 0) if c[2]]
+
+    def replay(stack, seq, strand):
+        while stack and len(stack[-1][0]) <= len(seq):
+            stack.pop()
+        tail = [(len(s), s, st) for s, st in reversed(stack)]
+        stack.append((seq, strand))
+        return tail
+
+    pairs, tt, qt = [], [], []
+    ts, qs = [], []
+    for trec, tstart, tseq in tch:
+        ttail = replay(ts, tseq, 0)
+        for qrec, qstart, qseq in qch:
+            qtail = replay(qs, qseq, 0)
+            assert replay(qs, qseq, 1) == qtail
+            pairs.append(dict(tseq=tseq, qseq=qseq, t_chunk_start=tstart, q_chunk_start=qstart, t_record=trec,
+                              q_record=qrec, q_record_len=len(Q[qrec])))
+            tt.append(ttail)
+            qt.append(qtail)
+    assert sum(1 for x in tt if x) > 0 and sum(1 for x in qt if x) > 0
+    prm = oracle.make_params(q_chunk=0, min_prob=0.9999, target_total=float(sum(len(t) for t in T)))
+    exp = _mm_records(g["out_default"].tobytes())
+    got = np.concatenate(oracle.align_batch(pairs, prm, threads=4, t_tails=tt, q_tails=qt))
+    ints = np.stack([got["target_id"], got["query_id"], got["query_size"], got["tstart"], got["qstart"], got["len"],
+                     got["reverse"]], 1).astype(np.int64).astype(np.int32)
+    assert np.array_equal(ints, exp["ints"])
+    assert np.array_equal(got["prob"], exp["prob"]) and np.array_equal(got["ident"], exp["ident"])
+    assert np.array_equal(got["ident"] * got["len"].astype(np.float64), exp["matches"])
+    plain = np.concatenate(oracle.align_batch(pairs, prm, threads=4))
+    assert len(plain) != len(got) or plain.tobytes() != got.tobytes()

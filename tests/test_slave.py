@@ -14,7 +14,8 @@ import numpy as np
 import pytest
 
 import hostbuild
-from conftest import ROOT
+import oracle
+from conftest import ROOT, chunk_records
 from master_stub import MasterStub
 from satsuma2_amd import synth
 
@@ -51,15 +52,31 @@ def test_slave_samples_matches_reference_slave(golden, tmp_path, per_exchange):
     assert got.tobytes() == exp.tobytes()
 
 
-def test_slave_asan_block_split_and_bad_block(golden, tmp_path):
-    """AddressSanitizer build; the two blocks split into four, plus a block naming chunks
-    that do not exist (skipped with a message, as no reference chunk exists for it)."""
+def test_slave_block_split_and_bad_block(golden, tmp_path):
+    """The two blocks split into four, one per exchange, plus a block naming chunks that do
+    not exist (skipped with a message, as no reference chunk exists for it)."""
     g, qf, tf = write_inputs(golden, tmp_path)
-    exe = hostbuild.emu_program("HomologyByXCorrSlave", sanitize=True)
+    exe = hostbuild.emu_program("HomologyByXCorrSlave")
     blocks = [(0, 6, 0, 1, 0), (0, 6, 2, 4, 0), (7, 13, 5, 7, 1), (7, 13, 8, 9, 1), (50, 60, 0, 1, 0)]
     got, p = run_slave(exe, qf, tf, blocks, len(g["results"]), per_exchange=1)
     assert got.tobytes() == g["results"].tobytes()
     assert "outside the 14 target / 10 query chunks" in p.stdout
+
+
+def test_slave_asan(golden, tmp_path):
+    """AddressSanitizer build of the slave and the emulated kernels on a few blocks (the
+    last chunks of each record, 64 and 3136 bytes long), against the oracle."""
+    g, qf, tf = write_inputs(golden, tmp_path)
+    t, q = g["t"].tobytes().upper(), g["q"].tobytes().upper()
+    tch, qch = chunk_records([t], 4096, 1024), chunk_records([q], 4096, 0)
+    blocks = [(12, 13, 8, 9, 0), (11, 13, 9, 9, 1)]
+    pairs = [dict(tseq=tch[ti][2], qseq=qch[qi][2], t_chunk_start=tch[ti][1], q_chunk_start=qch[qi][1],
+                  q_record_len=len(q), fast=fast)
+             for tf_, tt, qf_, qt, fast in blocks for qi in range(qf_, qt + 1) for ti in range(tf_, tt + 1)]
+    exp = np.concatenate(oracle.align_batch(pairs, oracle.make_params(target_total=float(len(t)))))
+    got, _ = run_slave(hostbuild.emu_program("HomologyByXCorrSlave", sanitize=True), qf, tf, blocks, len(exp),
+                       per_exchange=1)
+    assert got.tobytes() == exp.tobytes()
 
 
 def test_slave_min_len_flag(golden, tmp_path):
