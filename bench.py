@@ -11,8 +11,9 @@ sxc_batch_run over all 4096 pairs (target spectra, fused pair kernel, result ord
 Serving (config of the serving loop): two synthetic 50-Mbp genomes sharing 2,000 syntenic
 blocks, chunked as the slave chunks them (16,277 target chunks of 4096 bp every 3072 bp,
 12,208 query chunks), target spectra resident in HBM; battleship-shaped requests (seed
-lines of 64 chunk pairs, then the 24 x 24 matrix cells of hits) arrive as Poisson streams
-at swept rates; each rate reports request latency p50/p99 and chunk pairs per second.
+lines of one target chunk x 256 query chunks, then the 24 x 24 matrix cells of hits) arrive
+as Poisson streams offering 25%, 50% and 75% of the batched throughput; each rate reports
+request latency p50/p99 and chunk pairs per second.  The headline p50/p99 is at 50%.
 
 Printed: one JSON line with the batched throughput (`value`), the plain one-pair-per-call
 throughput beside it, the serving latencies, the dominant kernel's roofline figures from
@@ -44,7 +45,8 @@ B_PAIR = 263_168 + 458_752
 # WRITE_SIZE (x1.0) on the 4096-pair launch (profiles/uniform_pairs_4096_kernels.md):
 # (271,555 * 2 + 4,916) KiB / 4096 pairs.
 TRAFFIC_PER_PAIR = (271_555 * 2 + 4_916) * 1024 / 4096
-SERVE_RATES = (250.0, 1000.0, 2000.0)  # requests/s; 1000/s is the headline point
+SERVE_LOADS = (0.25, 0.5, 0.75)  # offered chunk pairs/s as fractions of the batched rate; 0.5 is the headline
+SEED_LINE = 256                  # query chunks per seed line
 
 
 def make_workload(n_pairs: int, seed: int = 1):
@@ -87,7 +89,7 @@ def cpu_baseline(T, Q, sample: int, threads: int):
             "sample": f"{sample} pairs of the same workload, both strands, oracle restatement"}
 
 
-def serve_bench(ctx, genome_bp: int, duration: float, rates, seed: int = 4):
+def serve_bench(ctx, genome_bp: int, duration: float, capacity: float, loads, seed: int = 4):
     from satsuma2_amd import synth
     from satsuma2_amd.api import Server, SlaveParams
     from satsuma2_amd.serve_load import Battleship, genome_chunks, run_stream
@@ -101,16 +103,20 @@ def serve_bench(ctx, genome_bp: int, duration: float, rates, seed: int = 4):
     create_s = time.perf_counter() - t0
     info = srv.stats()
     runs = []
-    for k, rate in enumerate(rates):
-        print(f"[bench] serving: {rate:.0f} requests/s for {duration:.0f} s", file=sys.stderr, flush=True)
-        prod = Battleship(len(tch), len(qch), np.random.default_rng(seed + k))
-        runs.append(run_stream(srv, prod, rate, duration, seed=seed + k))
+    for k, load in enumerate(loads):
+        rate = load * capacity / SEED_LINE
+        print(f"[bench] serving: {rate:.0f} requests/s ({load:.0%} load) for {duration:.0f} s", file=sys.stderr,
+              flush=True)
+        prod = Battleship(len(tch), len(qch), np.random.default_rng(seed + k), line=SEED_LINE)
+        r = run_stream(srv, prod, rate, duration, seed=seed + k)
+        r["offered_load"] = load
+        runs.append(r)
     info = srv.stats()
     srv.close()
     head = runs[min(1, len(runs) - 1)]
     return {"workload": f"two {genome_bp / 1e6:.0f}-Mbp synthetic genomes, {len(tch)} target x {len(qch)} query "
-                        f"chunks, battleship requests (seed lines of 64 pairs, 24x24 cells of hits)",
-            "p50_ms": head["p50_ms"], "p99_ms": head["p99_ms"], "at_requests_per_s": head["offered_requests_per_s"],
+                        f"chunks, battleship requests (seed lines of {SEED_LINE} pairs, 24x24 cells of hits)",
+            "p50_ms": head["p50_ms"], "p99_ms": head["p99_ms"], "at_load": head["offered_load"],
             "rates": runs, "create_s": round(create_s, 2), "resident_gb": round(info["resident_bytes"] / 1e9, 3),
             "slots": info["slots"], "max_pairs_per_launch": info["max_pairs_per_launch"],
             "launches": info["launches"]}
@@ -203,7 +209,7 @@ def main():
                      "kernel": "k_correlate<13>", "bytes_per_pair": B_PAIR},
     }
     if not args.no_serve:
-        sv = serve_bench(ctx, args.serve_bp, args.serve_seconds, SERVE_RATES)
+        sv = serve_bench(ctx, args.serve_bp, args.serve_seconds, value, SERVE_LOADS)
         out["serve_p50_ms"] = sv["p50_ms"]
         out["serve_p99_ms"] = sv["p99_ms"]
         out["serve"] = sv

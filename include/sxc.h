@@ -66,6 +66,11 @@ typedef struct {
   double cutoff_fast;   /* -cutoff_fast */
   double min_prob;      /* keep threshold: 0.99 on the slave path (m_minProb, :76) */
   double target_total;  /* sum of target record lengths (:405-408) */
+  int32_t prob_table;   /* -prob_table: keep by ProbTable lookup (analysis/ProbTable.cc:105-141)
+                           instead of GetMatchProbability (:184-187) */
+  int32_t pad_;
+  double table_prob;    /* the probability a table hit reports: ProbTable's cutoff, the slave's
+                           -min_prob (:413-421); a hit is kept if it is >= min_prob */
 } sxc_params;
 
 /* One chunk pair: chunk bytes and the chunk metadata FilterMatches uses (:168-219). */

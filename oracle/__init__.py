@@ -64,6 +64,8 @@ class Params(C.Structure):
         ("cutoff_fast", C.c_double),
         ("min_prob", C.c_double),
         ("target_total", C.c_double),
+        ("ptable", C.c_void_p),
+        ("table_prob", C.c_double),
     ]
 
 
@@ -126,6 +128,8 @@ def lib():
         L.sxo_align_batch_tails.argtypes = [C.POINTER(Pair), C.c_int32, C.POINTER(Params), C.c_void_p, C.c_void_p,
                                             C.c_int32, C.POINTER(Result), C.c_int32, C.POINTER(C.c_int32)]
         L.sxo_align_batch_tails.restype = C.c_int64
+        L.sxo_prob_table.argtypes = [C.c_double, C.c_int32, C.POINTER(C.c_double)]
+        L.sxo_prob_table.restype = None
         _lib = L
     return _lib
 
@@ -210,8 +214,17 @@ def reverse_complement(seq: bytes) -> bytes:
 
 
 def make_params(N=8192, q_chunk=4096, min_len=0, strands=2, cutoff=1.8, cutoff_fast=2.9, min_prob=0.99,
-                target_total=4096.0) -> Params:
-    return Params(N, q_chunk, min_len, strands, cutoff, cutoff_fast, min_prob, target_total)
+                target_total=4096.0, ptable=None, table_prob=0.0) -> Params:
+    """ptable: a float64 array from prob_table() (kept alive by the caller) for -prob_table."""
+    return Params(N, q_chunk, min_len, strands, cutoff, cutoff_fast, min_prob, target_total,
+                  ptable.ctypes.data if ptable is not None else None, table_prob)
+
+
+def prob_table(target_size: float, threads: int = 8) -> np.ndarray:
+    """ProbTable::Setup (analysis/ProbTable.cc:15-52) as a (512, 2048) float64 array."""
+    t = np.zeros((512, 2048), np.float64)
+    lib().sxo_prob_table(C.c_double(target_size), threads, t.ctypes.data_as(C.POINTER(C.c_double)))
+    return t
 
 
 class TailSeg(C.Structure):

@@ -198,7 +198,9 @@ int mode_detail(int argc, char** argv) {
 // out.bin: uint64 count, then count x t_result (raw struct bytes, 72 B each).
 // stdout: one line "pairs=<chunk pairs> seconds=<wall> threads=<T>".
 int mode_pairs(int argc, char** argv) {
-  if (argc < 7) die("usage: pairs query.fa target.fa pairs.txt out.bin threads [cutoff cutoff_fast minlen q_chunk t_chunk]");
+  if (argc < 7)
+    die("usage: pairs query.fa target.fa pairs.txt out.bin threads [cutoff cutoff_fast minlen q_chunk t_chunk "
+        "prob_table min_prob]");
   query_filename = argv[2];
   target_filename = argv[3];
   int threads = atoi(argv[6]);
@@ -207,8 +209,8 @@ int mode_pairs(int argc, char** argv) {
   minLen = argc > 9 ? atoi(argv[9]) : 0;
   queryChunk = argc > 10 ? atoi(argv[10]) : 4096;
   targetChunk = argc > 11 ? atoi(argv[11]) : 4096;
-  minProb = 0.9999;
-  prob_table = false;
+  prob_table = argc > 12 ? atoi(argv[12]) != 0 : false;
+  minProb = argc > 13 ? atof(argv[13]) : 0.9999;
 
   // analysis/HomologyByXCorrSlave.cc:389-408, verbatim call sequence
   queryRaw.Read(query_filename, queryNames);
@@ -221,6 +223,7 @@ int mode_pairs(int argc, char** argv) {
   targetRaw.clear();
   targetTotal = 0;
   for (int i = 0; i < cmTarget->GetCount(); i++) targetTotal += (double)cmTarget->GetSize(i);
+  if (prob_table) probt = ProbTable(targetTotal, minProb);  // analysis/HomologyByXCorrSlave.cc:413-421
 
   std::vector<t_pair> plist;
   FILE* pf = fopen(argv[4], "r");
@@ -291,6 +294,23 @@ int mode_chunks(int argc, char** argv) {
   return 0;
 }
 
+// probtable SIZE CUTOFF out.bin: ProbTable(SIZE, CUTOFF)'s 512 x 2048 thresholds (row 0,
+// which the reference leaves empty, as zeros).
+int mode_probtable(int argc, char** argv) {
+  if (argc < 5) die("usage: probtable size cutoff out.bin");
+  ProbTable pt(atof(argv[2]), atof(argv[3]));
+  FILE* f = fopen(argv[4], "wb");
+  if (!f) die("cannot open output");
+  std::vector<double> row(2048);
+  for (int i = 0; i < 512; ++i) {
+    const std::vector<double>& t = pt.m_table[i];
+    for (int j = 0; j < 2048; ++j) row[j] = j < (int)t.size() ? t[j] : 0.;
+    write_blob(f, row.data(), row.size() * sizeof(double));
+  }
+  fclose(f);
+  return 0;
+}
+
 // layout probe: prints sizeof/offsetof of the wire structs (analysis/WorkQueue.h:17-33)
 int mode_layout() {
   printf("t_pair %zu tF %zu tT %zu qF %zu qT %zu fast %zu sid %zu status %zu\n", sizeof(t_pair),
@@ -315,6 +335,7 @@ int main(int argc, char** argv) {
   if (m == "pairs") return mode_pairs(argc, argv);
   if (m == "layout") return mode_layout();
   if (m == "chunks") return mode_chunks(argc, argv);
+  if (m == "probtable") return mode_probtable(argc, argv);
   die("unknown mode");
   return 2;
 }

@@ -613,6 +613,79 @@ static double gc_adjust_expect(double gc, int n, double gc_target) {
   return r / (double)n / 2.;
 }
 
+/* ProbTable::GetMatchProbabilityRaw (analysis/ProbTable.cc:143-163). */
+static double prob_raw(int length, double ident, double ident_expect, double target_size) {
+  double p_match = ident_expect;
+  double s = sigma(p_match, length);
+  double m = p_match * (double)length;
+  double x = (double)length * ident;
+  double c = cdf(m, x, s);
+  double expect = c * target_size;
+  return exp(-expect);
+}
+
+typedef struct {
+  double target_size;
+  double* t;
+  int next;
+  pthread_mutex_t mu;
+} pt_job;
+
+static void* pt_worker(void* arg) {
+  pt_job* jb = (pt_job*)arg;
+  for (;;) {
+    pthread_mutex_lock(&jb->mu);
+    int i = jb->next++;
+    pthread_mutex_unlock(&jb->mu);
+    if (i >= SXO_PT_ROWS) break;
+    double* row = jb->t + (size_t)i * SXO_PT_LEN;
+    const double ident_expect = (double)i / ((double)SXO_PT_ROWS - 1);
+    row[0] = 2.;
+    for (int j = 1; j < SXO_PT_LEN; ++j) {
+      double min = 0, max = 1, mid;
+      while (max - min > 0.00000001) {
+        mid = (max + min) / 2.0;
+        if (prob_raw(j, mid, ident_expect, jb->target_size) != 0.) max = mid; else min = mid;
+      }
+      row[j] = (max + min) / 2.0;
+    }
+  }
+  return NULL;
+}
+
+void sxo_prob_table(double target_size, int32_t threads, double* t) {
+  memset(t, 0, sizeof(double) * SXO_PT_ROWS * SXO_PT_LEN);
+  pt_job jb;
+  jb.target_size = target_size;
+  jb.t = t;
+  jb.next = 1;
+  pthread_mutex_init(&jb.mu, NULL);
+  if (threads < 1) threads = 1;
+  pthread_t* th = (pthread_t*)malloc(sizeof(pthread_t) * (size_t)threads);
+  for (int k = 0; k < threads; ++k) pthread_create(&th[k], NULL, pt_worker, &jb);
+  for (int k = 0; k < threads; ++k) pthread_join(th[k], NULL);
+  free(th);
+  pthread_mutex_destroy(&jb.mu);
+}
+
+double sxo_prob_table_lookup(const double* t, const char* target, const char* query, int32_t st, int32_t sq,
+                             int32_t len, double cutoff, double* ident) {
+  codec_ready();
+  double gct = 0, gcq = 0, matches = 0;
+  for (int32_t i = 0; i < len; ++i) {
+    int a = ub(target[i + st]), b = ub(query[i + sq]);
+    matches += sxo_dna_equal(a, b);
+    gct += g_C[a] + g_G[a];
+    gcq += g_C[b] + g_G[b];
+  }
+  *ident = matches / (double)len;
+  double p_match = gc_adjust_expect(gcq, len, gct / (double)len);
+  int index = (int)(p_match * (SXO_PT_ROWS - 1)); /* ExpectToIndex */
+  if (index < 1) return 0.;
+  int l = len >= SXO_PT_LEN ? SXO_PT_LEN - 1 : len;
+  return (*ident >= t[(size_t)index * SXO_PT_LEN + l]) ? cutoff : 0.;
+}
+
 /* GetMatchProbabilityEx (analysis/AlignProbability.cc:62-127). */
 double sxo_match_probability(const char* t, const char* q, int32_t st, int32_t sq, int32_t length,
                              double target_size) {
@@ -747,9 +820,17 @@ static int32_t align_pair_ws(align_ws* ws, const sxo_pair* p, const sxo_params* 
       int32_t qstart = strand == 0 ? p->q_chunk_start + m->start_query
                                    : rc_query(p->q_chunk_start, m->start_query,
                                               prm->q_chunk > 0 ? prm->q_chunk : p->qlen, p->q_record_len);
-      double prob = sxo_match_probability(p->tseq, qs, m->start_target, m->start_query, len, prm->target_total);
-      if (prob < prm->min_prob) continue;
-      double ident = sxo_print_match_ident(qs, p->tseq, m->start_target, m->start_query, len);
+      double prob, ident;
+      if (prm->ptable) {
+        /* FilterMatches with prob_table (analysis/HomologyByXCorrSlave.cc:184-187) */
+        prob = sxo_prob_table_lookup(prm->ptable, p->tseq, qs, m->start_target, m->start_query, len,
+                                     prm->table_prob, &ident);
+        if (prob < prm->min_prob) continue;
+      } else {
+        prob = sxo_match_probability(p->tseq, qs, m->start_target, m->start_query, len, prm->target_total);
+        if (prob < prm->min_prob) continue;
+        ident = sxo_print_match_ident(qs, p->tseq, m->start_target, m->start_query, len);
+      }
       if (nres < cap) {
         sxo_result* r = &out[nres];
         memset(r, 0, sizeof *r);

@@ -62,7 +62,24 @@ typedef struct {
   double cutoff_fast;   /* -cutoff_fast */
   double min_prob;      /* effective keep threshold (0.99 on the slave path) */
   double target_total;  /* sum of target record lengths */
+  const double* ptable; /* -prob_table: sxo_prob_table output, or NULL for GetMatchProbabilityEx */
+  double table_prob;    /* the probability a table hit reports (ProbTable's cutoff, -min_prob) */
 } sxo_params;
+
+#define SXO_PT_ROWS 512
+#define SXO_PT_LEN 2048
+/* ProbTable::Setup (analysis/ProbTable.cc:15-52): t[i * 2048 + j] for i = 1..511, j = 1..2047
+ * is the identity found by bisection (width 1e-8) where GetMatchProbabilityRaw(j, ident,
+ * i / 511, target_size) stops being 0 (the reference tests the probability for non-zero,
+ * not against the cutoff); t[i * 2048] = 2; row 0 is never set (zeros here).  Rows are
+ * computed on `threads` threads. */
+void sxo_prob_table(double target_size, int32_t threads, double* t);
+/* ProbTable::GetMatchProbability (analysis/ProbTable.cc:105-141): `cutoff` if ident reaches
+ * the table's threshold, else 0; *ident = matches / len.  A GC expectation below 1/511 maps
+ * to row 0, which the reference never fills (it indexes an empty vector there); it is
+ * reported as 0 here. */
+double sxo_prob_table_lookup(const double* t, const char* target, const char* query, int32_t st, int32_t sq,
+                             int32_t len, double cutoff, double* ident);
 
 /* DNACodec tables (analysis/DNAVector.cc:13-57). */
 void sxo_codec(double* a256, double* c256, double* g256, double* t256, unsigned char* rc256);

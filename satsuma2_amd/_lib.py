@@ -41,7 +41,8 @@ class Result(C.Structure):
 class Params(C.Structure):
     _fields_ = [("N", C.c_int32), ("q_chunk", C.c_int32), ("min_len", C.c_int32), ("strands", C.c_int32),
                 ("cutoff", C.c_double), ("cutoff_fast", C.c_double), ("min_prob", C.c_double),
-                ("target_total", C.c_double)]
+                ("target_total", C.c_double), ("prob_table", C.c_int32), ("pad_", C.c_int32),
+                ("table_prob", C.c_double)]
 
 
 class Pair(C.Structure):

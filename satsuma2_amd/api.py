@@ -120,10 +120,12 @@ class SlaveParams:
     min_prob: float = 0.99  # m_minProb is fixed at 0.99 on the slave path (:76)
     strands: int = 2
     target_total: float = 0.0
+    prob_table: bool = False  # -prob_table: ProbTable lookup (analysis/ProbTable.cc:105-141)
+    table_prob: float = 0.9999  # -min_prob: what a table hit reports (the table's cutoff)
 
     def to_c(self) -> Params:
         return Params(2 * self.t_chunk, self.q_chunk, self.min_len, self.strands, self.cutoff, self.cutoff_fast,
-                      self.min_prob, self.target_total)
+                      self.min_prob, self.target_total, 1 if self.prob_table else 0, 0, self.table_prob)
 
 
 @dataclass

@@ -201,11 +201,9 @@ int main(int argc, char** argv) {
   const bool prob_table = P.boolean("-prob_table");
   const unsigned debug_max = (unsigned)P.i32("-debug_targets");
   const int device = P.i32("-device");
-  (void)P.f64("-min_prob");  // read but unused: FilterMatches keeps prob >= 0.99 (:76)
-  if (prob_table) {
-    printf("ERROR -prob_table is not supported by this build\n");
-    return 1;
-  }
+  // FilterMatches keeps prob >= 0.99 (m_minProb, :76); -min_prob is only the probability a
+  // ProbTable hit reports (:413-421, analysis/ProbTable.cc:132-136)
+  const double min_prob_flag = P.f64("-min_prob");
 
   time_t t0 = time(nullptr);
   Records qraw, traw;
@@ -237,6 +235,9 @@ int main(int argc, char** argv) {
   job.prm.cutoff_fast = cutoff_fast;
   job.prm.min_prob = 0.99;
   job.prm.target_total = target.total();
+  job.prm.prob_table = prob_table ? 1 : 0;
+  job.prm.pad_ = 0;
+  job.prm.table_prob = min_prob_flag;
   if (sxc_create(&job.ctx, device) != SXC_OK) {
     printf("ERROR %s\n", sxc_last_error());
     return 1;

@@ -30,6 +30,20 @@ int fail(int code, const std::string& msg) {
 
 using sxc::fail;
 
+int sxc::ctx_prob_table(sxc_ctx* ctx, double target_size, const double** out) {
+  auto it = ctx->ptables.find(target_size);
+  if (it == ctx->ptables.end()) {
+    std::vector<double> t((size_t)SXC_PT_ROWS * SXC_PT_LEN);
+    sxc::build_prob_table(target_size, t.data());
+    DevBuf& d = ctx->ptables[target_size];
+    HIPCHK(d.ensure(t.size() * sizeof(double)));
+    HIPCHK(hipMemcpy(d.p, t.data(), t.size() * sizeof(double), hipMemcpyHostToDevice));
+    it = ctx->ptables.find(target_size);
+  }
+  *out = it->second.as<double>();
+  return SXC_OK;
+}
+
 namespace {
 
 int check_N(int N, int* nb) {
@@ -144,6 +158,11 @@ int launch_pairs(sxc_ctx* ctx, sxc_batch_dev& b) {
   a.counts = b.counts.as<int32_t>();
   a.cap = b.slot_cap;
   a.stats = b.stats.as<int32_t>();
+  if (b.prm.prob_table) {
+    const int rc = sxc::ctx_prob_table(ctx, b.prm.target_total, &a.ptable);
+    if (rc) return rc;
+    a.table_prob = b.prm.table_prob;
+  }
   HIPCHK(sxc::launch_scan(b.nb, a, total, ctx->stream));
   return SXC_OK;
 }
@@ -431,6 +450,7 @@ void sxc_destroy(sxc_ctx* c) {
   (void)hipStreamSynchronize(c->stream);
   c->scratch.release();
   for (sxc_batch_dev& b : c->scratch_nb) b.release();
+  for (auto& kv : c->ptables) kv.second.release();
   for (DevBuf* b : {&c->p0, &c->p1, &c->p2, &c->p3, &c->p4, &c->p5, &c->p6}) b->release();
   (void)hipFree(c->d_tab);
   (void)hipFree(c->d_osc);

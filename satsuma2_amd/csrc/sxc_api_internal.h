@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <map>
 #include <mutex>
 #include <string>
 
@@ -83,4 +84,11 @@ struct sxc_ctx {
   sxc_batch_dev scratch;          // buffers reused by the one-shot entry points
   sxc_batch_dev scratch_nb[16];  // ... per length class when N is chosen per pair
   DevBuf p0, p1, p2, p3, p4, p5, p6;  // plain-API staging
+  std::map<double, DevBuf> ptables;   // ProbTable thresholds by target size (-prob_table)
 };
+
+namespace sxc {
+// The ProbTable thresholds for `target_size` in HBM, built on the host on first use.
+// Call with ctx->mu held.
+int ctx_prob_table(sxc_ctx* ctx, double target_size, const double** out);
+}  // namespace sxc

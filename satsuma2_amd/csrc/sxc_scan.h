@@ -44,6 +44,8 @@ struct ScanArgs {
   int32_t* counts;
   int32_t cap;
   int32_t* stats;            // optional {candidates, runs} per (pair, strand)
+  const double* ptable;      // optional ProbTable thresholds (SXC_PT_ROWS x SXC_PT_LEN): -prob_table
+  double table_prob;         // what a table hit reports (the table's cutoff)
 };
 
 constexpr int SCAN_NT = 256;  // threads per scan workgroup
@@ -191,6 +193,23 @@ SXC_HD inline bool tail_from_sums(double matches, double gct, double gcq, int le
   const double expect = cdf * target_size;
   prob = exp(-expect);
   return true;
+}
+
+// ProbTable::GetMatchProbability after the sums (analysis/ProbTable.cc:118-141): the table's
+// cutoff if the identity reaches the threshold for (expected identity, length), else 0.  An
+// expectation below 1/511 selects row 0, which the reference leaves empty; it reports 0.
+SXC_HD inline double table_from_sums(double matches, double gct, double gcq, int len, const double* ptable,
+                                     double table_prob, double& ident) {
+  ident = matches / (double)len;
+  const double gc_target = gct / (double)len;
+  const double at_target = 1. - gc_target;
+  double r = gcq * gc_target;
+  r += ((double)len - gcq) * at_target;
+  const double p = r / (double)len / 2.;
+  const int index = (int)(p * (SXC_PT_ROWS - 1));
+  if (index < 1) return 0.;
+  const int l = len >= SXC_PT_LEN ? SXC_PT_LEN - 1 : len;
+  return ident >= ptable[(size_t)index * SXC_PT_LEN + l] ? table_prob : 0.;
 }
 
 struct Lag {
@@ -378,7 +397,11 @@ SXC_HD inline void drain(Ex& ex, char* sm, const ScanArgs& a, const sxc_pair_dev
         int cm, ct, cq;
         packed_counts((const uint32_t*)(sm + SS::TPK), (const uint32_t*)(sm + SS::QPK), start_t, start_q, len, cm,
                       ct, cq);
-        if (!tail_from_sums((double)cm, (double)ct, (double)cq, len, a.target_total, a.z_skip, prob, ident)) continue;
+        if (a.ptable) {
+          prob = table_from_sums((double)cm, (double)ct, (double)cq, len, a.ptable, a.table_prob, ident);
+        } else if (!tail_from_sums((double)cm, (double)ct, (double)cq, len, a.target_total, a.z_skip, prob, ident)) {
+          continue;
+        }
       } else {
         double matches = 0, gct = 0, gcq = 0;
         for (int i = 0; i < len; ++i) {
@@ -387,7 +410,11 @@ SXC_HD inline void drain(Ex& ex, char* sm, const ScanArgs& a, const sxc_pair_dev
           gct += gcv[ca];
           gcq += gcv[cb];
         }
-        if (!tail_from_sums(matches, gct, gcq, len, a.target_total, a.z_skip, prob, ident)) continue;
+        if (a.ptable) {
+          prob = table_from_sums(matches, gct, gcq, len, a.ptable, a.table_prob, ident);
+        } else if (!tail_from_sums(matches, gct, gcq, len, a.target_total, a.z_skip, prob, ident)) {
+          continue;
+        }
       }
       if (prob < a.min_prob) continue;
       const int slot = ex.atomic_add_shared(&misc[SM_KEPT], 1);

@@ -98,6 +98,7 @@ struct sxc_server {
   int nb = 0;
   int P = 2048;
   DevBuf seq, tch, qch, spectra;
+  const double* ptable = nullptr;  // -prob_table thresholds (owned by the context)
   std::vector<ChunkMeta> tmeta, qmeta;
   int64_t resident_bytes = 0, slot_bytes = 0;
   std::vector<Slot> slots;
@@ -194,6 +195,8 @@ int launch_scan_slot(sxc_server* sv, Slot& sl) {
   a.counts = sl.counts.as<int32_t>();
   a.cap = sl.cap;
   a.stats = nullptr;
+  a.ptable = sv->ptable;
+  a.table_prob = sv->prm.table_prob;
   HIPCHK(sxc::launch_scan(sv->nb, a, 2 * sl.npairs, sl.s));
   const int total = 2 * sl.npairs;
   HIPCHK(sxc::launch_finalize(sl.counts.as<int32_t>(), total, sl.cap, sl.offsets.as<int64_t>(), sl.total.as<int64_t>(),
@@ -425,6 +428,10 @@ int server_init(sxc_server* sv, const sxc_chunk_desc* targets, int32_t nt, const
     return fail(SXC_ERR_NOMEM, "HBM budget too small: resident genome and spectra need " +
                                    std::to_string(sv->resident_bytes) + " bytes plus " +
                                    std::to_string(sv->slot_bytes) + " per launch slot");
+  if (sv->prm.prob_table) {
+    rc = sxc::ctx_prob_table(sv->ctx, sv->prm.target_total, &sv->ptable);
+    if (rc) return rc;
+  }
   hipStream_t s = sv->ctx->stream;
   HIPCHK(sv->seq.ensure(bytes.size()));
   HIPCHK(sv->tch.ensure(std::max<size_t>(1, ttab.size()) * sizeof(sxc_chunk)));

@@ -14,6 +14,11 @@
 #include <math.h>
 #include <string.h>
 
+#include <algorithm>
+#include <atomic>
+#include <thread>
+#include <vector>
+
 namespace sxc {
 
 static const char kLetters[] = "ACGTKMRYSWBVHDNX-";  // class ids 0..16; 17 = zero row
@@ -140,6 +145,42 @@ double keep_z_bound(double min_prob, double target_size) {
     if (0.5 * (1. + erf(mid)) >= 4. * cmax) hi = mid; else lo = mid;
   }
   return hi;
+}
+
+// ProbTable::Setup (analysis/ProbTable.cc:15-52) with GetMatchProbabilityRaw (:143-163):
+// for expected identity i / 511 and length j, the identity where exp(-cdf * size) stops
+// being 0, by bisection to a width of 1e-8 (the reference tests the probability for non-zero,
+// so the cutoff does not enter the thresholds).  Computed with the host's erf/exp, as the
+// reference does; rows are spread over the host's threads.
+void build_prob_table(double target_size, double* t) {
+  memset(t, 0, sizeof(double) * SXC_PT_ROWS * SXC_PT_LEN);
+  std::atomic<int> next(1);
+  auto work = [&]() {
+    for (;;) {
+      const int i = next.fetch_add(1);
+      if (i >= SXC_PT_ROWS) return;
+      double* row = t + (size_t)i * SXC_PT_LEN;
+      const double p = (double)i / ((double)SXC_PT_ROWS - 1);
+      row[0] = 2.;
+      for (int j = 1; j < SXC_PT_LEN; ++j) {
+        double lo = 0, hi = 1;
+        while (hi - lo > 0.00000001) {
+          const double mid = (hi + lo) / 2.0;
+          const double s = sqrt(p * (1. - p) * (double)j);
+          const double m = p * (double)j;
+          const double x = (double)j * mid;
+          const double cdf = 0.5 * (1. + erf((m - x) / s / 1.414213562));
+          if (exp(-(cdf * target_size)) != 0.) hi = mid; else lo = mid;
+        }
+        row[j] = (hi + lo) / 2.0;
+      }
+    }
+  };
+  const int nt = std::max(1, std::min(16, (int)std::thread::hardware_concurrency()));
+  std::vector<std::thread> th;
+  for (int k = 1; k < nt; ++k) th.emplace_back(work);
+  work();
+  for (auto& x : th) x.join();
 }
 
 }  // namespace sxc

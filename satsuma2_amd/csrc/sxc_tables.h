@@ -12,5 +12,7 @@ int class_of(int byte);
 bool class_has_thirds(int c);
 // z above which a run's probability is certainly below min_prob (see sxc_scan.h tail).
 double keep_z_bound(double min_prob, double target_size);
+// ProbTable thresholds for a target size: SXC_PT_ROWS x SXC_PT_LEN doubles (row 0 unused).
+void build_prob_table(double target_size, double* t);
 
 }  // namespace sxc

@@ -12,6 +12,8 @@
 #define SXC_NCLASS 18           // 16 IUPAC letters + '-' + "zero row" class
 #define SXC_CLASS_N 14          // class id of 'N'
 #define SXC_CLASS_ZERO 17       // bytes with an all-zero codec row
+#define SXC_PT_ROWS 512         // ProbTable rows: expected identity i / 511 (analysis/ProbTable.cc:24)
+#define SXC_PT_LEN 2048         // ProbTable columns: match length, clamped (analysis/ProbTable.cc:28, 79-80)
 
 // One chunk: bytes live at seq_base + off, `len` bases (uppercase ASCII < 128).
 typedef struct {

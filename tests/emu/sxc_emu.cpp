@@ -138,6 +138,13 @@ int run_batch(const sxc::HostBatch& hb, const sxc_params& prm, std::vector<sxc_r
   sc.counts = counts.data();
   sc.cap = cap;
   sc.stats = stats;
+  std::vector<double> ptable;
+  if (prm.prob_table) {
+    ptable.resize((size_t)SXC_PT_ROWS * SXC_PT_LEN);
+    sxc::build_prob_table(prm.target_total, ptable.data());
+    sc.ptable = ptable.data();
+    sc.table_prob = prm.table_prob;
+  }
   std::vector<char> ssm(sxc::ScanSmem<NB>::BYTES);
   for (int ps = 0; ps < total; ++ps) {
     EmuExec<int> ex(sxc::SCAN_NT);
@@ -191,6 +198,9 @@ int emu_align_batch(const sxc_pair* pairs, int n, const sxc_params* prm, sxc_res
                     int64_t* n_out, float* xc_out, int32_t* stats) {
   return emu_align_batch_tails(pairs, n, prm, nullptr, nullptr, out, cap, n_out, xc_out, stats);
 }
+
+// The library's ProbTable thresholds (sxc_tables.cpp), SXC_PT_ROWS x SXC_PT_LEN doubles.
+void emu_prob_table(double target_size, double* out) { sxc::build_prob_table(target_size, out); }
 
 // FFT of one signal through stage A + rounds (spectra_body with a given signal).
 int emu_fft_forward(const float* in, float* out, int N) {

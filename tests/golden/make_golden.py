@@ -17,6 +17,7 @@ Fixtures:
                      through the slave path and through the standalone CLI's match file
   cli_records.npz    12 + 12 records of uneven lengths through the standalone CLI under four
                      flag sets (match files, MultiMatches v3)
+  prob_table.npz     ProbTable thresholds (digest + rows) and the slave path with -prob_table
 """
 from __future__ import annotations
 
@@ -343,6 +344,37 @@ def gen_cli(tmp):
     np.savez_compressed(os.path.join(HERE, "cli_records.npz"), **d)
 
 
+PT_ROWS = [1, 64, 127, 200, 255, 383, 511]
+
+
+def gen_probtable(tmp):
+    """ProbTable thresholds for the samples slice's target size and the reference slave's
+    records with -prob_table on the samples slice blocks (cutoff 0.9999, and 0.9, which the
+    slave's fixed 0.99 keep threshold then rejects entirely)."""
+    g = np.load(os.path.join(HERE, "samples_slice.npz"))
+    size = float(len(g["t"]))
+    tb = os.path.join(tmp, "pt.bin")
+    run([DRIVER, "probtable", repr(size), "0.9999", tb])
+    t = np.fromfile(tb, np.float64).reshape(512, 2048)
+    qf, tf, pf = (os.path.join(tmp, x) for x in ("q.fa", "t.fa", "p.txt"))
+    synth.write_fasta(qf, [g["q"].tobytes()], ["dogX"])
+    synth.write_fasta(tf, [g["t"].tobytes()], ["humanX"])
+    with open(pf, "w") as f:
+        for b in g["pairs"]:
+            f.write(" ".join(str(int(x)) for x in b) + "\n")
+    out = {}
+    for name, cut in (("hi", "0.9999"), ("lo", "0.9")):
+        of = os.path.join(tmp, f"r_{name}.bin")
+        run([DRIVER, "pairs", qf, tf, pf, of, "1", "1.8", "2.9", "0", "4096", "4096", "1", cut])
+        r = read_results(of)
+        # the table keeps almost every run (171k records here): digest, count and a sample
+        out[f"results_{name}_sha"] = sha(r)
+        out[f"results_{name}_n"] = np.array([len(r)])
+        out[f"results_{name}_head"] = r[:256]
+    np.savez_compressed(os.path.join(HERE, "prob_table.npz"), target_size=np.array([size]), table_sha=sha(t),
+                        rows=t[PT_ROWS], row_idx=np.array(PT_ROWS, np.int32), **out)
+
+
 def main():
     subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "ref", f"REF={REF}"], check=True)
     with tempfile.TemporaryDirectory() as tmp:
@@ -353,6 +385,7 @@ def main():
         gen_uniform(tmp)
         gen_samples(tmp)
         gen_cli(tmp)
+        gen_probtable(tmp)
     for f in sorted(os.listdir(HERE)):
         if f.endswith(".npz"):
             print(f, os.path.getsize(os.path.join(HERE, f)))

@@ -114,3 +114,13 @@ def test_asan_uneven_pairs(emu_check, golden, tmp_path):
         N = int(g[f"N{k}"][0])
         lt, lq = g[f"len{k}"]
         _run_check(emu_check, tmp_path, [(unpack2(g[f"t{k}"], lt), unpack2(g[f"q{k}"], lq), 0)], N)
+
+
+def test_library_prob_table_golden(emu_lib, golden):
+    """The library's host-built ProbTable thresholds equal the reference's."""
+    import hashlib
+
+    g = golden("prob_table.npz")
+    t = np.zeros((512, 2048), np.float64)
+    emu_lib.emu_prob_table(C.c_double(float(g["target_size"][0])), t.ctypes.data_as(C.c_void_p))
+    assert hashlib.sha256(t.tobytes()).hexdigest() == str(g["table_sha"])

@@ -35,7 +35,7 @@ def test_abi_struct_layouts():
     assert C.sizeof(_lib.Result) == 72
     assert _lib.Result.prob.offset == 56 and _lib.Result.ident.offset == 64 and _lib.Result.reverse.offset == 48
     assert C.sizeof(_lib.SeqMatch) == 12
-    assert C.sizeof(_lib.Params) == 48
+    assert C.sizeof(_lib.Params) == 64 and _lib.Params.table_prob.offset == 56
     assert C.sizeof(_lib.Pair) == 56
 
 

@@ -399,3 +399,25 @@ def test_cli_signal_tails_golden(golden):
     assert np.array_equal(got["ident"] * got["len"].astype(np.float64), exp["matches"])
     plain = np.concatenate(oracle.align_batch(pairs, prm, threads=4))
     assert len(plain) != len(got) or plain.tobytes() != got.tobytes()
+
+
+def test_prob_table_golden(golden):
+    """ProbTable::Setup thresholds (analysis/ProbTable.cc:15-52) and the slave path with
+    -prob_table (analysis/HomologyByXCorrSlave.cc:184-187) against the reference's."""
+    g = golden("prob_table.npz")
+    size = float(g["target_size"][0])
+    t = oracle.prob_table(size)
+    assert sha(t) == str(g["table_sha"])
+    assert np.array_equal(t[g["row_idx"]], g["rows"])
+    s = golden("samples_slice.npz")
+    T, Q = s["t"].tobytes().upper(), s["q"].tobytes().upper()
+    tch, qch = chunk_records([T], 4096, 1024), chunk_records([Q], 4096, 0)
+    pairs = [dict(tseq=tch[ti][2], qseq=qch[qi][2], t_chunk_start=tch[ti][1], q_chunk_start=qch[qi][1],
+                  t_record=tch[ti][0], q_record=qch[qi][0], q_record_len=len(Q), fast=int(fast))
+             for tf, tt, qf, qt, fast in s["pairs"] for qi in range(qf, qt + 1) for ti in range(tf, tt + 1)]
+    for name, cut in (("hi", 0.9999), ("lo", 0.9)):
+        prm = oracle.make_params(target_total=size, ptable=t, table_prob=cut)
+        got = np.concatenate(oracle.align_batch(pairs, prm, threads=8, cap_per_pair=1 << 15))
+        assert len(got) == int(g[f"results_{name}_n"][0])
+        assert sha(got) == str(g[f"results_{name}_sha"]), name
+        assert got[:256].tobytes() == g[f"results_{name}_head"].tobytes()

@@ -150,3 +150,21 @@ def test_fasta_reader_and_chunker_match_reference(tmp_path, golden):
         ours = _stdout([hostbuild.seqio_check(), files, str(size), str(ov)])
         ref = _stdout([REF_DRIVER, "chunks", files, str(size), str(ov)])
         assert _lines(ours) == _lines(ref), (files, size, ov)
+
+
+@pytest.mark.parametrize("cut", ["0.9999", "0.9"])
+def test_slave_prob_table_matches_reference(golden, tmp_path, cut):
+    """-prob_table (analysis/HomologyByXCorrSlave.cc:184-187, 413-421): records equal the
+    reference slave's with the same flag (digest; 171,360 records at -min_prob 0.9999, none at
+    0.9, which the slave's fixed keep threshold of 0.99 rejects)."""
+    import hashlib
+
+    g, qf, tf = write_inputs(golden, tmp_path)
+    p = golden("prob_table.npz")
+    name = "hi" if cut == "0.9999" else "lo"
+    n = int(p[f"results_{name}_n"][0])
+    got, _ = run_slave(hostbuild.emu_program("HomologyByXCorrSlave"), qf, tf,
+                       [tuple(int(x) for x in r) for r in g["pairs"]], n,
+                       extra=("-prob_table", "true", "-min_prob", cut))
+    assert len(got) == n
+    assert hashlib.sha256(np.ascontiguousarray(got).tobytes()).hexdigest() == str(p[f"results_{name}_sha"])
