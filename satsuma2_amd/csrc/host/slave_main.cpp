@@ -4,14 +4,16 @@
 // loading and chunking (:386-409), same master exchange (:440-523) and the same t_result
 // records per block (align_target / Align / FilterMatches, :168-300).  What changes is
 // how the work runs: the reference gives each of its -p threads one t_pair block at a
-// time; here one submitter thread drains every queued block into a single batch of
-// chunk pairs and runs it through libsxc_hip.so, so the GPU sees thousands of chunk
-// pairs per launch.  The master exchange stays as it is: ask for blocks while fewer than
-// 2 * p are queued, and hand back every finished result on the next connection.
+// time.  Here both genomes and the spectra of every target chunk are put in HBM once, in a
+// serving loop (sxc_server_*): each block is submitted as it arrives and the loop packs
+// the pending chunk pairs of all blocks into shared launches.  If the genomes do not fit
+// the device, the slave falls back to draining the queued blocks into one batch at a time.
+// The master exchange stays as it is: ask for blocks while fewer than 2 * p wait besides the
+// p being worked on, and hand back every finished result on the next connection.
 //
-// Results of a batch are appended block by block in arrival order, each block in the
-// reference's order (query chunks outer, target chunks inner, forward strand then
-// reverse), i.e. the order the reference slave produces with -p 1.
+// Results are appended block by block in arrival order, each block in the reference's
+// order (query chunks outer, target chunks inner, forward strand then reverse), i.e. the
+// order the reference slave produces with -p 1.
 #include <arpa/inet.h>
 #include <errno.h>
 #include <netdb.h>
@@ -50,8 +52,11 @@ constexpr int kMaxPairsPerLaunch = 16384;
 struct State {
   std::mutex mu;
   std::condition_variable work_cv;   // blocks were queued or the run is over
-  std::condition_variable space_cv;  // the submitter took blocks off the queue
+  std::condition_variable space_cv;  // blocks left the queue or finished
+  std::condition_variable done_cv;   // a block was submitted to the serving loop
   std::deque<WirePair> blocks;
+  std::deque<uint64_t> running;      // serving-loop tickets, in arrival order
+  size_t busy = 0;                   // blocks taken off the queue and not finished
   std::vector<sxc_result> results;
   std::atomic<bool> finished{false};
 };
@@ -85,6 +90,7 @@ struct Job {
   const ChunkSet* target;
   sxc_params prm;
   sxc_ctx* ctx;
+  sxc_server* srv = nullptr;  // null: batch fallback
 };
 
 // Appends the chunk pairs of one block (align_target, :270-300) in reference order.
@@ -145,6 +151,13 @@ bool align_pairs(const Job& job, const std::vector<sxc_pair>& pairs, std::vector
   return true;
 }
 
+[[noreturn]] void device_failure(const char* what) {
+  printf("ERROR %s: %s\n", what, sxc_last_error());
+  fflush(stdout);
+  _exit(1);  // a device failure leaves the master's blocks unanswered; fail loudly
+}
+
+// Batch fallback: drains every queued block into one batch.
 void submitter(const Job& job, State* st) {
   for (;;) {
     std::vector<WirePair> take;
@@ -154,18 +167,99 @@ void submitter(const Job& job, State* st) {
       if (st->finished.load()) return;
       take.assign(st->blocks.begin(), st->blocks.end());
       st->blocks.clear();
+      st->busy += take.size();
     }
-    st->space_cv.notify_all();
     std::vector<sxc_pair> pairs;
     for (const WirePair& p : take) append_block(job, p, &pairs);
     std::vector<sxc_result> res;
-    if (!align_pairs(job, pairs, &res)) {
-      fflush(stdout);
-      _exit(1);  // a device failure leaves the master's blocks unanswered; fail loudly
+    if (!align_pairs(job, pairs, &res)) device_failure("aligning a batch");
+    {
+      std::lock_guard<std::mutex> g(st->mu);
+      st->results.insert(st->results.end(), res.begin(), res.end());
+      st->busy -= take.size();
     }
-    std::lock_guard<std::mutex> g(st->mu);
-    st->results.insert(st->results.end(), res.begin(), res.end());
+    st->space_cv.notify_all();
   }
+}
+
+// Serving loop: every block is a request as soon as it arrives.
+void serve_submitter(const Job& job, State* st) {
+  const int nt = (int)job.target->chunks.size(), nq = (int)job.query->chunks.size();
+  for (;;) {
+    std::vector<WirePair> take;
+    {
+      std::unique_lock<std::mutex> lk(st->mu);
+      st->work_cv.wait(lk, [&] { return st->finished.load() || !st->blocks.empty(); });
+      if (st->finished.load()) return;
+      take.assign(st->blocks.begin(), st->blocks.end());
+      st->blocks.clear();
+    }
+    for (const WirePair& p : take) {
+      if (p.target_from < 0 || p.target_to >= nt || p.query_from < 0 || p.query_to >= nq ||
+          p.target_from > p.target_to || p.query_from > p.query_to) {
+        printf("ERROR block t:%d-%d q:%d-%d is outside the %d target / %d query chunks; skipped\n", p.target_from,
+               p.target_to, p.query_from, p.query_to, nt, nq);
+        continue;
+      }
+      const sxc_block b{p.target_from, p.target_to, p.query_from, p.query_to, p.fast ? 1 : 0};
+      uint64_t ticket = 0;
+      if (sxc_server_submit(job.srv, &b, 1, &ticket) != SXC_OK) device_failure("submitting a block");
+      std::lock_guard<std::mutex> g(st->mu);
+      st->running.push_back(ticket);
+      ++st->busy;
+    }
+    st->done_cv.notify_all();
+  }
+}
+
+// Serving loop: collects blocks' records in arrival order.
+void serve_collector(const Job& job, State* st) {
+  std::vector<sxc_result> r;
+  for (;;) {
+    uint64_t ticket;
+    {
+      std::unique_lock<std::mutex> lk(st->mu);
+      st->done_cv.wait(lk, [&] { return st->finished.load() || !st->running.empty(); });
+      if (st->finished.load()) return;
+      ticket = st->running.front();
+    }
+    int64_t n = 0;
+    int rc;
+    while ((rc = sxc_server_wait(job.srv, ticket, 200, &n, nullptr)) == SXC_PENDING)
+      if (st->finished.load()) return;
+    if (rc != SXC_OK) device_failure("serving a block");
+    r.resize((size_t)std::max<int64_t>(n, 1));
+    if (sxc_server_results(job.srv, ticket, r.data(), (int64_t)r.size(), &n) != SXC_OK)
+      device_failure("collecting a block");
+    {
+      std::lock_guard<std::mutex> g(st->mu);
+      st->results.insert(st->results.end(), r.begin(), r.begin() + n);
+      st->running.pop_front();
+      --st->busy;
+    }
+    st->space_cv.notify_all();
+  }
+}
+
+// Both genomes and all target spectra resident, or null if they do not fit the device.
+sxc_server* make_server(const Job& job) {
+  auto descs = [](const ChunkSet& cs) {
+    std::vector<sxc_chunk_desc> d(cs.chunks.size());
+    for (size_t k = 0; k < cs.chunks.size(); ++k) {
+      const sxc_host::Chunk& c = cs.chunks[k];
+      d[k] = sxc_chunk_desc{c.data, c.len, c.start, c.record, cs.lengths[(size_t)c.record]};
+    }
+    return d;
+  };
+  const std::vector<sxc_chunk_desc> t = descs(*job.target), q = descs(*job.query);
+  sxc_server* srv = nullptr;
+  const int rc = sxc_server_create(job.ctx, &job.prm, t.data(), (int32_t)t.size(), q.data(), (int32_t)q.size(),
+                                   nullptr, &srv);
+  if (rc != SXC_OK) {
+    printf("serving loop unavailable (%s); aligning queued blocks in batches\n", sxc_last_error());
+    return nullptr;
+  }
+  return srv;
 }
 
 }  // namespace
@@ -243,8 +337,11 @@ int main(int argc, char** argv) {
     return 1;
   }
 
+  job.srv = make_server(job);
   State st;
-  std::thread worker(submitter, std::cref(job), &st);
+  std::thread worker(job.srv ? serve_submitter : submitter, std::cref(job), &st);
+  std::thread collector;
+  if (job.srv) collector = std::thread(serve_collector, std::cref(job), &st);
   printf("== Entering communication loop ==\ncomm loop for %s %d\n", master.c_str(), port);
   fflush(stdout);
 
@@ -253,7 +350,10 @@ int main(int argc, char** argv) {
     printf("ERROR resolving master hostname\n");
     st.finished = true;
     st.work_cv.notify_all();
+    st.done_cv.notify_all();
     worker.join();
+    if (collector.joinable()) collector.join();
+    sxc_server_destroy(job.srv);
     sxc_destroy(job.ctx);
     return -1;
   }
@@ -267,7 +367,9 @@ int main(int argc, char** argv) {
   while (!st.finished.load()) {
     {
       std::unique_lock<std::mutex> lk(st.mu);
-      if (st.blocks.size() >= (size_t)(2 * threads)) {
+      // the reference asks while fewer than 2 * p blocks wait, with up to p more being worked
+      // on by its threads; blocks in the serving loop count as waiting or being worked on
+      if (st.blocks.size() + st.busy >= (size_t)(3 * threads)) {
         st.space_cv.wait_for(lk, std::chrono::seconds(10));
         continue;
       }
@@ -339,10 +441,13 @@ int main(int argc, char** argv) {
     }
   }
   st.work_cv.notify_all();
+  st.done_cv.notify_all();
   printf("== Processing finished, waiting for the slaves to die ==\n");
   printf("TIME SPENT WORKING: %ld\n", (long)(time(nullptr) - t0));
   fflush(stdout);
   worker.join();
+  if (collector.joinable()) collector.join();
+  sxc_server_destroy(job.srv);
   sxc_destroy(job.ctx);
   return 0;
 }

@@ -9,6 +9,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <map>
 #include <vector>
 
 #include "../../satsuma2_amd/csrc/sxc_batch_host.h"
@@ -299,4 +300,69 @@ int sxc_align_batch(sxc_ctx*, const sxc_pair* pairs, int32_t n, const sxc_params
                     int64_t* n_out) {
   return emu_align_batch(pairs, n, prm, out, cap, n_out, nullptr, nullptr);
 }
+
+// The serving loop, run synchronously at submission.  SXC_EMU_NO_SERVER=1 makes creation
+// fail, as a device without room for the genomes does.
+struct sxc_server {
+  std::vector<sxc_chunk_desc> t, q;
+  sxc_params prm;
+  std::map<uint64_t, std::vector<sxc_result>> done;
+  uint64_t next = 1;
+};
+int sxc_server_create(sxc_ctx*, const sxc_params* prm, const sxc_chunk_desc* targets, int32_t nt,
+                      const sxc_chunk_desc* queries, int32_t nq, const sxc_server_config*, sxc_server** out) {
+  const char* off = getenv("SXC_EMU_NO_SERVER");
+  if (off && off[0] == '1') {
+    g_emu_err = "serving loop switched off";
+    return SXC_ERR_NOMEM;
+  }
+  *out = new sxc_server{std::vector<sxc_chunk_desc>(targets, targets + nt),
+                        std::vector<sxc_chunk_desc>(queries, queries + nq), *prm, {}, 1};
+  return SXC_OK;
+}
+int sxc_server_submit(sxc_server* sv, const sxc_block* blocks, int32_t nblocks, uint64_t* ticket) {
+  std::vector<sxc_pair> pairs;
+  for (int32_t k = 0; k < nblocks; ++k) {
+    const sxc_block& b = blocks[k];
+    if (b.target_from < 0 || b.target_to >= (int)sv->t.size() || b.query_from < 0 || b.query_to >= (int)sv->q.size())
+      return SXC_ERR_ARG;
+    for (int qi = b.query_from; qi <= b.query_to; ++qi)
+      for (int ti = b.target_from; ti <= b.target_to; ++ti) {
+        const sxc_chunk_desc &t = sv->t[(size_t)ti], &q = sv->q[(size_t)qi];
+        pairs.push_back(sxc_pair{t.seq, t.len, q.seq, q.len, t.start, q.start, t.record, q.record, q.record_len,
+                                 b.fast});
+      }
+  }
+  std::vector<sxc_result> r(pairs.size() * 64 + 1024);
+  int64_t n = 0;
+  int rc = emu_align_batch(pairs.data(), (int)pairs.size(), &sv->prm, r.data(), (int64_t)r.size(), &n, nullptr,
+                           nullptr);
+  if (rc == SXC_ERR_CAPACITY) {
+    r.resize((size_t)n);
+    rc = emu_align_batch(pairs.data(), (int)pairs.size(), &sv->prm, r.data(), (int64_t)r.size(), &n, nullptr,
+                         nullptr);
+  }
+  if (rc) return SXC_ERR_ARG;
+  r.resize((size_t)n);
+  *ticket = sv->next++;
+  sv->done[*ticket].swap(r);
+  return SXC_OK;
+}
+int sxc_server_wait(sxc_server* sv, uint64_t ticket, int32_t, int64_t* n_results, double* latency_ms) {
+  auto it = sv->done.find(ticket);
+  if (it == sv->done.end()) return SXC_ERR_ARG;
+  if (n_results) *n_results = (int64_t)it->second.size();
+  if (latency_ms) *latency_ms = 0;
+  return SXC_OK;
+}
+int sxc_server_results(sxc_server* sv, uint64_t ticket, sxc_result* out, int64_t cap, int64_t* n_out) {
+  auto it = sv->done.find(ticket);
+  if (it == sv->done.end()) return SXC_ERR_ARG;
+  *n_out = (int64_t)it->second.size();
+  if (*n_out > cap) return SXC_ERR_CAPACITY;
+  memcpy(out, it->second.data(), it->second.size() * sizeof(sxc_result));
+  sv->done.erase(it);
+  return SXC_OK;
+}
+void sxc_server_destroy(sxc_server* sv) { delete sv; }
 }  // extern "C"

@@ -74,3 +74,25 @@ def test_gpu_slave_many_blocks_vs_batch(tmp_path):
     got = _run(exe, qf, tf, blocks, len(exp), per_exchange=3, threads=2)
     assert len(got) == len(exp) > 0
     assert got.tobytes() == exp.tobytes()
+
+
+def test_gpu_slave_prob_table(golden, tmp_path):
+    """-prob_table: the records equal the reference slave's with the flag (171,360 on the
+    samples slice; the table lookup has no floating-point tail, so they are bit-exact)."""
+    import hashlib
+
+    g = golden("samples_slice.npz")
+    p = golden("prob_table.npz")
+    qf, tf = str(tmp_path / "q.fa"), str(tmp_path / "t.fa")
+    synth.write_fasta(qf, [g["q"].tobytes()], ["dogX"])
+    synth.write_fasta(tf, [g["t"].tobytes()], ["humanX"])
+    exe = hostbuild.product_program("HomologyByXCorrSlave")
+    n = int(p["results_hi_n"][0])
+    m = MasterStub([tuple(int(x) for x in r) for r in g["pairs"]], expect=n, per_exchange=2, deadline=300).start()
+    r = subprocess.run([exe, "-master", "127.0.0.1", "-port", str(m.port), "-sid", "1", "-q", qf, "-t", tf,
+                        "-prob_table", "true", "-min_prob", "0.9999"], capture_output=True, text=True, timeout=300)
+    m.done.wait(10)
+    assert m.error is None and r.returncode == 0, r.stdout + r.stderr
+    got = m.results()
+    assert len(got) == n
+    assert hashlib.sha256(np.ascontiguousarray(got).tobytes()).hexdigest() == str(p["results_hi_sha"])

@@ -30,10 +30,13 @@ def write_inputs(golden, tmp_path):
     return g, qf, tf
 
 
-def run_slave(exe, qf, tf, blocks, expect, per_exchange=2, threads=1, extra=(), timeout=600):
+def run_slave(exe, qf, tf, blocks, expect, per_exchange=2, threads=1, extra=(), timeout=600, batch=False):
+    """batch=True: the emulated library refuses the serving loop, so the slave takes its
+    batch fallback."""
     m = MasterStub(blocks, expect=expect, per_exchange=per_exchange, deadline=timeout).start()
+    env = dict(os.environ, SXC_EMU_NO_SERVER="1" if batch else "0")
     p = subprocess.run([exe, "-master", "127.0.0.1", "-port", str(m.port), "-sid", "3", "-p", str(threads),
-                        "-q", qf, "-t", tf, *extra], capture_output=True, text=True, timeout=timeout)
+                        "-q", qf, "-t", tf, *extra], capture_output=True, text=True, timeout=timeout, env=env)
     m.done.wait(10)
     assert m.error is None, m.error
     assert p.returncode == 0, p.stdout + p.stderr
@@ -41,12 +44,13 @@ def run_slave(exe, qf, tf, blocks, expect, per_exchange=2, threads=1, extra=(), 
     return m.results(), p
 
 
-@pytest.mark.parametrize("per_exchange", [2, 1])
-def test_slave_samples_matches_reference_slave(golden, tmp_path, per_exchange):
+@pytest.mark.parametrize("per_exchange,batch", [(2, False), (1, False), (2, True), (1, True)])
+def test_slave_samples_matches_reference_slave(golden, tmp_path, per_exchange, batch):
     g, qf, tf = write_inputs(golden, tmp_path)
     exe = hostbuild.emu_program("HomologyByXCorrSlave")
-    got, _ = run_slave(exe, qf, tf, [tuple(int(x) for x in r) for r in g["pairs"]], len(g["results"]),
-                       per_exchange=per_exchange)
+    got, p = run_slave(exe, qf, tf, [tuple(int(x) for x in r) for r in g["pairs"]], len(g["results"]),
+                       per_exchange=per_exchange, batch=batch)
+    assert ("aligning queued blocks in batches" in p.stdout) == batch
     exp = g["results"]
     assert len(got) == len(exp)
     assert got.tobytes() == exp.tobytes()
@@ -58,9 +62,10 @@ def test_slave_block_split_and_bad_block(golden, tmp_path):
     g, qf, tf = write_inputs(golden, tmp_path)
     exe = hostbuild.emu_program("HomologyByXCorrSlave")
     blocks = [(0, 6, 0, 1, 0), (0, 6, 2, 4, 0), (7, 13, 5, 7, 1), (7, 13, 8, 9, 1), (50, 60, 0, 1, 0)]
-    got, p = run_slave(exe, qf, tf, blocks, len(g["results"]), per_exchange=1)
-    assert got.tobytes() == g["results"].tobytes()
-    assert "outside the 14 target / 10 query chunks" in p.stdout
+    for batch in (False, True):
+        got, p = run_slave(exe, qf, tf, blocks, len(g["results"]), per_exchange=1, batch=batch)
+        assert got.tobytes() == g["results"].tobytes()
+        assert "outside the 14 target / 10 query chunks" in p.stdout
 
 
 def test_slave_asan(golden, tmp_path):
