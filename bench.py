@@ -15,6 +15,9 @@ lines of one target chunk x 256 query chunks, then the 24 x 24 matrix cells of h
 as Poisson streams offering 25%, 50% and 75% of the batched throughput; each rate reports
 request latency p50/p99 and chunk pairs per second.  The headline p50/p99 is at 50%.
 
+Uneven (config of the uneven batch): 20,000 pairs with target and query lengths log-uniform
+in [256, 16384] bp, each at its own FFT length (1024 ... 32768), in one resident batch.
+
 Printed: one JSON line with the batched throughput (`value`), the plain one-pair-per-call
 throughput beside it, the serving latencies, the dominant kernel's roofline figures from
 HIP events, and the reference CPU path timed on this host's cores on a bounded sample.
@@ -122,6 +125,35 @@ def serve_bench(ctx, genome_bp: int, duration: float, capacity: float, loads, se
             "launches": info["launches"]}
 
 
+def uneven_bench(ctx, n_pairs: int, steps: int, seed: int = 3):
+    """Uneven pairs in one batch: lengths log-uniform in [256, 16384] bp, each pair at its own
+    FFT length (next power of two >= q + t), one sub-batch per length class, the classes
+    running side by side on their own streams."""
+    import collections
+
+    from satsuma2_amd import synth
+    from satsuma2_amd.api import ChunkPair, ResidentBatch, SlaveParams
+
+    print(f"[bench] uneven: generating {n_pairs} pairs", file=sys.stderr, flush=True)
+    T, Q = synth.uneven_pairs(n_pairs, seed=seed)
+    pairs = [ChunkPair(T[r], Q[r], t_record=r, q_record=r, q_record_len=len(Q[r])) for r in range(n_pairs)]
+    classes = collections.Counter(synth.fft_len(len(q), len(t)) for t, q in zip(T, Q))
+    rb = ResidentBatch(ctx, pairs, SlaveParams(t_chunk=0, target_total=float(sum(len(t) for t in T))))
+    rb.run()
+    print(f"[bench] uneven: timing {steps} steps", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        rb.run()
+    wall = (time.perf_counter() - t0) / steps
+    n_res = len(rb.results())
+    rb.close()
+    return {"workload": f"uneven_pairs: {n_pairs} pairs, q and t log-uniform in [256, 16384] bp, GC 0.35-0.55, "
+                        "FFT length per pair, both strands",
+            "pairs_per_s": round(n_pairs / wall, 1), "ms_per_step": round(1e3 * wall, 3), "steps": steps,
+            "fft_len_classes": {str(k): v for k, v in sorted(classes.items())},
+            "bp": int(sum(len(t) + len(q) for t, q in zip(T, Q))), "results": n_res}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -134,6 +166,8 @@ def main():
     ap.add_argument("--serve-bp", type=int, default=50_000_000)
     ap.add_argument("--serve-seconds", type=float, default=4.0)
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--uneven-pairs", type=int, default=20_000)
+    ap.add_argument("--uneven-steps", type=int, default=5)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (chunk pairs shard by target chunk "
@@ -208,6 +242,8 @@ def main():
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": int(TRAFFIC_PER_PAIR * args.pairs),
                      "kernel": "k_correlate<13>", "bytes_per_pair": B_PAIR},
     }
+    if args.uneven_pairs > 0:
+        out["uneven"] = uneven_bench(ctx, args.uneven_pairs, args.uneven_steps)
     if not args.no_serve:
         sv = serve_bench(ctx, args.serve_bp, args.serve_seconds, value, SERVE_LOADS)
         out["serve_p50_ms"] = sv["p50_ms"]

@@ -218,7 +218,10 @@ int sxc_server_results(sxc_server* srv, uint64_t ticket, sxc_result* out, int64_
 int sxc_server_stats(sxc_server* srv, sxc_server_info* out);
 void sxc_server_destroy(sxc_server* srv);
 
-/* Device time of the last sxc_align_batch / sxc_align_pair / sxc_batch_run, by phase (milliseconds). */
+/* Time of the last sxc_align_batch / sxc_align_pair / sxc_batch_run (milliseconds): total_ms,
+ * upload_ms and download_ms are wall time on the host; the phases are device time summed over
+ * the length classes of the batch, which run side by side (so they can add up to more than
+ * total_ms). */
 typedef struct {
   double total_ms;
   double upload_ms;

@@ -64,10 +64,23 @@ float elapsed(hipEvent_t a, hipEvent_t b) {
 }
 
 // Upload a packed batch (inputs become resident in HBM).
+// The sub-batch's own stream, events and pinned control words (created on first use, on
+// the context's device).
+int batch_stream(sxc_batch_dev& b) {
+  if (b.s) return SXC_OK;
+  HIPCHK(hipStreamCreateWithFlags(&b.s, hipStreamNonBlocking));
+  for (hipEvent_t& e : b.ev) HIPCHK(hipEventCreate(&e));
+  HIPCHK(hipHostMalloc((void**)&b.h_ctl, 2 * sizeof(int64_t), hipHostMallocDefault));
+  return SXC_OK;
+}
+
 int batch_upload(sxc_ctx* ctx, sxc_batch_dev& b) {
+  (void)ctx;
   const sxc::HostBatch& hb = b.hb;
   const int N = b.prm.N;
-  hipStream_t s = ctx->stream;
+  int rc = batch_stream(b);
+  if (rc) return rc;
+  hipStream_t s = b.s;
   HIPCHK(b.seq.ensure(hb.seq.size()));
   HIPCHK(b.tch.ensure(hb.tchunks.size() * sizeof(sxc_chunk)));
   HIPCHK(b.qch.ensure(hb.qchunks.size() * sizeof(sxc_chunk)));
@@ -126,7 +139,7 @@ int launch_correlate(sxc_ctx* ctx, sxc_batch_dev& b) {
     a.segs = b.segs.as<sxc_tail_seg_dev>();
     a.src_sig = b.src_sig.as<float>();
   }
-  HIPCHK(sxc::launch_correlate(b.nb, a, total, ctx->stream));
+  HIPCHK(sxc::launch_correlate(b.nb, a, total, b.s));
   return SXC_OK;
 }
 
@@ -163,15 +176,21 @@ int launch_pairs(sxc_ctx* ctx, sxc_batch_dev& b) {
     if (rc) return rc;
     a.table_prob = b.prm.table_prob;
   }
-  HIPCHK(sxc::launch_scan(b.nb, a, total, ctx->stream));
+  HIPCHK(sxc::launch_scan(b.nb, a, total, b.s));
   return SXC_OK;
 }
 
-// Device-only work of a resident batch: target spectra, fused pair kernel, ordering.
-int batch_run(sxc_ctx* ctx, sxc_batch_dev& b) {
-  hipStream_t s = ctx->stream;
+// Device-only work of a resident batch, in two halves so that several sub-batches (one per
+// length class) run side by side on their own streams.  batch_begin queues target spectra,
+// the fused pair kernel, the record counts and their copy to pinned host memory, and returns
+// without waiting.  batch_end waits for the counts, re-runs the scan with larger slots if a
+// (pair, strand) kept more records than its slot holds, and orders the records.
+int batch_begin(sxc_ctx* ctx, sxc_batch_dev& b) {
+  int rc = batch_stream(b);
+  if (rc) return rc;
+  hipStream_t s = b.s;
   const int total = (int)b.hb.pairs.size() * b.prm.strands;
-  HIPCHK(hipEventRecord(ctx->ev[1], s));
+  HIPCHK(hipEventRecord(b.ev[0], s));
   if (!b.hb.schunks.empty()) {
     // signals of the sequences named by signal tails
     sxc::EncodeArgs ea{ctx->d_tab, ctx->d_ent[b.nb], b.seq.as<unsigned char>(), b.sch.as<sxc_chunk>(), 0,
@@ -198,54 +217,61 @@ int batch_run(sxc_ctx* ctx, sxc_batch_dev& b) {
     }
     HIPCHK(sxc::launch_spectra(b.nb, sa, s));
   }
-  HIPCHK(hipEventRecord(ctx->ev[2], s));
-  if (total == 0) {
-    b.n_results = 0;
-    HIPCHK(hipEventRecord(ctx->ev[6], s));
-    HIPCHK(hipEventRecord(ctx->ev[3], s));
-    HIPCHK(hipEventRecord(ctx->ev[4], s));
-    HIPCHK(hipEventSynchronize(ctx->ev[4]));
-    b.spectra_ms = elapsed(ctx->ev[1], ctx->ev[2]);
-    b.correlate_ms = b.scan_ms = b.finalize_ms = 0;
-    return SXC_OK;
-  }
-  int rc = launch_correlate(ctx, b);
+  HIPCHK(hipEventRecord(b.ev[1], s));
+  if (total == 0) return SXC_OK;
+  rc = launch_correlate(ctx, b);
   if (rc) return rc;
-  HIPCHK(hipEventRecord(ctx->ev[6], s));
+  HIPCHK(hipEventRecord(b.ev[2], s));
   rc = launch_pairs(ctx, b);
   if (rc) return rc;
-  HIPCHK(hipEventRecord(ctx->ev[3], s));
+  HIPCHK(hipEventRecord(b.ev[3], s));
   HIPCHK(b.offsets.ensure((size_t)total * sizeof(int64_t)));
   HIPCHK(b.total.ensure(sizeof(int64_t)));
   HIPCHK(b.maxc.ensure(sizeof(int32_t)));
   HIPCHK(sxc::launch_finalize(b.counts.as<int32_t>(), total, b.slot_cap, b.offsets.as<int64_t>(), b.total.as<int64_t>(),
                               b.maxc.as<int32_t>(), nullptr, nullptr, nullptr, s, true));
-  int32_t maxc = 0;
-  int64_t tot = 0;
-  HIPCHK(hipMemcpyAsync(&maxc, b.maxc.p, sizeof maxc, hipMemcpyDeviceToHost, s));
-  HIPCHK(hipMemcpyAsync(&tot, b.total.p, sizeof tot, hipMemcpyDeviceToHost, s));
+  b.h_ctl[0] = b.h_ctl[1] = 0;
+  HIPCHK(hipMemcpyAsync(&b.h_ctl[0], b.maxc.p, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipMemcpyAsync(&b.h_ctl[1], b.total.p, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+  return SXC_OK;
+}
+
+int batch_end(sxc_ctx* ctx, sxc_batch_dev& b) {
+  hipStream_t s = b.s;
+  const int total = (int)b.hb.pairs.size() * b.prm.strands;
+  if (total == 0) {
+    HIPCHK(hipEventSynchronize(b.ev[1]));
+    b.n_results = 0;
+    b.spectra_ms = elapsed(b.ev[0], b.ev[1]);
+    b.correlate_ms = b.scan_ms = b.finalize_ms = 0;
+    return SXC_OK;
+  }
   HIPCHK(hipStreamSynchronize(s));
+  const int32_t maxc = (int32_t)b.h_ctl[0];
+  int64_t tot = b.h_ctl[1];
   if (maxc > b.slot_cap) {
     // some (pair, strand) kept more records than its slot holds: enlarge and recompute
     b.slot_cap = maxc;
-    rc = launch_pairs(ctx, b);
+    const int rc = launch_pairs(ctx, b);
     if (rc) return rc;
     HIPCHK(sxc::launch_finalize(b.counts.as<int32_t>(), total, b.slot_cap, b.offsets.as<int64_t>(),
                                 b.total.as<int64_t>(), b.maxc.as<int32_t>(), nullptr, nullptr, nullptr, s, true));
-    HIPCHK(hipMemcpyAsync(&tot, b.total.p, sizeof tot, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(&b.h_ctl[1], b.total.p, sizeof(int64_t), hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
+    tot = b.h_ctl[1];
   }
   HIPCHK(b.final_.ensure((size_t)std::max<int64_t>(tot, 1) * sizeof(sxc_result_dev)));
   HIPCHK(sxc::launch_finalize(b.counts.as<int32_t>(), total, b.slot_cap, b.offsets.as<int64_t>(), nullptr, nullptr,
                               b.slots.as<sxc_result_dev>(), b.keys.as<uint64_t>(), b.final_.as<sxc_result_dev>(), s,
                               false));
-  HIPCHK(hipEventRecord(ctx->ev[4], s));
-  HIPCHK(hipEventSynchronize(ctx->ev[4]));
+  HIPCHK(hipEventRecord(b.ev[4], s));
+  HIPCHK(hipEventSynchronize(b.ev[4]));
   b.n_results = tot;
-  b.spectra_ms = elapsed(ctx->ev[1], ctx->ev[2]);
-  b.correlate_ms = elapsed(ctx->ev[2], ctx->ev[6]);
-  b.scan_ms = elapsed(ctx->ev[6], ctx->ev[3]);
-  b.finalize_ms = elapsed(ctx->ev[3], ctx->ev[4]);
+  b.spectra_ms = elapsed(b.ev[0], b.ev[1]);
+  b.correlate_ms = elapsed(b.ev[1], b.ev[2]);
+  b.scan_ms = elapsed(b.ev[2], b.ev[3]);
+  // includes the wait for the host to read the counts (and any re-run of the scan)
+  b.finalize_ms = elapsed(b.ev[3], b.ev[4]);
   return SXC_OK;
 }
 
@@ -348,10 +374,19 @@ int groups_prepare(std::vector<Group>& gs, const sxc_pair* pairs, int32_t n, con
   return SXC_OK;
 }
 
-int groups_run(sxc_ctx* ctx, std::vector<Group>& gs, sxc_timing& t) {
+// Runs every group; with `upload` the group's inputs are copied first.  All groups are
+// queued on their own streams before the first is waited for, so length classes overlap.
+int groups_run(sxc_ctx* ctx, std::vector<Group>& gs, sxc_timing& t, bool upload = false) {
   t.spectra_ms = t.correlate_ms = t.scan_ms = t.finalize_ms = 0;
   for (Group& g : gs) {
-    const int rc = batch_run(ctx, *g.b);
+    const auto t0 = std::chrono::steady_clock::now();
+    int rc = upload ? batch_upload(ctx, *g.b) : SXC_OK;
+    t.upload_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    if (!rc) rc = batch_begin(ctx, *g.b);
+    if (rc) return rc;
+  }
+  for (Group& g : gs) {
+    const int rc = batch_end(ctx, *g.b);
     if (rc) return rc;
     t.spectra_ms += g.b->spectra_ms;
     t.correlate_ms += g.b->correlate_ms;
@@ -474,14 +509,10 @@ int sxc_align_batch(sxc_ctx* ctx, const sxc_pair* pairs, int32_t n, const sxc_pa
   for (Group& gr : gs) gr.b = (prm->N != 0) ? &ctx->scratch : &ctx->scratch_nb[gr.nb];
   rc = groups_prepare(gs, pairs, n, prm, nullptr, nullptr);
   if (rc) return rc;
+  // uploads overlap the device work of earlier length classes
   const auto t0 = std::chrono::steady_clock::now();
-  for (Group& gr : gs) {
-    rc = batch_upload(ctx, *gr.b);
-    if (rc) return rc;
-  }
-  const auto t1 = std::chrono::steady_clock::now();
   sxc_timing t{};
-  rc = groups_run(ctx, gs, t);
+  rc = groups_run(ctx, gs, t, true);
   if (rc) return rc;
   const auto t2 = std::chrono::steady_clock::now();
   rc = groups_download(ctx, gs, n, out, cap, n_out);
@@ -489,7 +520,6 @@ int sxc_align_batch(sxc_ctx* ctx, const sxc_pair* pairs, int32_t n, const sxc_pa
   auto ms = [](std::chrono::steady_clock::time_point a, std::chrono::steady_clock::time_point b) {
     return std::chrono::duration<double, std::milli>(b - a).count();
   };
-  t.upload_ms = ms(t0, t1);
   t.download_ms = ms(t2, t3);
   t.total_ms = ms(t0, t3);
   ctx->last = t;
@@ -561,8 +591,9 @@ int sxc_batch_run(sxc_batch* bt) {
   int rc = set_device(bt->ctx);
   if (rc) return rc;
   sxc_timing t{};
+  const auto t0 = std::chrono::steady_clock::now();
   rc = groups_run(bt->ctx, bt->groups, t);
-  t.total_ms = t.spectra_ms + t.pair_ms + t.finalize_ms;
+  t.total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   bt->ctx->last = t;
   return rc;
 }

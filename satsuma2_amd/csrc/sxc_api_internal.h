@@ -65,10 +65,20 @@ struct sxc_batch_dev {
   int slot_cap = 64;
   int64_t n_results = 0;
   double spectra_ms = 0, correlate_ms = 0, scan_ms = 0, finalize_ms = 0;  // last run
+  // Own stream, phase events and pinned landing place for the record count, so that the
+  // sub-batches of different length classes run side by side.
+  hipStream_t s = nullptr;
+  hipEvent_t ev[5] = {};
+  int64_t* h_ctl = nullptr;  // [0] largest per-(pair, strand) count, [1] total records
   void release() {
+    if (s) (void)hipStreamSynchronize(s);
     for (DevBuf* b : {&seq, &tch, &qch, &pairs, &spectra, &which, &slots, &keys, &counts, &offsets, &total, &maxc,
                       &final_, &stats, &cands, &ncand, &sch, &sstr, &tails, &segs, &ttail, &src_sig})
       b->release();
+    for (hipEvent_t& e : ev)
+      if (e) (void)hipEventDestroy(e), e = nullptr;
+    if (h_ctl) (void)hipHostFree(h_ctl), h_ctl = nullptr;
+    if (s) (void)hipStreamDestroy(s), s = nullptr;
   }
 };
 
