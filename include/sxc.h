@@ -158,7 +158,11 @@ int sxc_batch_create(sxc_ctx* ctx, const sxc_pair* pairs, int32_t n, const sxc_p
  * array may be NULL; nseg = 0 means no tail. */
 int sxc_batch_create_tails(sxc_ctx* ctx, const sxc_pair* pairs, int32_t n, const sxc_params* prm,
                            const sxc_signal_tail* t_tail, const sxc_signal_tail* q_tail, sxc_batch** out);
+/* Runs the device work.  The first run is launched kernel by kernel (with phase timing);
+ * later runs replay the launch sequence captured as a HIP graph on the second run, unless
+ * graphs are switched off with sxc_batch_set_graph(batch, 0). */
 int sxc_batch_run(sxc_batch* batch);
+int sxc_batch_set_graph(sxc_batch* batch, int32_t on);
 int sxc_batch_results(sxc_batch* batch, sxc_result* out, int64_t cap, int64_t* n_out);
 /* Accepted lags (FindTop) and diagonal runs (DoOne) of the last run, summed over the batch. */
 int sxc_batch_stats(sxc_batch* batch, int64_t* candidates, int64_t* runs);
@@ -219,9 +223,10 @@ int sxc_server_stats(sxc_server* srv, sxc_server_info* out);
 void sxc_server_destroy(sxc_server* srv);
 
 /* Time of the last sxc_align_batch / sxc_align_pair / sxc_batch_run (milliseconds): total_ms,
- * upload_ms and download_ms are wall time on the host; the phases are device time summed over
- * the length classes of the batch, which run side by side (so they can add up to more than
- * total_ms). */
+ * upload_ms and download_ms are wall time on the host; the phases and device_ms are device
+ * time summed over the length classes of the batch, which run side by side (so they can add
+ * up to more than total_ms).  A run replayed from a captured graph (graph = 1) has no phase
+ * times, only device_ms. */
 typedef struct {
   double total_ms;
   double upload_ms;
@@ -233,6 +238,9 @@ typedef struct {
   int64_t runs;
   double correlate_ms;  /* encode + FFT + product + iFFT + peak search (part of pair_ms) */
   double scan_ms;       /* diagonal scan + probability tail (part of pair_ms) */
+  double device_ms;     /* first kernel to last copy of the run, per length class, summed */
+  int32_t graph;        /* 1 if the run was replayed from a captured graph */
+  int32_t pad_;
 } sxc_timing;
 int sxc_last_timing(sxc_ctx* ctx, sxc_timing* out);
 

@@ -55,7 +55,7 @@ class Timing(C.Structure):
     _fields_ = [("total_ms", C.c_double), ("upload_ms", C.c_double), ("spectra_ms", C.c_double),
                 ("pair_ms", C.c_double), ("finalize_ms", C.c_double), ("download_ms", C.c_double),
                 ("candidates", C.c_int64), ("runs", C.c_int64), ("correlate_ms", C.c_double),
-                ("scan_ms", C.c_double)]
+                ("scan_ms", C.c_double), ("device_ms", C.c_double), ("graph", C.c_int32), ("pad_", C.c_int32)]
 
 
 class TailSeg(C.Structure):
@@ -94,7 +94,8 @@ assert RESULT_DTYPE.itemsize == C.sizeof(Result) == 72
 
 EXPORTS = ["sxc_create", "sxc_destroy", "sxc_last_error", "sxc_version", "sxc_encode_signal", "sxc_fft_forward",
            "sxc_xcorr", "sxc_matchup", "sxc_match_probability", "sxc_align_pair", "sxc_align_batch",
-           "sxc_batch_create", "sxc_batch_create_tails", "sxc_batch_run", "sxc_batch_results", "sxc_batch_stats",
+           "sxc_batch_create", "sxc_batch_create_tails", "sxc_batch_run", "sxc_batch_set_graph", "sxc_batch_results",
+           "sxc_batch_stats",
            "sxc_batch_destroy", "sxc_last_timing", "sxc_server_create", "sxc_server_submit", "sxc_server_wait",
            "sxc_server_results", "sxc_server_stats", "sxc_server_destroy"]
 
@@ -131,6 +132,7 @@ def load() -> C.CDLL:
                                       C.POINTER(i64)]
         L.sxc_batch_create.argtypes = [vp, C.POINTER(Pair), i32, C.POINTER(Params), C.POINTER(vp)]
         L.sxc_batch_run.argtypes = [vp]
+        L.sxc_batch_set_graph.argtypes = [vp, i32]
         L.sxc_batch_results.argtypes = [vp, C.POINTER(Result), i64, C.POINTER(i64)]
         L.sxc_batch_stats.argtypes = [vp, C.POINTER(i64), C.POINTER(i64)]
         L.sxc_batch_destroy.argtypes = [vp]

@@ -202,6 +202,11 @@ class ResidentBatch:
     def run(self):
         check(self.ctx.L.sxc_batch_run(self._h))
 
+    def set_graph(self, on: bool):
+        """Replay runs from a captured HIP graph (the default from the second run on) or
+        launch kernel by kernel (phase timings available)."""
+        check(self.ctx.L.sxc_batch_set_graph(self._h, 1 if on else 0))
+
     def results(self) -> np.ndarray:
         cap = 1 << 12
         while True:

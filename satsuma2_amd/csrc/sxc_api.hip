@@ -114,10 +114,31 @@ int batch_upload(sxc_ctx* ctx, sxc_batch_dev& b) {
   return SXC_OK;
 }
 
-int launch_correlate(sxc_ctx* ctx, sxc_batch_dev& b) {
+// Device buffers of a run, sized before anything is queued (nothing allocates while a
+// launch sequence is being captured into a graph).
+int batch_alloc(sxc_ctx* ctx, sxc_batch_dev& b) {
   const int total = (int)b.hb.pairs.size() * b.prm.strands;
   HIPCHK(b.cands.ensure((size_t)total * b.prm.N * sizeof(uint16_t)));
   HIPCHK(b.ncand.ensure((size_t)total * sizeof(int32_t)));
+  HIPCHK(b.slots.ensure((size_t)total * b.slot_cap * sizeof(sxc_result_dev)));
+  HIPCHK(b.keys.ensure((size_t)total * b.slot_cap * sizeof(uint64_t)));
+  HIPCHK(b.counts.ensure((size_t)total * sizeof(int32_t)));
+  HIPCHK(b.stats.ensure((size_t)total * 2 * sizeof(int32_t)));
+  HIPCHK(b.offsets.ensure((size_t)total * sizeof(int64_t)));
+  HIPCHK(b.total.ensure(sizeof(int64_t)));
+  HIPCHK(b.maxc.ensure(sizeof(int32_t)));
+  if (b.out_cap == 0) b.out_cap = std::max<int64_t>(1024, 2 * (int64_t)total);
+  HIPCHK(b.final_.ensure((size_t)b.out_cap * sizeof(sxc_result_dev)));
+  b.ptable = nullptr;
+  if (b.prm.prob_table) {
+    const int rc = sxc::ctx_prob_table(ctx, b.prm.target_total, &b.ptable);
+    if (rc) return rc;
+  }
+  return SXC_OK;
+}
+
+int launch_correlate(sxc_ctx* ctx, sxc_batch_dev& b) {
+  const int total = (int)b.hb.pairs.size() * b.prm.strands;
   sxc::PairArgs a{};
   a.tab = ctx->d_tab;
   a.osc = ctx->d_osc;
@@ -145,10 +166,6 @@ int launch_correlate(sxc_ctx* ctx, sxc_batch_dev& b) {
 
 int launch_pairs(sxc_ctx* ctx, sxc_batch_dev& b) {
   const int total = (int)b.hb.pairs.size() * b.prm.strands;
-  HIPCHK(b.slots.ensure((size_t)total * b.slot_cap * sizeof(sxc_result_dev)));
-  HIPCHK(b.keys.ensure((size_t)total * b.slot_cap * sizeof(uint64_t)));
-  HIPCHK(b.counts.ensure((size_t)total * sizeof(int32_t)));
-  HIPCHK(b.stats.ensure((size_t)total * 2 * sizeof(int32_t)));
   sxc::ScanArgs a{};
   a.tab = ctx->d_tab;
   a.seq = b.seq.as<unsigned char>();
@@ -172,8 +189,7 @@ int launch_pairs(sxc_ctx* ctx, sxc_batch_dev& b) {
   a.cap = b.slot_cap;
   a.stats = b.stats.as<int32_t>();
   if (b.prm.prob_table) {
-    const int rc = sxc::ctx_prob_table(ctx, b.prm.target_total, &a.ptable);
-    if (rc) return rc;
+    a.ptable = b.ptable;
     a.table_prob = b.prm.table_prob;
   }
   HIPCHK(sxc::launch_scan(b.nb, a, total, b.s));
@@ -181,16 +197,18 @@ int launch_pairs(sxc_ctx* ctx, sxc_batch_dev& b) {
 }
 
 // Device-only work of a resident batch, in two halves so that several sub-batches (one per
-// length class) run side by side on their own streams.  batch_begin queues target spectra,
-// the fused pair kernel, the record counts and their copy to pinned host memory, and returns
-// without waiting.  batch_end waits for the counts, re-runs the scan with larger slots if a
-// (pair, strand) kept more records than its slot holds, and orders the records.
-int batch_begin(sxc_ctx* ctx, sxc_batch_dev& b) {
-  int rc = batch_stream(b);
-  if (rc) return rc;
+// length class) run side by side on their own streams.  batch_begin queues the whole run --
+// target spectra, the fused pair kernel, the record counts, the ordered records and the copy
+// of the count and total to pinned host memory -- and returns without waiting; from the
+// second run on, the sequence is captured once as a graph and replayed.  batch_end waits and,
+// if a (pair, strand) kept more records than its slot holds or the records overflow the
+// output buffer, grows the buffer and redoes what it needs eagerly.
+
+// Kernels and copies of one run on b.s (no allocation, no host synchronisation).
+int batch_enqueue(sxc_ctx* ctx, sxc_batch_dev& b, bool events) {
   hipStream_t s = b.s;
   const int total = (int)b.hb.pairs.size() * b.prm.strands;
-  HIPCHK(hipEventRecord(b.ev[0], s));
+  if (events) HIPCHK(hipEventRecord(b.ev[0], s));
   if (!b.hb.schunks.empty()) {
     // signals of the sequences named by signal tails
     sxc::EncodeArgs ea{ctx->d_tab, ctx->d_ent[b.nb], b.seq.as<unsigned char>(), b.sch.as<sxc_chunk>(), 0,
@@ -217,22 +235,91 @@ int batch_begin(sxc_ctx* ctx, sxc_batch_dev& b) {
     }
     HIPCHK(sxc::launch_spectra(b.nb, sa, s));
   }
-  HIPCHK(hipEventRecord(b.ev[1], s));
-  if (total == 0) return SXC_OK;
-  rc = launch_correlate(ctx, b);
+  if (events) HIPCHK(hipEventRecord(b.ev[1], s));
+  int rc = launch_correlate(ctx, b);
   if (rc) return rc;
-  HIPCHK(hipEventRecord(b.ev[2], s));
+  if (events) HIPCHK(hipEventRecord(b.ev[2], s));
   rc = launch_pairs(ctx, b);
   if (rc) return rc;
-  HIPCHK(hipEventRecord(b.ev[3], s));
-  HIPCHK(b.offsets.ensure((size_t)total * sizeof(int64_t)));
-  HIPCHK(b.total.ensure(sizeof(int64_t)));
-  HIPCHK(b.maxc.ensure(sizeof(int32_t)));
-  HIPCHK(sxc::launch_finalize(b.counts.as<int32_t>(), total, b.slot_cap, b.offsets.as<int64_t>(), b.total.as<int64_t>(),
-                              b.maxc.as<int32_t>(), nullptr, nullptr, nullptr, s, true));
-  b.h_ctl[0] = b.h_ctl[1] = 0;
+  if (events) HIPCHK(hipEventRecord(b.ev[3], s));
+  HIPCHK(sxc::launch_count_offsets(b.counts.as<int32_t>(), total, b.slot_cap, b.offsets.as<int64_t>(),
+                                   b.total.as<int64_t>(), b.maxc.as<int32_t>(), s));
+  HIPCHK(sxc::launch_order_records(b.counts.as<int32_t>(), total, b.slot_cap, b.offsets.as<int64_t>(),
+                                   b.slots.as<sxc_result_dev>(), b.keys.as<uint64_t>(),
+                                   b.final_.as<sxc_result_dev>(), b.out_cap, s));
   HIPCHK(hipMemcpyAsync(&b.h_ctl[0], b.maxc.p, sizeof(int32_t), hipMemcpyDeviceToHost, s));
   HIPCHK(hipMemcpyAsync(&b.h_ctl[1], b.total.p, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+  if (events) HIPCHK(hipEventRecord(b.ev[4], s));
+  return SXC_OK;
+}
+
+// Everything a captured graph depends on besides the batch's (fixed) inputs.
+uint64_t graph_signature(const sxc_batch_dev& b) {
+  uint64_t h = 1469598103934665603ull;
+  auto mix = [&](uint64_t v) { h = (h ^ v) * 1099511628211ull; };
+  for (const DevBuf* d : {&b.seq, &b.tch, &b.qch, &b.pairs, &b.spectra, &b.which, &b.slots, &b.keys, &b.counts,
+                          &b.offsets, &b.total, &b.maxc, &b.final_, &b.stats, &b.cands, &b.ncand, &b.sch, &b.sstr,
+                          &b.tails, &b.segs, &b.ttail, &b.src_sig})
+    mix((uint64_t)(uintptr_t)d->p);
+  mix((uint64_t)(uintptr_t)b.ptable);
+  mix((uint64_t)(uintptr_t)b.h_ctl);
+  mix((uint64_t)b.slot_cap);
+  mix((uint64_t)b.out_cap);
+  // launch shapes and scalar arguments
+  mix(b.hb.pairs.size());
+  mix(b.hb.tchunks.size());
+  mix(b.hb.qchunks.size());
+  mix(b.hb.schunks.size());
+  mix(b.hb.tails.size());
+  mix((uint64_t)b.nb);
+  const unsigned char* pb = (const unsigned char*)&b.prm;
+  for (size_t i = 0; i < sizeof b.prm; ++i) mix(pb[i]);
+  return h;
+}
+
+int batch_begin(sxc_ctx* ctx, sxc_batch_dev& b) {
+  int rc = batch_stream(b);
+  if (rc) return rc;
+  hipStream_t s = b.s;
+  const int total = (int)b.hb.pairs.size() * b.prm.strands;
+  b.last_graph = false;
+  if (total == 0) {
+    // no pairs (and so no chunks): nothing to run
+    HIPCHK(hipEventRecord(b.ev[0], s));
+    HIPCHK(hipEventRecord(b.ev[1], s));
+    return SXC_OK;
+  }
+  rc = batch_alloc(ctx, b);
+  if (rc) return rc;
+  b.h_ctl[0] = b.h_ctl[1] = 0;
+  if (b.use_graph && b.runs > 0) {
+    const uint64_t sig = graph_signature(b);
+    if (!b.gexec || b.gsig != sig) {
+      if (b.gexec) HIPCHK(hipGraphExecDestroy(b.gexec));
+      b.gexec = nullptr;
+      hipGraph_t g = nullptr;
+      HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+      rc = batch_enqueue(ctx, b, false);
+      const hipError_t e = hipStreamEndCapture(s, &g);
+      if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+      }
+      HIPCHK(e);
+      const hipError_t ei = hipGraphInstantiateWithFlags(&b.gexec, g, 0);
+      (void)hipGraphDestroy(g);
+      HIPCHK(ei);
+      b.gsig = sig;
+    }
+    HIPCHK(hipEventRecord(b.ev[0], s));
+    HIPCHK(hipGraphLaunch(b.gexec, s));
+    HIPCHK(hipEventRecord(b.ev[4], s));
+    b.last_graph = true;
+  } else {
+    rc = batch_enqueue(ctx, b, true);
+    if (rc) return rc;
+  }
+  ++b.runs;
   return SXC_OK;
 }
 
@@ -244,34 +331,48 @@ int batch_end(sxc_ctx* ctx, sxc_batch_dev& b) {
     b.n_results = 0;
     b.spectra_ms = elapsed(b.ev[0], b.ev[1]);
     b.correlate_ms = b.scan_ms = b.finalize_ms = 0;
+    b.device_ms = b.spectra_ms;
     return SXC_OK;
   }
   HIPCHK(hipStreamSynchronize(s));
+  if (b.last_graph) {
+    b.spectra_ms = b.correlate_ms = b.scan_ms = b.finalize_ms = 0;
+  } else {
+    b.spectra_ms = elapsed(b.ev[0], b.ev[1]);
+    b.correlate_ms = elapsed(b.ev[1], b.ev[2]);
+    b.scan_ms = elapsed(b.ev[2], b.ev[3]);
+    b.finalize_ms = elapsed(b.ev[3], b.ev[4]);
+  }
+  b.device_ms = elapsed(b.ev[0], b.ev[4]);
   const int32_t maxc = (int32_t)b.h_ctl[0];
   int64_t tot = b.h_ctl[1];
+  bool order = false;
   if (maxc > b.slot_cap) {
     // some (pair, strand) kept more records than its slot holds: enlarge and recompute
     b.slot_cap = maxc;
-    const int rc = launch_pairs(ctx, b);
+    int rc = batch_alloc(ctx, b);
     if (rc) return rc;
-    HIPCHK(sxc::launch_finalize(b.counts.as<int32_t>(), total, b.slot_cap, b.offsets.as<int64_t>(),
-                                b.total.as<int64_t>(), b.maxc.as<int32_t>(), nullptr, nullptr, nullptr, s, true));
+    rc = launch_pairs(ctx, b);
+    if (rc) return rc;
+    HIPCHK(sxc::launch_count_offsets(b.counts.as<int32_t>(), total, b.slot_cap, b.offsets.as<int64_t>(),
+                                     b.total.as<int64_t>(), b.maxc.as<int32_t>(), s));
     HIPCHK(hipMemcpyAsync(&b.h_ctl[1], b.total.p, sizeof(int64_t), hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
     tot = b.h_ctl[1];
+    order = true;
   }
-  HIPCHK(b.final_.ensure((size_t)std::max<int64_t>(tot, 1) * sizeof(sxc_result_dev)));
-  HIPCHK(sxc::launch_finalize(b.counts.as<int32_t>(), total, b.slot_cap, b.offsets.as<int64_t>(), nullptr, nullptr,
-                              b.slots.as<sxc_result_dev>(), b.keys.as<uint64_t>(), b.final_.as<sxc_result_dev>(), s,
-                              false));
-  HIPCHK(hipEventRecord(b.ev[4], s));
-  HIPCHK(hipEventSynchronize(b.ev[4]));
+  if (tot > b.out_cap) {
+    b.out_cap = tot + tot / 4;
+    HIPCHK(b.final_.ensure((size_t)b.out_cap * sizeof(sxc_result_dev)));
+    order = true;
+  }
+  if (order) {
+    HIPCHK(sxc::launch_order_records(b.counts.as<int32_t>(), total, b.slot_cap, b.offsets.as<int64_t>(),
+                                     b.slots.as<sxc_result_dev>(), b.keys.as<uint64_t>(),
+                                     b.final_.as<sxc_result_dev>(), b.out_cap, s));
+    HIPCHK(hipStreamSynchronize(s));
+  }
   b.n_results = tot;
-  b.spectra_ms = elapsed(b.ev[0], b.ev[1]);
-  b.correlate_ms = elapsed(b.ev[1], b.ev[2]);
-  b.scan_ms = elapsed(b.ev[2], b.ev[3]);
-  // includes the wait for the host to read the counts (and any re-run of the scan)
-  b.finalize_ms = elapsed(b.ev[3], b.ev[4]);
   return SXC_OK;
 }
 
@@ -388,6 +489,8 @@ int groups_run(sxc_ctx* ctx, std::vector<Group>& gs, sxc_timing& t, bool upload 
   for (Group& g : gs) {
     const int rc = batch_end(ctx, *g.b);
     if (rc) return rc;
+    t.device_ms += g.b->device_ms;
+    t.graph |= g.b->last_graph ? 1 : 0;
     t.spectra_ms += g.b->spectra_ms;
     t.correlate_ms += g.b->correlate_ms;
     t.scan_ms += g.b->scan_ms;
@@ -506,7 +609,10 @@ int sxc_align_batch(sxc_ctx* ctx, const sxc_pair* pairs, int32_t n, const sxc_pa
   std::vector<Group> gs;
   rc = group_pairs(pairs, n, prm, gs);
   if (rc) return rc;
-  for (Group& gr : gs) gr.b = (prm->N != 0) ? &ctx->scratch : &ctx->scratch_nb[gr.nb];
+  for (Group& gr : gs) {
+    gr.b = (prm->N != 0) ? &ctx->scratch : &ctx->scratch_nb[gr.nb];
+    gr.b->use_graph = false;  // new inputs every call: launched kernel by kernel
+  }
   rc = groups_prepare(gs, pairs, n, prm, nullptr, nullptr);
   if (rc) return rc;
   // uploads overlap the device work of earlier length classes
@@ -596,6 +702,13 @@ int sxc_batch_run(sxc_batch* bt) {
   t.total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   bt->ctx->last = t;
   return rc;
+}
+
+int sxc_batch_set_graph(sxc_batch* bt, int32_t on) {
+  if (!bt) return fail(SXC_ERR_ARG, "null batch");
+  std::lock_guard<std::mutex> g(bt->ctx->mu);
+  for (Group& gr : bt->groups) gr.b->use_graph = on != 0;
+  return SXC_OK;
 }
 
 int sxc_batch_results(sxc_batch* bt, sxc_result* out, int64_t cap, int64_t* n_out) {

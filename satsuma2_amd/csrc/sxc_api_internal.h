@@ -64,14 +64,25 @@ struct sxc_batch_dev {
   int nb = 0;
   int slot_cap = 64;
   int64_t n_results = 0;
-  double spectra_ms = 0, correlate_ms = 0, scan_ms = 0, finalize_ms = 0;  // last run
+  double spectra_ms = 0, correlate_ms = 0, scan_ms = 0, finalize_ms = 0;  // last run, by phase
+  double device_ms = 0;  // last run, first kernel to last copy
   // Own stream, phase events and pinned landing place for the record count, so that the
   // sub-batches of different length classes run side by side.
   hipStream_t s = nullptr;
   hipEvent_t ev[5] = {};
   int64_t* h_ctl = nullptr;  // [0] largest per-(pair, strand) count, [1] total records
+  int64_t out_cap = 0;       // records final_ holds
+  const double* ptable = nullptr;
+  // The run's launch sequence captured as a graph, replayed while the buffers it was
+  // captured with (signature) stay in place.
+  bool use_graph = true;
+  bool last_graph = false;  // the last run replayed the graph
+  int runs = 0;
+  hipGraphExec_t gexec = nullptr;
+  uint64_t gsig = 0;
   void release() {
     if (s) (void)hipStreamSynchronize(s);
+    if (gexec) (void)hipGraphExecDestroy(gexec), gexec = nullptr;
     for (DevBuf* b : {&seq, &tch, &qch, &pairs, &spectra, &which, &slots, &keys, &counts, &offsets, &total, &maxc,
                       &final_, &stats, &cands, &ncand, &sch, &sstr, &tails, &segs, &ttail, &src_sig})
       b->release();

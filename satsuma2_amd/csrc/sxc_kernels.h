@@ -15,9 +15,14 @@ hipError_t launch_encode(int nb, const EncodeArgs& a, int n, hipStream_t s);
 hipError_t launch_xcorr(int nb, const XcorrArgs& a, int n, hipStream_t s);
 hipError_t launch_prob(const ProbArgs& a, hipStream_t s);
 hipError_t launch_order_runs(const int32_t* runs, const uint64_t* keys, int n, int32_t* out, hipStream_t s);
-// scan_only: exclusive scan of counts into offsets/total/max_count; else pack slots into out.
-hipError_t launch_finalize(const int32_t* counts, int n, int cap, int64_t* offsets, int64_t* total,
-                           int32_t* max_count, const sxc_result_dev* slots, const uint64_t* keys,
-                           sxc_result_dev* out, hipStream_t s, bool scan_only);
+// Exclusive scan of the per-(pair, strand) record counts (clipped at cap) into offsets, their
+// total and the largest unclipped count.
+hipError_t launch_count_offsets(const int32_t* counts, int n, int cap, int64_t* offsets, int64_t* total,
+                                int32_t* max_count, hipStream_t s);
+// Orders each (pair, strand)'s slot records and packs them at its offset; records at or past
+// out_cap are dropped.
+hipError_t launch_order_records(const int32_t* counts, int n, int cap, const int64_t* offsets,
+                                const sxc_result_dev* slots, const uint64_t* keys, sxc_result_dev* out,
+                                int64_t out_cap, hipStream_t s);
 
 }  // namespace sxc

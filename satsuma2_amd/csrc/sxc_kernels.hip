@@ -177,9 +177,11 @@ __global__ __launch_bounds__(1024) void k_scan_counts(const int32_t* counts, int
 
 // Orders each (pair, strand)'s kept records by (lag rank, run start) -- the order
 // FilterMatches pushes them -- and packs them at their offsets.
+// Records that would land at or beyond out_cap are not written (the host compares the
+// total with out_cap and packs again into a larger buffer).
 __global__ __launch_bounds__(256) void k_finalize(const sxc_result_dev* slots, const uint64_t* keys,
                                                   const int32_t* counts, const int64_t* offsets, int cap,
-                                                  sxc_result_dev* out) {
+                                                  sxc_result_dev* out, int64_t out_cap) {
   const int ps = blockIdx.x;
   const int n = counts[ps] < cap ? counts[ps] : cap;
   const size_t base = (size_t)ps * cap;
@@ -187,7 +189,8 @@ __global__ __launch_bounds__(256) void k_finalize(const sxc_result_dev* slots, c
     const uint64_t ke = keys[base + e];
     int rank = 0;
     for (int k = 0; k < n; ++k) rank += keys[base + k] < ke;
-    out[offsets[ps] + rank] = slots[base + e];
+    const int64_t at = offsets[ps] + rank;
+    if (at < out_cap) out[at] = slots[base + e];
   }
 }
 
@@ -272,15 +275,18 @@ hipError_t launch_prob(const ProbArgs& a, hipStream_t s) {
   return hipGetLastError();
 }
 
-hipError_t launch_finalize(const int32_t* counts, int n, int cap, int64_t* offsets, int64_t* total,
-                           int32_t* max_count, const sxc_result_dev* slots, const uint64_t* keys,
-                           sxc_result_dev* out, hipStream_t s, bool scan_only) {
+hipError_t launch_count_offsets(const int32_t* counts, int n, int cap, int64_t* offsets, int64_t* total,
+                                int32_t* max_count, hipStream_t s) {
   if (n <= 0) return hipSuccess;
-  if (scan_only) {
-    hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(1024), 0, s, counts, n, cap, offsets, total, max_count);
-  } else {
-    hipLaunchKernelGGL(k_finalize, dim3(n), dim3(256), 0, s, slots, keys, counts, offsets, cap, out);
-  }
+  hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(1024), 0, s, counts, n, cap, offsets, total, max_count);
+  return hipGetLastError();
+}
+
+hipError_t launch_order_records(const int32_t* counts, int n, int cap, const int64_t* offsets,
+                                const sxc_result_dev* slots, const uint64_t* keys, sxc_result_dev* out,
+                                int64_t out_cap, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_finalize, dim3(n), dim3(256), 0, s, slots, keys, counts, offsets, cap, out, out_cap);
   return hipGetLastError();
 }
 

@@ -199,8 +199,8 @@ int launch_scan_slot(sxc_server* sv, Slot& sl) {
   a.table_prob = sv->prm.table_prob;
   HIPCHK(sxc::launch_scan(sv->nb, a, 2 * sl.npairs, sl.s));
   const int total = 2 * sl.npairs;
-  HIPCHK(sxc::launch_finalize(sl.counts.as<int32_t>(), total, sl.cap, sl.offsets.as<int64_t>(), sl.total.as<int64_t>(),
-                              sl.maxc.as<int32_t>(), nullptr, nullptr, nullptr, sl.s, true));
+  HIPCHK(sxc::launch_count_offsets(sl.counts.as<int32_t>(), total, sl.cap, sl.offsets.as<int64_t>(),
+                                   sl.total.as<int64_t>(), sl.maxc.as<int32_t>(), sl.s));
   HIPCHK(hipMemcpyAsync(sl.h_offsets, sl.offsets.p, (size_t)total * sizeof(int64_t), hipMemcpyDeviceToHost, sl.s));
   HIPCHK(hipMemcpyAsync(sl.h_offsets + total, sl.total.p, sizeof(int64_t), hipMemcpyDeviceToHost, sl.s));
   HIPCHK(hipMemcpyAsync(sl.h_maxc, sl.maxc.p, sizeof(int32_t), hipMemcpyDeviceToHost, sl.s));
@@ -290,9 +290,9 @@ int slot_pack(sxc_server* sv, Slot& sl) {
     sl.h_final_cap = n;
     HIPCHK(hipHostMalloc((void**)&sl.h_final, (size_t)n * sizeof(sxc_result), hipHostMallocDefault));
   }
-  HIPCHK(sxc::launch_finalize(sl.counts.as<int32_t>(), total, sl.cap, sl.offsets.as<int64_t>(), nullptr, nullptr,
-                              sl.outs.as<sxc_result_dev>(), sl.keys.as<uint64_t>(), sl.final_.as<sxc_result_dev>(),
-                              sl.s, false));
+  HIPCHK(sxc::launch_order_records(sl.counts.as<int32_t>(), total, sl.cap, sl.offsets.as<int64_t>(),
+                                   sl.outs.as<sxc_result_dev>(), sl.keys.as<uint64_t>(),
+                                   sl.final_.as<sxc_result_dev>(), n, sl.s));
   if (n > 0) HIPCHK(hipMemcpyAsync(sl.h_final, sl.final_.p, (size_t)n * sizeof(sxc_result), hipMemcpyDeviceToHost, sl.s));
   HIPCHK(hipEventRecord(sl.ev_pack, sl.s));
   sl.state = PACKING;

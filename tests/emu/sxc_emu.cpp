@@ -289,6 +289,7 @@ int sxc_batch_run(sxc_batch* b) {
   b->res.swap(r);
   return SXC_OK;
 }
+int sxc_batch_set_graph(sxc_batch*, int32_t) { return SXC_OK; }
 int sxc_batch_results(sxc_batch* b, sxc_result* out, int64_t cap, int64_t* n_out) {
   *n_out = (int64_t)b->res.size();
   if (*n_out > cap) return SXC_ERR_CAPACITY;

@@ -151,6 +151,52 @@ def test_resident_batch_repeatable(ctx, golden):
     rb.close()
 
 
+def test_resident_batch_graph_replays(ctx, golden):
+    """Runs after the first replay a captured HIP graph.  Every replay, with other eager work
+    and synchronisations between them, equals the oracle; so does the first (eager) run, in
+    which tandem-repeat pairs overflow both the per-(pair, strand) record slots (> 64) and
+    the first output buffer (> 1024 records), and an eager run after graphs are switched off."""
+    from satsuma2_amd.api import ChunkPair, HomologyByXCorr, ResidentBatch, SlaveParams
+
+    rng = np.random.default_rng(5)
+    T, Q = [], []
+    for period in (40, 60):
+        unit = rng.choice(list(b"ACGT"), period).astype(np.uint8)
+        t = np.resize(unit, 4096)
+        q = t.copy()
+        m = rng.random(4096) < 0.12
+        q[m] = rng.choice(list(b"ACGT"), int(m.sum()))
+        T.append(t.tobytes())
+        Q.append(q.tobytes())
+    g = golden("uniform_pairs.npz")
+    for r in range(6):
+        T.append(unpack2(g["t"][r], 4096))
+        Q.append(unpack2(g["q"][r], 4096))
+    P = len(T)
+    prm = SlaveParams(target_total=float(4096 * 64))
+    pairs = [ChunkPair(T[r], Q[r], t_record=r, q_record=r, q_record_len=4096) for r in range(P)]
+    ref = np.concatenate(oracle.align_batch(
+        [dict(tseq=T[r], qseq=Q[r], t_record=r, q_record=r, q_record_len=4096) for r in range(P)],
+        oracle.make_params(target_total=float(4096 * 64))))
+    assert len(ref) > 1024 and np.bincount(ref["query_id"] * 2 + ref["reverse"]).max() > 64
+    other = HomologyByXCorr(ctx, prm)
+    rb = ResidentBatch(ctx, pairs, prm)
+    for k in range(5):
+        rb.run()
+        tm = ctx.timing()
+        assert tm.graph == (1 if k > 0 else 0), k
+        assert tm.device_ms > 0, k
+        assert_results_equal(rb.results(), ref, f"run {k}")
+        # eager work of the program between replays
+        assert_results_equal(other.align_pair(pairs[2]), ref[ref["query_id"] == 2], f"eager after run {k}")
+    rb.set_graph(False)
+    rb.run()
+    tm = ctx.timing()
+    assert tm.graph == 0 and tm.correlate_ms > 0 and tm.scan_ms > 0
+    assert_results_equal(rb.results(), ref, "eager again")
+    rb.close()
+
+
 def test_edge_pairs(ctx, golden):
     g = golden("edge_pairs.npz")
     for name in g["names"]:
