@@ -182,6 +182,10 @@ def main():
     T, Q, pairs = make_workload(args.pairs)
     params = SlaveParams(target_total=float(4096 * args.pairs))
     ctx = Context(0)
+    print("[bench] autotuning the FFT kernel geometries", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    tune = ctx.autotune(512, 32768)
+    tune_s = time.perf_counter() - t0
     rb = ResidentBatch(ctx, pairs, params)
 
     print("[bench] warmup", file=sys.stderr, flush=True)
@@ -254,6 +258,12 @@ def main():
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": int(TRAFFIC_PER_PAIR * args.pairs),
                      "kernel": "k_correlate<13>", "bytes_per_pair": B_PAIR},
     }
+    out["autotune"] = {"seconds": round(tune_s, 2), "classes": {
+        str(N): {"pairs_kernel": list(ctx.geometry(N, "pairs")), "spectra_kernel": list(ctx.geometry(N, "spectra")),
+                 "candidates": [[e["stage_a"], e["round_levels"], round(e["correlate_ms"], 4),
+                                 round(e["spectra_ms"], 4)] for e in tune if e["N"] == N]}
+        for N in sorted({e["N"] for e in tune})},
+        "columns": "stage_a, round_levels, pair kernel ms, spectra kernel ms (synthetic batch per length)"}
     if args.uneven_pairs > 0:
         out["uneven"] = uneven_bench(ctx, args.uneven_pairs, args.uneven_steps)
     if not args.no_serve:

@@ -87,6 +87,15 @@ class ServerInfo(C.Structure):
                 ("slots", C.c_int32), ("max_pairs_per_launch", C.c_int32), ("device_ms", C.c_double)]
 
 
+class TuneEntry(C.Structure):
+    _fields_ = [("N", C.c_int32), ("stage_a", C.c_int32), ("round_levels", C.c_int32), ("pairs", C.c_int32),
+                ("correlate_ms", C.c_double), ("spectra_ms", C.c_double), ("chosen_correlate", C.c_int32),
+                ("chosen_spectra", C.c_int32)]
+
+
+SXC_KERNEL_PAIRS = 0
+SXC_KERNEL_SPECTRA = 1
+
 RESULT_DTYPE = np.dtype([("query_id", "<u8"), ("target_id", "<u8"), ("query_size", "<u8"), ("qstart", "<u8"),
                          ("tstart", "<u8"), ("len", "<u8"), ("reverse", "u1"), ("pad_", "u1", (7,)),
                          ("prob", "<f8"), ("ident", "<f8")])
@@ -97,7 +106,8 @@ EXPORTS = ["sxc_create", "sxc_destroy", "sxc_last_error", "sxc_version", "sxc_en
            "sxc_batch_create", "sxc_batch_create_tails", "sxc_batch_run", "sxc_batch_set_graph", "sxc_batch_results",
            "sxc_batch_stats",
            "sxc_batch_destroy", "sxc_last_timing", "sxc_server_create", "sxc_server_submit", "sxc_server_wait",
-           "sxc_server_results", "sxc_server_stats", "sxc_server_destroy"]
+           "sxc_server_results", "sxc_server_stats", "sxc_server_destroy", "sxc_autotune", "sxc_set_geometry",
+           "sxc_get_geometry"]
 
 _lib = None
 _lock = threading.Lock()
@@ -133,6 +143,9 @@ def load() -> C.CDLL:
         L.sxc_batch_create.argtypes = [vp, C.POINTER(Pair), i32, C.POINTER(Params), C.POINTER(vp)]
         L.sxc_batch_run.argtypes = [vp]
         L.sxc_batch_set_graph.argtypes = [vp, i32]
+        L.sxc_autotune.argtypes = [vp, i32, i32, C.POINTER(TuneEntry), i32, C.POINTER(i32)]
+        L.sxc_set_geometry.argtypes = [vp, i32, i32, i32, i32]
+        L.sxc_get_geometry.argtypes = [vp, i32, i32, C.POINTER(i32), C.POINTER(i32)]
         L.sxc_batch_results.argtypes = [vp, C.POINTER(Result), i64, C.POINTER(i64)]
         L.sxc_batch_stats.argtypes = [vp, C.POINTER(i64), C.POINTER(i64)]
         L.sxc_batch_destroy.argtypes = [vp]

@@ -56,6 +56,29 @@ class Context:
     def handle(self):
         return self._h
 
+    # ---- FFT kernel geometry (autotuner)
+    def autotune(self, N_lo: int = 8192, N_hi: int = 0) -> list:
+        """Times every compiled FFT geometry of the pair and target-spectra kernels for each
+        length in [N_lo, N_hi] (sxc_autotune) and keeps the fastest; returns one dict per
+        (length, geometry)."""
+        N_hi = N_hi or N_lo
+        cap = 64
+        out = (_lib.TuneEntry * cap)()
+        n = C.c_int32(0)
+        check(self.L.sxc_autotune(self._h, N_lo, N_hi, out, cap, C.byref(n)))
+        return [{f: getattr(out[i], f) for f, _ in _lib.TuneEntry._fields_} for i in range(n.value)]
+
+    def set_geometry(self, N: int, kernel: str, stage_a: int, round_levels: int):
+        """kernel: "pairs" or "spectra"."""
+        k = _lib.SXC_KERNEL_PAIRS if kernel == "pairs" else _lib.SXC_KERNEL_SPECTRA
+        check(self.L.sxc_set_geometry(self._h, N, k, stage_a, round_levels))
+
+    def geometry(self, N: int, kernel: str = "pairs"):
+        k = _lib.SXC_KERNEL_PAIRS if kernel == "pairs" else _lib.SXC_KERNEL_SPECTRA
+        sa, km = C.c_int32(0), C.c_int32(0)
+        check(self.L.sxc_get_geometry(self._h, N, k, C.byref(sa), C.byref(km)))
+        return sa.value, km.value
+
     # ---- plain entry points (one reference call each)
     def encode(self, seq: bytes, N: int) -> np.ndarray:
         out = np.zeros((4, N), np.float32)

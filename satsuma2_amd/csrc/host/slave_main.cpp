@@ -282,6 +282,7 @@ int main(int argc, char** argv) {
   P.add("-debug_targets", "int",
         "number of targets to receive before exiting (for debug purposes mostly, don't use)", "0");
   P.add("-device", "int", "HIP device to run on", "0");
+  P.add("-autotune", "bool", "time the compiled FFT kernel geometries at this length and run the fastest", "1");
   P.parse();
 
   const std::string master = P.str("-master");
@@ -295,6 +296,7 @@ int main(int argc, char** argv) {
   const bool prob_table = P.boolean("-prob_table");
   const unsigned debug_max = (unsigned)P.i32("-debug_targets");
   const int device = P.i32("-device");
+  const bool autotune = P.boolean("-autotune");
   // FilterMatches keeps prob >= 0.99 (m_minProb, :76); -min_prob is only the probability a
   // ProbTable hit reports (:413-421, analysis/ProbTable.cc:132-136)
   const double min_prob_flag = P.f64("-min_prob");
@@ -335,6 +337,14 @@ int main(int argc, char** argv) {
   if (sxc_create(&job.ctx, device) != SXC_OK) {
     printf("ERROR %s\n", sxc_last_error());
     return 1;
+  }
+  if (autotune) {
+    sxc_tune_entry tune[16];
+    int32_t nt = 0;
+    if (sxc_autotune(job.ctx, job.prm.N, job.prm.N, tune, 16, &nt) != SXC_OK) {
+      printf("ERROR %s\n", sxc_last_error());
+      return 1;
+    }
   }
 
   job.srv = make_server(job);

@@ -150,6 +150,7 @@ int main(int argc, char** argv) {
   P.add("-pairs", "string", "valid block pairs to process", "");
   P.add("-line", "double", "pick one block and compare to the query", "-1.0");
   P.add("-device", "int", "HIP device to run on", "0");
+  P.add("-autotune", "bool", "time the compiled FFT kernel geometries at this length and run the fastest", "1");
   P.parse();
 
   const std::string qfile = P.str("-q"), tfile = P.str("-t"), output = P.str("-o");
@@ -163,6 +164,7 @@ int main(int argc, char** argv) {
   const double cutoff = P.f64("-cutoff");
   const double one_line = P.f64("-line");
   const int device = P.i32("-device");
+  const bool autotune = P.boolean("-autotune");
   (void)P.i32("-cq");
   (void)P.i32("-ct");
 
@@ -271,6 +273,14 @@ int main(int argc, char** argv) {
   if (sxc_create(&ctx, device) != SXC_OK) {
     printf("ERROR %s\n", sxc_last_error());
     return 1;
+  }
+  if (autotune) {
+    sxc_tune_entry tune[16];
+    int32_t nt = 0;
+    if (sxc_autotune(ctx, prm.N, prm.N, tune, 16, &nt) != SXC_OK) {
+      printf("ERROR %s\n", sxc_last_error());
+      return 1;
+    }
   }
   std::vector<sxc_result> res;
   for (size_t at = 0; at < pairs.size(); at += kMaxPairsPerLaunch) {

@@ -160,7 +160,7 @@ int launch_correlate(sxc_ctx* ctx, sxc_batch_dev& b) {
     a.segs = b.segs.as<sxc_tail_seg_dev>();
     a.src_sig = b.src_sig.as<float>();
   }
-  HIPCHK(sxc::launch_correlate(b.nb, a, total, b.s));
+  HIPCHK(sxc::launch_correlate(ctx->geo_corr[b.nb], a, total, b.s));
   return SXC_OK;
 }
 
@@ -204,6 +204,30 @@ int launch_pairs(sxc_ctx* ctx, sxc_batch_dev& b) {
 // if a (pair, strand) kept more records than its slot holds or the records overflow the
 // output buffer, grows the buffer and redoes what it needs eagerly.
 
+// The four spectra of every target chunk of the batch, into b.spectra.
+int launch_target_spectra(sxc_ctx* ctx, sxc_batch_dev& b) {
+  if (b.hb.tchunks.empty()) return SXC_OK;
+  sxc::SpecArgs sa{};
+  sa.tab = ctx->d_tab;
+  sa.osc = ctx->d_osc;
+  sa.ent_lut = ctx->d_ent[b.nb];
+  sa.seq = b.seq.as<unsigned char>();
+  sa.chunks = b.tch.as<sxc_chunk>();
+  sa.which = b.which.as<int32_t>();
+  sa.n = (int)b.hb.tchunks.size();
+  sa.spectra = b.spectra.as<float>();
+  sa.dst = b.which.as<int32_t>();
+  sa.signals = nullptr;
+  if (!b.hb.tails.empty()) {
+    sa.chunk_tail = b.ttail.as<int32_t>();
+    sa.tails = b.tails.as<sxc_tail_dev>();
+    sa.segs = b.segs.as<sxc_tail_seg_dev>();
+    sa.src_sig = b.src_sig.as<float>();
+  }
+  HIPCHK(sxc::launch_spectra(ctx->geo_spec[b.nb], sa, b.s));
+  return SXC_OK;
+}
+
 // Kernels and copies of one run on b.s (no allocation, no host synchronisation).
 int batch_enqueue(sxc_ctx* ctx, sxc_batch_dev& b, bool events) {
   hipStream_t s = b.s;
@@ -215,28 +239,10 @@ int batch_enqueue(sxc_ctx* ctx, sxc_batch_dev& b, bool events) {
                        b.sstr.as<int32_t>(), b.src_sig.as<float>()};
     HIPCHK(sxc::launch_encode(b.nb, ea, (int)b.hb.schunks.size(), s));
   }
-  if (!b.hb.tchunks.empty()) {
-    sxc::SpecArgs sa{};
-    sa.tab = ctx->d_tab;
-    sa.osc = ctx->d_osc;
-    sa.ent_lut = ctx->d_ent[b.nb];
-    sa.seq = b.seq.as<unsigned char>();
-    sa.chunks = b.tch.as<sxc_chunk>();
-    sa.which = b.which.as<int32_t>();
-    sa.n = (int)b.hb.tchunks.size();
-    sa.spectra = b.spectra.as<float>();
-    sa.dst = b.which.as<int32_t>();
-    sa.signals = nullptr;
-    if (!b.hb.tails.empty()) {
-      sa.chunk_tail = b.ttail.as<int32_t>();
-      sa.tails = b.tails.as<sxc_tail_dev>();
-      sa.segs = b.segs.as<sxc_tail_seg_dev>();
-      sa.src_sig = b.src_sig.as<float>();
-    }
-    HIPCHK(sxc::launch_spectra(b.nb, sa, s));
-  }
+  int rc = launch_target_spectra(ctx, b);
+  if (rc) return rc;
   if (events) HIPCHK(hipEventRecord(b.ev[1], s));
-  int rc = launch_correlate(ctx, b);
+  rc = launch_correlate(ctx, b);
   if (rc) return rc;
   if (events) HIPCHK(hipEventRecord(b.ev[2], s));
   rc = launch_pairs(ctx, b);
@@ -254,7 +260,7 @@ int batch_enqueue(sxc_ctx* ctx, sxc_batch_dev& b, bool events) {
 }
 
 // Everything a captured graph depends on besides the batch's (fixed) inputs.
-uint64_t graph_signature(const sxc_batch_dev& b) {
+uint64_t graph_signature(const sxc_ctx* ctx, const sxc_batch_dev& b) {
   uint64_t h = 1469598103934665603ull;
   auto mix = [&](uint64_t v) { h = (h ^ v) * 1099511628211ull; };
   for (const DevBuf* d : {&b.seq, &b.tch, &b.qch, &b.pairs, &b.spectra, &b.which, &b.slots, &b.keys, &b.counts,
@@ -272,6 +278,8 @@ uint64_t graph_signature(const sxc_batch_dev& b) {
   mix(b.hb.schunks.size());
   mix(b.hb.tails.size());
   mix((uint64_t)b.nb);
+  mix((uint64_t)ctx->geo_corr[b.nb]);
+  mix((uint64_t)ctx->geo_spec[b.nb]);
   const unsigned char* pb = (const unsigned char*)&b.prm;
   for (size_t i = 0; i < sizeof b.prm; ++i) mix(pb[i]);
   return h;
@@ -293,7 +301,7 @@ int batch_begin(sxc_ctx* ctx, sxc_batch_dev& b) {
   if (rc) return rc;
   b.h_ctl[0] = b.h_ctl[1] = 0;
   if (b.use_graph && b.runs > 0) {
-    const uint64_t sig = graph_signature(b);
+    const uint64_t sig = graph_signature(ctx, b);
     if (!b.gexec || b.gsig != sig) {
       if (b.gexec) HIPCHK(hipGraphExecDestroy(b.gexec));
       b.gexec = nullptr;
@@ -545,6 +553,151 @@ int groups_download(sxc_ctx* ctx, std::vector<Group>& gs, int32_t n, sxc_result*
   return SXC_OK;
 }
 
+// ---------------------------------------------------------------- autotuner
+//
+// Times every compiled geometry of the pair kernel and the target-spectra kernel for one FFT
+// length on a synthetic batch shaped like the slave's (target and query chunks of N/2 bases,
+// both strands, half the pairs carrying a planted homology), checks that all geometries
+// accept the same lags, and makes the fastest the context's choice.  Called with ctx->mu held.
+
+uint64_t tune_rand(uint64_t& st) {
+  st = st * 6364136223846793005ull + 1442695040888963407ull;
+  return st >> 33;
+}
+
+int tune_class(sxc_ctx* ctx, int nb, std::vector<sxc_tune_entry>& out) {
+  const int N = 1 << nb, L = N / 2;
+  const int P = std::min(16384, std::max(256, (1 << 24) >> nb));
+  std::vector<std::string> T((size_t)P), Q((size_t)P);
+  uint64_t st = 0x5eed0000ull + (uint64_t)nb;
+  const char acgt[4] = {'A', 'C', 'G', 'T'};
+  auto base = [&]() {
+    const uint64_t r = tune_rand(st) % 1000;  // GC 0.41
+    return r < 295 ? 'A' : r < 500 ? 'C' : r < 705 ? 'G' : 'T';
+  };
+  for (int k = 0; k < P; ++k) {
+    T[(size_t)k].resize((size_t)L);
+    Q[(size_t)k].resize((size_t)L);
+    for (int i = 0; i < L; ++i) T[(size_t)k][(size_t)i] = base();
+    for (int i = 0; i < L; ++i) Q[(size_t)k][(size_t)i] = base();
+    if (k % 2 == 0) {  // a homology of L/4 bases at 90% identity
+      const int len = L / 4, at_t = (int)(tune_rand(st) % (uint64_t)(L - len)), at_q = (int)(tune_rand(st) % (uint64_t)(L - len));
+      for (int i = 0; i < len; ++i)
+        Q[(size_t)k][(size_t)(at_q + i)] =
+            (tune_rand(st) % 10 == 0) ? acgt[tune_rand(st) % 4] : T[(size_t)k][(size_t)(at_t + i)];
+    }
+  }
+  std::vector<sxc_pair> pairs((size_t)P);
+  for (int k = 0; k < P; ++k) {
+    sxc_pair& p = pairs[(size_t)k];
+    p = sxc_pair{};
+    p.tseq = T[(size_t)k].data();
+    p.tlen = L;
+    p.qseq = Q[(size_t)k].data();
+    p.qlen = L;
+    p.t_record = k;
+    p.q_record = k;
+    p.q_record_len = L;
+  }
+  sxc_params prm{};
+  prm.N = N;
+  prm.q_chunk = L;
+  prm.strands = 2;
+  prm.cutoff = 1.8;
+  prm.cutoff_fast = 2.9;
+  prm.min_prob = 0.99;
+  prm.target_total = (double)L * P;
+  sxc_batch_dev b;
+  int rc = batch_prepare(b, pairs.data(), P, &prm);
+  if (!rc) rc = batch_upload(ctx, b);
+  if (!rc) rc = batch_alloc(ctx, b);
+  if (rc) {
+    b.release();
+    return rc;
+  }
+  const int total = 2 * P;
+  int geos[8];
+  const int ng = sxc::fft_geometries(nb, geos, 8);
+  std::vector<int32_t> ref_n, got_n((size_t)total);
+  std::vector<uint16_t> ref_c, got_c((size_t)total * N);
+  const size_t first = out.size();
+  double best_c = 1e300, best_s = 1e300;
+  int pick_c = nb, pick_s = nb;
+  const int reps = 3;
+  for (int gi = 0; gi < ng && !rc; ++gi) {
+    const int g = geos[gi];
+    ctx->geo_spec[nb] = ctx->geo_corr[nb] = g;
+    double ms_s = 0, ms_c = 0;
+    for (int r = 0; r <= reps && !rc; ++r) {  // the first repetition warms up
+      if (hipEventRecord(b.ev[0], b.s) != hipSuccess) rc = fail(SXC_ERR_HIP, "hipEventRecord");
+      if (!rc) rc = launch_target_spectra(ctx, b);
+      if (!rc && hipEventRecord(b.ev[1], b.s) != hipSuccess) rc = fail(SXC_ERR_HIP, "hipEventRecord");
+      if (!rc) rc = launch_correlate(ctx, b);
+      if (!rc && hipEventRecord(b.ev[2], b.s) != hipSuccess) rc = fail(SXC_ERR_HIP, "hipEventRecord");
+      if (!rc && hipStreamSynchronize(b.s) != hipSuccess) rc = fail(SXC_ERR_HIP, "hipStreamSynchronize");
+      if (!rc && r > 0) {
+        ms_s += elapsed(b.ev[0], b.ev[1]);
+        ms_c += elapsed(b.ev[1], b.ev[2]);
+      }
+    }
+    if (rc) break;
+    // every geometry must accept the same lags (the same FFTReal arithmetic, bit for bit)
+    if (hipMemcpy(got_n.data(), b.ncand.p, got_n.size() * sizeof(int32_t), hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(got_c.data(), b.cands.p, got_c.size() * sizeof(uint16_t), hipMemcpyDeviceToHost) != hipSuccess) {
+      rc = fail(SXC_ERR_HIP, "hipMemcpy");
+      break;
+    }
+    for (int ps = 0; ps < total; ++ps)
+      std::fill(got_c.begin() + (size_t)ps * N + std::min(N, std::max(0, got_n[(size_t)ps])),
+                got_c.begin() + (size_t)(ps + 1) * N, (uint16_t)0);
+    if (gi == 0) {
+      ref_n = got_n;
+      ref_c = got_c;
+    } else if (got_n != ref_n || got_c != ref_c) {
+      rc = fail(SXC_ERR_HIP, "autotune: FFT geometry " + std::to_string(g) + " of N = " + std::to_string(N) +
+                                 " accepts different lags than the default geometry");
+      break;
+    }
+    sxc_tune_entry e{};
+    e.N = N;
+    e.stage_a = sxc::geo_sa(g);
+    e.round_levels = sxc::geo_km(g);
+    e.pairs = P;
+    e.correlate_ms = ms_c / reps;
+    e.spectra_ms = ms_s / reps;
+    out.push_back(e);
+    if (e.correlate_ms < best_c) best_c = e.correlate_ms, pick_c = g;
+    if (e.spectra_ms < best_s) best_s = e.spectra_ms, pick_s = g;
+  }
+  b.release();
+  if (rc) {
+    ctx->geo_corr[nb] = ctx->geo_spec[nb] = nb;
+    out.resize(first);
+    return rc;
+  }
+  ctx->geo_corr[nb] = pick_c;
+  ctx->geo_spec[nb] = pick_s;
+  for (size_t i = first; i < out.size(); ++i) {
+    sxc_tune_entry& e = out[i];
+    e.chosen_correlate = e.stage_a == sxc::geo_sa(pick_c) && e.round_levels == sxc::geo_km(pick_c);
+    e.chosen_spectra = e.stage_a == sxc::geo_sa(pick_s) && e.round_levels == sxc::geo_km(pick_s);
+  }
+  return SXC_OK;
+}
+
+// The compiled geometry of length 2^nb with the given stage-A width and round depth.
+int find_geometry(int nb, int sa, int km, int* geo) {
+  int geos[8];
+  const int ng = sxc::fft_geometries(nb, geos, 8);
+  for (int i = 0; i < ng; ++i)
+    if (sxc::geo_sa(geos[i]) == sa && sxc::geo_km(geos[i]) == km) {
+      *geo = geos[i];
+      return SXC_OK;
+    }
+  return fail(SXC_ERR_ARG, "no compiled FFT geometry with stage A " + std::to_string(sa) + " and " +
+                               std::to_string(km) + "-level rounds for N = " + std::to_string(1 << nb));
+}
+
 }  // namespace
 
 extern "C" {
@@ -639,6 +792,50 @@ int sxc_align_pair(sxc_ctx* ctx, const sxc_pair* pair, const sxc_params* prm, sx
   const int rc = sxc_align_batch(ctx, pair, 1, prm, out, cap, &n);
   *n_out = (int32_t)n;
   return rc;
+}
+
+int sxc_autotune(sxc_ctx* ctx, int32_t N_lo, int32_t N_hi, sxc_tune_entry* out, int32_t cap, int32_t* n_out) {
+  if (!ctx || !n_out || (cap > 0 && !out)) return fail(SXC_ERR_ARG, "null argument");
+  *n_out = 0;
+  int lo, hi;
+  int rc = check_N(N_lo, &lo);
+  if (!rc) rc = check_N(N_hi, &hi);
+  if (rc) return rc;
+  std::lock_guard<std::mutex> g(ctx->mu);
+  rc = set_device(ctx);
+  if (rc) return rc;
+  std::vector<sxc_tune_entry> res;
+  for (int nb = lo; nb <= hi && !rc; ++nb) rc = tune_class(ctx, nb, res);
+  if (rc) return rc;
+  *n_out = (int32_t)res.size();
+  if ((int64_t)res.size() > cap) return fail(SXC_ERR_CAPACITY, "output capacity too small");
+  std::copy(res.begin(), res.end(), out);
+  return SXC_OK;
+}
+
+int sxc_set_geometry(sxc_ctx* ctx, int32_t N, int32_t kernel, int32_t stage_a, int32_t round_levels) {
+  if (!ctx) return fail(SXC_ERR_ARG, "null context");
+  if (kernel != SXC_KERNEL_PAIRS && kernel != SXC_KERNEL_SPECTRA) return fail(SXC_ERR_ARG, "unknown kernel");
+  int nb, geo = 0;
+  int rc = check_N(N, &nb);
+  if (!rc) rc = find_geometry(nb, stage_a, round_levels, &geo);
+  if (rc) return rc;
+  std::lock_guard<std::mutex> g(ctx->mu);
+  (kernel == SXC_KERNEL_PAIRS ? ctx->geo_corr : ctx->geo_spec)[nb] = geo;
+  return SXC_OK;
+}
+
+int sxc_get_geometry(sxc_ctx* ctx, int32_t N, int32_t kernel, int32_t* stage_a, int32_t* round_levels) {
+  if (!ctx || !stage_a || !round_levels) return fail(SXC_ERR_ARG, "null argument");
+  if (kernel != SXC_KERNEL_PAIRS && kernel != SXC_KERNEL_SPECTRA) return fail(SXC_ERR_ARG, "unknown kernel");
+  int nb;
+  const int rc = check_N(N, &nb);
+  if (rc) return rc;
+  std::lock_guard<std::mutex> g(ctx->mu);
+  const int geo = (kernel == SXC_KERNEL_PAIRS ? ctx->geo_corr : ctx->geo_spec)[nb];
+  *stage_a = sxc::geo_sa(geo);
+  *round_levels = sxc::geo_km(geo);
+  return SXC_OK;
 }
 
 int sxc_last_timing(sxc_ctx* ctx, sxc_timing* out) {

@@ -106,6 +106,10 @@ struct sxc_ctx {
   sxc_batch_dev scratch_nb[16];  // ... per length class when N is chosen per pair
   DevBuf p0, p1, p2, p3, p4, p5, p6;  // plain-API staging
   std::map<double, DevBuf> ptables;   // ProbTable thresholds by target size (-prob_table)
+  // FFT geometry code per log2 length for the pair kernel and the target-spectra kernel
+  // (a plain log2 length: the default geometry; set by sxc_autotune / sxc_set_geometry)
+  int geo_corr[16] = {0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15};
+  int geo_spec[16] = {0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15};
 };
 
 namespace sxc {

@@ -59,12 +59,24 @@ SXC_HD constexpr int pad(int p) { return p + (p >> 5); }
 #define SXC_KMAX 3
 #endif
 
+// A kernel's FFT geometry packed in one template int ("geometry code"): log2 of the
+// transform length in bits 0-4, bit 5 set for a 32-wide stage A, the LDS round depth in
+// bits 6-7.  A plain log2 length (bits 5-7 clear) is the default geometry for that length.
+// Every geometry computes the same FFTReal dataflow; they differ in how positions map to
+// threads and how many levels one LDS round carries.
+SXC_HD constexpr int geo_code(int nb, int sa, int km) { return nb | (sa == 32 ? 32 : 0) | (km << 6); }
+SXC_HD constexpr int geo_nb(int g) { return g & 31; }
+SXC_HD constexpr int geo_sa(int g) { return ((g & 32) || geo_nb(g) >= 15) ? 32 : ((g >> 6) ? 16 : SXC_STAGE_A); }
+SXC_HD constexpr int geo_km(int g) { return (g >> 6) ? (g >> 6) : SXC_KMAX; }
+
 template <int NB>
 struct Geo {
-  static constexpr int N = 1 << NB;
+  static constexpr int LOG = geo_nb(NB);                   // log2(N)
+  static constexpr int N = 1 << LOG;
   static constexpr int H = N / 2;
   // 16 or 32 positions per thread in stage A; N = 32768 needs 32 to stay at 1024 threads
-  static constexpr int SA = (NB >= 15) ? 32 : SXC_STAGE_A;
+  static constexpr int SA = geo_sa(NB);
+  static constexpr int KM = geo_km(NB);                    // levels per LDS round at most
   static constexpr int LSA = SA == 32 ? 5 : 4;             // log2(SA)
   static constexpr int NT = (N / SA < 64) ? 64 : N / SA;   // threads per workgroup
   static constexpr int YLEN = N + N / 32;                  // padded LDS floats
@@ -157,8 +169,8 @@ SXC_HD inline void level_contig(float* v, const Twiddle& tw) {
 // Time position read by slot r of stage-A group g (pass 1+2 reads x[br[4k+m]]).
 template <int NB>
 SXC_HD inline int stageA_pos(int g, int r) {
-  constexpr int L = Geo<NB>::LSA;
-  return (int)((bitrev((unsigned)r, L) << (NB - L)) | bitrev((unsigned)g, NB - L));
+  constexpr int L = Geo<NB>::LSA, LG = Geo<NB>::LOG;
+  return (int)((bitrev((unsigned)r, L) << (LG - L)) | bitrev((unsigned)g, LG - L));
 }
 
 // Forward levels pass1+2, pass3, pass_3 (and pass_4 for SA = 32) (FFTReal.hpp:466-528,
@@ -310,7 +322,7 @@ SXC_HD inline void unit_level_special(float* v, const Twiddle& tw) {
 // Round geometry for NB with K levels from M.
 template <int NB, int K, int M>
 struct RoundGeo {
-  static constexpr int N = 1 << NB;
+  static constexpr int N = Geo<NB>::N;
   static constexpr int HM = 1 << (M - 1);
   static constexpr int UPB = HM / 2;                  // units per big block
   static constexpr int BIG = (1 << (K + 1)) * HM;     // big block length
@@ -365,12 +377,12 @@ SXC_HD inline void round_unit(float* Y, int u, const Twiddle& tw) {
   }
 }
 
-// Level schedule above stage A: levels LSA .. NB-1 in rounds of KMAX, the remainder last.
+// Level schedule above stage A: levels LSA .. log2(N)-1 in rounds of KM, the remainder last.
 template <int NB>
 struct Schedule {
   static constexpr int M0 = Geo<NB>::LSA;
-  static constexpr int KM = SXC_KMAX;
-  static constexpr int R = NB - M0;
+  static constexpr int KM = Geo<NB>::KM;
+  static constexpr int R = Geo<NB>::LOG - M0;
   static constexpr int NR = (R + KM - 1) / KM;
   static constexpr int k(int r) { return (r < R / KM) ? KM : (R - KM * (R / KM)); }
   static constexpr int m(int r) { return M0 + KM * r; }

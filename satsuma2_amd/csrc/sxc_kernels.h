@@ -7,10 +7,14 @@
 
 namespace sxc {
 
-hipError_t launch_correlate(int nb, const PairArgs& a, int total, hipStream_t s);
+// The FFT kernels take a geometry code (geo_code in sxc_fft.h; a plain log2 length is the
+// default geometry for that length); hipErrorInvalidValue if it was not compiled.
+hipError_t launch_correlate(int geo, const PairArgs& a, int total, hipStream_t s);
+// Compiled geometries of length 2^nb (distinct stage-A width / round depth), default first.
+int fft_geometries(int nb, int* out, int cap);
 hipError_t launch_scan(int nb, const ScanArgs& a, int total, hipStream_t s);
 hipError_t launch_peaks(int nb, const float* xc, double cutoff, uint16_t* cands, int32_t* ncand, hipStream_t s);
-hipError_t launch_spectra(int nb, const SpecArgs& a, hipStream_t s);
+hipError_t launch_spectra(int geo, const SpecArgs& a, hipStream_t s);
 hipError_t launch_encode(int nb, const EncodeArgs& a, int n, hipStream_t s);
 hipError_t launch_xcorr(int nb, const XcorrArgs& a, int n, hipStream_t s);
 hipError_t launch_prob(const ProbArgs& a, hipStream_t s);

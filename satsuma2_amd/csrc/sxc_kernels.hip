@@ -4,53 +4,12 @@
 // double arithmetic are never contracted into FMA; see sxc_fft.h).
 #include <hip/hip_runtime.h>
 
+#include "sxc_fft_kernels.h"
 #include "sxc_kernels.h"
 #include "sxc_pipeline.h"
 #include "sxc_scan.h"
 
 namespace sxc {
-
-// Device executor: one HIP thread per emulated thread; per-thread state in registers.
-// Each phase receives an opaque copy of the thread index, so the compiler recomputes the
-// phase's (cheap) LDS addresses instead of hoisting every round's addresses out of the
-// channel loop and keeping them all live.
-template <class R>
-struct DevExec {
-  R& r;
-  template <class F>
-  __device__ __forceinline__ void each(F&& f) {
-    int t = (int)threadIdx.x;
-    asm volatile("" : "+v"(t));
-    f(t);
-  }
-  __device__ __forceinline__ void sync() { __syncthreads(); }
-  template <class T>
-  __device__ __forceinline__ T& reg(int) {
-    return r;
-  }
-  __device__ __forceinline__ int atomic_add_shared(int* p, int v) { return atomicAdd(p, v); }
-};
-
-// Workgroup -> (pair, strand): the two strands of a pair read the same resident target
-// spectra, so they are placed 8 workgroups apart, which the dispatcher sends to the same
-// XCD (and its L2) under round-robin XCD assignment.
-__device__ __forceinline__ int xcd_pair_strand(int b, int strands, int total) {
-  if (strands != 2) return b;
-  const int grp = b / 16, r = b % 16;
-  const int ps = (grp * 8 + (r % 8)) * 2 + (r / 8);
-  return ps < total ? ps : -1;
-}
-
-template <int NB>
-__global__ __launch_bounds__(Geo<NB>::NT) void k_correlate(PairArgs a, int total) {
-  __shared__ __align__(16) char sm[Smem<NB>::BYTES];
-  Regs<NB> r;
-  DevExec<Regs<NB>> ex{r};
-  // the grid is rounded up to a multiple of 16 for the strand interleave
-  const int ps = xcd_pair_strand((int)blockIdx.x, a.strands, total);
-  if (ps < 0) return;
-  correlate_body<NB>(ex, sm, a, ps);
-}
 
 struct NoRegs {};
 
@@ -69,14 +28,6 @@ __global__ __launch_bounds__(Geo<NB>::NT) void k_peaks(const float* xc, double c
   Regs<NB> r;
   DevExec<Regs<NB>> ex{r};
   peaks_body<NB>(ex, sm, xc, cutoff, cands, ncand);
-}
-
-template <int NB>
-__global__ __launch_bounds__(Geo<NB>::NT) void k_spectra(SpecArgs a) {
-  __shared__ __align__(16) char sm[Smem<NB>::BYTES];
-  Regs<NB> r;
-  DevExec<Regs<NB>> ex{r};
-  spectra_body<NB>(ex, sm, a, (int)blockIdx.x);
 }
 
 // Plain-API kernels ----------------------------------------------------------------
@@ -226,12 +177,42 @@ hipError_t launch_order_runs(const int32_t* runs, const uint64_t* keys, int n, i
     default: return hipErrorInvalidValue; \
   }
 
-hipError_t launch_correlate(int nb, const PairArgs& a, int total, hipStream_t s) {
-  const int grid = (a.strands == 2) ? ((total + 15) / 16) * 16 : total;
-#define CALL(X) hipLaunchKernelGGL(k_correlate<X>, dim3(grid), dim3(Geo<X>::NT), 0, s, a, total)
-  SXC_NB_SWITCH(nb, CALL)
-#undef CALL
-  return hipGetLastError();
+const FftEntry* fft_table_default(int* n) {
+  static const FftEntry t[] = {SXC_FFT_ENTRY(9) SXC_FFT_ENTRY(10) SXC_FFT_ENTRY(11) SXC_FFT_ENTRY(12)
+                                   SXC_FFT_ENTRY(13) SXC_FFT_ENTRY(14) SXC_FFT_ENTRY(15)};
+  *n = (int)(sizeof(t) / sizeof(t[0]));
+  return t;
+}
+
+const FftEntry* fft_entry(int geo) {
+  for (auto tf : {fft_table_default, fft_table_r2, fft_table_s32}) {
+    int n = 0;
+    const FftEntry* t = tf(&n);
+    for (int i = 0; i < n; ++i)
+      if (t[i].geo == geo) return &t[i];
+  }
+  return nullptr;
+}
+
+int fft_geometries(int nb, int* out, int cap) {
+  int n = 0;
+  for (auto tf : {fft_table_default, fft_table_r2, fft_table_s32}) {
+    int nt = 0;
+    const FftEntry* t = tf(&nt);
+    for (int i = 0; i < nt; ++i) {
+      const int g = t[i].geo;
+      if (geo_nb(g) != nb) continue;
+      bool dup = false;  // the same stage-A width and round depth under another code
+      for (int j = 0; j < n; ++j) dup |= geo_sa(out[j]) == geo_sa(g) && geo_km(out[j]) == geo_km(g);
+      if (!dup && n < cap) out[n++] = g;
+    }
+  }
+  return n;
+}
+
+hipError_t launch_correlate(int geo, const PairArgs& a, int total, hipStream_t s) {
+  const FftEntry* e = fft_entry(geo);
+  return e ? e->correlate(a, total, s) : hipErrorInvalidValue;
 }
 
 hipError_t launch_scan(int nb, const ScanArgs& a, int total, hipStream_t s) {
@@ -248,11 +229,9 @@ hipError_t launch_peaks(int nb, const float* xc, double cutoff, uint16_t* cands,
   return hipGetLastError();
 }
 
-hipError_t launch_spectra(int nb, const SpecArgs& a, hipStream_t s) {
-#define CALL(X) hipLaunchKernelGGL(k_spectra<X>, dim3(a.n), dim3(Geo<X>::NT), 0, s, a)
-  SXC_NB_SWITCH(nb, CALL)
-#undef CALL
-  return hipGetLastError();
+hipError_t launch_spectra(int geo, const SpecArgs& a, hipStream_t s) {
+  const FftEntry* e = fft_entry(geo);
+  return e ? e->spectra(a, s) : hipErrorInvalidValue;
 }
 
 hipError_t launch_encode(int nb, const EncodeArgs& a, int n, hipStream_t s) {

@@ -116,7 +116,7 @@ struct Smem {
   static constexpr int RCT = QUART + 4 * SXC_NCLASS;           // uchar[128] reverse complement
   // strand-applied chunk bytes: an LDS copy up to N = 16384; at 32768 the plan would not fit
   // the 160 KB of a workgroup, and the bytes are read from global memory instead
-  static constexpr bool SEQ_IN_LDS = NB <= 14;
+  static constexpr bool SEQ_IN_LDS = G::LOG <= 14;
   static constexpr int QSEQ = (RCT + 128 + 15) & ~15;          // uchar[N] when SEQ_IN_LDS
   static constexpr int BYTES = (QSEQ + (SEQ_IN_LDS ? N : 0) + 15) & ~15;
   // after the peaks phase Y holds the accepted lags
@@ -295,7 +295,7 @@ SXC_HD inline float tail_at(const TailRef& t, int p, int ch) {
     if (p < t.seg[mid].end) hi = mid; else lo = mid + 1;
   }
   if (lo >= t.n) return 0.f;
-  return t.sig[((size_t)t.seg[lo].src * 4 + ch) * (size_t)(1 << NB) + p];
+  return t.sig[((size_t)t.seg[lo].src * 4 + ch) * (size_t)Geo<NB>::N + p];
 }
 
 // Signal value of channel ch at time position p (CCSignal::SeqToPCM, :118-131).

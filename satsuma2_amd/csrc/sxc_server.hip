@@ -96,6 +96,7 @@ struct sxc_server {
   sxc_ctx* ctx = nullptr;
   sxc_params prm{};
   int nb = 0;
+  int geo_corr = 0, geo_spec = 0;  // FFT geometry codes of the pair and spectra kernels
   int P = 2048;
   DevBuf seq, tch, qch, spectra;
   const double* ptable = nullptr;  // -prob_table thresholds (owned by the context)
@@ -264,7 +265,7 @@ int slot_fill(sxc_server* sv, Slot& sl) {
   a.cutoff_fast = sv->prm.cutoff_fast;
   a.cands = sl.cands.as<uint16_t>();
   a.ncand = sl.ncand.as<int32_t>();
-  HIPCHK(sxc::launch_correlate(sv->nb, a, 2 * sl.npairs, sl.s));
+  HIPCHK(sxc::launch_correlate(sv->geo_corr, a, 2 * sl.npairs, sl.s));
   int rc = launch_scan_slot(sv, sl);
   if (rc) return rc;
   sl.state = COUNTING;
@@ -462,7 +463,7 @@ int server_init(sxc_server* sv, const sxc_chunk_desc* targets, int32_t nt, const
     sa.n = n;
     sa.spectra = sv->spectra.as<float>();
     sa.dst = which.as<int32_t>();
-    HIPCHK(sxc::launch_spectra(sv->nb, sa, s));
+    HIPCHK(sxc::launch_spectra(sv->geo_spec, sa, s));
     HIPCHK(hipStreamSynchronize(s));  // idx is reused
   }
   which.release();
@@ -501,6 +502,8 @@ int sxc_server_create(sxc_ctx* ctx, const sxc_params* prm, const sxc_chunk_desc*
   sv->prm = *prm;
   sv->prm.strands = 2;
   sv->nb = nb;
+  sv->geo_corr = ctx->geo_corr[nb];  // the context's geometry choice when the server starts
+  sv->geo_spec = ctx->geo_spec[nb];
   const int rc = server_init(sv, targets, nt, queries, nq, cfg);
   if (rc) {
     server_release(sv);

@@ -5,6 +5,7 @@
 // boundaries, LDS is a heap buffer.  Built with AddressSanitizer by tests/test_emu.py so
 // every LDS and global index the kernels compute is bounds-checked on the CPU before the
 // same code runs on a GPU.  Not part of the product.
+#include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -58,11 +59,13 @@ const Tables& tables() {
   return t;
 }
 
-template <int NB>
+// GEO: the geometry code of the FFT kernels (a plain log2 length: the default geometry).
+template <int GEO>
 int run_batch(const sxc::HostBatch& hb, const sxc_params& prm, std::vector<sxc_result_dev>& res, int cap,
               float* xc_out, int32_t* stats) {
-  using G = sxc::Geo<NB>;
-  using SM = sxc::Smem<NB>;
+  constexpr int NB = sxc::geo_nb(GEO);
+  using G = sxc::Geo<GEO>;
+  using SM = sxc::Smem<GEO>;
   const Tables& T = tables();
   std::vector<float> spectra((size_t)hb.tchunks.size() * 4 * G::N);
   std::vector<int32_t> which(hb.tchunks.size()), dst(hb.tchunks.size());
@@ -73,9 +76,9 @@ int run_batch(const sxc::HostBatch& hb, const sxc_params& prm, std::vector<sxc_r
   sxc::EncodeArgs ea{&T.tab, T.ent[NB].data(), hb.seq.data(), hb.schunks.data(), 0, hb.sstrand.data(),
                      src_sig.data()};
   for (int k = 0; k < (int)hb.schunks.size(); ++k) {
-    EmuExec<sxc::Regs<NB>> ex(G::NT);
+    EmuExec<sxc::Regs<GEO>> ex(G::NT);
     memset(sm.data(), 0xCD, sm.size());
-    sxc::encode_body<NB>(ex, sm.data(), ea, k);
+    sxc::encode_body<GEO>(ex, sm.data(), ea, k);
   }
   const bool tails = !hb.tails.empty();
   sxc::SpecArgs sa{&T.tab, T.osc.data(), T.ent[NB].data(), hb.seq.data(), hb.tchunks.data(), which.data(),
@@ -83,9 +86,9 @@ int run_batch(const sxc::HostBatch& hb, const sxc_params& prm, std::vector<sxc_r
                    tails ? hb.ttail.data() : nullptr, tails ? hb.tails.data() : nullptr,
                    tails ? hb.segs.data() : nullptr, tails ? src_sig.data() : nullptr};
   for (int k = 0; k < sa.n; ++k) {
-    EmuExec<sxc::Regs<NB>> ex(G::NT);
+    EmuExec<sxc::Regs<GEO>> ex(G::NT);
     memset(sm.data(), 0xCD, sm.size());
-    sxc::spectra_body<NB>(ex, sm.data(), sa, k);
+    sxc::spectra_body<GEO>(ex, sm.data(), sa, k);
   }
   const int total = (int)hb.pairs.size() * prm.strands;
   std::vector<sxc_result_dev> slots((size_t)total * cap);
@@ -114,9 +117,9 @@ int run_batch(const sxc::HostBatch& hb, const sxc_params& prm, std::vector<sxc_r
     pa.src_sig = src_sig.data();
   }
   for (int ps = 0; ps < total; ++ps) {
-    EmuExec<sxc::Regs<NB>> ex(G::NT);
+    EmuExec<sxc::Regs<GEO>> ex(G::NT);
     memset(sm.data(), 0xCD, sm.size());
-    sxc::correlate_body<NB>(ex, sm.data(), pa, ps);
+    sxc::correlate_body<GEO>(ex, sm.data(), pa, ps);
   }
   sxc::ScanArgs sc{};
   sc.tab = &T.tab;
@@ -178,16 +181,27 @@ int emu_align_batch_tails(const sxc_pair* pairs, int n, const sxc_params* prm, c
   std::vector<sxc_result_dev> res;
   const int slot_cap = 4096;
   int rc;
-  switch (sxc::nb_of(prm->N)) {
-    case 9: rc = run_batch<9>(hb, *prm, res, slot_cap, xc_out, stats); break;
-    case 10: rc = run_batch<10>(hb, *prm, res, slot_cap, xc_out, stats); break;
-    case 11: rc = run_batch<11>(hb, *prm, res, slot_cap, xc_out, stats); break;
-    case 12: rc = run_batch<12>(hb, *prm, res, slot_cap, xc_out, stats); break;
-    case 13: rc = run_batch<13>(hb, *prm, res, slot_cap, xc_out, stats); break;
-    case 14: rc = run_batch<14>(hb, *prm, res, slot_cap, xc_out, stats); break;
-    case 15: rc = run_batch<15>(hb, *prm, res, slot_cap, xc_out, stats); break;
+  // SXC_EMU_GEO="<stage A>,<round levels>" runs that geometry of the FFT kernels (the
+  // alternatives the library compiles for the autotuner); unset: the default geometry
+  int geo = sxc::nb_of(prm->N);
+  if (const char* v = getenv("SXC_EMU_GEO")) {
+    int sa = 0, km = 0;
+    if (sscanf(v, "%d,%d", &sa, &km) != 2) return -1;
+    geo = sxc::geo_code(geo, sa, km);
+  }
+#define SXC_EMU_CASE(G) \
+  case (G): rc = run_batch<(G)>(hb, *prm, res, slot_cap, xc_out, stats); break;
+  switch (geo) {
+    SXC_EMU_CASE(9) SXC_EMU_CASE(10) SXC_EMU_CASE(11) SXC_EMU_CASE(12) SXC_EMU_CASE(13) SXC_EMU_CASE(14)
+    SXC_EMU_CASE(15)
+    SXC_EMU_CASE(sxc::geo_code(9, 16, 2)) SXC_EMU_CASE(sxc::geo_code(10, 16, 2)) SXC_EMU_CASE(sxc::geo_code(11, 16, 2))
+    SXC_EMU_CASE(sxc::geo_code(12, 16, 2)) SXC_EMU_CASE(sxc::geo_code(13, 16, 2)) SXC_EMU_CASE(sxc::geo_code(14, 16, 2))
+    SXC_EMU_CASE(sxc::geo_code(15, 32, 2))
+    SXC_EMU_CASE(sxc::geo_code(11, 32, 3)) SXC_EMU_CASE(sxc::geo_code(12, 32, 3)) SXC_EMU_CASE(sxc::geo_code(13, 32, 3))
+    SXC_EMU_CASE(sxc::geo_code(14, 32, 3))
     default: return -1;
   }
+#undef SXC_EMU_CASE
   if (rc) return rc;
   *n_out = (int64_t)res.size();
   if ((int64_t)res.size() > cap) return -3;
@@ -366,4 +380,15 @@ int sxc_server_results(sxc_server* sv, uint64_t ticket, sxc_result* out, int64_t
   return SXC_OK;
 }
 void sxc_server_destroy(sxc_server* sv) { delete sv; }
+// One geometry per length here: nothing to time.
+int sxc_autotune(sxc_ctx*, int32_t, int32_t, sxc_tune_entry*, int32_t, int32_t* n_out) {
+  *n_out = 0;
+  return SXC_OK;
+}
+int sxc_set_geometry(sxc_ctx*, int32_t, int32_t, int32_t, int32_t) { return SXC_OK; }
+int sxc_get_geometry(sxc_ctx*, int32_t N, int32_t, int32_t* sa, int32_t* km) {
+  *sa = N >= 32768 ? 32 : 16;
+  *km = 3;
+  return SXC_OK;
+}
 }  // extern "C"

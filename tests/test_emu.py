@@ -77,6 +77,36 @@ def test_emulated_batch_uniform(emu_lib, golden):
         assert st[ps, 0] == len(oracle.findtop(oxc, 2.9 if p["fast"] else 1.8))
 
 
+# The alternative FFT geometries the library compiles for its autotuner (stage-A width,
+# levels per LDS round), by log2 length; each must reproduce the oracle bit for bit.
+GEOMETRIES = [(nb, 16, 2) for nb in range(9, 15)] + [(15, 32, 2)] + [(nb, 32, 3) for nb in range(11, 15)]
+
+
+@pytest.mark.parametrize("nb,sa,km", GEOMETRIES)
+def test_emulated_geometries(emu_lib, nb, sa, km, monkeypatch):
+    N = 1 << nb
+    rng = np.random.default_rng(nb * 100 + sa + km)
+    pairs = []
+    for r in range(2):
+        t = rng.choice(np.frombuffer(b"ACGT", np.uint8), N // 2)
+        q = rng.choice(np.frombuffer(b"ACGT", np.uint8), N // 2)
+        ln = N // 8
+        q[N // 5: N // 5 + ln] = t[N // 7: N // 7 + ln]  # a homology
+        pairs.append(dict(tseq=t.tobytes(), qseq=q.tobytes(), t_record=r, q_record=r))
+    monkeypatch.setenv("SXC_EMU_GEO", f"{sa},{km}")
+    got, xc, _ = emu_batch(emu_lib, pairs, N=N, target_total=1e6, want_xc=True)
+    ref = np.concatenate(oracle.align_batch([dict(p, q_record_len=N // 2) for p in pairs],
+                                            oracle.make_params(N=N, target_total=1e6)))
+    assert len(ref) > 0
+    for c in COLS:
+        assert np.array_equal(got[c], ref[c]), c
+    for ps in range(4):
+        p = pairs[ps // 2]
+        qs = p["qseq"] if ps % 2 == 0 else oracle.reverse_complement(p["qseq"])
+        oxc = oracle.xcorr(oracle.encode(p["tseq"], N), oracle.encode(qs, N))
+        assert np.array_equal(xc[ps].view(np.uint32), oxc.view(np.uint32)), ps
+
+
 def _write_input(path, pairs, N, target_total=1e6):
     with open(path, "wb") as f:
         f.write(struct.pack("<3i", N, 2, len(pairs)))

@@ -197,6 +197,76 @@ def test_resident_batch_graph_replays(ctx, golden):
     rb.close()
 
 
+# Alternative FFT geometries compiled for the autotuner: (log2 N, stage-A width, levels per
+# LDS round).  Each must reproduce the oracle bit for bit.
+GEOMETRIES = [(nb, 16, 2) for nb in range(9, 15)] + [(15, 32, 2)] + [(nb, 32, 3) for nb in range(11, 15)]
+
+
+def _homology_pairs(N, n, seed):
+    rng = np.random.default_rng(seed)
+    out = []
+    for r in range(n):
+        t = rng.choice(np.frombuffer(b"ACGT", np.uint8), N // 2)
+        q = rng.choice(np.frombuffer(b"ACGT", np.uint8), N // 2)
+        ln = N // 8
+        q[N // 5: N // 5 + ln] = t[N // 7: N // 7 + ln]
+        out.append((t.tobytes(), q.tobytes()))
+    return out
+
+
+@pytest.mark.parametrize("nb,sa,km", GEOMETRIES)
+def test_fft_geometry_parity(ctx, nb, sa, km):
+    from satsuma2_amd.api import ChunkPair, HomologyByXCorr, SlaveParams
+
+    N = 1 << nb
+    tq = _homology_pairs(N, 8, nb * 7 + sa + km)
+    pairs = [ChunkPair(t, q, t_record=r, q_record=r, q_record_len=N // 2) for r, (t, q) in enumerate(tq)]
+    ref = np.concatenate(oracle.align_batch(
+        [dict(tseq=t, qseq=q, t_record=r, q_record=r, q_record_len=N // 2) for r, (t, q) in enumerate(tq)],
+        oracle.make_params(N=N, target_total=1e6)))
+    assert len(ref) > 0
+    dflt = ctx.geometry(N, "pairs"), ctx.geometry(N, "spectra")
+    try:
+        ctx.set_geometry(N, "pairs", sa, km)
+        ctx.set_geometry(N, "spectra", sa, km)
+        assert ctx.geometry(N, "pairs") == (sa, km)
+        got = HomologyByXCorr(ctx, SlaveParams(t_chunk=N // 2, target_total=1e6)).align_batch(pairs)
+        assert_results_equal(got, ref, f"geometry {nb},{sa},{km}")
+    finally:
+        ctx.set_geometry(N, "pairs", *dflt[0])
+        ctx.set_geometry(N, "spectra", *dflt[1])
+
+
+def test_autotune(ctx, golden):
+    """The autotuner times every geometry of every length, keeps one per kernel and length,
+    and batches run in the chosen geometries give the golden records."""
+    from satsuma2_amd.api import ChunkPair, HomologyByXCorr, SlaveParams
+
+    entries = ctx.autotune(512, 32768)
+    by_n = {}
+    for e in entries:
+        by_n.setdefault(e["N"], []).append(e)
+        assert e["correlate_ms"] > 0 and e["spectra_ms"] > 0 and e["pairs"] >= 256
+    assert sorted(by_n) == [1 << nb for nb in range(9, 16)]
+    for N, es in by_n.items():
+        assert len(es) >= 2
+        assert sum(e["chosen_correlate"] for e in es) == 1 and sum(e["chosen_spectra"] for e in es) == 1
+        pick = [e for e in es if e["chosen_correlate"]][0]
+        assert ctx.geometry(N, "pairs") == (pick["stage_a"], pick["round_levels"])
+        assert pick["correlate_ms"] == min(e["correlate_ms"] for e in es)
+    g = golden("uniform_pairs.npz")
+    P = 64
+    T = [unpack2(g["t"][r], 4096) for r in range(P)]
+    Q = [unpack2(g["q"][r], 4096) for r in range(P)]
+    pairs = [ChunkPair(T[r], Q[r], t_record=r, q_record=r, q_record_len=4096, fast=int(g["fast"][r]))
+             for r in range(P)]
+    got = HomologyByXCorr(ctx, SlaveParams(target_total=float(4096 * g["t"].shape[0]))).align_batch(pairs)
+    assert_results_equal(got, g["results"][g["results"]["query_id"] < P], "after autotune")
+    for N in by_n:  # back to the defaults for the tests that follow
+        ctx.set_geometry(N, "pairs", 32 if N == 32768 else 16, 3)
+        ctx.set_geometry(N, "spectra", 32 if N == 32768 else 16, 3)
+
+
 def test_edge_pairs(ctx, golden):
     g = golden("edge_pairs.npz")
     for name in g["names"]:
