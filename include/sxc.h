@@ -200,14 +200,19 @@ typedef struct {
   int64_t hbm_budget;            /* device bytes for resident data + slots; 0 = 3/4 of free memory */
   int32_t max_pairs_per_launch;  /* 0 = 2048 */
   int32_t max_slots;             /* launches in flight; 0 = 3 */
+  int32_t no_query_spectra;      /* 1: do not keep query spectra resident (0: keep both strands'
+                                    spectra of every query chunk when they fit the budget, so a
+                                    served pair skips the query's encode and forward transforms) */
+  int32_t pad_;
 } sxc_server_config;
 
 typedef struct {
   int64_t requests_done, pairs_done, launches, results;
-  int64_t resident_bytes;        /* genome bytes, chunk tables and target spectra */
+  int64_t resident_bytes;        /* genome bytes, chunk tables, target (and query) spectra */
   int64_t slot_bytes;            /* device bytes of one launch slot */
   int32_t slots, max_pairs_per_launch;
   double device_ms;              /* summed device time of all launches */
+  int64_t query_spectra_bytes;   /* resident query spectra (0: the pair kernel transforms queries) */
 } sxc_server_info;
 
 /* prm->strands is ignored (both strands, as Align). */

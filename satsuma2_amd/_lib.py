@@ -78,13 +78,15 @@ class Block(C.Structure):
 
 
 class ServerConfig(C.Structure):
-    _fields_ = [("hbm_budget", C.c_int64), ("max_pairs_per_launch", C.c_int32), ("max_slots", C.c_int32)]
+    _fields_ = [("hbm_budget", C.c_int64), ("max_pairs_per_launch", C.c_int32), ("max_slots", C.c_int32),
+                ("no_query_spectra", C.c_int32), ("pad_", C.c_int32)]
 
 
 class ServerInfo(C.Structure):
     _fields_ = [("requests_done", C.c_int64), ("pairs_done", C.c_int64), ("launches", C.c_int64),
                 ("results", C.c_int64), ("resident_bytes", C.c_int64), ("slot_bytes", C.c_int64),
-                ("slots", C.c_int32), ("max_pairs_per_launch", C.c_int32), ("device_ms", C.c_double)]
+                ("slots", C.c_int32), ("max_pairs_per_launch", C.c_int32), ("device_ms", C.c_double),
+                ("query_spectra_bytes", C.c_int64)]
 
 
 class TuneEntry(C.Structure):

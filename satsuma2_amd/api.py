@@ -274,7 +274,8 @@ class Server:
     (target_from, target_to, query_from, query_to, fast) taken as they arrive."""
 
     def __init__(self, ctx: Context, params: SlaveParams, targets: Sequence[Chunk], queries: Sequence[Chunk],
-                 hbm_budget: int = 0, max_pairs_per_launch: int = 0, max_slots: int = 0):
+                 hbm_budget: int = 0, max_pairs_per_launch: int = 0, max_slots: int = 0,
+                 query_spectra: bool = True):
         self.ctx = ctx
         self._keep = [c.seq for c in targets] + [c.seq for c in queries]
 
@@ -287,7 +288,7 @@ class Server:
 
         self._t, self._q = descs(targets), descs(queries)
         self._prm = params.to_c()
-        cfg = _lib.ServerConfig(hbm_budget, max_pairs_per_launch, max_slots)
+        cfg = _lib.ServerConfig(hbm_budget, max_pairs_per_launch, max_slots, 0 if query_spectra else 1, 0)
         self._h = C.c_void_p()
         check(ctx.L.sxc_server_create(ctx.handle, C.byref(self._prm), self._t, len(targets), self._q, len(queries),
                                       C.byref(cfg), C.byref(self._h)))

@@ -29,6 +29,7 @@ struct DevExec {
     return r;
   }
   __device__ __forceinline__ int atomic_add_shared(int* p, int v) { return atomicAdd(p, v); }
+  __device__ __forceinline__ int atomic_add_global(int* p, int v) { return atomicAdd(p, v); }
 };
 
 // Workgroup -> (pair, strand): the two strands of a pair read the same resident target

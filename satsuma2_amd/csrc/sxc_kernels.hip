@@ -216,7 +216,8 @@ hipError_t launch_correlate(int geo, const PairArgs& a, int total, hipStream_t s
 }
 
 hipError_t launch_scan(int nb, const ScanArgs& a, int total, hipStream_t s) {
-#define CALL(X) hipLaunchKernelGGL(k_scan<X>, dim3(total), dim3(SCAN_NT), 0, s, a)
+  const int grid = total * (a.split > 1 ? a.split : 1);
+#define CALL(X) hipLaunchKernelGGL(k_scan<X>, dim3(grid), dim3(SCAN_NT), 0, s, a)
   SXC_NB_SWITCH(nb, CALL)
 #undef CALL
   return hipGetLastError();

@@ -41,6 +41,11 @@ struct PairArgs {
   const sxc_tail_dev* tails;     // signal tails (pairs[].q_tail indexes them), may be null
   const sxc_tail_seg_dev* segs;
   const float* src_sig;          // tail source signals, 4*N floats each
+  // optional resident query spectra: 4*N floats per (query chunk, strand), at
+  // qspectra + (2 * q_chunk + strand) * 4N; the pair kernel then skips the query's encode
+  // and forward transforms (the spectra are those transforms' output, bit for bit)
+  const float* qspectra;
+  int32_t* counts;               // optional: set to 0 per (pair, strand) for a split scan
 };
 
 struct SpecArgs {
@@ -58,6 +63,7 @@ struct SpecArgs {
   const sxc_tail_dev* tails;
   const sxc_tail_seg_dev* segs;
   const float* src_sig;
+  int32_t strand;            // 1: spectra of the reverse complement (resident query spectra)
 };
 
 struct EncodeArgs {
@@ -388,7 +394,7 @@ SXC_HD inline void spectra_body(Ex& ex, char* sm, const SpecArgs& a, int k) {
   const Twiddle tw{(const float*)(sm + SM::LUT), a.osc};
   SeqView qs{nullptr, nullptr, ck.len};
   if (!a.signals) {
-    qs = phase_prepare<NB>(ex, sm, a.tab, a.ent_lut, a.seq + ck.off, ck.len, ck.flags & 1, 0);
+    qs = phase_prepare<NB>(ex, sm, a.tab, a.ent_lut, a.seq + ck.off, ck.len, ck.flags & 1, a.strand);
   } else {
     float* lut = (float*)(sm + SM::LUT);
     ex.each([&](int tid) {
@@ -443,7 +449,8 @@ SXC_HD inline void product(Ex& ex, float* Y, const float* T) {
 // natural order in Y.  Per-thread accumulators stay in registers across channels.
 template <int NB, class Ex>
 SXC_HD inline void correlate(Ex& ex, char* sm, const sxc_tables* tab, const Twiddle& tw, const SeqView& qs,
-                             const float* T4, const float* qsig4, const TailRef& tail) {
+                             const float* T4, const float* qsig4, const TailRef& tail,
+                             const float* qspec4 = nullptr) {
   using SM = Smem<NB>;
   using G = Geo<NB>;
   float* Y = (float*)(sm + SM::Y);
@@ -453,7 +460,15 @@ SXC_HD inline void correlate(Ex& ex, char* sm, const sxc_tables* tab, const Twid
   });
   const float mul = (float)(1.0 / (double)G::N);  // FFTReal::rescale (FFTReal.hpp:264)
   for (int ch = 0; ch < 4; ++ch) {
-    forward_channel<NB>(ex, sm, tab, tw, qs, ch, qsig4 ? qsig4 + (size_t)ch * G::N : nullptr, tail);
+    if (qspec4) {
+      const float* qch = qspec4 + (size_t)ch * G::N;
+      ex.each([&](int tid) {
+        for (int p = tid; p < G::N; p += G::NT) Y[pad(p)] = qch[p];
+      });
+      ex.sync();
+    } else {
+      forward_channel<NB>(ex, sm, tab, tw, qs, ch, qsig4 ? qsig4 + (size_t)ch * G::N : nullptr, tail);
+    }
     product<NB>(ex, Y, T4 + (size_t)ch * G::N);
     inv_rounds<NB, Schedule<NB>::NR - 1>(ex, Y, tw);
     ex.each([&](int g) {
@@ -560,9 +575,23 @@ SXC_HD inline void correlate_body(Ex& ex, char* sm, const PairArgs& a, int ps) {
   const sxc_pair_dev pr = a.pairs[pair];
   const sxc_chunk qc = a.qchunks[pr.q_chunk];
   const Twiddle tw{(const float*)(sm + SM::LUT), a.osc};
-  const SeqView qs = phase_prepare<NB>(ex, sm, a.tab, a.ent_lut, a.seq + qc.off, qc.len, qc.flags & 1, strand);
-  correlate<NB>(ex, sm, a.tab, tw, qs, a.spectra + (size_t)pr.t_chunk * 4 * G::N, nullptr,
-                tail_ref<NB>(a.tails, a.segs, a.src_sig, pr.q_tail));
+  if (a.qspectra) {
+    // resident query spectra: no encode; only the twiddle table and the counters to set up
+    float* lut = (float*)(sm + SM::LUT);
+    int* misc0 = (int*)(sm + SM::MISC);
+    ex.each([&](int tid) {
+      for (int i = tid; i < 2048; i += G::NT) lut[i] = a.tab->lut12[i];
+      if (tid < 16) misc0[tid] = 0;
+    });
+    ex.sync();
+    correlate<NB>(ex, sm, a.tab, tw, SeqView{nullptr, nullptr, qc.len}, a.spectra + (size_t)pr.t_chunk * 4 * G::N,
+                  nullptr, TailRef{nullptr, 0, nullptr},
+                  a.qspectra + ((size_t)pr.q_chunk * 2 + strand) * 4 * G::N);
+  } else {
+    const SeqView qs = phase_prepare<NB>(ex, sm, a.tab, a.ent_lut, a.seq + qc.off, qc.len, qc.flags & 1, strand);
+    correlate<NB>(ex, sm, a.tab, tw, qs, a.spectra + (size_t)pr.t_chunk * 4 * G::N, nullptr,
+                  tail_ref<NB>(a.tails, a.segs, a.src_sig, pr.q_tail));
+  }
   if (a.xc_out) {
     const float* Y = (const float*)(sm + SM::Y);
     ex.each([&](int tid) {
@@ -576,7 +605,10 @@ SXC_HD inline void correlate_body(Ex& ex, char* sm, const PairArgs& a, int ps) {
   uint16_t* dst = a.cands + (size_t)ps * G::N;
   ex.each([&](int tid) {
     for (int i = tid; i < n; i += G::NT) dst[i] = cand[i];
-    if (tid == 0) a.ncand[ps] = n;
+    if (tid == 0) {
+      a.ncand[ps] = n;
+      if (a.counts) a.counts[ps] = 0;
+    }
   });
 }
 

@@ -46,6 +46,11 @@ struct ScanArgs {
   int32_t* stats;            // optional {candidates, runs} per (pair, strand)
   const double* ptable;      // optional ProbTable thresholds (SXC_PT_ROWS x SXC_PT_LEN): -prob_table
   double table_prob;         // what a table hit reports (the table's cutoff)
+  // workgroups per (pair, strand), each scanning a contiguous share of its accepted lags
+  // (0 or 1: one).  With more than one, records take their slots through a global atomic
+  // on counts[ps], which must be zero at launch (the pair kernel clears it), and stats
+  // runs are added atomically.
+  int32_t split;
 };
 
 constexpr int SCAN_NT = 256;  // threads per scan workgroup
@@ -382,7 +387,7 @@ SXC_HD inline void drain(Ex& ex, char* sm, const ScanArgs& a, const sxc_pair_dev
       const int start_q = start_t + shift;
       const uint64_t key = ((uint64_t)(uint32_t)rank << 32) | (uint32_t)ns;
       if (a.mode == 1) {
-        const int slot = ex.atomic_add_shared(&misc[SM_KEPT], 1);
+        const int slot = a.split > 1 ? ex.atomic_add_global(&a.counts[ps], 1) : ex.atomic_add_shared(&misc[SM_KEPT], 1);
         if (slot >= a.cap) continue;
         const size_t at = (size_t)ps * a.cap + slot;
         a.runs[3 * at] = start_t;
@@ -417,7 +422,7 @@ SXC_HD inline void drain(Ex& ex, char* sm, const ScanArgs& a, const sxc_pair_dev
         }
       }
       if (prob < a.min_prob) continue;
-      const int slot = ex.atomic_add_shared(&misc[SM_KEPT], 1);
+      const int slot = a.split > 1 ? ex.atomic_add_global(&a.counts[ps], 1) : ex.atomic_add_shared(&misc[SM_KEPT], 1);
       if (slot >= a.cap) continue;
       const int tstart = pr.t_chunk_start + start_t;
       // RCQuery (analysis/HomologyByXCorrSlave.cc:56-60) on the reverse strand, with the
@@ -446,9 +451,11 @@ SXC_HD inline void drain(Ex& ex, char* sm, const ScanArgs& a, const sxc_pair_dev
 }
 
 template <int NB, class Ex>
-SXC_HD inline void scan_body(Ex& ex, char* sm, const ScanArgs& a, int ps) {
+SXC_HD inline void scan_body(Ex& ex, char* sm, const ScanArgs& a, int blk) {
   using SS = ScanSmem<NB>;
   constexpr int N = 1 << NB;
+  const int split = a.split > 1 ? a.split : 1;
+  const int ps = blk / split, part = blk % split;
   const sxc_pair_dev pr = a.pairs[ps / a.strands];
   const int strand = ps % a.strands;
   const sxc_chunk tch = a.tchunks[pr.t_chunk];
@@ -551,11 +558,13 @@ SXC_HD inline void scan_body(Ex& ex, char* sm, const ScanArgs& a, int ps) {
   }
   const uint16_t* cands = fast ? (const uint16_t*)(sm + SS::CAND) : gcands;
   const int ncand = misc[SM_NCAND];
+  // this workgroup's share of the lags (all of them unless the scan is split)
+  const int lo = (int)((int64_t)ncand * part / split), hi = (int)((int64_t)ncand * (part + 1) / split);
   constexpr int CE = SCAN_NT / SCAN_GSZ;
   int runs = 0;
-  for (int e0 = 0; e0 < ncand; e0 += CE) {
+  for (int e0 = lo; e0 < hi; e0 += CE) {
     const int qstart = misc[SM_QN];
-    const int e1 = (e0 + CE < ncand) ? e0 + CE : ncand;
+    const int e1 = (e0 + CE < hi) ? e0 + CE : hi;
     scan_epoch<NB>(ex, sm, cands, e0, e1, SCAN_GSZ);
     const int qn = misc[SM_QN];
     if (qn > SS::QCAP) {
@@ -576,7 +585,7 @@ SXC_HD inline void scan_body(Ex& ex, char* sm, const ScanArgs& a, int ps) {
         if (tid == 0) misc[SM_QN] = 0;
       });
       ex.sync();
-    } else if (qn > SS::QCAP / 2 || e1 == ncand) {
+    } else if (qn > SS::QCAP / 2 || e1 == hi) {
       drain<NB>(ex, sm, a, pr, cands, ps, qn);
       runs += qn;
       ex.each([&](int tid) {
@@ -587,10 +596,15 @@ SXC_HD inline void scan_body(Ex& ex, char* sm, const ScanArgs& a, int ps) {
   }
   ex.each([&](int tid) {
     if (tid == 0) {
-      a.counts[ps] = misc[SM_KEPT];
-      if (a.stats) {
-        a.stats[2 * ps] = ncand;
-        a.stats[2 * ps + 1] = runs;
+      if (split == 1) {
+        a.counts[ps] = misc[SM_KEPT];
+        if (a.stats) {
+          a.stats[2 * ps] = ncand;
+          a.stats[2 * ps + 1] = runs;
+        }
+      } else if (a.stats) {
+        if (part == 0) a.stats[2 * ps] = ncand;
+        ex.atomic_add_global(&a.stats[2 * ps + 1], runs);
       }
     }
   });

@@ -98,10 +98,11 @@ struct sxc_server {
   int nb = 0;
   int geo_corr = 0, geo_spec = 0;  // FFT geometry codes of the pair and spectra kernels
   int P = 2048;
-  DevBuf seq, tch, qch, spectra;
+  int cus = 256;  // compute units of the device
+  DevBuf seq, tch, qch, spectra, qspectra;
   const double* ptable = nullptr;  // -prob_table thresholds (owned by the context)
   std::vector<ChunkMeta> tmeta, qmeta;
-  int64_t resident_bytes = 0, slot_bytes = 0;
+  int64_t resident_bytes = 0, slot_bytes = 0, qspec_bytes = 0;
   std::vector<Slot> slots;
   std::mutex mu;
   std::condition_variable work_cv, done_cv;
@@ -168,7 +169,9 @@ int64_t slot_bytes_for(int P, int N, int cap) {
   return (int64_t)P * sizeof(sxc_pair_dev) + total * N * 2 + total * 4 + total * cap * (72 + 8 + 72) + total * 12 + 64;
 }
 
-int launch_scan_slot(sxc_server* sv, Slot& sl) {
+// split: workgroups per (pair, strand) (counts must have been cleared by the pair kernel
+// when above 1).
+int launch_scan_slot(sxc_server* sv, Slot& sl, int split) {
   // Room for cap records of every (pair, strand) of this launch.  The buffers only grow;
   // a launch may have more pairs than the one that last grew them.
   const size_t nrec = (size_t)2 * sl.npairs * sl.cap;
@@ -198,6 +201,7 @@ int launch_scan_slot(sxc_server* sv, Slot& sl) {
   a.stats = nullptr;
   a.ptable = sv->ptable;
   a.table_prob = sv->prm.table_prob;
+  a.split = split;
   HIPCHK(sxc::launch_scan(sv->nb, a, 2 * sl.npairs, sl.s));
   const int total = 2 * sl.npairs;
   HIPCHK(sxc::launch_count_offsets(sl.counts.as<int32_t>(), total, sl.cap, sl.offsets.as<int64_t>(),
@@ -265,8 +269,13 @@ int slot_fill(sxc_server* sv, Slot& sl) {
   a.cutoff_fast = sv->prm.cutoff_fast;
   a.cands = sl.cands.as<uint16_t>();
   a.ncand = sl.ncand.as<int32_t>();
+  a.qspectra = sv->qspec_bytes > 0 ? sv->qspectra.as<float>() : nullptr;
+  a.counts = sl.counts.as<int32_t>();  // cleared per (pair, strand) for a split scan
   HIPCHK(sxc::launch_correlate(sv->geo_corr, a, 2 * sl.npairs, sl.s));
-  int rc = launch_scan_slot(sv, sl);
+  // A launch too small to fill the device splits each (pair, strand)'s lags over up to 4
+  // scan workgroups (3 resident per CU), which shortens its latency.
+  const int split = std::max(1, std::min(4, 3 * sv->cus / (2 * sl.npairs)));
+  int rc = launch_scan_slot(sv, sl, split);
   if (rc) return rc;
   sl.state = COUNTING;
   ++sv->st_launches;
@@ -279,7 +288,7 @@ int slot_pack(sxc_server* sv, Slot& sl) {
   const int total = 2 * sl.npairs;
   if (*sl.h_maxc > sl.cap) {
     sl.cap = *sl.h_maxc;
-    int rc = launch_scan_slot(sv, sl);  // grows the record buffers to total x cap
+    int rc = launch_scan_slot(sv, sl, 1);  // grows the record buffers to total x cap
     if (rc) return rc;
     HIPCHK(hipEventSynchronize(sl.ev_count));
   }
@@ -416,6 +425,7 @@ int server_init(sxc_server* sv, const sxc_chunk_desc* targets, int32_t nt, const
   if (rc) return rc;
   bytes.resize(bytes.size() + 16, 0);
   sv->P = cfg && cfg->max_pairs_per_launch > 0 ? cfg->max_pairs_per_launch : 2048;
+  HIPCHK(hipDeviceGetAttribute(&sv->cus, hipDeviceAttributeMultiprocessorCount, sv->ctx->device));
   const int max_slots = cfg && cfg->max_slots > 0 ? cfg->max_slots : 3;
   size_t free_b = 0, total_b = 0;
   HIPCHK(hipMemGetInfo(&free_b, &total_b));
@@ -423,6 +433,13 @@ int server_init(sxc_server* sv, const sxc_chunk_desc* targets, int32_t nt, const
   sv->resident_bytes = (int64_t)bytes.size() + (int64_t)(ttab.size() + qtab.size()) * (int64_t)sizeof(sxc_chunk) +
                        (int64_t)nt * 4 * N * (int64_t)sizeof(float);
   sv->slot_bytes = slot_bytes_for(sv->P, N, 64);
+  // both strands' spectra of every query chunk, if they leave room for at least one slot
+  const int64_t qspec_bytes = (int64_t)nq * 2 * 4 * N * (int64_t)sizeof(float);
+  const bool want_q = !(cfg && cfg->no_query_spectra) && nq > 0;
+  if (want_q && budget - sv->resident_bytes - qspec_bytes >= sv->slot_bytes) {
+    sv->resident_bytes += qspec_bytes;
+    sv->qspec_bytes = qspec_bytes;
+  }
   const int64_t room = budget - sv->resident_bytes;
   const int nslots = (int)std::min<int64_t>(max_slots, room > 0 ? room / sv->slot_bytes : 0);
   if (nslots < 1)
@@ -444,9 +461,9 @@ int server_init(sxc_server* sv, const sxc_chunk_desc* targets, int32_t nt, const
   if (!qtab.empty())
     HIPCHK(hipMemcpyAsync(sv->qch.p, qtab.data(), qtab.size() * sizeof(sxc_chunk), hipMemcpyHostToDevice, s));
   // resident target spectra, computed once in launches of up to 65536 chunks
-  DevBuf which;
+  DevBuf which, qdst;
   const int step = 65536;
-  HIPCHK(which.ensure((size_t)std::min<int32_t>(nt, step) * sizeof(int32_t) + 4));
+  HIPCHK(which.ensure((size_t)std::min<int32_t>(std::max(nt, nq), step) * sizeof(int32_t) + 4));
   std::vector<int32_t> idx;
   for (int32_t at = 0; at < nt; at += step) {
     const int n = std::min<int32_t>(step, nt - at);
@@ -466,6 +483,40 @@ int server_init(sxc_server* sv, const sxc_chunk_desc* targets, int32_t nt, const
     HIPCHK(sxc::launch_spectra(sv->geo_spec, sa, s));
     HIPCHK(hipStreamSynchronize(s));  // idx is reused
   }
+  if (sv->qspec_bytes > 0) {
+    // resident query spectra: slot 2 q + strand holds query chunk q's (strand 1: reverse
+    // complement) four spectra, the transforms the pair kernel would otherwise compute
+    HIPCHK(sv->qspectra.ensure((size_t)sv->qspec_bytes));
+    std::vector<int32_t> dst;
+    for (int32_t at = 0; at < nq; at += step) {
+      const int n = std::min<int32_t>(step, nq - at);
+      for (int strand = 0; strand < 2; ++strand) {
+        idx.resize((size_t)n);
+        dst.resize((size_t)n);
+        for (int i = 0; i < n; ++i) {
+          idx[(size_t)i] = at + i;
+          dst[(size_t)i] = 2 * (at + i) + strand;
+        }
+        HIPCHK(qdst.ensure((size_t)n * sizeof(int32_t) + 4));
+        HIPCHK(hipMemcpyAsync(which.p, idx.data(), (size_t)n * sizeof(int32_t), hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(qdst.p, dst.data(), (size_t)n * sizeof(int32_t), hipMemcpyHostToDevice, s));
+        sxc::SpecArgs sa{};
+        sa.tab = sv->ctx->d_tab;
+        sa.osc = sv->ctx->d_osc;
+        sa.ent_lut = sv->ctx->d_ent[sv->nb];
+        sa.seq = sv->seq.as<unsigned char>();
+        sa.chunks = sv->qch.as<sxc_chunk>();
+        sa.which = which.as<int32_t>();
+        sa.n = n;
+        sa.spectra = sv->qspectra.as<float>();
+        sa.dst = qdst.as<int32_t>();
+        sa.strand = strand;
+        HIPCHK(sxc::launch_spectra(sv->geo_spec, sa, s));
+        HIPCHK(hipStreamSynchronize(s));  // idx and dst are reused
+      }
+    }
+    qdst.release();
+  }
   which.release();
   sv->slots.resize((size_t)nslots);
   for (Slot& sl : sv->slots) {
@@ -480,7 +531,7 @@ void server_release(sxc_server* sv) {
     if (sl.s) (void)hipStreamSynchronize(sl.s);
     slot_free(sl);
   }
-  for (DevBuf* b : {&sv->seq, &sv->tch, &sv->qch, &sv->spectra}) b->release();
+  for (DevBuf* b : {&sv->seq, &sv->tch, &sv->qch, &sv->spectra, &sv->qspectra}) b->release();
 }
 
 }  // namespace
@@ -590,6 +641,7 @@ int sxc_server_stats(sxc_server* sv, sxc_server_info* out) {
   out->slots = (int32_t)sv->slots.size();
   out->max_pairs_per_launch = sv->P;
   out->device_ms = sv->st_device_ms;
+  out->query_spectra_bytes = sv->qspec_bytes;
   return SXC_OK;
 }
 

@@ -38,6 +38,7 @@ struct EmuExec {
     *p += v;
     return o;
   }
+  int atomic_add_global(int* p, int v) { return atomic_add_shared(p, v); }
 };
 
 struct Tables {
@@ -116,6 +117,25 @@ int run_batch(const sxc::HostBatch& hb, const sxc_params& prm, std::vector<sxc_r
     pa.segs = hb.segs.data();
     pa.src_sig = src_sig.data();
   }
+  // SXC_EMU_QSPEC=1: both strands' query spectra computed first (as the serving loop keeps
+  // them resident) and read by the pair kernel instead of transforming the queries
+  std::vector<float> qspec;
+  if (getenv("SXC_EMU_QSPEC") && !tails) {
+    const int nq = (int)hb.qchunks.size();
+    qspec.resize((size_t)nq * 2 * 4 * G::N);
+    std::vector<int32_t> qw(1), qd(1);
+    for (int q = 0; q < nq; ++q)
+      for (int strand = 0; strand < 2; ++strand) {
+        qw[0] = q;
+        qd[0] = 2 * q + strand;
+        sxc::SpecArgs qa{&T.tab, T.osc.data(), T.ent[NB].data(), hb.seq.data(), hb.qchunks.data(), qw.data(), 1,
+                         qspec.data(), qd.data(), nullptr, nullptr, nullptr, nullptr, nullptr, strand};
+        EmuExec<sxc::Regs<GEO>> ex(G::NT);
+        memset(sm.data(), 0xCD, sm.size());
+        sxc::spectra_body<GEO>(ex, sm.data(), qa, 0);
+      }
+    pa.qspectra = qspec.data();
+  }
   for (int ps = 0; ps < total; ++ps) {
     EmuExec<sxc::Regs<GEO>> ex(G::NT);
     memset(sm.data(), 0xCD, sm.size());
@@ -150,10 +170,19 @@ int run_batch(const sxc::HostBatch& hb, const sxc_params& prm, std::vector<sxc_r
     sc.table_prob = prm.table_prob;
   }
   std::vector<char> ssm(sxc::ScanSmem<NB>::BYTES);
-  for (int ps = 0; ps < total; ++ps) {
+  // SXC_EMU_SPLIT=k: k scan workgroups per (pair, strand), counts cleared first (as the pair
+  // kernel does for a split scan), run in reverse order
+  if (const char* v = getenv("SXC_EMU_SPLIT")) sc.split = atoi(v);
+  const int split = sc.split > 1 ? sc.split : 1;
+  if (split > 1) {
+    std::fill(counts.begin(), counts.end(), 0);
+    if (stats)
+      for (int ps = 0; ps < total; ++ps) stats[2 * ps + 1] = 0;
+  }
+  for (int blk = total * split - 1; blk >= 0; --blk) {
     EmuExec<int> ex(sxc::SCAN_NT);
     memset(ssm.data(), 0xCD, ssm.size());
-    sxc::scan_body<NB>(ex, ssm.data(), sc, ps);
+    sxc::scan_body<NB>(ex, ssm.data(), sc, blk);
   }
   res.clear();
   for (int ps = 0; ps < total; ++ps) {

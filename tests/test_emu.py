@@ -77,6 +77,39 @@ def test_emulated_batch_uniform(emu_lib, golden):
         assert st[ps, 0] == len(oracle.findtop(oxc, 2.9 if p["fast"] else 1.8))
 
 
+def test_emulated_resident_query_spectra(emu_lib, golden, monkeypatch):
+    """The pair kernel reading both strands' query spectra precomputed by the spectra kernel
+    (the serving loop's resident query spectra) gives the oracle's records and xc bits."""
+    g = golden("uniform_pairs.npz")
+    pairs = [dict(tseq=unpack2(g["t"][r], 4096), qseq=unpack2(g["q"][r], 4096), t_record=r, q_record=r,
+                  fast=int(g["fast"][r])) for r in range(4)]
+    monkeypatch.setenv("SXC_EMU_QSPEC", "1")
+    got, xc, _ = emu_batch(emu_lib, pairs, want_xc=True)
+    ref = np.concatenate(oracle.align_batch(
+        [dict(p, q_record_len=4096) for p in pairs], oracle.make_params(target_total=4096.0 * 64)))
+    for c in COLS:
+        assert np.array_equal(got[c], ref[c]), c
+    for ps in range(8):
+        p = pairs[ps // 2]
+        qs = p["qseq"] if ps % 2 == 0 else oracle.reverse_complement(p["qseq"])
+        oxc = oracle.xcorr(oracle.encode(p["tseq"], 8192), oracle.encode(qs, 8192))
+        assert np.array_equal(xc[ps].view(np.uint32), oxc.view(np.uint32))
+
+
+@pytest.mark.parametrize("split", [2, 3])
+def test_emulated_split_scan(emu_lib, golden, monkeypatch, split):
+    """A (pair, strand)'s lags shared by several scan workgroups (the serving loop's small
+    launches): the same records, candidate and run counts as one workgroup."""
+    g = golden("uniform_pairs.npz")
+    pairs = [dict(tseq=unpack2(g["t"][r], 4096), qseq=unpack2(g["q"][r], 4096), t_record=r, q_record=r,
+                  fast=int(g["fast"][r])) for r in range(4)]
+    one, _, st1 = emu_batch(emu_lib, pairs)
+    monkeypatch.setenv("SXC_EMU_SPLIT", str(split))
+    got, _, st = emu_batch(emu_lib, pairs)
+    assert len(one) > 0 and got.tobytes() == one.tobytes()
+    assert np.array_equal(st, st1)
+
+
 # The alternative FFT geometries the library compiles for its autotuner (stage-A width,
 # levels per LDS round), by log2 length; each must reproduce the oracle bit for bit.
 GEOMETRIES = [(nb, 16, 2) for nb in range(9, 15)] + [(15, 32, 2)] + [(nb, 32, 3) for nb in range(11, 15)]

@@ -112,7 +112,27 @@ def test_server_alone_equals_batch_and_mix(genomes, served):
         assert srv.results(tickets[i]).tobytes() == alone[i].tobytes(), i
     st = srv.stats()
     assert st["slots"] == 3 and st["max_pairs_per_launch"] == 256
+    # served pairs read both strands' query spectra from HBM instead of transforming queries
+    assert st["query_spectra_bytes"] == len(qch) * 2 * 4 * 8192 * 4
     assert st["pairs_done"] >= 2 * sum((b[1] - b[0] + 1) * (b[3] - b[2] + 1) for r in reqs for b in r)
+
+
+def test_server_without_query_spectra(genomes):
+    """The same records when the pair kernel transforms each query itself."""
+    from satsuma2_amd.api import Chunk, Context, Server, SlaveParams
+    T, Q, tch, qch = genomes
+    ctx = Context(0)
+    params = SlaveParams(target_total=float(len(T) + 4096))
+    srv = Server(ctx, params, [Chunk(s, st, r, rl) for s, st, r, rl in tch],
+                 [Chunk(s, st, r, rl) for s, st, r, rl in qch], max_pairs_per_launch=256, max_slots=2,
+                 query_spectra=False)
+    assert srv.stats()["query_spectra_bytes"] == 0
+    reqs = _requests(tch, qch, np.random.default_rng(11), n=6)
+    for blocks in reqs:
+        got = srv.results(srv.submit(blocks))
+        assert got.tobytes() == _batch_records(ctx, params, tch, qch, blocks).tobytes()
+    srv.close()
+    ctx.close()
 
 
 def test_server_latency_and_edge_requests(served, genomes):
