@@ -54,6 +54,17 @@ SERVE_LOADS = (0.25, 0.5, 0.75)  # offered chunk pairs/s as fractions of the bat
 SEED_LINE = 256                  # query chunks per seed line
 
 
+def kernel_name(ctx, N: int = 8192) -> str:
+    """The pair kernel's instantiation name for the context's geometry at length N (a plain
+    log2 length for the default geometry, else the packed geometry code)."""
+    sa, km, occ, lean = ctx.geometry(N, "pairs")
+    nb = N.bit_length() - 1
+    default = (32 if nb >= 15 else 16, 3, 0, 0)
+    code = nb if (sa, km, occ, lean) == default else (
+        nb | (32 if sa == 32 else 0) | (km << 6) | (256 if occ else 0) | (512 if lean else 0))
+    return f"k_correlate<{code}>"
+
+
 def make_workload(n_pairs: int, seed: int = 1):
     from satsuma2_amd import synth
     from satsuma2_amd.api import ChunkPair
@@ -168,6 +179,7 @@ def main():
     ap.add_argument("--serve-bp", type=int, default=50_000_000)
     ap.add_argument("--serve-seconds", type=float, default=4.0)
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true", help="default FFT kernel geometries (for profiling)")
     ap.add_argument("--uneven-pairs", type=int, default=20_000)
     ap.add_argument("--uneven-steps", type=int, default=5)
     args = ap.parse_args()
@@ -182,10 +194,12 @@ def main():
     T, Q, pairs = make_workload(args.pairs)
     params = SlaveParams(target_total=float(4096 * args.pairs))
     ctx = Context(0)
-    print("[bench] autotuning the FFT kernel geometries", file=sys.stderr, flush=True)
-    t0 = time.perf_counter()
-    tune = ctx.autotune(512, 32768)
-    tune_s = time.perf_counter() - t0
+    tune, tune_s = [], 0.0
+    if not args.no_autotune:
+        print("[bench] autotuning the FFT kernel geometries", file=sys.stderr, flush=True)
+        t0 = time.perf_counter()
+        tune = ctx.autotune(512, 32768)
+        tune_s = time.perf_counter() - t0
     rb = ResidentBatch(ctx, pairs, params)
 
     print("[bench] warmup", file=sys.stderr, flush=True)
@@ -218,13 +232,15 @@ def main():
     rb.close()
 
     # plain drop-in: one chunk pair per call, host buffers in and out (as the slave calls Align)
-    h = HomologyByXCorr(ctx, params)
-    for p in pairs[:4]:
-        h.align_pair(p)
-    t0 = time.perf_counter()
-    for p in pairs[: args.plain_pairs]:
-        h.align_pair(p)
-    plain = args.plain_pairs / (time.perf_counter() - t0)
+    plain = float("nan")
+    if args.plain_pairs > 0:
+        h = HomologyByXCorr(ctx, params)
+        for p in pairs[:4]:
+            h.align_pair(p)
+        t0 = time.perf_counter()
+        for p in pairs[: args.plain_pairs]:
+            h.align_pair(p)
+        plain = args.plain_pairs / (time.perf_counter() - t0)
 
     k_ms = float(np.mean(cor_ms))
     achieved = B_PAIR * args.pairs / (k_ms * 1e-3) / 1e9
@@ -246,8 +262,8 @@ def main():
                    "keep_prob": 0.99},
         "graph_steps": graphed,
         "eager_ms_per_step": round(eager_ms, 3),
-        "plain_pairs_per_s": round(plain, 1),
-        "batched_over_plain": round(value / plain, 2),
+        "plain_pairs_per_s": round(plain, 1) if plain == plain else None,
+        "batched_over_plain": round(value / plain, 2) if plain == plain else None,
         "phase_ms": {"target_spectra": round(float(np.mean(spec_ms)), 3),
                      "correlate": round(float(np.mean(cor_ms)), 3), "scan": round(float(np.mean(scan_ms)), 3),
                      "finalize": round(float(np.mean(fin_ms)), 3)},
@@ -256,14 +272,16 @@ def main():
         "diagonal_runs": runs,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": int(TRAFFIC_PER_PAIR * args.pairs),
-                     "kernel": "k_correlate<13>", "bytes_per_pair": B_PAIR},
+                     "kernel": kernel_name(ctx), "bytes_per_pair": B_PAIR},
     }
-    out["autotune"] = {"seconds": round(tune_s, 2), "classes": {
-        str(N): {"pairs_kernel": list(ctx.geometry(N, "pairs")), "spectra_kernel": list(ctx.geometry(N, "spectra")),
-                 "candidates": [[e["stage_a"], e["round_levels"], round(e["correlate_ms"], 4),
-                                 round(e["spectra_ms"], 4)] for e in tune if e["N"] == N]}
-        for N in sorted({e["N"] for e in tune})},
-        "columns": "stage_a, round_levels, pair kernel ms, spectra kernel ms (synthetic batch per length)"}
+    if tune:
+        out["autotune"] = {"seconds": round(tune_s, 2), "classes": {
+            str(N): {"pairs_kernel": list(ctx.geometry(N, "pairs")), "spectra_kernel": list(ctx.geometry(N, "spectra")),
+                     "candidates": [list(e["geo"]) + [round(e["correlate_ms"], 4), round(e["spectra_ms"], 4)]
+                                    for e in tune if e["N"] == N]}
+            for N in sorted({e["N"] for e in tune})},
+            "columns": "stage_a, round_levels, three_per_cu, lean_lds, pair kernel ms, spectra kernel ms "
+                       "(synthetic batch per length)"}
     if args.uneven_pairs > 0:
         out["uneven"] = uneven_bench(ctx, args.uneven_pairs, args.uneven_steps)
     if not args.no_serve:

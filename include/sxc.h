@@ -231,18 +231,24 @@ void sxc_server_destroy(sxc_server* srv);
  *
  * Each FFT length has several compiled kernel geometries that compute the same FFTReal
  * dataflow bit for bit and differ in how it maps onto the GPU: the stage-A width (16 or 32
- * positions per thread: the radix of the register-resident first levels) and the LDS round
- * depth (at most 2 or 3 butterfly levels per pass over LDS).  sxc_autotune times every
+ * positions per thread: the radix of the register-resident first levels), the LDS round
+ * depth (at most 2 or 3 butterfly levels per pass over LDS), a register budget for more
+ * workgroups per CU, and a lean LDS plan (see sxc_geometry).  sxc_autotune times every
  * geometry of the pair kernel and of the target-spectra kernel for each length in
  * [N_lo, N_hi] on a synthetic batch (chunks of N/2 bases, both strands), checks that all
  * geometries accept identical lags, and makes the fastest the context's choice for later
  * batches and servers.  out receives one entry per (length, geometry).  The reference has
  * one CPU FFT and no counterpart; sxc_set_geometry applies a saved choice without timing. */
 typedef struct {
+  int32_t stage_a;          /* positions per thread in stage A (16 or 32) */
+  int32_t round_levels;     /* butterfly levels per LDS round, at most (2 or 3) */
+  int32_t three_per_cu;     /* 1: register budget for three workgroups per CU */
+  int32_t lean_lds;         /* 1: twiddles and chunk bytes read through the caches, not LDS */
+} sxc_geometry; /* one compiled FFT kernel geometry */
+typedef struct {
   int32_t N;
-  int32_t stage_a;          /* positions per thread in stage A */
-  int32_t round_levels;     /* butterfly levels per LDS round, at most */
   int32_t pairs;            /* chunk pairs of the synthetic batch */
+  sxc_geometry geo;
   double correlate_ms;      /* pair kernel over the batch (both strands), mean of 3 launches */
   double spectra_ms;        /* target-spectra kernel over the batch's target chunks */
   int32_t chosen_correlate; /* 1: the context now runs the pair kernel in this geometry */
@@ -251,8 +257,8 @@ typedef struct {
 #define SXC_KERNEL_PAIRS 0
 #define SXC_KERNEL_SPECTRA 1
 int sxc_autotune(sxc_ctx* ctx, int32_t N_lo, int32_t N_hi, sxc_tune_entry* out, int32_t cap, int32_t* n_out);
-int sxc_set_geometry(sxc_ctx* ctx, int32_t N, int32_t kernel, int32_t stage_a, int32_t round_levels);
-int sxc_get_geometry(sxc_ctx* ctx, int32_t N, int32_t kernel, int32_t* stage_a, int32_t* round_levels);
+int sxc_set_geometry(sxc_ctx* ctx, int32_t N, int32_t kernel, const sxc_geometry* geo);
+int sxc_get_geometry(sxc_ctx* ctx, int32_t N, int32_t kernel, sxc_geometry* geo);
 
 /* Time of the last sxc_align_batch / sxc_align_pair / sxc_batch_run (milliseconds): total_ms,
  * upload_ms and download_ms are wall time on the host; the phases and device_ms are device

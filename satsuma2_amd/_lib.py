@@ -89,10 +89,14 @@ class ServerInfo(C.Structure):
                 ("query_spectra_bytes", C.c_int64)]
 
 
+class Geometry(C.Structure):
+    _fields_ = [("stage_a", C.c_int32), ("round_levels", C.c_int32), ("three_per_cu", C.c_int32),
+                ("lean_lds", C.c_int32)]
+
+
 class TuneEntry(C.Structure):
-    _fields_ = [("N", C.c_int32), ("stage_a", C.c_int32), ("round_levels", C.c_int32), ("pairs", C.c_int32),
-                ("correlate_ms", C.c_double), ("spectra_ms", C.c_double), ("chosen_correlate", C.c_int32),
-                ("chosen_spectra", C.c_int32)]
+    _fields_ = [("N", C.c_int32), ("pairs", C.c_int32), ("geo", Geometry), ("correlate_ms", C.c_double),
+                ("spectra_ms", C.c_double), ("chosen_correlate", C.c_int32), ("chosen_spectra", C.c_int32)]
 
 
 SXC_KERNEL_PAIRS = 0
@@ -146,8 +150,8 @@ def load() -> C.CDLL:
         L.sxc_batch_run.argtypes = [vp]
         L.sxc_batch_set_graph.argtypes = [vp, i32]
         L.sxc_autotune.argtypes = [vp, i32, i32, C.POINTER(TuneEntry), i32, C.POINTER(i32)]
-        L.sxc_set_geometry.argtypes = [vp, i32, i32, i32, i32]
-        L.sxc_get_geometry.argtypes = [vp, i32, i32, C.POINTER(i32), C.POINTER(i32)]
+        L.sxc_set_geometry.argtypes = [vp, i32, i32, C.POINTER(Geometry)]
+        L.sxc_get_geometry.argtypes = [vp, i32, i32, C.POINTER(Geometry)]
         L.sxc_batch_results.argtypes = [vp, C.POINTER(Result), i64, C.POINTER(i64)]
         L.sxc_batch_stats.argtypes = [vp, C.POINTER(i64), C.POINTER(i64)]
         L.sxc_batch_destroy.argtypes = [vp]

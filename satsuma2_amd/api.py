@@ -16,7 +16,7 @@ from __future__ import annotations
 
 import ctypes as C
 from dataclasses import dataclass
-from typing import List, Sequence
+from typing import List, NamedTuple, Sequence
 
 import numpy as np
 
@@ -31,6 +31,18 @@ def _fp(a: np.ndarray):
 
 def _b(s) -> bytes:
     return s if isinstance(s, bytes) else bytes(s)
+
+
+class Geometry(NamedTuple):
+    """A compiled FFT kernel geometry (sxc_geometry)."""
+    stage_a: int = 16
+    round_levels: int = 3
+    three_per_cu: int = 0
+    lean_lds: int = 0
+
+    @staticmethod
+    def of(g) -> "Geometry":
+        return Geometry(g.stage_a, g.round_levels, g.three_per_cu, g.lean_lds)
 
 
 class Context:
@@ -60,24 +72,32 @@ class Context:
     def autotune(self, N_lo: int = 8192, N_hi: int = 0) -> list:
         """Times every compiled FFT geometry of the pair and target-spectra kernels for each
         length in [N_lo, N_hi] (sxc_autotune) and keeps the fastest; returns one dict per
-        (length, geometry)."""
+        (length, geometry): N, pairs, geo (a Geometry tuple), correlate_ms, spectra_ms,
+        chosen_correlate, chosen_spectra."""
         N_hi = N_hi or N_lo
-        cap = 64
+        cap = 128
         out = (_lib.TuneEntry * cap)()
         n = C.c_int32(0)
         check(self.L.sxc_autotune(self._h, N_lo, N_hi, out, cap, C.byref(n)))
-        return [{f: getattr(out[i], f) for f, _ in _lib.TuneEntry._fields_} for i in range(n.value)]
+        res = []
+        for i in range(n.value):
+            e = out[i]
+            res.append(dict(N=e.N, pairs=e.pairs, geo=Geometry.of(e.geo), correlate_ms=e.correlate_ms,
+                            spectra_ms=e.spectra_ms, chosen_correlate=e.chosen_correlate,
+                            chosen_spectra=e.chosen_spectra))
+        return res
 
-    def set_geometry(self, N: int, kernel: str, stage_a: int, round_levels: int):
+    def set_geometry(self, N: int, kernel: str, geo: "Geometry"):
         """kernel: "pairs" or "spectra"."""
         k = _lib.SXC_KERNEL_PAIRS if kernel == "pairs" else _lib.SXC_KERNEL_SPECTRA
-        check(self.L.sxc_set_geometry(self._h, N, k, stage_a, round_levels))
+        g = _lib.Geometry(*geo)
+        check(self.L.sxc_set_geometry(self._h, N, k, C.byref(g)))
 
-    def geometry(self, N: int, kernel: str = "pairs"):
+    def geometry(self, N: int, kernel: str = "pairs") -> "Geometry":
         k = _lib.SXC_KERNEL_PAIRS if kernel == "pairs" else _lib.SXC_KERNEL_SPECTRA
-        sa, km = C.c_int32(0), C.c_int32(0)
-        check(self.L.sxc_get_geometry(self._h, N, k, C.byref(sa), C.byref(km)))
-        return sa.value, km.value
+        g = _lib.Geometry()
+        check(self.L.sxc_get_geometry(self._h, N, k, C.byref(g)))
+        return Geometry.of(g)
 
     # ---- plain entry points (one reference call each)
     def encode(self, seq: bytes, N: int) -> np.ndarray:

@@ -24,8 +24,9 @@ ARCH = os.environ.get("SXC_OFFLOAD_ARCH", "gfx950")
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall", "-Wno-unused-function",
           "-Wno-unknown-pragmas"]
 DEVICE = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
-SOURCES = ["sxc_kernels.hip", "sxc_kernels_r2.hip", "sxc_kernels_s32.hip", "sxc_api.hip", "sxc_server.hip",
-           "sxc_tables.cpp"]
+SOURCES = ["sxc_kernels.hip", "sxc_kernels_r2.hip", "sxc_kernels_s32.hip", "sxc_kernels_occ.hip", "sxc_kernels_lean.hip",
+           "sxc_api.hip",
+           "sxc_server.hip", "sxc_tables.cpp"]
 # Host programs linked against the library (plain C++, no device code).
 PROGRAMS = {
     "HomologyByXCorrSlave": ["host/slave_main.cpp", "host/sxc_seqio.cpp"],

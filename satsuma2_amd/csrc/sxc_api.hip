@@ -555,6 +555,12 @@ int groups_download(sxc_ctx* ctx, std::vector<Group>& gs, int32_t n, sxc_result*
 
 // ---------------------------------------------------------------- autotuner
 //
+// Geometry codes (sxc_fft.h) <-> the ABI's sxc_geometry.
+
+sxc_geometry geometry_of(int g) {
+  return sxc_geometry{sxc::geo_sa(g), sxc::geo_km(g), sxc::geo_occ(g), sxc::geo_lean(g)};
+}
+//
 // Times every compiled geometry of the pair kernel and the target-spectra kernel for one FFT
 // length on a synthetic batch shaped like the slave's (target and query chunks of N/2 bases,
 // both strands, half the pairs carrying a planted homology), checks that all geometries
@@ -660,8 +666,7 @@ int tune_class(sxc_ctx* ctx, int nb, std::vector<sxc_tune_entry>& out) {
     }
     sxc_tune_entry e{};
     e.N = N;
-    e.stage_a = sxc::geo_sa(g);
-    e.round_levels = sxc::geo_km(g);
+    e.geo = geometry_of(g);
     e.pairs = P;
     e.correlate_ms = ms_c / reps;
     e.spectra_ms = ms_s / reps;
@@ -677,25 +682,31 @@ int tune_class(sxc_ctx* ctx, int nb, std::vector<sxc_tune_entry>& out) {
   }
   ctx->geo_corr[nb] = pick_c;
   ctx->geo_spec[nb] = pick_s;
+  const sxc_geometry pc = geometry_of(pick_c), ps_ = geometry_of(pick_s);
   for (size_t i = first; i < out.size(); ++i) {
     sxc_tune_entry& e = out[i];
-    e.chosen_correlate = e.stage_a == sxc::geo_sa(pick_c) && e.round_levels == sxc::geo_km(pick_c);
-    e.chosen_spectra = e.stage_a == sxc::geo_sa(pick_s) && e.round_levels == sxc::geo_km(pick_s);
+    e.chosen_correlate = memcmp(&e.geo, &pc, sizeof pc) == 0;
+    e.chosen_spectra = memcmp(&e.geo, &ps_, sizeof ps_) == 0;
   }
   return SXC_OK;
 }
 
-// The compiled geometry of length 2^nb with the given stage-A width and round depth.
-int find_geometry(int nb, int sa, int km, int* geo) {
+// The compiled geometry of length 2^nb matching `want`.
+int find_geometry(int nb, const sxc_geometry& want, int* geo) {
   int geos[8];
   const int ng = sxc::fft_geometries(nb, geos, 8);
-  for (int i = 0; i < ng; ++i)
-    if (sxc::geo_sa(geos[i]) == sa && sxc::geo_km(geos[i]) == km) {
+  for (int i = 0; i < ng; ++i) {
+    const sxc_geometry g = geometry_of(geos[i]);
+    if (g.stage_a == want.stage_a && g.round_levels == want.round_levels && g.three_per_cu == want.three_per_cu &&
+        g.lean_lds == want.lean_lds) {
       *geo = geos[i];
       return SXC_OK;
     }
-  return fail(SXC_ERR_ARG, "no compiled FFT geometry with stage A " + std::to_string(sa) + " and " +
-                               std::to_string(km) + "-level rounds for N = " + std::to_string(1 << nb));
+  }
+  return fail(SXC_ERR_ARG, "no compiled FFT geometry (stage A " + std::to_string(want.stage_a) + ", " +
+                               std::to_string(want.round_levels) + "-level rounds, three per CU " +
+                               std::to_string(want.three_per_cu) + ", lean LDS " + std::to_string(want.lean_lds) +
+                               ") for N = " + std::to_string(1 << nb));
 }
 
 }  // namespace
@@ -813,28 +824,27 @@ int sxc_autotune(sxc_ctx* ctx, int32_t N_lo, int32_t N_hi, sxc_tune_entry* out, 
   return SXC_OK;
 }
 
-int sxc_set_geometry(sxc_ctx* ctx, int32_t N, int32_t kernel, int32_t stage_a, int32_t round_levels) {
-  if (!ctx) return fail(SXC_ERR_ARG, "null context");
+int sxc_set_geometry(sxc_ctx* ctx, int32_t N, int32_t kernel, const sxc_geometry* want) {
+  if (!ctx || !want) return fail(SXC_ERR_ARG, "null argument");
   if (kernel != SXC_KERNEL_PAIRS && kernel != SXC_KERNEL_SPECTRA) return fail(SXC_ERR_ARG, "unknown kernel");
   int nb, geo = 0;
   int rc = check_N(N, &nb);
-  if (!rc) rc = find_geometry(nb, stage_a, round_levels, &geo);
+  if (!rc) rc = find_geometry(nb, *want, &geo);
   if (rc) return rc;
   std::lock_guard<std::mutex> g(ctx->mu);
   (kernel == SXC_KERNEL_PAIRS ? ctx->geo_corr : ctx->geo_spec)[nb] = geo;
   return SXC_OK;
 }
 
-int sxc_get_geometry(sxc_ctx* ctx, int32_t N, int32_t kernel, int32_t* stage_a, int32_t* round_levels) {
-  if (!ctx || !stage_a || !round_levels) return fail(SXC_ERR_ARG, "null argument");
+int sxc_get_geometry(sxc_ctx* ctx, int32_t N, int32_t kernel, sxc_geometry* out) {
+  if (!ctx || !out) return fail(SXC_ERR_ARG, "null argument");
   if (kernel != SXC_KERNEL_PAIRS && kernel != SXC_KERNEL_SPECTRA) return fail(SXC_ERR_ARG, "unknown kernel");
   int nb;
   const int rc = check_N(N, &nb);
   if (rc) return rc;
   std::lock_guard<std::mutex> g(ctx->mu);
   const int geo = (kernel == SXC_KERNEL_PAIRS ? ctx->geo_corr : ctx->geo_spec)[nb];
-  *stage_a = sxc::geo_sa(geo);
-  *round_levels = sxc::geo_km(geo);
+  *out = geometry_of(geo);
   return SXC_OK;
 }
 

@@ -61,13 +61,22 @@ SXC_HD constexpr int pad(int p) { return p + (p >> 5); }
 
 // A kernel's FFT geometry packed in one template int ("geometry code"): log2 of the
 // transform length in bits 0-4, bit 5 set for a 32-wide stage A, the LDS round depth in
-// bits 6-7.  A plain log2 length (bits 5-7 clear) is the default geometry for that length.
+// bits 6-7, bit 8 asking the compiler to fit three workgroups per CU (a register budget of
+// 3 x waves-per-workgroup / 4 waves per SIMD), bit 9 for a lean LDS plan (twiddles and
+// chunk bytes read through the caches instead of staged in LDS, so more workgroups fit a
+// CU).  A plain log2 length (bits 5-9 clear) is the default geometry for that length.
 // Every geometry computes the same FFTReal dataflow; they differ in how positions map to
-// threads and how many levels one LDS round carries.
-SXC_HD constexpr int geo_code(int nb, int sa, int km) { return nb | (sa == 32 ? 32 : 0) | (km << 6); }
+// threads, how many levels one LDS round carries and how many workgroups share a CU.
+SXC_HD constexpr int geo_code(int nb, int sa, int km, int occ = 0, int lean = 0) {
+  return nb | (sa == 32 ? 32 : 0) | (km << 6) | (occ ? 256 : 0) | (lean ? 512 : 0);
+}
+SXC_HD constexpr int geo_lean(int g) { return (g >> 9) & 1; }
 SXC_HD constexpr int geo_nb(int g) { return g & 31; }
-SXC_HD constexpr int geo_sa(int g) { return ((g & 32) || geo_nb(g) >= 15) ? 32 : ((g >> 6) ? 16 : SXC_STAGE_A); }
-SXC_HD constexpr int geo_km(int g) { return (g >> 6) ? (g >> 6) : SXC_KMAX; }
+SXC_HD constexpr int geo_sa(int g) {
+  return ((g & 32) || geo_nb(g) >= 15) ? 32 : (((g >> 6) & 3) ? 16 : SXC_STAGE_A);
+}
+SXC_HD constexpr int geo_km(int g) { return ((g >> 6) & 3) ? ((g >> 6) & 3) : SXC_KMAX; }
+SXC_HD constexpr int geo_occ(int g) { return (g >> 8) & 1; }
 
 template <int NB>
 struct Geo {

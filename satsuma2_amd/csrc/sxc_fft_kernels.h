@@ -61,16 +61,52 @@ __global__ __launch_bounds__(Geo<G>::NT) void k_spectra(SpecArgs a) {
   spectra_body<G>(ex, sm, a, (int)blockIdx.x);
 }
 
+// The same kernels with a register budget that lets three workgroups share a CU (geometry
+// codes with bit 8; whether the extra occupancy beats the spills it may cost is for the
+// autotuner to measure).
+template <int G>
+constexpr int occ_waves() {
+  return (3 * (Geo<G>::NT / 64) + 3) / 4;  // waves per SIMD for 3 workgroups
+}
+
+template <int G>
+__global__ __launch_bounds__(Geo<G>::NT) __attribute__((amdgpu_waves_per_eu(occ_waves<G>())))
+void k_correlate_occ(PairArgs a, int total) {
+  __shared__ __align__(16) char sm[Smem<G>::BYTES];
+  Regs<G> r;
+  DevExec<Regs<G>> ex{r};
+  const int ps = xcd_pair_strand((int)blockIdx.x, a.strands, total);
+  if (ps < 0) return;
+  correlate_body<G>(ex, sm, a, ps);
+}
+
+template <int G>
+__global__ __launch_bounds__(Geo<G>::NT) __attribute__((amdgpu_waves_per_eu(occ_waves<G>())))
+void k_spectra_occ(SpecArgs a) {
+  __shared__ __align__(16) char sm[Smem<G>::BYTES];
+  Regs<G> r;
+  DevExec<Regs<G>> ex{r};
+  spectra_body<G>(ex, sm, a, (int)blockIdx.x);
+}
+
 template <int G>
 hipError_t launch_correlate_g(const PairArgs& a, int total, hipStream_t s) {
   const int grid = (a.strands == 2) ? ((total + 15) / 16) * 16 : total;
-  hipLaunchKernelGGL(k_correlate<G>, dim3(grid), dim3(Geo<G>::NT), 0, s, a, total);
+  if constexpr (geo_occ(G)) {
+    hipLaunchKernelGGL(k_correlate_occ<G>, dim3(grid), dim3(Geo<G>::NT), 0, s, a, total);
+  } else {
+    hipLaunchKernelGGL(k_correlate<G>, dim3(grid), dim3(Geo<G>::NT), 0, s, a, total);
+  }
   return hipGetLastError();
 }
 
 template <int G>
 hipError_t launch_spectra_g(const SpecArgs& a, hipStream_t s) {
-  hipLaunchKernelGGL(k_spectra<G>, dim3(a.n), dim3(Geo<G>::NT), 0, s, a);
+  if constexpr (geo_occ(G)) {
+    hipLaunchKernelGGL(k_spectra_occ<G>, dim3(a.n), dim3(Geo<G>::NT), 0, s, a);
+  } else {
+    hipLaunchKernelGGL(k_spectra<G>, dim3(a.n), dim3(Geo<G>::NT), 0, s, a);
+  }
   return hipGetLastError();
 }
 
@@ -83,11 +119,14 @@ struct FftEntry {
 #define SXC_FFT_ENTRY(G) ::sxc::FftEntry{(G), &::sxc::launch_correlate_g<(G)>, &::sxc::launch_spectra_g<(G)>},
 
 // Launch tables of the translation units (sxc_kernels.hip: defaults; sxc_kernels_r2.hip:
-// 2-level LDS rounds; sxc_kernels_s32.hip: 32-wide stage A).  Host functions returning a
+// 2-level LDS rounds; sxc_kernels_s32.hip: 32-wide stage A; sxc_kernels_occ.hip: three
+// workgroups per CU; sxc_kernels_lean.hip: lean LDS plan).  Host functions returning a
 // function-local table: a namespace-scope constant table would also be emitted for the
 // device, where the host launchers it points to do not exist.
 const FftEntry* fft_table_default(int* n);
 const FftEntry* fft_table_r2(int* n);
 const FftEntry* fft_table_s32(int* n);
+const FftEntry* fft_table_occ(int* n);
+const FftEntry* fft_table_lean(int* n);
 
 }  // namespace sxc

@@ -185,7 +185,7 @@ const FftEntry* fft_table_default(int* n) {
 }
 
 const FftEntry* fft_entry(int geo) {
-  for (auto tf : {fft_table_default, fft_table_r2, fft_table_s32}) {
+  for (auto tf : {fft_table_default, fft_table_r2, fft_table_s32, fft_table_occ, fft_table_lean}) {
     int n = 0;
     const FftEntry* t = tf(&n);
     for (int i = 0; i < n; ++i)
@@ -196,14 +196,16 @@ const FftEntry* fft_entry(int geo) {
 
 int fft_geometries(int nb, int* out, int cap) {
   int n = 0;
-  for (auto tf : {fft_table_default, fft_table_r2, fft_table_s32}) {
+  for (auto tf : {fft_table_default, fft_table_r2, fft_table_s32, fft_table_occ, fft_table_lean}) {
     int nt = 0;
     const FftEntry* t = tf(&nt);
     for (int i = 0; i < nt; ++i) {
       const int g = t[i].geo;
       if (geo_nb(g) != nb) continue;
-      bool dup = false;  // the same stage-A width and round depth under another code
-      for (int j = 0; j < n; ++j) dup |= geo_sa(out[j]) == geo_sa(g) && geo_km(out[j]) == geo_km(g);
+      bool dup = false;  // the same geometry under another code
+      for (int j = 0; j < n; ++j)
+        dup |= geo_sa(out[j]) == geo_sa(g) && geo_km(out[j]) == geo_km(g) && geo_occ(out[j]) == geo_occ(g) &&
+               geo_lean(out[j]) == geo_lean(g);
       if (!dup && n < cap) out[n++] = g;
     }
   }

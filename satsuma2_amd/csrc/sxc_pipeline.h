@@ -105,29 +105,42 @@ struct Smem {
   static constexpr int NWIN = 512;  // max entropy windows (L <= N, win = N/512)
   // byte offsets
   static constexpr int Y = 0;                                  // float[YLEN]: FFT / xc / scan area
-  static constexpr int LUT = Y + 4 * G::YLEN;                  // float[2048]
-  static constexpr int ENT = LUT + 4 * 2048;                   // float[NWIN]
+  // lean plan: twiddles and chunk bytes are read through the caches, not staged here
+  static constexpr bool LEAN = geo_lean(NB) != 0;
+  static constexpr int LUT = Y + 4 * G::YLEN;                  // float[2048] twiddles
+  // the peaks phase's per-thread counts reuse the twiddle area (the transforms are done)
+  static constexpr int CNT = LUT;                              // int32[NT]
+  static_assert(4 * NT <= 4 * 2048, "counts fit the twiddle area");
+  static constexpr int ENT = LUT + (LEAN ? 4 * NT : 4 * 2048);  // float[NWIN]
   static constexpr int ENV = ENT + 4 * NWIN;                   // double[max(N/256, 1)]
   static constexpr int NENV = (N / 256) > 0 ? N / 256 : 1;
   static constexpr int OFF = ENV + 8 * NENV;                   // double[4] channel means
   static constexpr int DSUM = OFF + 32;                        // double[4] sequential sums
   static constexpr int MISC = DSUM + 32;                       // int32[16]
-  static constexpr int CNT = MISC + 64;                        // int32[NT] per-thread counts
-  static constexpr int CODEC = CNT + 4 * NT;                   // double[18][4]
-  static constexpr int EQUAL = CODEC + 8 * 4 * SXC_NCLASS;     // double[18][18]
-  static constexpr int GC = EQUAL + 8 * SXC_NCLASS * SXC_NCLASS;  // double[18]
-  static constexpr int CLS = GC + 8 * SXC_NCLASS;              // uchar[128] byte -> class
-  static constexpr int SCORE = CLS + 128;                      // uchar[18*18]
-  static constexpr int QUART = SCORE + SXC_NCLASS * SXC_NCLASS;  // uchar[18][4]
+  static constexpr int CODEC = MISC + 64;                      // double[18][4]
+  static constexpr int CLS = CODEC + 8 * 4 * SXC_NCLASS;       // uchar[128] byte -> class
+  static constexpr int QUART = CLS + 128;                      // uchar[18][4]
   static constexpr int RCT = QUART + 4 * SXC_NCLASS;           // uchar[128] reverse complement
   // strand-applied chunk bytes: an LDS copy up to N = 16384; at 32768 the plan would not fit
   // the 160 KB of a workgroup, and the bytes are read from global memory instead
-  static constexpr bool SEQ_IN_LDS = G::LOG <= 14;
+  static constexpr bool SEQ_IN_LDS = G::LOG <= 14 && !LEAN;
   static constexpr int QSEQ = (RCT + 128 + 15) & ~15;          // uchar[N] when SEQ_IN_LDS
   static constexpr int BYTES = (QSEQ + (SEQ_IN_LDS ? N : 0) + 15) & ~15;
   // after the peaks phase Y holds the accepted lags
   static constexpr int CAND = Y;                               // uint16[N]
 };
+
+// The twiddle table a kernel reads: the LDS copy, or the constant table itself under the
+// lean plan.
+template <int NB>
+SXC_HD inline const float* twiddle_table(const char* sm, const sxc_tables* tab) {
+  if constexpr (Smem<NB>::LEAN) {
+    return tab->lut12;
+  } else {
+    return (const float*)(sm + Smem<NB>::LUT);
+  }
+}
+
 
 enum { MISC_L = 0, MISC_THIRDS = 1, MISC_NCAND = 2, MISC_KEPT = 3, MISC_Q4 = 4 /* 4 ints */, MISC_RUNS = 8 };
 
@@ -171,24 +184,17 @@ SXC_HD inline SeqView phase_prepare(Ex& ex, char* sm, const sxc_tables* tab, con
   float* lut = (float*)(sm + SM::LUT);
   unsigned char* cls = (unsigned char*)(sm + SM::CLS);
   unsigned char* rct = (unsigned char*)(sm + SM::RCT);
-  unsigned char* score = (unsigned char*)(sm + SM::SCORE);
   unsigned char* quart = (unsigned char*)(sm + SM::QUART);
   double* codec = (double*)(sm + SM::CODEC);
-  double* equal = (double*)(sm + SM::EQUAL);
-  double* gcv = (double*)(sm + SM::GC);
   int* misc = (int*)(sm + SM::MISC);
   ex.each([&](int tid) {
-    for (int i = tid; i < 2048; i += G::NT) lut[i] = tab->lut12[i];
+    if constexpr (!SM::LEAN)
+      for (int i = tid; i < 2048; i += G::NT) lut[i] = tab->lut12[i];
     for (int i = tid; i < 128; i += G::NT) cls[i] = tab->cls[i];
-    for (int i = tid; i < SXC_NCLASS * SXC_NCLASS; i += G::NT) {
-      score[i] = (&tab->score[0][0])[i];
-      equal[i] = (&tab->equal[0][0])[i];
-    }
     for (int i = tid; i < 4 * SXC_NCLASS; i += G::NT) {
       quart[i] = (&tab->quarters[0][0])[i];
       codec[i] = (&tab->codec[0][0])[i];
     }
-    for (int i = tid; i < SXC_NCLASS; i += G::NT) gcv[i] = tab->gc[i];
     for (int i = tid; i < 128; i += G::NT) rct[i] = tab->rc[i];
     if constexpr (SM::SEQ_IN_LDS) {
       // DNAVector::ReverseComplement (analysis/DNAVector.cc:482-521)
@@ -391,11 +397,11 @@ SXC_HD inline void spectra_body(Ex& ex, char* sm, const SpecArgs& a, int k) {
   using SM = Smem<NB>;
   using G = Geo<NB>;
   const sxc_chunk ck = a.chunks[a.which[k]];
-  const Twiddle tw{(const float*)(sm + SM::LUT), a.osc};
+  const Twiddle tw{twiddle_table<NB>(sm, a.tab), a.osc};
   SeqView qs{nullptr, nullptr, ck.len};
   if (!a.signals) {
     qs = phase_prepare<NB>(ex, sm, a.tab, a.ent_lut, a.seq + ck.off, ck.len, ck.flags & 1, a.strand);
-  } else {
+  } else if constexpr (!SM::LEAN) {
     float* lut = (float*)(sm + SM::LUT);
     ex.each([&](int tid) {
       for (int i = tid; i < 2048; i += G::NT) lut[i] = a.tab->lut12[i];
@@ -574,13 +580,14 @@ SXC_HD inline void correlate_body(Ex& ex, char* sm, const PairArgs& a, int ps) {
   const int strand = ps % a.strands;
   const sxc_pair_dev pr = a.pairs[pair];
   const sxc_chunk qc = a.qchunks[pr.q_chunk];
-  const Twiddle tw{(const float*)(sm + SM::LUT), a.osc};
+  const Twiddle tw{twiddle_table<NB>(sm, a.tab), a.osc};
   if (a.qspectra) {
     // resident query spectra: no encode; only the twiddle table and the counters to set up
     float* lut = (float*)(sm + SM::LUT);
     int* misc0 = (int*)(sm + SM::MISC);
     ex.each([&](int tid) {
-      for (int i = tid; i < 2048; i += G::NT) lut[i] = a.tab->lut12[i];
+      if constexpr (!SM::LEAN)
+        for (int i = tid; i < 2048; i += G::NT) lut[i] = a.tab->lut12[i];
       if (tid < 16) misc0[tid] = 0;
     });
     ex.sync();

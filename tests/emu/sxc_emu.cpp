@@ -214,9 +214,9 @@ int emu_align_batch_tails(const sxc_pair* pairs, int n, const sxc_params* prm, c
   // alternatives the library compiles for the autotuner); unset: the default geometry
   int geo = sxc::nb_of(prm->N);
   if (const char* v = getenv("SXC_EMU_GEO")) {
-    int sa = 0, km = 0;
-    if (sscanf(v, "%d,%d", &sa, &km) != 2) return -1;
-    geo = sxc::geo_code(geo, sa, km);
+    int sa = 0, km = 0, lean = 0;
+    if (sscanf(v, "%d,%d,%d", &sa, &km, &lean) < 2) return -1;
+    geo = sxc::geo_code(geo, sa, km, 0, lean);
   }
 #define SXC_EMU_CASE(G) \
   case (G): rc = run_batch<(G)>(hb, *prm, res, slot_cap, xc_out, stats); break;
@@ -228,6 +228,8 @@ int emu_align_batch_tails(const sxc_pair* pairs, int n, const sxc_params* prm, c
     SXC_EMU_CASE(sxc::geo_code(15, 32, 2))
     SXC_EMU_CASE(sxc::geo_code(11, 32, 3)) SXC_EMU_CASE(sxc::geo_code(12, 32, 3)) SXC_EMU_CASE(sxc::geo_code(13, 32, 3))
     SXC_EMU_CASE(sxc::geo_code(14, 32, 3))
+    SXC_EMU_CASE(sxc::geo_code(12, 32, 3, 0, 1)) SXC_EMU_CASE(sxc::geo_code(12, 16, 2, 0, 1))
+    SXC_EMU_CASE(sxc::geo_code(13, 32, 3, 0, 1)) SXC_EMU_CASE(sxc::geo_code(14, 32, 3, 0, 1))
     default: return -1;
   }
 #undef SXC_EMU_CASE
@@ -414,10 +416,9 @@ int sxc_autotune(sxc_ctx*, int32_t, int32_t, sxc_tune_entry*, int32_t, int32_t* 
   *n_out = 0;
   return SXC_OK;
 }
-int sxc_set_geometry(sxc_ctx*, int32_t, int32_t, int32_t, int32_t) { return SXC_OK; }
-int sxc_get_geometry(sxc_ctx*, int32_t N, int32_t, int32_t* sa, int32_t* km) {
-  *sa = N >= 32768 ? 32 : 16;
-  *km = 3;
+int sxc_set_geometry(sxc_ctx*, int32_t, int32_t, const sxc_geometry*) { return SXC_OK; }
+int sxc_get_geometry(sxc_ctx*, int32_t N, int32_t, sxc_geometry* g) {
+  *g = sxc_geometry{N >= 32768 ? 32 : 16, 3, 0, 0};
   return SXC_OK;
 }
 }  // extern "C"

@@ -112,13 +112,16 @@ def test_emulated_split_scan(emu_lib, golden, monkeypatch, split):
 
 # The alternative FFT geometries the library compiles for its autotuner (stage-A width,
 # levels per LDS round), by log2 length; each must reproduce the oracle bit for bit.
-GEOMETRIES = [(nb, 16, 2) for nb in range(9, 15)] + [(15, 32, 2)] + [(nb, 32, 3) for nb in range(11, 15)]
+# (log2 N, stage-A width, levels per LDS round, lean LDS plan); the register-budget variants
+# run the same code as their plain counterparts on the CPU.
+GEOMETRIES = ([(nb, 16, 2, 0) for nb in range(9, 15)] + [(15, 32, 2, 0)] + [(nb, 32, 3, 0) for nb in range(11, 15)]
+              + [(12, 32, 3, 1), (12, 16, 2, 1), (13, 32, 3, 1), (14, 32, 3, 1)])
 
 
-@pytest.mark.parametrize("nb,sa,km", GEOMETRIES)
-def test_emulated_geometries(emu_lib, nb, sa, km, monkeypatch):
+@pytest.mark.parametrize("nb,sa,km,lean", GEOMETRIES)
+def test_emulated_geometries(emu_lib, nb, sa, km, lean, monkeypatch):
     N = 1 << nb
-    rng = np.random.default_rng(nb * 100 + sa + km)
+    rng = np.random.default_rng(nb * 100 + sa + km + 7 * lean)
     pairs = []
     for r in range(2):
         t = rng.choice(np.frombuffer(b"ACGT", np.uint8), N // 2)
@@ -126,7 +129,7 @@ def test_emulated_geometries(emu_lib, nb, sa, km, monkeypatch):
         ln = N // 8
         q[N // 5: N // 5 + ln] = t[N // 7: N // 7 + ln]  # a homology
         pairs.append(dict(tseq=t.tobytes(), qseq=q.tobytes(), t_record=r, q_record=r))
-    monkeypatch.setenv("SXC_EMU_GEO", f"{sa},{km}")
+    monkeypatch.setenv("SXC_EMU_GEO", f"{sa},{km},{lean}")
     got, xc, _ = emu_batch(emu_lib, pairs, N=N, target_total=1e6, want_xc=True)
     ref = np.concatenate(oracle.align_batch([dict(p, q_record_len=N // 2) for p in pairs],
                                             oracle.make_params(N=N, target_total=1e6)))

@@ -199,7 +199,10 @@ def test_resident_batch_graph_replays(ctx, golden):
 
 # Alternative FFT geometries compiled for the autotuner: (log2 N, stage-A width, levels per
 # LDS round).  Each must reproduce the oracle bit for bit.
-GEOMETRIES = [(nb, 16, 2) for nb in range(9, 15)] + [(15, 32, 2)] + [(nb, 32, 3) for nb in range(11, 15)]
+# (log2 N, stage-A width, levels per LDS round, three workgroups per CU, lean LDS)
+GEOMETRIES = ([(nb, 16, 2, 0, 0) for nb in range(9, 15)] + [(15, 32, 2, 0, 0)]
+              + [(nb, 32, 3, 0, 0) for nb in range(11, 15)] + [(13, 16, 3, 1, 0), (13, 16, 2, 1, 0)]
+              + [(12, 32, 3, 0, 1), (12, 16, 2, 0, 1), (13, 32, 3, 0, 1), (14, 32, 3, 0, 1)])
 
 
 def _homology_pairs(N, n, seed):
@@ -214,9 +217,9 @@ def _homology_pairs(N, n, seed):
     return out
 
 
-@pytest.mark.parametrize("nb,sa,km", GEOMETRIES)
-def test_fft_geometry_parity(ctx, nb, sa, km):
-    from satsuma2_amd.api import ChunkPair, HomologyByXCorr, SlaveParams
+@pytest.mark.parametrize("nb,sa,km,occ,lean", GEOMETRIES)
+def test_fft_geometry_parity(ctx, nb, sa, km, occ, lean):
+    from satsuma2_amd.api import ChunkPair, Geometry, HomologyByXCorr, SlaveParams
 
     N = 1 << nb
     tq = _homology_pairs(N, 8, nb * 7 + sa + km)
@@ -227,20 +230,20 @@ def test_fft_geometry_parity(ctx, nb, sa, km):
     assert len(ref) > 0
     dflt = ctx.geometry(N, "pairs"), ctx.geometry(N, "spectra")
     try:
-        ctx.set_geometry(N, "pairs", sa, km)
-        ctx.set_geometry(N, "spectra", sa, km)
-        assert ctx.geometry(N, "pairs") == (sa, km)
+        ctx.set_geometry(N, "pairs", Geometry(sa, km, occ, lean))
+        ctx.set_geometry(N, "spectra", Geometry(sa, km, occ, lean))
+        assert ctx.geometry(N, "pairs") == (sa, km, occ, lean)
         got = HomologyByXCorr(ctx, SlaveParams(t_chunk=N // 2, target_total=1e6)).align_batch(pairs)
-        assert_results_equal(got, ref, f"geometry {nb},{sa},{km}")
+        assert_results_equal(got, ref, f"geometry {nb},{sa},{km},{occ},{lean}")
     finally:
-        ctx.set_geometry(N, "pairs", *dflt[0])
-        ctx.set_geometry(N, "spectra", *dflt[1])
+        ctx.set_geometry(N, "pairs", dflt[0])
+        ctx.set_geometry(N, "spectra", dflt[1])
 
 
 def test_autotune(ctx, golden):
     """The autotuner times every geometry of every length, keeps one per kernel and length,
     and batches run in the chosen geometries give the golden records."""
-    from satsuma2_amd.api import ChunkPair, HomologyByXCorr, SlaveParams
+    from satsuma2_amd.api import ChunkPair, Geometry, HomologyByXCorr, SlaveParams
 
     entries = ctx.autotune(512, 32768)
     by_n = {}
@@ -252,7 +255,7 @@ def test_autotune(ctx, golden):
         assert len(es) >= 2
         assert sum(e["chosen_correlate"] for e in es) == 1 and sum(e["chosen_spectra"] for e in es) == 1
         pick = [e for e in es if e["chosen_correlate"]][0]
-        assert ctx.geometry(N, "pairs") == (pick["stage_a"], pick["round_levels"])
+        assert ctx.geometry(N, "pairs") == pick["geo"]
         assert pick["correlate_ms"] == min(e["correlate_ms"] for e in es)
     g = golden("uniform_pairs.npz")
     P = 64
@@ -263,8 +266,8 @@ def test_autotune(ctx, golden):
     got = HomologyByXCorr(ctx, SlaveParams(target_total=float(4096 * g["t"].shape[0]))).align_batch(pairs)
     assert_results_equal(got, g["results"][g["results"]["query_id"] < P], "after autotune")
     for N in by_n:  # back to the defaults for the tests that follow
-        ctx.set_geometry(N, "pairs", 32 if N == 32768 else 16, 3)
-        ctx.set_geometry(N, "spectra", 32 if N == 32768 else 16, 3)
+        ctx.set_geometry(N, "pairs", Geometry(32 if N == 32768 else 16, 3, 0, 0))
+        ctx.set_geometry(N, "spectra", Geometry(32 if N == 32768 else 16, 3, 0, 0))
 
 
 def test_edge_pairs(ctx, golden):
