@@ -61,8 +61,8 @@ SXC_HD constexpr int pad(int p) { return p + (p >> 5); }
 
 // A kernel's FFT geometry packed in one template int ("geometry code"): log2 of the
 // transform length in bits 0-4, bit 5 set for a 32-wide stage A, the LDS round depth in
-// bits 6-7, bit 8 asking the compiler to fit three workgroups per CU (a register budget of
-// 3 x waves-per-workgroup / 4 waves per SIMD), bit 9 for a lean LDS plan (twiddles and
+// bits 6-7, bit 8 asking the compiler for a register budget that fits as many workgroups
+// per CU as the LDS plan allows (up to three), bit 9 for a lean LDS plan (twiddles and
 // chunk bytes read through the caches instead of staged in LDS, so more workgroups fit a
 // CU).  A plain log2 length (bits 5-9 clear) is the default geometry for that length.
 // Every geometry computes the same FFTReal dataflow; they differ in how positions map to

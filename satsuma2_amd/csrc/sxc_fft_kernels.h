@@ -61,12 +61,16 @@ __global__ __launch_bounds__(Geo<G>::NT) void k_spectra(SpecArgs a) {
   spectra_body<G>(ex, sm, a, (int)blockIdx.x);
 }
 
-// The same kernels with a register budget that lets three workgroups share a CU (geometry
-// codes with bit 8; whether the extra occupancy beats the spills it may cost is for the
-// autotuner to measure).
+// The same kernels with a register budget that lets as many workgroups share a CU as its
+// LDS plan allows, up to three (geometry codes with bit 8; whether the extra occupancy
+// beats the spills it may cost is for the autotuner to measure).
 template <int G>
 constexpr int occ_waves() {
-  return (3 * (Geo<G>::NT / 64) + 3) / 4;  // waves per SIMD for 3 workgroups
+  // waves per SIMD for as many workgroups as the LDS plan lets share a CU (3 at most)
+  constexpr int by_lds = 163840 / Smem<G>::BYTES;
+  constexpr int wg = by_lds < 1 ? 1 : (by_lds > 3 ? 3 : by_lds);
+  constexpr int w = (wg * (Geo<G>::NT / 64) + 3) / 4;
+  return w < 1 ? 1 : (w > 8 ? 8 : w);
 }
 
 template <int G>

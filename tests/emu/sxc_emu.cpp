@@ -230,6 +230,8 @@ int emu_align_batch_tails(const sxc_pair* pairs, int n, const sxc_params* prm, c
     SXC_EMU_CASE(sxc::geo_code(14, 32, 3))
     SXC_EMU_CASE(sxc::geo_code(12, 32, 3, 0, 1)) SXC_EMU_CASE(sxc::geo_code(12, 16, 2, 0, 1))
     SXC_EMU_CASE(sxc::geo_code(13, 32, 3, 0, 1)) SXC_EMU_CASE(sxc::geo_code(14, 32, 3, 0, 1))
+    SXC_EMU_CASE(sxc::geo_code(11, 32, 3, 0, 1)) SXC_EMU_CASE(sxc::geo_code(11, 16, 2, 0, 1))
+    SXC_EMU_CASE(sxc::geo_code(13, 32, 2, 0, 1))
     default: return -1;
   }
 #undef SXC_EMU_CASE

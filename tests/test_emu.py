@@ -115,7 +115,8 @@ def test_emulated_split_scan(emu_lib, golden, monkeypatch, split):
 # (log2 N, stage-A width, levels per LDS round, lean LDS plan); the register-budget variants
 # run the same code as their plain counterparts on the CPU.
 GEOMETRIES = ([(nb, 16, 2, 0) for nb in range(9, 15)] + [(15, 32, 2, 0)] + [(nb, 32, 3, 0) for nb in range(11, 15)]
-              + [(12, 32, 3, 1), (12, 16, 2, 1), (13, 32, 3, 1), (14, 32, 3, 1)])
+              + [(12, 32, 3, 1), (12, 16, 2, 1), (13, 32, 3, 1), (14, 32, 3, 1), (11, 32, 3, 1), (11, 16, 2, 1),
+                 (13, 32, 2, 1)])
 
 
 @pytest.mark.parametrize("nb,sa,km,lean", GEOMETRIES)

@@ -202,7 +202,8 @@ def test_resident_batch_graph_replays(ctx, golden):
 # (log2 N, stage-A width, levels per LDS round, three workgroups per CU, lean LDS)
 GEOMETRIES = ([(nb, 16, 2, 0, 0) for nb in range(9, 15)] + [(15, 32, 2, 0, 0)]
               + [(nb, 32, 3, 0, 0) for nb in range(11, 15)] + [(13, 16, 3, 1, 0), (13, 16, 2, 1, 0)]
-              + [(12, 32, 3, 0, 1), (12, 16, 2, 0, 1), (13, 32, 3, 0, 1), (14, 32, 3, 0, 1)])
+              + [(12, 32, 3, 0, 1), (12, 16, 2, 0, 1), (13, 32, 3, 0, 1), (14, 32, 3, 0, 1), (11, 32, 3, 0, 1),
+                 (11, 16, 2, 0, 1), (13, 32, 2, 0, 1), (14, 32, 3, 1, 1)])
 
 
 def _homology_pairs(N, n, seed):
