@@ -59,25 +59,30 @@ constexpr int SCAN_GSZ = 16;  // threads per lag in the batched phase
 constexpr int SXC_MIN_MATCHES = SXC_WIN_THRESHOLD / 100 + 1;
 static_assert(100 * SXC_MIN_MATCHES > SXC_WIN_THRESHOLD && 100 * (SXC_MIN_MATCHES - 1) <= SXC_WIN_THRESHOLD, "");
 
+// NB: log2 N in bits 0-4; bit 5 selects the kernel for pairs whose two chunks are pure
+// A/C/G/T, whose LDS plan holds only the packed bases, the run queue and the counters (the
+// accepted lags are read from global memory), so twice as many of its workgroups share a
+// CU.  The other kernel takes every other pair.  Each exits at once on the other's pairs.
 template <int NB>
 struct ScanSmem {
-  static constexpr int N = 1 << NB;
+  static constexpr int LOG = NB & 31;
+  static constexpr bool PURE = (NB & 32) != 0;
+  static constexpr int N = 1 << LOG;
   static constexpr int QCAP = N / 4 + 64;  // runs of one lag: npos / 2 + 1 with npos <= N / 2
   static constexpr int PKW = N / 16 + 4;                // 2-bit packed words (+3 words of read-ahead)
-  static constexpr int TCODE = 0;                       // uchar[N]
-  static constexpr int QCODE = TCODE + N;               // uchar[N]
-  static constexpr int TPK = QCODE + N;                 // uint32[PKW]: 16 bases per word (pure A/C/G/T only)
+  static constexpr int TCODE = 0;                       // uchar[N] (general kernel)
+  static constexpr int QCODE = TCODE + (PURE ? 0 : N);  // uchar[N] (general kernel)
+  static constexpr int TPK = QCODE + (PURE ? 0 : N);    // uint32[PKW]: 16 bases per word (pure A/C/G/T only)
   static constexpr int QPK = TPK + 4 * PKW;             // uint32[PKW]
   static constexpr int QUEUE = QPK + 4 * PKW;           // int2[QCAP]: {lag rank, ns | ne << 16}
   static constexpr int SUMM = QUEUE + 8 * QCAP;         // int2[SCAN_NT]
+  static constexpr int NTAB = PURE ? 0 : 1;             // the score tables (general kernel)
   static constexpr int EQUAL = SUMM + 8 * SCAN_NT;      // double[18][18]
-  static constexpr int GC = EQUAL + 8 * SXC_NCLASS * SXC_NCLASS;  // double[18]
-  static constexpr int SCORE = GC + 8 * SXC_NCLASS;     // uchar[18][18]
-  static constexpr int CLS = SCORE + SXC_NCLASS * SXC_NCLASS;  // uchar[128]
-  static constexpr int RC = CLS + 128;                  // uchar[128]
-  static constexpr int MISC = (RC + 128 + 15) & ~15;    // int[16]
-  // pure A/C/G/T pairs: after packing, the code area holds the accepted lags (uint16[N])
-  static constexpr int CAND = TCODE;
+  static constexpr int GC = EQUAL + NTAB * 8 * SXC_NCLASS * SXC_NCLASS;  // double[18]
+  static constexpr int SCORE = GC + NTAB * 8 * SXC_NCLASS;              // uchar[18][18]
+  static constexpr int CLS = SCORE + NTAB * SXC_NCLASS * SXC_NCLASS;     // uchar[128]
+  static constexpr int RC = CLS + NTAB * 128;                           // uchar[128]
+  static constexpr int MISC = (RC + NTAB * 128 + 15) & ~15;             // int[16]
   static constexpr int BYTES = MISC + 64;
 };
 
@@ -238,7 +243,7 @@ template <int NB, class Ex>
 SXC_HD inline void scan_epoch(Ex& ex, char* sm, const uint16_t* cands, int c0, int c1, int gsz) {
   using SS = ScanSmem<NB>;
   constexpr int INF = 0x7fffffff;
-  constexpr int H = (1 << NB) / 2;
+  constexpr int H = SS::N / 2;
   const unsigned char* tc = (const unsigned char*)(sm + SS::TCODE);
   const unsigned char* qc = (const unsigned char*)(sm + SS::QCODE);
   const unsigned char* score = (const unsigned char*)(sm + SS::SCORE);
@@ -313,7 +318,7 @@ SXC_HD inline void scan_epoch(Ex& ex, char* sm, const uint16_t* cands, int c0, i
           e1 = e0;
         }
         (void)e3;
-      } else if (na < nb) {
+      } else if (!SS::PURE && na < nb) {
         const unsigned char* tp = tc + g.i0;
         const unsigned char* qp = qc + g.i0 + g.shift;
         auto sc = [&](int n) -> int {
@@ -366,8 +371,8 @@ SXC_HD inline void scan_epoch(Ex& ex, char* sm, const uint16_t* cands, int c0, i
 template <int NB, class Ex>
 SXC_HD inline void drain(Ex& ex, char* sm, const ScanArgs& a, const sxc_pair_dev& pr, const uint16_t* cands, int ps,
                          int n) {
-  constexpr int H = (1 << NB) / 2;
   using SS = ScanSmem<NB>;
+  constexpr int H = SS::N / 2;
   const unsigned char* tc = (const unsigned char*)(sm + SS::TCODE);
   const unsigned char* qc = (const unsigned char*)(sm + SS::QCODE);
   const double* equal = (const double*)(sm + SS::EQUAL);
@@ -398,7 +403,7 @@ SXC_HD inline void drain(Ex& ex, char* sm, const ScanArgs& a, const sxc_pair_dev
       }
       if (len < a.min_len) continue;  // FilterMatches: GetLength() < m_minLen
       double ident, prob = 0.;
-      if (fast) {
+      if (SS::PURE || fast) {
         int cm, ct, cq;
         packed_counts((const uint32_t*)(sm + SS::TPK), (const uint32_t*)(sm + SS::QPK), start_t, start_q, len, cm,
                       ct, cq);
@@ -453,13 +458,15 @@ SXC_HD inline void drain(Ex& ex, char* sm, const ScanArgs& a, const sxc_pair_dev
 template <int NB, class Ex>
 SXC_HD inline void scan_body(Ex& ex, char* sm, const ScanArgs& a, int blk) {
   using SS = ScanSmem<NB>;
-  constexpr int N = 1 << NB;
+  constexpr int N = SS::N;
   const int split = a.split > 1 ? a.split : 1;
   const int ps = blk / split, part = blk % split;
   const sxc_pair_dev pr = a.pairs[ps / a.strands];
   const int strand = ps % a.strands;
   const sxc_chunk tch = a.tchunks[pr.t_chunk];
   const sxc_chunk qch = a.qchunks[pr.q_chunk];
+  const bool fast = (tch.flags & 2) && (qch.flags & 2);  // both pure A/C/G/T
+  if (fast != SS::PURE) return;                           // the other kernel's pair
   const sxc_tables* tab = a.tab;
   unsigned char* tc = (unsigned char*)(sm + SS::TCODE);
   unsigned char* qc = (unsigned char*)(sm + SS::QCODE);
@@ -472,23 +479,24 @@ SXC_HD inline void scan_body(Ex& ex, char* sm, const ScanArgs& a, int blk) {
   const uint16_t* gcands = a.cands + (size_t)ps * N;
   unsigned char* cls = (unsigned char*)(sm + SS::CLS);
   unsigned char* rc = (unsigned char*)(sm + SS::RC);
-  const bool fast = (tch.flags & 2) && (qch.flags & 2);  // both pure A/C/G/T
-  // stage the chunk bytes (16-byte aligned in HBM) and the tables in LDS
+  // the general kernel stages the chunk bytes (16-byte aligned in HBM) and the tables in LDS
   ex.each([&](int tid) {
-    const uint32_t* ts4 = (const uint32_t*)ts;
-    const uint32_t* qs4 = (const uint32_t*)qs;
-    uint32_t* tc4 = (uint32_t*)tc;
-    uint32_t* qc4 = (uint32_t*)qc;
-    for (int i = tid; i < (tch.len + 3) / 4; i += SCAN_NT) tc4[i] = ts4[i];
-    for (int i = tid; i < (qch.len + 3) / 4; i += SCAN_NT) qc4[i] = qs4[i];
-    for (int i = tid; i < SXC_NCLASS * SXC_NCLASS; i += SCAN_NT) {
-      equal[i] = (&tab->equal[0][0])[i];
-      score[i] = (&tab->score[0][0])[i];
-    }
-    for (int i = tid; i < SXC_NCLASS; i += SCAN_NT) gcv[i] = tab->gc[i];
-    for (int i = tid; i < 128; i += SCAN_NT) {
-      cls[i] = tab->cls[i];
-      rc[i] = tab->rc[i];
+    if constexpr (!SS::PURE) {
+      const uint32_t* ts4 = (const uint32_t*)ts;
+      const uint32_t* qs4 = (const uint32_t*)qs;
+      uint32_t* tc4 = (uint32_t*)tc;
+      uint32_t* qc4 = (uint32_t*)qc;
+      for (int i = tid; i < (tch.len + 3) / 4; i += SCAN_NT) tc4[i] = ts4[i];
+      for (int i = tid; i < (qch.len + 3) / 4; i += SCAN_NT) qc4[i] = qs4[i];
+      for (int i = tid; i < SXC_NCLASS * SXC_NCLASS; i += SCAN_NT) {
+        equal[i] = (&tab->equal[0][0])[i];
+        score[i] = (&tab->score[0][0])[i];
+      }
+      for (int i = tid; i < SXC_NCLASS; i += SCAN_NT) gcv[i] = tab->gc[i];
+      for (int i = tid; i < 128; i += SCAN_NT) {
+        cls[i] = tab->cls[i];
+        rc[i] = tab->rc[i];
+      }
     }
     if (tid == 0) {
       misc[SM_QN] = 0;
@@ -502,8 +510,9 @@ SXC_HD inline void scan_body(Ex& ex, char* sm, const ScanArgs& a, int blk) {
   });
   ex.sync();
   const int Lt = tch.len, Lq = qch.len;
-  if (fast) {
-    // 2-bit codes A=0, C=1, G=2, T=3 from the ASCII letters; complement = code ^ 3
+  if constexpr (SS::PURE) {
+    // 2-bit codes A=0, C=1, G=2, T=3 from the ASCII letters (read from HBM); complement =
+    // code ^ 3
     uint32_t* tpk = (uint32_t*)(sm + SS::TPK);
     uint32_t* qpk = (uint32_t*)(sm + SS::QPK);
     ex.each([&](int tid) {
@@ -512,24 +521,18 @@ SXC_HD inline void scan_body(Ex& ex, char* sm, const ScanArgs& a, int blk) {
         for (int b = 0; b < 16; ++b) {
           const int i = 16 * w + b;
           if (i < Lt) {
-            const unsigned x = tc[i];
+            const unsigned x = ts[i];
             vt |= (((x >> 1) ^ (x >> 2)) & 3u) << (2 * b);
           }
           if (i < Lq) {
             // DNAVector::ReverseComplement for strand 1 (analysis/DNAVector.cc:482-521)
-            const unsigned x = strand ? qc[Lq - 1 - i] : qc[i];
+            const unsigned x = strand ? qs[Lq - 1 - i] : qs[i];
             vq |= ((((x >> 1) ^ (x >> 2)) & 3u) ^ (strand ? 3u : 0u)) << (2 * b);
           }
         }
         tpk[w] = vt;
         qpk[w] = vq;
       }
-    });
-    ex.sync();
-    uint16_t* lc = (uint16_t*)(sm + SS::CAND);
-    const int nc = misc[SM_NCAND];
-    ex.each([&](int tid) {
-      for (int i = tid; i < nc; i += SCAN_NT) lc[i] = gcands[i];
     });
     ex.sync();
   } else {
@@ -556,7 +559,7 @@ SXC_HD inline void scan_body(Ex& ex, char* sm, const ScanArgs& a, int blk) {
     });
     ex.sync();
   }
-  const uint16_t* cands = fast ? (const uint16_t*)(sm + SS::CAND) : gcands;
+  const uint16_t* cands = gcands;
   const int ncand = misc[SM_NCAND];
   // this workgroup's share of the lags (all of them unless the scan is split)
   const int lo = (int)((int64_t)ncand * part / split), hi = (int)((int64_t)ncand * (part + 1) / split);
