@@ -13,8 +13,22 @@ namespace sxc {
 
 struct NoRegs {};
 
+// The pure-A/C/G/T scan kernel gets a register budget for as many of its 4-wave
+// workgroups per CU as its LDS plan admits, up to 6 waves per SIMD (measured on the 4096-pair
+// workload at N = 8192: 4 workgroups 2.28 ms, 5: 2.16 ms, 6: 2.09 ms, 7: 2.11 ms; the spills
+// the budget costs are cheaper than the latency the extra waves hide).
 template <int NB>
-__global__ __launch_bounds__(SCAN_NT) void k_scan(ScanArgs a) {
+constexpr int scan_waves() {
+  if constexpr (!ScanSmem<NB>::PURE) {
+    return 1;
+  } else {
+    constexpr int wg = 163840 / ScanSmem<NB>::BYTES;
+    return wg < 1 ? 1 : (wg > 6 ? 6 : wg);
+  }
+}
+
+template <int NB>
+__global__ __launch_bounds__(SCAN_NT) __attribute__((amdgpu_waves_per_eu(scan_waves<NB>()))) void k_scan(ScanArgs a) {
   __shared__ __align__(16) char sm[ScanSmem<NB>::BYTES];
   NoRegs r;
   DevExec<NoRegs> ex{r};

@@ -565,11 +565,16 @@ SXC_HD inline void scan_body(Ex& ex, char* sm, const ScanArgs& a, int blk) {
   const int lo = (int)((int64_t)ncand * part / split), hi = (int)((int64_t)ncand * (part + 1) / split);
   constexpr int CE = SCAN_NT / SCAN_GSZ;
   int runs = 0;
+  // Queue entries before the current epoch, kept per thread: every thread must see the same
+  // count, and the shared counter may already hold the next epoch's pushes of a faster wave.
+  int queued = 0;
   for (int e0 = lo; e0 < hi; e0 += CE) {
-    const int qstart = misc[SM_QN];
+    const int qstart = queued;
     const int e1 = (e0 + CE < hi) ? e0 + CE : hi;
     scan_epoch<NB>(ex, sm, cands, e0, e1, SCAN_GSZ);
     const int qn = misc[SM_QN];
+    ex.sync();  // every thread has read the count before any thread pushes again
+    queued = qn;
     if (qn > SS::QCAP) {
       // the epoch overflowed the queue: keep what came before it, redo its lags one by one
       drain<NB>(ex, sm, a, pr, cands, ps, qstart);
@@ -588,6 +593,7 @@ SXC_HD inline void scan_body(Ex& ex, char* sm, const ScanArgs& a, int blk) {
         if (tid == 0) misc[SM_QN] = 0;
       });
       ex.sync();
+      queued = 0;
     } else if (qn > SS::QCAP / 2 || e1 == hi) {
       drain<NB>(ex, sm, a, pr, cands, ps, qn);
       runs += qn;
@@ -595,6 +601,7 @@ SXC_HD inline void scan_body(Ex& ex, char* sm, const ScanArgs& a, int blk) {
         if (tid == 0) misc[SM_QN] = 0;
       });
       ex.sync();
+      queued = 0;
     }
   }
   ex.each([&](int tid) {
