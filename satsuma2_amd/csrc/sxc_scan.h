@@ -74,8 +74,17 @@ struct ScanSmem {
   static constexpr int QCODE = TCODE + (PURE ? 0 : N);  // uchar[N] (general kernel)
   static constexpr int TPK = QCODE + (PURE ? 0 : N);    // uint32[PKW]: 16 bases per word (pure A/C/G/T only)
   static constexpr int QPK = TPK + 4 * PKW;             // uint32[PKW]
-  static constexpr int QUEUE = QPK + 4 * PKW;           // int2[QCAP]: {lag rank, ns | ne << 16}
-  static constexpr int SUMM = QUEUE + 8 * QCAP;         // int2[SCAN_NT]
+  // From N = 16384 the queue takes one epoch of 8 lags at a time (drained after every
+  // epoch) in 4-byte entries {lag - first lag of the epoch (3 bits), ns (14), ne (15)}, which
+  // halves the largest part of the plan; below, 16-lag epochs accumulate in 8-byte entries
+  // {lag rank, ns | ne << 16} until the queue is half full.
+  static constexpr bool COMPACT = LOG >= 14;
+  static constexpr int GSZ = COMPACT ? 32 : SCAN_GSZ;   // threads per lag in an epoch
+  static constexpr int CE = SCAN_NT / GSZ;              // lags per epoch
+  static constexpr int QE = COMPACT ? 4 : 8;            // bytes per queue entry
+  static_assert(!COMPACT || (CE <= 8 && LOG <= 15), "compact queue entries: 3 + 14 + 15 bits");
+  static constexpr int QUEUE = QPK + 4 * PKW;           // queue entries [QCAP]
+  static constexpr int SUMM = QUEUE + QE * QCAP;        // int2[SCAN_NT]
   static constexpr int NTAB = PURE ? 0 : 1;             // the score tables (general kernel)
   static constexpr int EQUAL = SUMM + 8 * SCAN_NT;      // double[18][18]
   static constexpr int GC = EQUAL + NTAB * 8 * SXC_NCLASS * SXC_NCLASS;  // double[18]
@@ -255,8 +264,12 @@ SXC_HD inline void scan_epoch(Ex& ex, char* sm, const uint16_t* cands, int c0, i
   auto push = [&](int rank, int ns, int ne) {
     const int k = ex.atomic_add_shared(&misc[SM_QN], 1);
     if (k < SS::QCAP) {
-      queue[2 * k] = rank;
-      queue[2 * k + 1] = ns | (ne << 16);
+      if constexpr (SS::COMPACT) {
+        queue[k] = (int)((unsigned)(rank - c0) | ((unsigned)ns << 3) | ((unsigned)ne << 17));
+      } else {
+        queue[2 * k] = rank;
+        queue[2 * k + 1] = ns | (ne << 16);
+      }
     }
   };
   ex.each([&](int tid) {
@@ -368,9 +381,10 @@ SXC_HD inline void scan_epoch(Ex& ex, char* sm, const uint16_t* cands, int c0, i
 }
 
 // Drains queue entries [0, n): mode 0 filters into t_results, mode 1 writes runs.
+// rank_base: the first lag of the epoch the entries come from (compact queue).
 template <int NB, class Ex>
 SXC_HD inline void drain(Ex& ex, char* sm, const ScanArgs& a, const sxc_pair_dev& pr, const uint16_t* cands, int ps,
-                         int n) {
+                         int n, int rank_base) {
   using SS = ScanSmem<NB>;
   constexpr int H = SS::N / 2;
   const unsigned char* tc = (const unsigned char*)(sm + SS::TCODE);
@@ -383,8 +397,17 @@ SXC_HD inline void drain(Ex& ex, char* sm, const ScanArgs& a, const sxc_pair_dev
   const int strand = ps % a.strands;
   ex.each([&](int tid) {
     for (int k = tid; k < n; k += SCAN_NT) {
-      const int rank = queue[2 * k];
-      const int ns = queue[2 * k + 1] & 0xffff, ne = (queue[2 * k + 1] >> 16) & 0xffff;
+      int rank, ns, ne;
+      if constexpr (SS::COMPACT) {
+        const unsigned v = (unsigned)queue[k];
+        rank = rank_base + (int)(v & 7u);
+        ns = (int)((v >> 3) & 0x3fffu);
+        ne = (int)(v >> 17);
+      } else {
+        rank = queue[2 * k];
+        ns = queue[2 * k + 1] & 0xffff;
+        ne = (queue[2 * k + 1] >> 16) & 0xffff;
+      }
       const int shift = (int)cands[rank] - H;
       const int i0 = shift < 0 ? -shift : 0;
       const int start_t = i0 + ns - SXC_MIN_LEN;
@@ -563,7 +586,7 @@ SXC_HD inline void scan_body(Ex& ex, char* sm, const ScanArgs& a, int blk) {
   const int ncand = misc[SM_NCAND];
   // this workgroup's share of the lags (all of them unless the scan is split)
   const int lo = (int)((int64_t)ncand * part / split), hi = (int)((int64_t)ncand * (part + 1) / split);
-  constexpr int CE = SCAN_NT / SCAN_GSZ;
+  constexpr int CE = SS::CE;
   int runs = 0;
   // Queue entries before the current epoch, kept per thread: every thread must see the same
   // count, and the shared counter may already hold the next epoch's pushes of a faster wave.
@@ -571,13 +594,13 @@ SXC_HD inline void scan_body(Ex& ex, char* sm, const ScanArgs& a, int blk) {
   for (int e0 = lo; e0 < hi; e0 += CE) {
     const int qstart = queued;
     const int e1 = (e0 + CE < hi) ? e0 + CE : hi;
-    scan_epoch<NB>(ex, sm, cands, e0, e1, SCAN_GSZ);
+    scan_epoch<NB>(ex, sm, cands, e0, e1, SS::GSZ);
     const int qn = misc[SM_QN];
     ex.sync();  // every thread has read the count before any thread pushes again
     queued = qn;
     if (qn > SS::QCAP) {
       // the epoch overflowed the queue: keep what came before it, redo its lags one by one
-      drain<NB>(ex, sm, a, pr, cands, ps, qstart);
+      drain<NB>(ex, sm, a, pr, cands, ps, qstart, e0);  // (compact queue: empty)
       runs += qstart;
       for (int c = e0; c < e1; ++c) {
         ex.each([&](int tid) {
@@ -586,7 +609,7 @@ SXC_HD inline void scan_body(Ex& ex, char* sm, const ScanArgs& a, int blk) {
         ex.sync();
         scan_epoch<NB>(ex, sm, cands, c, c + 1, SCAN_NT);
         const int n1 = misc[SM_QN];
-        drain<NB>(ex, sm, a, pr, cands, ps, n1);
+        drain<NB>(ex, sm, a, pr, cands, ps, n1, c);
         runs += n1;
       }
       ex.each([&](int tid) {
@@ -594,8 +617,8 @@ SXC_HD inline void scan_body(Ex& ex, char* sm, const ScanArgs& a, int blk) {
       });
       ex.sync();
       queued = 0;
-    } else if (qn > SS::QCAP / 2 || e1 == hi) {
-      drain<NB>(ex, sm, a, pr, cands, ps, qn);
+    } else if (SS::COMPACT || qn > SS::QCAP / 2 || e1 == hi) {
+      drain<NB>(ex, sm, a, pr, cands, ps, qn, e0);
       runs += qn;
       ex.each([&](int tid) {
         if (tid == 0) misc[SM_QN] = 0;
