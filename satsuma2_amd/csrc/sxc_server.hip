@@ -84,6 +84,8 @@ struct Slot {
   int32_t* h_maxc = nullptr;        // pinned
   sxc_result* h_final = nullptr;    // pinned, h_final_cap records
   int64_t h_final_cap = 0;
+  int64_t out_cap = 0;              // records final_ holds
+  int64_t copied = 0;               // ordered records the launch itself copied to h_final
   int cap = kDefaultCap;            // records per (pair, strand) of the current launch
   int npairs = 0;
   std::vector<Piece> pieces;
@@ -141,6 +143,7 @@ int slot_alloc(sxc_server* sv, Slot& sl) {
   HIPCHK(sl.total.ensure(sizeof(int64_t)));
   HIPCHK(sl.maxc.ensure(sizeof(int32_t)));
   HIPCHK(sl.final_.ensure(total * kDefaultCap * sizeof(sxc_result_dev)));
+  sl.out_cap = (int64_t)total * kDefaultCap;
   HIPCHK(hipHostMalloc((void**)&sl.h_pairs, (size_t)P * sizeof(sxc_pair_dev), hipHostMallocDefault));
   HIPCHK(hipHostMalloc((void**)&sl.h_offsets, (total + 1) * sizeof(int64_t), hipHostMallocDefault));
   HIPCHK(hipHostMalloc((void**)&sl.h_maxc, sizeof(int32_t), hipHostMallocDefault));
@@ -170,8 +173,10 @@ int64_t slot_bytes_for(int P, int N, int cap) {
 }
 
 // split: workgroups per (pair, strand) (counts must have been cleared by the pair kernel
-// when above 1).
-int launch_scan_slot(sxc_server* sv, Slot& sl, int split) {
+// when above 1).  order: also order the records into final_ (up to out_cap) and copy the
+// first of them to the host in the same pass, so a launch whose records fit needs no
+// second round trip.
+int launch_scan_slot(sxc_server* sv, Slot& sl, int split, bool order) {
   // Room for cap records of every (pair, strand) of this launch.  The buffers only grow;
   // a launch may have more pairs than the one that last grew them.
   const size_t nrec = (size_t)2 * sl.npairs * sl.cap;
@@ -209,6 +214,16 @@ int launch_scan_slot(sxc_server* sv, Slot& sl, int split) {
   HIPCHK(hipMemcpyAsync(sl.h_offsets, sl.offsets.p, (size_t)total * sizeof(int64_t), hipMemcpyDeviceToHost, sl.s));
   HIPCHK(hipMemcpyAsync(sl.h_offsets + total, sl.total.p, sizeof(int64_t), hipMemcpyDeviceToHost, sl.s));
   HIPCHK(hipMemcpyAsync(sl.h_maxc, sl.maxc.p, sizeof(int32_t), hipMemcpyDeviceToHost, sl.s));
+  sl.copied = 0;
+  if (order) {
+    HIPCHK(sxc::launch_order_records(sl.counts.as<int32_t>(), total, sl.cap, sl.offsets.as<int64_t>(),
+                                     sl.outs.as<sxc_result_dev>(), sl.keys.as<uint64_t>(),
+                                     sl.final_.as<sxc_result_dev>(), sl.out_cap, sl.s));
+    // records are few per pair (mostly none): a prefix sized by the launch usually holds all
+    sl.copied = std::min<int64_t>(std::min(sl.out_cap, sl.h_final_cap), std::max(512, sl.npairs));
+    HIPCHK(hipMemcpyAsync(sl.h_final, sl.final_.p, (size_t)sl.copied * sizeof(sxc_result), hipMemcpyDeviceToHost,
+                          sl.s));
+  }
   HIPCHK(hipEventRecord(sl.ev_count, sl.s));
   return SXC_OK;
 }
@@ -275,44 +290,64 @@ int slot_fill(sxc_server* sv, Slot& sl) {
   // A launch too small to fill the device splits each (pair, strand)'s lags over up to 4
   // scan workgroups (3 resident per CU), which shortens its latency.
   const int split = std::max(1, std::min(4, 3 * sv->cus / (2 * sl.npairs)));
-  int rc = launch_scan_slot(sv, sl, split);
+  int rc = launch_scan_slot(sv, sl, split, true);
   if (rc) return rc;
   sl.state = COUNTING;
   ++sv->st_launches;
   return SXC_OK;
 }
 
-// Counts are back: redo the scan with room for every record if a (pair, strand) kept
-// more than the slot holds, then order the records and copy them back.
+void slot_complete(sxc_server* sv, Slot& sl, hipEvent_t last);
+
+// Counts are back (and, usually, the ordered records with them).  If a (pair, strand) kept
+// more records than the slot holds, redo the scan with room for all; if the records
+// overflowed the output buffer, order them again into a larger one; copy whatever the
+// launch did not bring back.  Completes the slot at once when nothing is missing.
 int slot_pack(sxc_server* sv, Slot& sl) {
   const int total = 2 * sl.npairs;
+  bool order = false;
   if (*sl.h_maxc > sl.cap) {
     sl.cap = *sl.h_maxc;
-    int rc = launch_scan_slot(sv, sl, 1);  // grows the record buffers to total x cap
+    int rc = launch_scan_slot(sv, sl, 1, false);  // grows the record buffers to total x cap
     if (rc) return rc;
     HIPCHK(hipEventSynchronize(sl.ev_count));
+    order = true;
   }
   const int64_t n = sl.h_offsets[total];
-  HIPCHK(sl.final_.ensure((size_t)std::max<int64_t>(n, 1) * sizeof(sxc_result_dev)));
+  if (n > sl.out_cap) {
+    HIPCHK(sl.final_.ensure((size_t)n * sizeof(sxc_result_dev)));
+    sl.out_cap = n;
+    order = true;
+  }
   if (n > sl.h_final_cap) {
     if (sl.h_final) HIPCHK(hipHostFree(sl.h_final));
     sl.h_final = nullptr;
     sl.h_final_cap = n;
     HIPCHK(hipHostMalloc((void**)&sl.h_final, (size_t)n * sizeof(sxc_result), hipHostMallocDefault));
+    sl.copied = 0;  // the records copied so far went to the freed buffer
   }
-  HIPCHK(sxc::launch_order_records(sl.counts.as<int32_t>(), total, sl.cap, sl.offsets.as<int64_t>(),
-                                   sl.outs.as<sxc_result_dev>(), sl.keys.as<uint64_t>(),
-                                   sl.final_.as<sxc_result_dev>(), n, sl.s));
-  if (n > 0) HIPCHK(hipMemcpyAsync(sl.h_final, sl.final_.p, (size_t)n * sizeof(sxc_result), hipMemcpyDeviceToHost, sl.s));
+  if (order) {
+    HIPCHK(sxc::launch_order_records(sl.counts.as<int32_t>(), total, sl.cap, sl.offsets.as<int64_t>(),
+                                     sl.outs.as<sxc_result_dev>(), sl.keys.as<uint64_t>(),
+                                     sl.final_.as<sxc_result_dev>(), sl.out_cap, sl.s));
+    sl.copied = 0;
+  }
+  if (n <= sl.copied) {
+    slot_complete(sv, sl, sl.ev_count);
+    return SXC_OK;
+  }
+  HIPCHK(hipMemcpyAsync(sl.h_final + sl.copied, sl.final_.as<sxc_result_dev>() + sl.copied,
+                        (size_t)(n - sl.copied) * sizeof(sxc_result), hipMemcpyDeviceToHost, sl.s));
   HIPCHK(hipEventRecord(sl.ev_pack, sl.s));
   sl.state = PACKING;
   return SXC_OK;
 }
 
-// Records are on the host: hand each request its pairs' records.  Called with sv->mu held.
-void slot_complete(sxc_server* sv, Slot& sl) {
+// Records are on the host: hand each request its pairs' records.  `last`: the slot's last
+// completed event.  Called with sv->mu held.
+void slot_complete(sxc_server* sv, Slot& sl, hipEvent_t last) {
   float ms = 0.f;
-  if (hipEventElapsedTime(&ms, sl.ev_start, sl.ev_pack) == hipSuccess) sv->st_device_ms += ms;
+  if (hipEventElapsedTime(&ms, sl.ev_start, last) == hipSuccess) sv->st_device_ms += ms;
   const int total = 2 * sl.npairs;
   const auto now = Clock::now();
   for (const Piece& pc : sl.pieces) {
@@ -359,7 +394,7 @@ void dispatch_loop(sxc_server* sv) {
         progressed = true;
       }
       if (sl.state == PACKING && hipEventQuery(sl.ev_pack) == hipSuccess) {
-        slot_complete(sv, sl);
+        slot_complete(sv, sl, sl.ev_pack);
         progressed = true;
       }
     }
