@@ -93,6 +93,12 @@ void sxc_destroy(sxc_ctx* ctx);
 const char* sxc_last_error(void);
 const char* sxc_version(void);
 
+/* Host-side codec helpers (no device work; the tables the kernels use).
+ * DNAVector::ReverseComplement on seq[0, len) in place (analysis/DNAVector.cc:482-521, the
+ * DNACodec map of :13-57); DNA_Equal(a, b) (analysis/DNAVector.cc:390-403). */
+int sxc_reverse_complement(char* seq, int32_t len);
+double sxc_dna_equal(char a, char b);
+
 /* CCSignal::SetSequence(seq, N) (analysis/CrossCorr.cc:179-206): out = 4*N floats, channels A, C, G, T. */
 int sxc_encode_signal(sxc_ctx* ctx, const char* seq, int32_t len, int32_t N, float* out);
 

@@ -31,6 +31,8 @@ SOURCES = ["sxc_kernels.hip", "sxc_kernels_r2.hip", "sxc_kernels_s32.hip", "sxc_
 PROGRAMS = {
     "HomologyByXCorrSlave": ["host/slave_main.cpp", "host/sxc_seqio.cpp"],
     "HomologyByXCorr": ["host/xcorr_main.cpp", "host/sxc_seqio.cpp"],
+    # a caller of the reference-compatible C++ classes (include/satsuma_compat/)
+    "compat_align": ["host/compat_align.cpp"],
 }
 CXX = os.environ.get("CXX", "g++")
 
@@ -65,7 +67,7 @@ def build(verbose: bool = True) -> str:
         print(f"built {LIB}")
     for name, srcs in PROGRAMS.items():
         exe = os.path.join(LIBDIR, name)
-        cmd = [CXX, "-O2", "-std=c++17", "-Wall", "-pthread", "-o", exe + ".tmp",
+        cmd = [CXX, "-O2", "-std=c++17", "-Wall", "-pthread", f"-I{os.path.join(os.path.dirname(PKG), 'include')}", "-o", exe + ".tmp",
                *[os.path.join(CSRC, s) for s in srcs], f"-L{LIBDIR}", "-lsxc_hip", "-Wl,-rpath,$ORIGIN"]
         r = subprocess.run(cmd, capture_output=True, text=True)
         if r.returncode != 0:

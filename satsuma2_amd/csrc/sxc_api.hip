@@ -714,6 +714,31 @@ int find_geometry(int nb, const sxc_geometry& want, int* geo) {
 extern "C" {
 
 const char* sxc_last_error(void) { return sxc::g_err.c_str(); }
+
+static const sxc_tables& host_tables() {
+  static const sxc_tables t = [] {
+    sxc_tables x;
+    sxc::build_tables(&x);
+    return x;
+  }();
+  return t;
+}
+
+int sxc_reverse_complement(char* seq, int32_t len) {
+  if (len < 0 || (len > 0 && !seq)) return fail(SXC_ERR_ARG, "null argument");
+  const sxc_tables& t = host_tables();
+  for (int32_t i = 0, j = len - 1; i < (len + 1) / 2; ++i, --j) {
+    const char a = (char)t.rc[(unsigned char)seq[i] & 127], b = (char)t.rc[(unsigned char)seq[j] & 127];
+    seq[i] = b;
+    seq[j] = a;
+  }
+  return SXC_OK;
+}
+
+double sxc_dna_equal(char a, char b) {
+  const sxc_tables& t = host_tables();
+  return t.equal[t.cls[(unsigned char)a & 127]][t.cls[(unsigned char)b & 127]];
+}
 const char* sxc_version(void) { return "sxc 0.1 gfx950"; }
 
 int sxc_create(sxc_ctx** out, int device) {

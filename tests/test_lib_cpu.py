@@ -28,6 +28,28 @@ def test_library_exports_every_declared_symbol():
     assert set(_lib.EXPORTS) <= set(names)
 
 
+def test_host_codec_helpers_match_oracle():
+    """sxc_reverse_complement / sxc_dna_equal (host tables, no device) against the oracle's
+    DNAVector::ReverseComplement and DNA_Equal over every byte."""
+    import oracle
+    from satsuma2_amd import _lib
+
+    L = _lib.load()
+    L.sxc_dna_equal.argtypes = [C.c_char, C.c_char]
+    L.sxc_dna_equal.restype = C.c_double
+    L.sxc_reverse_complement.argtypes = [C.c_char_p, C.c_int32]
+    ol = oracle.lib()
+    for a in range(128):
+        for b in range(128):
+            assert L.sxc_dna_equal(bytes([a]), bytes([b])) == ol.sxo_dna_equal(a, b), (a, b)
+    rng = np.random.default_rng(7)
+    for n in (0, 1, 2, 7, 100):
+        seq = bytes(rng.integers(0, 128, n, dtype=np.uint8)) if n != 100 else b"ACGTNRYKMSWBDHVX-acgt" * 4 + b"ACGT" * 4
+        buf = C.create_string_buffer(seq, max(1, len(seq)))
+        assert L.sxc_reverse_complement(buf, len(seq)) == 0
+        assert buf.raw[: len(seq)] == oracle.reverse_complement(seq), seq
+
+
 def test_abi_struct_layouts():
     from satsuma2_amd import _lib
 
