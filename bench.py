@@ -14,11 +14,17 @@ Serving (config of the serving loop): two synthetic 50-Mbp genomes sharing 2,000
 blocks, chunked as the slave chunks them (16,277 target chunks of 4096 bp every 3072 bp,
 12,208 query chunks), target spectra resident in HBM; battleship-shaped requests (seed
 lines of one target chunk x 256 query chunks, then the 24 x 24 matrix cells of hits) arrive
-as Poisson streams offering 25%, 50% and 75% of the batched throughput; each rate reports
-request latency p50/p99 and chunk pairs per second.  The headline p50/p99 is at 50%.
+as streams offering 25%, 50% and 75% of the batched throughput in chunk pairs per second
+(exponential gaps scaled by each request's pairs); each rate reports request latency p50/p99
+and chunk pairs per second.  The headline p50/p99 is at 50%.
 
 Uneven (config of the uneven batch): 20,000 pairs with target and query lengths log-uniform
 in [256, 16384] bp, each at its own FFT length (1024 ... 32768), in one resident batch.
+
+With --large (config of the large served run, not part of the default run): two 200-Mbp
+genomes with GC drawn per 100-kbp window, tandem repeats over 10% of each, five
+transposon-like families of 1,000 copies and 5,000 syntenic blocks, served at 25% and 50% of
+the batched rate.
 
 Printed: one JSON line with the batched throughput (`value`), the plain one-pair-per-call
 throughput beside it, the serving latencies, the dominant kernel's roofline figures from
@@ -52,6 +58,7 @@ B_PAIR = 263_168 + 458_752
 TRAFFIC_PER_PAIR = (271_555 * 2 + 4_916) * 1024 / 4096
 SERVE_LOADS = (0.25, 0.5, 0.75)  # offered chunk pairs/s as fractions of the batched rate; 0.5 is the headline
 SEED_LINE = 256                  # query chunks per seed line
+LARGE_LOADS = (0.25, 0.5)        # offered loads of the large served run
 
 
 def kernel_name(ctx, N: int = 8192) -> str:
@@ -105,13 +112,19 @@ def cpu_baseline(T, Q, sample: int, threads: int):
             "sample": f"{sample} pairs of the same workload, both strands, oracle restatement"}
 
 
-def serve_bench(ctx, genome_bp: int, duration: float, capacity: float, loads, seed: int = 4):
+def serve_bench(ctx, genome_bp: int, duration: float, capacity: float, loads, seed: int = 4, large: bool = False):
     from satsuma2_amd import synth
     from satsuma2_amd.api import Server, SlaveParams
     from satsuma2_amd.serve_load import Battleship, genome_chunks, run_stream
 
     print(f"[bench] serving: generating two {genome_bp / 1e6:.0f}-Mbp genomes", file=sys.stderr, flush=True)
-    T, Q, _ = synth.syntenic_genomes(length=genome_bp, n_blocks=max(1, genome_bp * 2000 // 50_000_000), seed=seed)
+    t0 = time.perf_counter()
+    if large:
+        T, Q, _ = synth.large_genomes(length=genome_bp, n_blocks=max(1, genome_bp * 5000 // 200_000_000), seed=seed)
+    else:
+        T, Q, _ = synth.syntenic_genomes(length=genome_bp, n_blocks=max(1, genome_bp * 2000 // 50_000_000),
+                                         seed=seed)
+    gen_s = time.perf_counter() - t0
     tch = genome_chunks(T, 4096, 3072)
     qch = genome_chunks(Q, 4096, 4096)
     t0 = time.perf_counter()
@@ -120,18 +133,21 @@ def serve_bench(ctx, genome_bp: int, duration: float, capacity: float, loads, se
     info = srv.stats()
     runs = []
     for k, load in enumerate(loads):
-        rate = load * capacity / SEED_LINE
-        print(f"[bench] serving: {rate:.0f} requests/s ({load:.0%} load) for {duration:.0f} s", file=sys.stderr,
-              flush=True)
+        print(f"[bench] serving: {load * capacity:.0f} chunk pairs/s offered ({load:.0%} of the batched rate) "
+              f"for {duration:.0f} s", file=sys.stderr, flush=True)
         prod = Battleship(len(tch), len(qch), np.random.default_rng(seed + k), line=SEED_LINE)
-        r = run_stream(srv, prod, rate, duration, seed=seed + k)
+        r = run_stream(srv, prod, 0.0, duration, seed=seed + k, pairs_rate=load * capacity)
         r["offered_load"] = load
         runs.append(r)
     info = srv.stats()
     srv.close()
     head = runs[min(1, len(runs) - 1)]
-    return {"workload": f"two {genome_bp / 1e6:.0f}-Mbp synthetic genomes, {len(tch)} target x {len(qch)} query "
-                        f"chunks, battleship requests (seed lines of {SEED_LINE} pairs, 24x24 cells of hits)",
+    kind = ("GC 0.38-0.45 per 100-kbp window, 10% tandem repeats, 5 transposon families x 1000 copies, "
+            "5,000 syntenic blocks" if large else "2,000 syntenic blocks")
+    return {"workload": f"two {genome_bp / 1e6:.0f}-Mbp synthetic genomes ({kind}), {len(tch)} target x "
+                        f"{len(qch)} query chunks, battleship requests (seed lines of {SEED_LINE} pairs, 24x24 "
+                        f"cells of hits)",
+            "generate_s": round(gen_s, 1),
             "p50_ms": head["p50_ms"], "p99_ms": head["p99_ms"], "at_load": head["offered_load"],
             "rates": runs, "create_s": round(create_s, 2), "resident_gb": round(info["resident_bytes"] / 1e9, 3),
             "slots": info["slots"], "max_pairs_per_launch": info["max_pairs_per_launch"],
@@ -179,6 +195,9 @@ def main():
     ap.add_argument("--serve-bp", type=int, default=50_000_000)
     ap.add_argument("--serve-seconds", type=float, default=4.0)
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--large", action="store_true",
+                    help="also serve the large whole-genome run (two 200-Mbp genomes with repeats; minutes)")
+    ap.add_argument("--large-bp", type=int, default=200_000_000)
     ap.add_argument("--no-autotune", action="store_true", help="default FFT kernel geometries (for profiling)")
     ap.add_argument("--uneven-pairs", type=int, default=20_000)
     ap.add_argument("--uneven-steps", type=int, default=5)
@@ -289,6 +308,8 @@ def main():
         out["serve_p50_ms"] = sv["p50_ms"]
         out["serve_p99_ms"] = sv["p99_ms"]
         out["serve"] = sv
+    if args.large:
+        out["serve_large"] = serve_bench(ctx, args.large_bp, args.serve_seconds, value, LARGE_LOADS, seed=5, large=True)
     if not args.no_cpu:
         threads = max(1, min(16, os.cpu_count() or 1))
         print(f"[bench] cpu baseline on {threads} threads", file=sys.stderr, flush=True)

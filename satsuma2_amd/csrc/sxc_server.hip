@@ -100,6 +100,11 @@ struct sxc_server {
   int nb = 0;
   int geo_corr = 0, geo_spec = 0;  // FFT geometry codes of the pair and spectra kernels
   int P = 2048;
+  // Adapted to the request stream by the dispatcher: the record room a launch starts with
+  // per (pair, strand) (raised to the largest count an overflowing launch needed), and the
+  // mean records per pair (sizes the prefix of records a launch copies back with its counts).
+  int cap_hint = kDefaultCap;
+  double records_per_pair = 0.;
   int cus = 256;  // compute units of the device
   DevBuf seq, tch, qch, spectra, qspectra;
   const double* ptable = nullptr;  // -prob_table thresholds (owned by the context)
@@ -220,7 +225,9 @@ int launch_scan_slot(sxc_server* sv, Slot& sl, int split, bool order) {
                                      sl.outs.as<sxc_result_dev>(), sl.keys.as<uint64_t>(),
                                      sl.final_.as<sxc_result_dev>(), sl.out_cap, sl.s));
     // records are few per pair (mostly none): a prefix sized by the launch usually holds all
-    sl.copied = std::min<int64_t>(std::min(sl.out_cap, sl.h_final_cap), std::max(512, sl.npairs));
+    const double want = 2.0 * sv->records_per_pair * sl.npairs;
+    sl.copied = std::min<int64_t>(std::min(sl.out_cap, sl.h_final_cap),
+                                  std::max<int64_t>(std::max(512, sl.npairs), (int64_t)want));
     HIPCHK(hipMemcpyAsync(sl.h_final, sl.final_.p, (size_t)sl.copied * sizeof(sxc_result), hipMemcpyDeviceToHost,
                           sl.s));
   }
@@ -233,7 +240,7 @@ int launch_scan_slot(sxc_server* sv, Slot& sl, int split, bool order) {
 int slot_fill(sxc_server* sv, Slot& sl) {
   sl.pieces.clear();
   sl.npairs = 0;
-  sl.cap = kDefaultCap;  // an earlier launch's overflow does not carry over
+  sl.cap = sv->cap_hint;
   while (sl.npairs < sv->P && !sv->pending.empty()) {
     Request* r = sv->pending.front();
     Piece pc{r, r->dispatched, 0, sl.npairs};
@@ -307,13 +314,15 @@ int slot_pack(sxc_server* sv, Slot& sl) {
   const int total = 2 * sl.npairs;
   bool order = false;
   if (*sl.h_maxc > sl.cap) {
+    // the same pass again with room for every record (the record buffers grow to total x
+    // cap), without waiting here: the slot comes back through COUNTING; later launches
+    // start with that much room (up to 1024 records per (pair, strand))
     sl.cap = *sl.h_maxc;
-    int rc = launch_scan_slot(sv, sl, 1, false);  // grows the record buffers to total x cap
-    if (rc) return rc;
-    HIPCHK(hipEventSynchronize(sl.ev_count));
-    order = true;
+    sv->cap_hint = std::min(1024, std::max(sv->cap_hint, sl.cap));
+    return launch_scan_slot(sv, sl, 1, true);
   }
   const int64_t n = sl.h_offsets[total];
+  sv->records_per_pair = 0.9 * sv->records_per_pair + 0.1 * (double)n / std::max(1, sl.npairs);
   if (n > sl.out_cap) {
     HIPCHK(sl.final_.ensure((size_t)n * sizeof(sxc_result_dev)));
     sl.out_cap = n;

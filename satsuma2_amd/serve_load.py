@@ -8,7 +8,8 @@ analysis/SatsumaSynteny2.cc:484-599) asks for two kinds of t_pair blocks:
     blocks of target x query chunks with the normal cutoff.
 `Battleship` produces both from a pair of chunked genomes: seeds at random until hits come
 back, then the 24 x 24-chunk matrix cell of every hit not asked for yet.
-`run_stream` offers requests as a Poisson stream at a given rate to a `Server` and reports
+`run_stream` offers requests as a Poisson stream (a rate of requests, or of chunk pairs with
+gaps scaled by each request's pairs) to a `Server` and reports
 per-request latency percentiles (submission to completion, measured by the server) and the
 chunk pairs per second completed.
 """
@@ -55,7 +56,8 @@ class Battleship:
         rev = recs["reverse"] != 0
         qs = np.where(rev, recs["query_size"].astype(np.int64) - qs - recs["len"].astype(np.int64), qs)
         qi = np.maximum(qs, 0) // self.q_step
-        cells = np.unique(np.stack([ti // self.cell, qi // self.cell], 1), axis=0)
+        keys = np.unique((ti // self.cell) * (1 << 32) + qi // self.cell)
+        cells = np.stack([keys >> 32, keys & 0xFFFFFFFF], 1)
         with self.lock:
             for cx, cy in cells.tolist():
                 if (cx, cy) in self.asked:
@@ -75,7 +77,12 @@ def _pairs(blocks: Sequence[Block]) -> int:
     return sum((b[1] - b[0] + 1) * (b[3] - b[2] + 1) for b in blocks)
 
 
-def run_stream(server, producer: Battleship, rate: float, duration: float, seed: int = 0) -> dict:
+def run_stream(server, producer: Battleship, rate: float, duration: float, seed: int = 0,
+               pairs_rate: float = 0.0) -> dict:
+    """Offers requests for `duration` seconds: a Poisson stream of `rate` requests/s, or, with
+    pairs_rate > 0, requests spaced by exponential gaps whose mean is the request's chunk
+    pairs / pairs_rate (the offered chunk pairs per second, whatever mix of seeds and cells
+    the search produces)."""
     rng = np.random.default_rng(seed)
     done: collections.deque = collections.deque()
     tickets: collections.deque = collections.deque()
@@ -100,17 +107,20 @@ def run_stream(server, producer: Battleship, rate: float, duration: float, seed:
     t0 = time.perf_counter()
     t_next = t0
     submitted = 0
+    submitted_pairs = 0
     while True:
-        t_next += float(rng.exponential(1.0 / rate))
+        blocks = producer.next()
+        mean_gap = _pairs(blocks) / pairs_rate if pairs_rate > 0 else 1.0 / rate
+        t_next += float(rng.exponential(mean_gap))
         if t_next - t0 > duration:
             break
         delay = t_next - time.perf_counter()
         if delay > 0:
             time.sleep(delay)
-        blocks = producer.next()
         t_sub = time.perf_counter()
         ticket = server.submit(blocks)
         submitted += 1
+        submitted_pairs += _pairs(blocks)
         with cv:
             tickets.append((ticket, t_sub, _pairs(blocks)))
             cv.notify()
@@ -121,7 +131,9 @@ def run_stream(server, producer: Battleship, rate: float, duration: float, seed:
     lat = np.array([d[1] for d in done])
     end = max(d[0] - t0 + d[1] / 1e3 for d in done) if done else duration
     pairs = sum(d[2] for d in done)
-    return {"offered_requests_per_s": rate, "requests": submitted, "seconds": round(end, 3),
+    return {"offered_requests_per_s": round(submitted / duration, 1),
+            "offered_pairs_per_s": round(submitted_pairs / duration, 1), "requests": submitted,
+            "seconds": round(end, 3),
             "pairs_per_s": round(pairs / end, 1), "mean_pairs_per_request": round(pairs / max(1, len(done)), 1),
             "p50_ms": round(float(np.percentile(lat, 50)), 3) if len(lat) else None,
             "p99_ms": round(float(np.percentile(lat, 99)), 3) if len(lat) else None,

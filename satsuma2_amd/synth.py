@@ -192,3 +192,51 @@ def syntenic_genomes(length: int = 50_000_000, n_blocks: int = 2000, seed: int =
         q[qs:qs + seg.size] = seg
         blocks.append((ts, qs, int(seg.size), inv))
     return t.tobytes(), q.tobytes(), blocks
+
+
+def large_genomes(length: int = 200_000_000, n_blocks: int = 5000, seed: int = 5, gc_window: int = 100_000,
+                  repeat_frac: float = 0.10, n_families: int = 5, copies: int = 1000):
+    """Two genomes for the large served run: GC uniform in [0.38, 0.45] per 100-kbp window;
+    syntenic blocks as in syntenic_genomes (lengths log-uniform 5-200 kbp, identity 75-95%,
+    1% indels, 10% inverted); then, in each genome independently, tandem repeats (period
+    1-6, length 100-1000 bp) until they cover repeat_frac of it, and n_families
+    transposon-like families (300-6000 bp) of `copies` copies each at 80-95% identity.
+    Returns (target bytes, query bytes, blocks)."""
+    rng = np.random.default_rng(seed)
+
+    def base():
+        g = np.empty(length, np.uint8)
+        for w in range(0, length, gc_window):
+            n = min(gc_window, length - w)
+            g[w:w + n] = random_seq_fast(rng, n, float(rng.uniform(0.38, 0.45)))
+        return g
+
+    t, q = base(), base()
+    lens = np.exp(rng.uniform(np.log(5_000), np.log(200_000), n_blocks)).astype(np.int64)
+    blocks = []
+    for k in range(n_blocks):
+        ln = int(lens[k])
+        ts = int(rng.integers(0, length - ln))
+        qs = int(min(max(ts + rng.normal(0, 4e6), 0), length - ln))
+        seg = mutate_fast(rng, t[ts:ts + ln], float(rng.uniform(0.75, 0.95)), 0.01)[:ln]
+        inv = bool(rng.random() < 0.1)
+        if inv:
+            seg = revcomp(seg)
+        q[qs:qs + seg.size] = seg
+        blocks.append((ts, qs, int(seg.size), inv))
+    fams = [random_seq_fast(rng, int(rng.integers(300, 6001)), 0.42) for _ in range(n_families)]
+    for g in (t, q):
+        covered = 0
+        while covered < repeat_frac * length:
+            period, ln = int(rng.integers(1, 7)), int(rng.integers(100, 1001))
+            at = int(rng.integers(0, length - ln))
+            g[at:at + ln] = np.resize(ALPHA[rng.integers(0, 4, period)], ln)
+            covered += ln
+        for fam in fams:
+            for _ in range(copies):
+                cp = mutate_fast(rng, fam, float(rng.uniform(0.80, 0.95)), 0.01)
+                if rng.random() < 0.5:
+                    cp = revcomp(cp)
+                at = int(rng.integers(0, length - cp.size))
+                g[at:at + cp.size] = cp
+    return t.tobytes(), q.tobytes(), blocks
