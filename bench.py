@@ -176,11 +176,27 @@ def uneven_bench(ctx, n_pairs: int, steps: int, seed: int = 3):
     wall = (time.perf_counter() - t0) / steps
     n_res = len(rb.results())
     rb.close()
+    # each length class alone, launched kernel by kernel (device ms per phase; the batch runs
+    # the classes side by side)
+    per_class = {}
+    tot = float(sum(len(t) for t in T))
+    for N in sorted(classes):
+        idx = [r for r in range(n_pairs) if synth.fft_len(len(Q[r]), len(T[r])) == N]
+        cb = ResidentBatch(ctx, [pairs[r] for r in idx], SlaveParams(t_chunk=0, target_total=tot))
+        cb.set_graph(False)
+        cb.run()
+        t1 = time.perf_counter()
+        cb.run()
+        tm = ctx.timing()
+        per_class[str(N)] = {"pairs": len(idx), "ms": round(1e3 * (time.perf_counter() - t1), 3),
+                             "spectra_ms": round(tm.spectra_ms, 3), "correlate_ms": round(tm.correlate_ms, 3),
+                             "scan_ms": round(tm.scan_ms, 3), "pair_kernel": kernel_name(ctx, N)}
+        cb.close()
     return {"workload": f"uneven_pairs: {n_pairs} pairs, q and t log-uniform in [256, 16384] bp, GC 0.35-0.55, "
                         "FFT length per pair, both strands",
             "pairs_per_s": round(n_pairs / wall, 1), "ms_per_step": round(1e3 * wall, 3), "steps": steps,
             "fft_len_classes": {str(k): v for k, v in sorted(classes.items())},
-            "bp": int(sum(len(t) + len(q) for t, q in zip(T, Q))), "results": n_res}
+            "bp": int(sum(len(t) + len(q) for t, q in zip(T, Q))), "results": n_res, "classes_alone": per_class}
 
 
 def main():
