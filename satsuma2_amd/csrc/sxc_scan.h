@@ -164,6 +164,18 @@ SXC_HD inline uint64_t match_block(const uint32_t* tpk, const uint32_t* qpk, int
 
 SXC_HD inline int popc64(uint64_t v) { return __builtin_popcountll(v); }
 
+SXC_HD inline uint32_t rev32(uint32_t x) {
+#if defined(__clang__)
+  return __builtin_bitreverse32(x);
+#else
+  x = ((x >> 1) & 0x55555555u) | ((x & 0x55555555u) << 1);
+  x = ((x >> 2) & 0x33333333u) | ((x & 0x33333333u) << 2);
+  x = ((x >> 4) & 0x0f0f0f0fu) | ((x & 0x0f0f0f0fu) << 4);
+  x = ((x >> 8) & 0x00ff00ffu) | ((x & 0x00ff00ffu) << 8);
+  return (x >> 16) | (x << 16);
+#endif
+}
+
 // matches in the window [n - 45, n], n = 32k + r
 SXC_HD inline int window_count(uint64_t e0, uint64_t e1, uint64_t e2, int r) {
   const uint64_t low = (r == 31) ? ~0ull : ((1ull << (2 * r + 2)) - 1);
@@ -299,12 +311,21 @@ SXC_HD inline void scan_epoch(Ex& ex, char* sm, const uint16_t* cands, int c0, i
           const uint64_t e0 = match_block(tpk, qpk, ti, qj, k);
           // m(n - 46) for the block: offsets r < 14 come from block k-2, r >= 14 from block k-1
           const uint64_t minus = (e2 >> 36) | (e1 << 28);
-          uint32_t f = 0;
+          // 2-bit group r of z holds m(n) + 1 - m(n - 46), in {0, 1, 2}: no carry crosses a
+          // group, so each 32-bit half is one add.  D = C - 19 is negative exactly where the
+          // window is below the threshold; its sign bits shift into g, position 0 first.
+          const uint32_t zl = (uint32_t)e0 + (~(uint32_t)minus & 0x55555555u);
+          const uint32_t zh = (uint32_t)(e0 >> 32) + (~(uint32_t)(minus >> 32) & 0x55555555u);
+          int D = C - SXC_MIN_MATCHES;
+          uint32_t g = 0;
           static_for<0, 32>([&](auto R) {
             constexpr int r = decltype(R)::value;
-            C += (int)((e0 >> (2 * r)) & 1) - (int)((minus >> (2 * r)) & 1);
-            f |= (C >= SXC_MIN_MATCHES ? 1u : 0u) << r;
+            const uint32_t zr = (r < 16) ? (zl >> (2 * r)) & 3u : (zh >> (2 * (r - 16))) & 3u;
+            D += (int)zr - 1;
+            g = (g << 1) | ((uint32_t)D >> 31);
           });
+          C = D + SXC_MIN_MATCHES;
+          uint32_t f = ~rev32(g);
           if (32 * k < 46) f &= (46 - 32 * k >= 32) ? 0u : ~((1u << (46 - 32 * k)) - 1);  // flags from n = 46
           const int r0 = (k == k0) ? na - 32 * k : 0;
           const int r1 = (k == klast) ? nb - 32 * k : 32;
