@@ -54,7 +54,6 @@ struct ScanArgs {
 };
 
 constexpr int SCAN_NT = 256;  // threads per scan workgroup
-constexpr int SCAN_GSZ = 16;  // threads per lag in the batched phase
 // smallest count c of identical A/C/G/T bases with 100 * c > 1889 (19 of 46)
 constexpr int SXC_MIN_MATCHES = SXC_WIN_THRESHOLD / 100 + 1;
 static_assert(100 * SXC_MIN_MATCHES > SXC_WIN_THRESHOLD && 100 * (SXC_MIN_MATCHES - 1) <= SXC_WIN_THRESHOLD, "");
@@ -74,28 +73,37 @@ struct ScanSmem {
   static constexpr int QCODE = TCODE + (PURE ? 0 : N);  // uchar[N] (general kernel)
   static constexpr int TPK = QCODE + (PURE ? 0 : N);    // uint32[PKW]: 16 bases per word (pure A/C/G/T only)
   static constexpr int QPK = TPK + 4 * PKW;             // uint32[PKW]
-  // From N = 16384 the queue takes one epoch of 8 lags at a time (drained after every
-  // epoch) in 4-byte entries {lag - first lag of the epoch (3 bits), ns (14), ne (15)}, which
-  // halves the largest part of the plan; below, 16-lag epochs accumulate in 8-byte entries
-  // {lag rank, ns | ne << 16} until the queue is half full.
-  static constexpr bool COMPACT = LOG >= 14;
-  static constexpr int GSZ = COMPACT ? 32 : SCAN_GSZ;   // threads per lag in an epoch
-  static constexpr int CE = SCAN_NT / GSZ;              // lags per epoch
+  // The pure kernel up to N = 8192 (BMRUNS) keeps each epoch's flags in a bitmap of WPL
+  // words per lag and turns its rising edges into runs afterwards; the other kernels and
+  // lengths scan contiguous segments per thread and stitch runs across them.  BMRUNS, and
+  // every kernel from N = 16384, take one epoch at a time (drained after every epoch) in
+  // 4-byte queue entries {lag - first lag of the epoch (RB bits), ns (NSB = LOG - 1), ne
+  // (LOG)}; otherwise 16-lag epochs accumulate in 8-byte entries {lag rank, ns | ne << 16}
+  // until the queue is half full.
+  static constexpr bool BMRUNS = PURE && LOG <= 13;
+  static constexpr bool COMPACT = BMRUNS || LOG >= 14;
+  static constexpr int CE = (COMPACT && !BMRUNS) ? 8 : 16;  // lags per epoch
+  static constexpr int GSZ = SCAN_NT / CE;              // threads per lag in an epoch
+  static constexpr int NSB = LOG - 1;                   // ns < N / 2
+  static constexpr int RB = 32 - NSB - LOG;             // ne <= N / 2 takes LOG bits
+  static_assert(!COMPACT || (RB > 0 && (1 << RB) >= CE), "compact queue entries");
   static constexpr int QE = COMPACT ? 4 : 8;            // bytes per queue entry
-  static_assert(!COMPACT || (CE <= 8 && LOG <= 15), "compact queue entries: 3 + 14 + 15 bits");
+  static constexpr int WPL = N / 64;                    // bitmap words per lag: npos <= N / 2
   static constexpr int QUEUE = QPK + 4 * PKW;           // queue entries [QCAP]
-  static constexpr int SUMM = QUEUE + QE * QCAP;        // int2[SCAN_NT]
+  static constexpr int SUMM = QUEUE + QE * QCAP;        // int2[SCAN_NT] (segments)
+  static constexpr int BITMAP = SUMM;                   // uint32[CE][WPL] (BMRUNS)
+  static constexpr int RLIST = BITMAP + 4 * CE * WPL;   // uint16[CE * WPL]: blocks with a rising edge (BMRUNS)
+  static constexpr int EQUAL = SUMM + (BMRUNS ? 6 * CE * WPL : 8 * SCAN_NT);  // double[18][18]
   static constexpr int NTAB = PURE ? 0 : 1;             // the score tables (general kernel)
-  static constexpr int EQUAL = SUMM + 8 * SCAN_NT;      // double[18][18]
   static constexpr int GC = EQUAL + NTAB * 8 * SXC_NCLASS * SXC_NCLASS;  // double[18]
   static constexpr int SCORE = GC + NTAB * 8 * SXC_NCLASS;              // uchar[18][18]
   static constexpr int CLS = SCORE + NTAB * SXC_NCLASS * SXC_NCLASS;     // uchar[128]
   static constexpr int RC = CLS + NTAB * 128;                           // uchar[128]
-  static constexpr int MISC = (RC + NTAB * 128 + 15) & ~15;             // int[16]
-  static constexpr int BYTES = MISC + 64;
+  static constexpr int MISC = (RC + NTAB * 128 + 15) & ~15;             // int[16 + CE]
+  static constexpr int BYTES = MISC + 64 + 4 * CE;
 };
 
-enum { SM_QN = 0, SM_KEPT = 1, SM_RUNS = 2, SM_NCAND = 3, SM_LT = 4, SM_LQ = 5, SM_FAST = 6 };
+enum { SM_QN = 0, SM_KEPT = 1, SM_RUNS = 2, SM_NCAND = 3, SM_LT = 4, SM_LQ = 5, SM_FAST = 6, SM_NRB = 7, SM_NPOS = 16 };
 
 // Base code: class id (0..16) of a codec letter, or 128 | byte for a byte whose codec
 // row is zero.  Two bases are identical bytes iff their codes are equal.
@@ -259,9 +267,136 @@ SXC_HD inline Lag lag_geometry(int idx, int H, int Lq, int Lt) {
   return g;
 }
 
-// Scans lags [c0, c1) with gsz threads per lag; runs are pushed to the LDS queue.
+// Queue entry of a run (both kernels): compact {lag - first lag of the epoch, ns, ne}
+// or the 8-byte {lag rank, ns | ne << 16}.
 template <int NB, class Ex>
-SXC_HD inline void scan_epoch(Ex& ex, char* sm, const uint16_t* cands, int c0, int c1, int gsz) {
+SXC_HD inline void push_run(Ex& ex, char* sm, int c0, int rank, int ns, int ne) {
+  using SS = ScanSmem<NB>;
+  int* misc = (int*)(sm + SS::MISC);
+  int* queue = (int*)(sm + SS::QUEUE);
+  const int k = ex.atomic_add_shared(&misc[SM_QN], 1);
+  if (k < SS::QCAP) {
+    if constexpr (SS::COMPACT) {
+      queue[k] = (int)((unsigned)(rank - c0) | ((unsigned)ns << SS::RB) | ((unsigned)ne << (SS::RB + SS::NSB)));
+    } else {
+      queue[2 * k] = rank;
+      queue[2 * k + 1] = ns | (ne << 16);
+    }
+  }
+}
+
+// Pure A/C/G/T lags [c0, c1), gsz threads per lag.  Score = 100 * [match], and a window
+// sum > 1889 <=> >= 19 matches.  C(n) = C(n - 1) + m(n) - m(n - 46) with m(< 0) = 0 equals
+// the reference's running `match` (analysis/CrossCorr.cc:700-705).  Each thread takes whole
+// 32-position blocks of one lag, updates every position branch-free and stores the block's
+// flags (window above the threshold, positions >= 46 and < npos) in the epoch's LDS
+// bitmap.  A second pass over the bitmap words, spread over all threads, turns every rising
+// edge at ns and the first clear flag after it at ne (or npos, where the loop ends) into
+// the run [ns, ne) -- the runs DoOne opens and closes.
+template <int NB, class Ex>
+SXC_HD inline void scan_epoch_bitmap(Ex& ex, char* sm, const uint16_t* cands, int c0, int c1, int gsz) {
+  using SS = ScanSmem<NB>;
+  constexpr int H = SS::N / 2;
+  constexpr int WPL = SS::WPL;
+  int* misc = (int*)(sm + SS::MISC);
+  int* npos_of = misc + SM_NPOS;
+  uint32_t* bm = (uint32_t*)(sm + SS::BITMAP);
+  const uint32_t* tpk = (const uint32_t*)(sm + SS::TPK);
+  const uint32_t* qpk = (const uint32_t*)(sm + SS::QPK);
+  const int Lt = misc[SM_LT], Lq = misc[SM_LQ];
+  const int nl = c1 - c0;
+  ex.each([&](int tid) {
+    const int j = tid / gsz;
+    const int l = tid % gsz;
+    if (j >= nl) return;
+    const Lag g = lag_geometry(cands[c0 + j], H, Lq, Lt);
+    if (l == 0) npos_of[j] = g.npos;
+    // whole blocks per thread: no block is written by two threads
+    const int S = ((g.npos + gsz - 1) / gsz + 31) & ~31;
+    const int na = l * S;
+    const int nb = (na + S < g.npos) ? na + S : g.npos;
+    if (na >= nb) return;
+    uint32_t* row = bm + j * WPL;
+    const int ti = g.i0, qj = g.i0 + g.shift;
+    const int k0 = na >> 5, klast = (nb - 1) >> 5;
+    uint64_t e2 = match_block(tpk, qpk, ti, qj, k0 - 2);
+    uint64_t e1 = match_block(tpk, qpk, ti, qj, k0 - 1);
+    int C = window_count(e1, e2, match_block(tpk, qpk, ti, qj, k0 - 3), 31);  // C at 32 k0 - 1
+    uint32_t prev = (na - 1 >= 46 && C >= SXC_MIN_MATCHES) ? 1u : 0u;    // flag at 32 k0 - 1
+    uint16_t* rlist = (uint16_t*)(sm + SS::RLIST);
+    for (int k = k0; k <= klast; ++k) {
+      const uint64_t e0 = match_block(tpk, qpk, ti, qj, k);
+      // m(n - 46) for the block: offsets r < 14 come from block k-2, r >= 14 from block k-1
+      const uint64_t minus = (e2 >> 36) | (e1 << 28);
+      // 2-bit group r of z holds m(n) + 1 - m(n - 46), in {0, 1, 2}: no carry crosses a
+      // group, so each 32-bit half is one add.  D = C - 19 is negative exactly where the
+      // window is below the threshold; its sign bits shift into g, position 0 first.
+      const uint32_t zl = (uint32_t)e0 + (~(uint32_t)minus & 0x55555555u);
+      const uint32_t zh = (uint32_t)(e0 >> 32) + (~(uint32_t)(minus >> 32) & 0x55555555u);
+      int D = C - SXC_MIN_MATCHES;
+      uint32_t gbits = 0;
+      static_for<0, 32>([&](auto R) {
+        constexpr int r = decltype(R)::value;
+        const uint32_t zr = (r < 16) ? (zl >> (2 * r)) & 3u : (zh >> (2 * (r - 16))) & 3u;
+        D += (int)zr - 1;
+        gbits = (gbits << 1) | ((uint32_t)D >> 31);
+      });
+      C = D + SXC_MIN_MATCHES;
+      uint32_t f = ~rev32(gbits);
+      if (32 * k < 46) f &= (46 - 32 * k >= 32) ? 0u : ~((1u << (46 - 32 * k)) - 1);  // flags from n = 46
+      if (k == klast && nb - 32 * k < 32) f &= (1u << (nb - 32 * k)) - 1;
+      row[k] = f;
+      if (f & ~((f << 1) | prev)) rlist[ex.atomic_add_shared(&misc[SM_NRB], 1)] = (uint16_t)(j * WPL + k);
+      prev = f >> 31;
+      e2 = e1;
+      e1 = e0;
+    }
+  });
+  ex.sync();
+  int* queue = (int*)(sm + SS::QUEUE);
+  const uint16_t* rlist = (const uint16_t*)(sm + SS::RLIST);
+  ex.each([&](int tid) {
+    const int nrb = misc[SM_NRB];
+    for (int x = tid; x < nrb; x += SCAN_NT) {
+      const int e = rlist[x];
+      const int j = e / WPL, w = e % WPL;
+      const int npos = npos_of[j];
+      const int W = (npos + 31) >> 5;
+      const uint32_t* row = bm + j * WPL;
+      const uint32_t f = row[w];
+      const uint32_t carry = w > 0 ? row[w - 1] >> 31 : 0u;
+      uint32_t rises = f & ~((f << 1) | carry);
+      int at = ex.atomic_add_shared(&misc[SM_QN], __builtin_popcount(rises));
+      while (rises) {
+        const int r = __builtin_ctz(rises);
+        rises &= rises - 1;
+        const uint32_t clear = ~f & (~0u << r);  // bit r is set: the first clear bit is above it
+        int ne;
+        if (clear) {
+          ne = 32 * w + __builtin_ctz(clear);
+        } else {
+          int w2 = w + 1;
+          while (w2 < W && row[w2] == ~0u) ++w2;
+          ne = w2 < W ? 32 * w2 + __builtin_ctz(~row[w2]) : npos;
+        }
+        if (at < SS::QCAP)
+          queue[at] = (int)((unsigned)j | ((unsigned)(32 * w + r) << SS::RB) | ((unsigned)ne << (SS::RB + SS::NSB)));
+        ++at;
+      }
+    }
+  });
+  ex.sync();
+  ex.each([&](int tid) {
+    if (tid == 0) misc[SM_NRB] = 0;
+  });
+  ex.sync();
+}
+
+// Lags [c0, c1) with gsz threads per lag, each thread scanning a contiguous segment: it
+// recomputes the window at its segment start, records the runs that open and close inside
+// its segment, and a second phase closes runs that cross segment boundaries.
+template <int NB, class Ex>
+SXC_HD inline void scan_epoch_segments(Ex& ex, char* sm, const uint16_t* cands, int c0, int c1, int gsz) {
   using SS = ScanSmem<NB>;
   constexpr int INF = 0x7fffffff;
   constexpr int H = SS::N / 2;
@@ -269,28 +404,18 @@ SXC_HD inline void scan_epoch(Ex& ex, char* sm, const uint16_t* cands, int c0, i
   const unsigned char* qc = (const unsigned char*)(sm + SS::QCODE);
   const unsigned char* score = (const unsigned char*)(sm + SS::SCORE);
   int* misc = (int*)(sm + SS::MISC);
-  int* queue = (int*)(sm + SS::QUEUE);
   int* summ = (int*)(sm + SS::SUMM);
   const int Lt = misc[SM_LT], Lq = misc[SM_LQ];
   const bool fast = misc[SM_FAST] != 0;
-  auto push = [&](int rank, int ns, int ne) {
-    const int k = ex.atomic_add_shared(&misc[SM_QN], 1);
-    if (k < SS::QCAP) {
-      if constexpr (SS::COMPACT) {
-        queue[k] = (int)((unsigned)(rank - c0) | ((unsigned)ns << 3) | ((unsigned)ne << 17));
-      } else {
-        queue[2 * k] = rank;
-        queue[2 * k + 1] = ns | (ne << 16);
-      }
-    }
-  };
+  auto push = [&](int rank, int ns, int ne) { push_run<NB>(ex, sm, c0, rank, ns, ne); };
   ex.each([&](int tid) {
     const int c = c0 + tid / gsz;
     const int l = tid % gsz;
     int first_end = INF, open = -1;
     if (c < c1) {
       const Lag g = lag_geometry(cands[c], H, Lq, Lt);
-      const int S = (g.npos + gsz - 1) / gsz;
+      // pure A/C/G/T segments start on 32-position blocks: no block is scanned by two threads
+      const int S = fast ? (((g.npos + gsz - 1) / gsz + 31) & ~31) : (g.npos + gsz - 1) / gsz;
       const int na = l * S;
       const int nb = (na + S < g.npos) ? na + S : g.npos;
       if (na < nb && fast) {
@@ -401,6 +526,16 @@ SXC_HD inline void scan_epoch(Ex& ex, char* sm, const uint16_t* cands, int c0, i
   ex.sync();
 }
 
+// Scans lags [c0, c1) with gsz threads per lag; runs are pushed to the LDS queue.
+template <int NB, class Ex>
+SXC_HD inline void scan_epoch(Ex& ex, char* sm, const uint16_t* cands, int c0, int c1, int gsz) {
+  if constexpr (ScanSmem<NB>::BMRUNS) {
+    scan_epoch_bitmap<NB>(ex, sm, cands, c0, c1, gsz);
+  } else {
+    scan_epoch_segments<NB>(ex, sm, cands, c0, c1, gsz);
+  }
+}
+
 // Drains queue entries [0, n): mode 0 filters into t_results, mode 1 writes runs.
 // rank_base: the first lag of the epoch the entries come from (compact queue).
 template <int NB, class Ex>
@@ -421,9 +556,9 @@ SXC_HD inline void drain(Ex& ex, char* sm, const ScanArgs& a, const sxc_pair_dev
       int rank, ns, ne;
       if constexpr (SS::COMPACT) {
         const unsigned v = (unsigned)queue[k];
-        rank = rank_base + (int)(v & 7u);
-        ns = (int)((v >> 3) & 0x3fffu);
-        ne = (int)(v >> 17);
+        rank = rank_base + (int)(v & ((1u << SS::RB) - 1));
+        ns = (int)((v >> SS::RB) & ((1u << SS::NSB) - 1));
+        ne = (int)(v >> (SS::RB + SS::NSB));
       } else {
         rank = queue[2 * k];
         ns = queue[2 * k + 1] & 0xffff;
@@ -544,6 +679,7 @@ SXC_HD inline void scan_body(Ex& ex, char* sm, const ScanArgs& a, int blk) {
     }
     if (tid == 0) {
       misc[SM_QN] = 0;
+      misc[SM_NRB] = 0;
       misc[SM_KEPT] = 0;
       misc[SM_RUNS] = 0;
       misc[SM_NCAND] = a.ncand[ps];
