@@ -67,22 +67,24 @@ struct ScanSmem {
   static constexpr int LOG = NB & 31;
   static constexpr bool PURE = (NB & 32) != 0;
   static constexpr int N = 1 << LOG;
-  static constexpr int QCAP = N / 4 + 64;  // runs of one lag: npos / 2 + 1 with npos <= N / 2
+  // The segment scans' queue holds every run one lag can produce (npos / 2 + 1, npos <= N / 2);
+  // the bitmap scan flushes its queue whenever it would overflow.
+  static constexpr int QCAP = (PURE ? 1024 : N / 4 + 64);
   static constexpr int PKW = N / 16 + 4;                // 2-bit packed words (+3 words of read-ahead)
   static constexpr int TCODE = 0;                       // uchar[N] (general kernel)
   static constexpr int QCODE = TCODE + (PURE ? 0 : N);  // uchar[N] (general kernel)
   static constexpr int TPK = QCODE + (PURE ? 0 : N);    // uint32[PKW]: 16 bases per word (pure A/C/G/T only)
   static constexpr int QPK = TPK + 4 * PKW;             // uint32[PKW]
-  // The pure kernel up to N = 8192 (BMRUNS) keeps each epoch's flags in a bitmap of WPL
-  // words per lag and turns its rising edges into runs afterwards; the other kernels and
-  // lengths scan contiguous segments per thread and stitch runs across them.  BMRUNS, and
-  // every kernel from N = 16384, take one epoch at a time (drained after every epoch) in
+  // The pure kernel (BMRUNS) keeps each epoch's flags in a bitmap of WPL words per lag and
+  // turns its rising edges into runs afterwards; the general kernel scans contiguous
+  // segments per thread and stitches runs across them.  BMRUNS, and the general kernel from
+  // N = 16384, take one epoch at a time (drained after every epoch) in
   // 4-byte queue entries {lag - first lag of the epoch (RB bits), ns (NSB = LOG - 1), ne
   // (LOG)}; otherwise 16-lag epochs accumulate in 8-byte entries {lag rank, ns | ne << 16}
   // until the queue is half full.
-  static constexpr bool BMRUNS = PURE && LOG <= 13;
+  static constexpr bool BMRUNS = PURE;
   static constexpr bool COMPACT = BMRUNS || LOG >= 14;
-  static constexpr int CE = (COMPACT && !BMRUNS) ? 8 : 16;  // lags per epoch
+  static constexpr int CE = LOG >= 14 ? 8 : 16;         // lags per epoch
   static constexpr int GSZ = SCAN_NT / CE;              // threads per lag in an epoch
   static constexpr int NSB = LOG - 1;                   // ns < N / 2
   static constexpr int RB = 32 - NSB - LOG;             // ne <= N / 2 takes LOG bits
@@ -293,8 +295,8 @@ SXC_HD inline void push_run(Ex& ex, char* sm, int c0, int rank, int ns, int ne) 
 // bitmap.  A second pass over the bitmap words, spread over all threads, turns every rising
 // edge at ns and the first clear flag after it at ne (or npos, where the loop ends) into
 // the run [ns, ne) -- the runs DoOne opens and closes.
-template <int NB, class Ex>
-SXC_HD inline void scan_epoch_bitmap(Ex& ex, char* sm, const uint16_t* cands, int c0, int c1, int gsz) {
+template <int NB, class Ex, class Flush>
+SXC_HD inline void scan_epoch_bitmap(Ex& ex, char* sm, const uint16_t* cands, int c0, int c1, int gsz, Flush&& flush) {
   using SS = ScanSmem<NB>;
   constexpr int H = SS::N / 2;
   constexpr int WPL = SS::WPL;
@@ -346,6 +348,7 @@ SXC_HD inline void scan_epoch_bitmap(Ex& ex, char* sm, const uint16_t* cands, in
       if (32 * k < 46) f &= (46 - 32 * k >= 32) ? 0u : ~((1u << (46 - 32 * k)) - 1);  // flags from n = 46
       if (k == klast && nb - 32 * k < 32) f &= (1u << (nb - 32 * k)) - 1;
       row[k] = f;
+      // (the same increment on every lane: one atomic per wave)
       if (f & ~((f << 1) | prev)) rlist[ex.atomic_add_shared(&misc[SM_NRB], 1)] = (uint16_t)(j * WPL + k);
       prev = f >> 31;
       e2 = e1;
@@ -355,37 +358,55 @@ SXC_HD inline void scan_epoch_bitmap(Ex& ex, char* sm, const uint16_t* cands, in
   ex.sync();
   int* queue = (int*)(sm + SS::QUEUE);
   const uint16_t* rlist = (const uint16_t*)(sm + SS::RLIST);
-  ex.each([&](int tid) {
-    const int nrb = misc[SM_NRB];
-    for (int x = tid; x < nrb; x += SCAN_NT) {
-      const int e = rlist[x];
-      const int j = e / WPL, w = e % WPL;
-      const int npos = npos_of[j];
-      const int W = (npos + 31) >> 5;
-      const uint32_t* row = bm + j * WPL;
-      const uint32_t f = row[w];
-      const uint32_t carry = w > 0 ? row[w - 1] >> 31 : 0u;
-      uint32_t rises = f & ~((f << 1) | carry);
-      int at = ex.atomic_add_shared(&misc[SM_QN], __builtin_popcount(rises));
-      while (rises) {
-        const int r = __builtin_ctz(rises);
-        rises &= rises - 1;
-        const uint32_t clear = ~f & (~0u << r);  // bit r is set: the first clear bit is above it
-        int ne;
-        if (clear) {
-          ne = 32 * w + __builtin_ctz(clear);
-        } else {
-          int w2 = w + 1;
-          while (w2 < W && row[w2] == ~0u) ++w2;
-          ne = w2 < W ? 32 * w2 + __builtin_ctz(~row[w2]) : npos;
+  // All listed blocks at once; if their runs overflowed the queue, again in rounds of blocks
+  // that fit it (a word holds at most 16 rising edges), the queue flushed after each round.
+  const int nrb = misc[SM_NRB];
+  auto extract = [&](int x0, int x1) {
+    ex.each([&](int tid) {
+      for (int x = x0 + tid; x < x1; x += SCAN_NT) {
+        const int e = rlist[x];
+        const int j = e / WPL, w = e % WPL;
+        const int npos = npos_of[j];
+        const int W = (npos + 31) >> 5;
+        const uint32_t* row = bm + j * WPL;
+        const uint32_t f = row[w];
+        const uint32_t carry = w > 0 ? row[w - 1] >> 31 : 0u;
+        uint32_t rises = f & ~((f << 1) | carry);
+        while (rises) {
+          const int r = __builtin_ctz(rises);
+          rises &= rises - 1;
+          const uint32_t clear = ~f & (~0u << r);  // bit r is set: the first clear bit is above it
+          int ne;
+          if (clear) {
+            ne = 32 * w + __builtin_ctz(clear);
+          } else {
+            int w2 = w + 1;
+            while (w2 < W && row[w2] == ~0u) ++w2;
+            ne = w2 < W ? 32 * w2 + __builtin_ctz(~row[w2]) : npos;
+          }
+          const int at = ex.atomic_add_shared(&misc[SM_QN], 1);  // one atomic per wave
+          if (at < SS::QCAP)
+            queue[at] = (int)((unsigned)j | ((unsigned)(32 * w + r) << SS::RB) | ((unsigned)ne << (SS::RB + SS::NSB)));
         }
-        if (at < SS::QCAP)
-          queue[at] = (int)((unsigned)j | ((unsigned)(32 * w + r) << SS::RB) | ((unsigned)ne << (SS::RB + SS::NSB)));
-        ++at;
       }
+    });
+    ex.sync();
+  };
+  extract(0, nrb);
+  const int qn = misc[SM_QN];
+  if (qn <= SS::QCAP) {
+    flush(qn);
+  } else {
+    ex.sync();  // every thread has read the count
+    ex.each([&](int tid) {
+      if (tid == 0) misc[SM_QN] = 0;
+    });
+    ex.sync();
+    for (int x0 = 0; x0 < nrb; x0 += SS::QCAP / 16) {
+      extract(x0, (x0 + SS::QCAP / 16 < nrb) ? x0 + SS::QCAP / 16 : nrb);
+      flush(misc[SM_QN]);
     }
-  });
-  ex.sync();
+  }
   ex.each([&](int tid) {
     if (tid == 0) misc[SM_NRB] = 0;
   });
@@ -524,16 +545,6 @@ SXC_HD inline void scan_epoch_segments(Ex& ex, char* sm, const uint16_t* cands, 
     push(c, open, end);
   });
   ex.sync();
-}
-
-// Scans lags [c0, c1) with gsz threads per lag; runs are pushed to the LDS queue.
-template <int NB, class Ex>
-SXC_HD inline void scan_epoch(Ex& ex, char* sm, const uint16_t* cands, int c0, int c1, int gsz) {
-  if constexpr (ScanSmem<NB>::BMRUNS) {
-    scan_epoch_bitmap<NB>(ex, sm, cands, c0, c1, gsz);
-  } else {
-    scan_epoch_segments<NB>(ex, sm, cands, c0, c1, gsz);
-  }
 }
 
 // Drains queue entries [0, n): mode 0 filters into t_results, mode 1 writes runs.
@@ -751,7 +762,18 @@ SXC_HD inline void scan_body(Ex& ex, char* sm, const ScanArgs& a, int blk) {
   for (int e0 = lo; e0 < hi; e0 += CE) {
     const int qstart = queued;
     const int e1 = (e0 + CE < hi) ? e0 + CE : hi;
-    scan_epoch<NB>(ex, sm, cands, e0, e1, SS::GSZ);
+    if constexpr (SS::BMRUNS) {
+      scan_epoch_bitmap<NB>(ex, sm, cands, e0, e1, SS::GSZ, [&](int n) {
+        drain<NB>(ex, sm, a, pr, cands, ps, n, e0);
+        runs += n;
+        ex.each([&](int tid) {
+          if (tid == 0) misc[SM_QN] = 0;
+        });
+        ex.sync();
+      });
+      continue;
+    }
+    scan_epoch_segments<NB>(ex, sm, cands, e0, e1, SS::GSZ);
     const int qn = misc[SM_QN];
     ex.sync();  // every thread has read the count before any thread pushes again
     queued = qn;
@@ -764,7 +786,7 @@ SXC_HD inline void scan_body(Ex& ex, char* sm, const ScanArgs& a, int blk) {
           if (tid == 0) misc[SM_QN] = 0;
         });
         ex.sync();
-        scan_epoch<NB>(ex, sm, cands, c, c + 1, SCAN_NT);
+        scan_epoch_segments<NB>(ex, sm, cands, c, c + 1, SCAN_NT);
         const int n1 = misc[SM_QN];
         drain<NB>(ex, sm, a, pr, cands, ps, n1, c);
         runs += n1;

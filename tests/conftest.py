@@ -60,3 +60,16 @@ def chunk_records(records, size, overlap):
                 seg = b""
             out.append((rid, start, seg))
     return out
+
+
+def dense_run_pair(a=40, b=12, seed=3, length=4096):
+    """A chunk repeating (a random bases, one fixed b-base block): on the diagonals at
+    multiples of a + b the window crosses the threshold about once per period, so one
+    16-lag epoch of the scan holds more runs (1,081) than the run queue (1,024 entries)."""
+    rng = np.random.default_rng(seed)
+    blk = rng.choice(list(b"ACGT"), b).astype(np.uint8)
+    units = []
+    for _ in range(length // (a + b) + 1):
+        units.append(rng.choice(list(b"ACGT"), a).astype(np.uint8))
+        units.append(blk)
+    return np.concatenate(units)[:length].tobytes()

@@ -13,7 +13,7 @@ import pytest
 
 import hostbuild
 import oracle
-from conftest import lcg_noise, unpack2
+from conftest import dense_run_pair, lcg_noise, unpack2
 
 COLS = ["query_id", "target_id", "query_size", "qstart", "tstart", "len", "reverse", "prob", "ident"]
 
@@ -75,6 +75,23 @@ def test_emulated_batch_uniform(emu_lib, golden):
         oxc = oracle.xcorr(oracle.encode(p["tseq"], 8192), oracle.encode(qs, 8192))
         assert np.array_equal(xc[ps].view(np.uint32), oxc.view(np.uint32))
         assert st[ps, 0] == len(oracle.findtop(oxc, 2.9 if p["fast"] else 1.8))
+
+
+def test_emulated_dense_runs(emu_lib):
+    """More runs in one epoch than the queue holds: the pure kernel extracts them in rounds
+    that each fit the queue.  Records and run counts equal the oracle's."""
+    t = dense_run_pair()
+    pairs = [dict(tseq=t, qseq=t, t_record=0, q_record=0)]
+    got, _, st = emu_batch(emu_lib, pairs)
+    ref = oracle.align_batch([dict(pairs[0], q_record_len=4096)], oracle.make_params(target_total=4096.0 * 64))[0]
+    assert len(ref) > 0
+    for c in COLS:
+        assert np.array_equal(got[c], ref[c]), c
+    for strand in range(2):
+        qs = t if strand == 0 else oracle.reverse_complement(t)
+        oxc = oracle.xcorr(oracle.encode(t, 8192), oracle.encode(qs, 8192))
+        assert st[strand, 1] == len(oracle.matchup(qs, t, oxc, 1.8))
+    assert st[0, 1] > 1024
 
 
 def test_emulated_resident_query_spectra(emu_lib, golden, monkeypatch):

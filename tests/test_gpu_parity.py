@@ -15,7 +15,7 @@ import numpy as np
 import pytest
 
 import oracle
-from conftest import chunk_records, lcg_noise, unpack2
+from conftest import chunk_records, dense_run_pair, lcg_noise, unpack2
 
 pytestmark = pytest.mark.gpu
 
@@ -148,6 +148,25 @@ def test_resident_batch_repeatable(ctx, golden):
         assert_results_equal(rb.results(), ref, "resident")
     cands, runs = rb.stats()
     assert cands > 0 and runs > 0
+    rb.close()
+
+
+def test_dense_runs(ctx):
+    """A pair whose scan holds more runs in one epoch than the run queue (1,024 entries):
+    the runs are extracted in rounds.  Records and the total run count equal the oracle's."""
+    from satsuma2_amd.api import ChunkPair, ResidentBatch, SlaveParams
+
+    t = dense_run_pair()
+    ref = oracle.align_batch([dict(tseq=t, qseq=t, q_record_len=4096)], oracle.make_params(target_total=4096.0 * 64))[0]
+    nruns = 0
+    for strand in range(2):
+        qs = t if strand == 0 else oracle.reverse_complement(t)
+        oxc = oracle.xcorr(oracle.encode(t, 8192), oracle.encode(qs, 8192))
+        nruns += len(oracle.matchup(qs, t, oxc, 1.8))
+    rb = ResidentBatch(ctx, [ChunkPair(t, t, q_record_len=4096)], SlaveParams(target_total=4096.0 * 64))
+    rb.run()
+    assert_results_equal(rb.results(), ref, "dense runs")
+    assert rb.stats()[1] == nruns > 1024
     rb.close()
 
 
