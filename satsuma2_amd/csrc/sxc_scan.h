@@ -236,6 +236,20 @@ SXC_HD inline bool tail_from_sums(double matches, double gct, double gcq, int le
   return true;
 }
 
+// The z of tail_from_sums estimated in float from the integer sums: its error (|m - x| off
+// by at most ~5e-7 * len, s by a few ulp) is far below the margin, so a true result implies
+// z >= z_skip in FP64 as well and the run can be dropped without the FP64 divisions.
+SXC_HD inline bool z_clearly_skipped(int matches, int gct, int gcq, int len, double z_skip) {
+  const float fl = (float)len;
+  const float gt = (float)gct / fl;
+  const float r = (float)gcq * gt + (float)(len - gcq) * (1.f - gt);
+  const float p = r / fl * 0.5f;
+  const float s = sqrtf(p * (1.f - p) * fl);
+  const float z = (p * fl - (float)matches) / s * 0.70710678f;
+  const float margin = 1e-5f * (fl / s + fabsf(z) + 1.f);
+  return z - margin >= (float)z_skip;  // false for NaN (s = 0) and for z_skip = inf
+}
+
 // ProbTable::GetMatchProbability after the sums (analysis/ProbTable.cc:118-141): the table's
 // cutoff if the identity reaches the threshold for (expected identity, length), else 0.  An
 // expectation below 1/511 selects row 0, which the reference leaves empty; it reports 0.
@@ -599,7 +613,8 @@ SXC_HD inline void drain(Ex& ex, char* sm, const ScanArgs& a, const sxc_pair_dev
                       ct, cq);
         if (a.ptable) {
           prob = table_from_sums((double)cm, (double)ct, (double)cq, len, a.ptable, a.table_prob, ident);
-        } else if (!tail_from_sums((double)cm, (double)ct, (double)cq, len, a.target_total, a.z_skip, prob, ident)) {
+        } else if (z_clearly_skipped(cm, ct, cq, len, a.z_skip) ||
+                   !tail_from_sums((double)cm, (double)ct, (double)cq, len, a.target_total, a.z_skip, prob, ident)) {
           continue;
         }
       } else {

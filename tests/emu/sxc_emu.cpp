@@ -5,6 +5,7 @@
 // boundaries, LDS is a heap buffer.  Built with AddressSanitizer by tests/test_emu.py so
 // every LDS and global index the kernels compute is bounds-checked on the CPU before the
 // same code runs on a GPU.  Not part of the product.
+#include <math.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -281,6 +282,47 @@ int emu_fft_forward(const float* in, float* out, int N) {
   }
   memcpy(out, spec.data(), sizeof(float) * N);
   return 0;
+}
+
+// The float pre-filter of the tail (z_clearly_skipped) against the FP64 z of
+// tail_from_sums on n random sums: with z_skip just above the exact z the filter must never
+// drop the run (returns the number of such violations); fired[0] counts the samples where
+// it dropped the run for z_skip well below z.
+int64_t emu_check_z_filter(int64_t n, uint64_t seed, int64_t* fired) {
+  uint64_t x = seed * 2654435761ull + 1;
+  auto next = [&]() {
+    x ^= x << 13;
+    x ^= x >> 7;
+    x ^= x << 17;
+    return x;
+  };
+  int64_t bad = 0, hit = 0;
+  const double deltas[] = {1e-12, 1e-9, 1e-6, 1e-4};
+  for (int64_t i = 0; i < n; ++i) {
+    const int len = 46 + (int)(next() % 16339);
+    const int cm = (int)(next() % (uint64_t)(len + 1));
+    const int ct = (int)(next() % (uint64_t)(len + 1));
+    const int cq = (int)(next() % (uint64_t)(len + 1));
+    // z exactly as tail_from_sums computes it
+    const double ident = (double)cm / (double)len;
+    const double gc_target = (double)ct / (double)len;
+    const double at_target = 1. - gc_target;
+    double r = (double)cq * gc_target;
+    r += ((double)len - (double)cq) * at_target;
+    const double p = r / (double)len / 2.;
+    const double sd = sqrt(p * (1. - p) * (double)len);
+    const double m = p * (double)len;
+    const double xx = (double)len * ident;
+    const double z = (m - xx) / sd / 1.414213562;
+    if (!std::isfinite(z)) continue;
+    for (double d : deltas) {
+      const double above = z + d * (1. + fabs(z));
+      if (sxc::z_clearly_skipped(cm, ct, cq, len, above)) ++bad;
+    }
+    if (sxc::z_clearly_skipped(cm, ct, cq, len, z - 1e-3 * (1. + fabs(z)) - 0.01)) ++hit;
+  }
+  *fired = hit;
+  return bad;
 }
 
 }  // extern "C"

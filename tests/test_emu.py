@@ -208,3 +208,15 @@ def test_library_prob_table_golden(emu_lib, golden):
     t = np.zeros((512, 2048), np.float64)
     emu_lib.emu_prob_table(C.c_double(float(g["target_size"][0])), t.ctypes.data_as(C.c_void_p))
     assert hashlib.sha256(t.tobytes()).hexdigest() == str(g["table_sha"])
+
+
+def test_tail_float_prefilter_is_one_sided(emu_lib):
+    """The drain drops a run on a float estimate of the tail's z only where the FP64 z is
+    beyond the keep bound too: with the bound just above the exact z (by 1e-12 .. 1e-4,
+    relative) the filter never fires on 10^6 random sums, and it does fire with the bound
+    below z (so it is not vacuous)."""
+    emu_lib.emu_check_z_filter.restype = C.c_int64
+    fired = C.c_int64(0)
+    bad = emu_lib.emu_check_z_filter(C.c_int64(1_000_000), C.c_uint64(7), C.byref(fired))
+    assert bad == 0
+    assert fired.value > 900_000
