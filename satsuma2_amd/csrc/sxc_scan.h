@@ -174,6 +174,15 @@ SXC_HD inline uint64_t match_block(const uint32_t* tpk, const uint32_t* qpk, int
 
 SXC_HD inline int popc64(uint64_t v) { return __builtin_popcountll(v); }
 
+// bits [s, s + 32) of hi:lo, 0 <= s < 32 (a funnel shift: one v_alignbit)
+SXC_HD inline uint32_t funnel32(uint32_t hi, uint32_t lo, int s) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_amdgcn_alignbit(hi, lo, (unsigned)s);
+#else
+  return (lo >> s) | ((hi << 1) << (31 - s));
+#endif
+}
+
 SXC_HD inline uint32_t rev32(uint32_t x) {
 #if defined(__clang__)
   return __builtin_bitreverse32(x);
@@ -340,15 +349,27 @@ SXC_HD inline void scan_epoch_bitmap(Ex& ex, char* sm, const uint16_t* cands, in
     int C = window_count(e1, e2, match_block(tpk, qpk, ti, qj, k0 - 3), 31);  // C at 32 k0 - 1
     uint32_t prev = (na - 1 >= 46 && C >= SXC_MIN_MATCHES) ? 1u : 0u;    // flag at 32 k0 - 1
     uint16_t* rlist = (uint16_t*)(sm + SS::RLIST);
-    for (int k = k0; k <= klast; ++k) {
-      const uint64_t e0 = match_block(tpk, qpk, ti, qj, k);
+    // The block's 32 bases of each side are words w, w + 1, w + 2 shifted right by the
+    // diagonal's bit offset; word w + 2 of one block is word w of the next.
+    const int tb = (ti & 15) * 2, qb = (qj & 15) * 2;
+    const uint32_t* tw = tpk + (ti >> 4) + 2 * k0;
+    const uint32_t* qw = qpk + (qj >> 4) + 2 * k0;
+    uint32_t t0 = tw[0], q0 = qw[0];
+    uint32_t e1l = (uint32_t)e1, e1h = (uint32_t)(e1 >> 32), e2h = (uint32_t)(e2 >> 32);
+    for (int k = k0; k <= klast; ++k, tw += 2, qw += 2) {
+      const uint32_t t1 = tw[1], t2 = tw[2], q1 = qw[1], q2 = qw[2];
+      const uint32_t xl = funnel32(t1, t0, tb) ^ funnel32(q1, q0, qb);
+      const uint32_t xh = funnel32(t2, t1, tb) ^ funnel32(q2, q1, qb);
+      t0 = t2;
+      q0 = q2;
+      const uint32_t e0l = ~(xl | (xl >> 1)) & 0x55555555u, e0h = ~(xh | (xh >> 1)) & 0x55555555u;
       // m(n - 46) for the block: offsets r < 14 come from block k-2, r >= 14 from block k-1
-      const uint64_t minus = (e2 >> 36) | (e1 << 28);
+      const uint32_t ml = funnel32(e1l, e2h, 4), mh = funnel32(e1h, e1l, 4);
       // 2-bit group r of z holds m(n) + 1 - m(n - 46), in {0, 1, 2}: no carry crosses a
       // group, so each 32-bit half is one add.  D = C - 19 is negative exactly where the
       // window is below the threshold; its sign bits shift into g, position 0 first.
-      const uint32_t zl = (uint32_t)e0 + (~(uint32_t)minus & 0x55555555u);
-      const uint32_t zh = (uint32_t)(e0 >> 32) + (~(uint32_t)(minus >> 32) & 0x55555555u);
+      const uint32_t zl = e0l + (~ml & 0x55555555u);
+      const uint32_t zh = e0h + (~mh & 0x55555555u);
       int D = C - SXC_MIN_MATCHES;
       uint32_t gbits = 0;
       static_for<0, 32>([&](auto R) {
@@ -365,8 +386,9 @@ SXC_HD inline void scan_epoch_bitmap(Ex& ex, char* sm, const uint16_t* cands, in
       // (the same increment on every lane: one atomic per wave)
       if (f & ~((f << 1) | prev)) rlist[ex.atomic_add_shared(&misc[SM_NRB], 1)] = (uint16_t)(j * WPL + k);
       prev = f >> 31;
-      e2 = e1;
-      e1 = e0;
+      e2h = e1h;
+      e1l = e0l;
+      e1h = e0h;
     }
   });
   ex.sync();
