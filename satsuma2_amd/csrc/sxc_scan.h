@@ -3,18 +3,22 @@
 // One workgroup per (pair, strand).  SeqAnalyzer::DoOne (OPTIM_DOONE,
 // analysis/CrossCorr.cc:667-724) slides a 46-wide window of integer scores
 // (int)(100 * DNA_EqualAmb + 0.5) along the diagonal of each lag FindTop accepted and
-// opens a run while the window sum exceeds 1889.  Here a lag's diagonal is split into
-// contiguous segments, one per thread; each thread recomputes the window at its segment
-// start, records the runs that open and close inside its segment, and a second phase
-// closes runs that cross segment boundaries.  Runs go to an LDS queue; the queue is
-// drained by all threads at once, one run per thread, through GetMatchProbabilityEx
+// opens a run while the window sum exceeds 1889.  Lags are taken in epochs of 8 or 16.
+// Pairs of pure A/C/G/T chunks (scan_epoch_bitmap): threads take whole 32-position blocks of
+// a lag, keep the window's flags in an LDS bitmap, and a second pass turns each rising edge
+// into a run.  Other pairs (scan_epoch_segments): a lag's diagonal is split into contiguous
+// segments, one per thread; each thread recomputes the window at its segment start,
+// records the runs that open and close inside its segment, and a second phase closes runs
+// that cross segment boundaries.  Runs go to an LDS queue; the queue is drained by all
+// threads at once, one run per thread, through GetMatchProbabilityEx
 // (analysis/AlignProbability.cc:62-127) and FilterMatches
 // (analysis/HomologyByXCorrSlave.cc:168-219).  Kept records carry the key
 // (lag rank, run start) so the finalize kernel restores the reference's order.
 //
-// Queue capacity holds every run one lag can produce (runs need a below-threshold gap,
-// so at most npos/2 + 1 per lag, npos <= N/2); an epoch of several lags that overflows
-// is discarded and redone one lag at a time.
+// Queue capacity: the segment scans' queue holds every run one lag can produce (runs need
+// a below-threshold gap, so at most npos/2 + 1 per lag, npos <= N/2), and an epoch of
+// several lags that overflows is discarded and redone one lag at a time; the bitmap scan
+// re-extracts an overflowing epoch's runs in rounds that each fit its queue.
 #pragma once
 #include <math.h>
 
