@@ -50,13 +50,14 @@ inline std::string pack_batch(const sxc_pair* in, int n, int N, HostBatch& hb,
     if (len > N) return "sequence longer than the FFT length N";
     c.off = (int64_t)hb.seq.size();
     c.len = len;
-    c.flags = 2;  // bit 1: only A, C, G, T (scores are 100 or 0)
+    c.flags = 2 | 4;  // bit 1: only A, C, G, T (scores are 100 or 0); bit 2: only A, C, G, T, N
     for (int i = 0; i < len; ++i) {
       const unsigned char b = (unsigned char)s[i];
       if (b >= 128) return "sequence byte >= 128";
       const int cl = class_of(b);
       if (class_has_thirds(cl)) c.flags |= 1;
       if (cl > 3) c.flags &= ~2;
+      if (cl > 3 && cl != SXC_CLASS_N) c.flags &= ~4;
       hb.seq.push_back(b);
     }
     while (hb.seq.size() % 16) hb.seq.push_back(0);  // next chunk 16-byte aligned

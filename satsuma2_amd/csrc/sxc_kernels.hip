@@ -232,11 +232,12 @@ hipError_t launch_correlate(int geo, const PairArgs& a, int total, hipStream_t s
 }
 
 hipError_t launch_scan(int nb, const ScanArgs& a, int total, hipStream_t s) {
-  // the pure-A/C/G/T kernel and the general one over the same grid; each workgroup of
-  // either kernel exits at once on the other kernel's pairs
+  // the pure-A/C/G/T kernel, its A/C/G/T/N variant and the general one over the same grid;
+  // each workgroup exits at once on the other kernels' pairs
   const int grid = total * (a.split > 1 ? a.split : 1);
 #define CALL(X)                                                            \
   hipLaunchKernelGGL(k_scan<(X) | 32>, dim3(grid), dim3(SCAN_NT), 0, s, a); \
+  hipLaunchKernelGGL(k_scan<(X) | 64>, dim3(grid), dim3(SCAN_NT), 0, s, a); \
   hipLaunchKernelGGL(k_scan<X>, dim3(grid), dim3(SCAN_NT), 0, s, a)
   SXC_NB_SWITCH(nb, CALL)
 #undef CALL

@@ -61,15 +61,22 @@ constexpr int SCAN_NT = 256;  // threads per scan workgroup
 // smallest count c of identical A/C/G/T bases with 100 * c > 1889 (19 of 46)
 constexpr int SXC_MIN_MATCHES = SXC_WIN_THRESHOLD / 100 + 1;
 static_assert(100 * SXC_MIN_MATCHES > SXC_WIN_THRESHOLD && 100 * (SXC_MIN_MATCHES - 1) <= SXC_WIN_THRESHOLD, "");
+// With N: a position involving an N scores 25 (N:N and N against A/C/G/T), so a window of M
+// identical A/C/G/T bases and K positions with an N exceeds 1889 <=> 4 M + K >= 76.
+constexpr int SXC_MIN_QUARTERS = SXC_WIN_THRESHOLD / 25 + 1;
+static_assert(25 * SXC_MIN_QUARTERS > SXC_WIN_THRESHOLD && 25 * (SXC_MIN_QUARTERS - 1) <= SXC_WIN_THRESHOLD, "");
 
 // NB: log2 N in bits 0-4; bit 5 selects the kernel for pairs whose two chunks are pure
 // A/C/G/T, whose LDS plan holds only the packed bases, the run queue and the counters (the
 // accepted lags are read from global memory), so twice as many of its workgroups share a
-// CU.  The other kernel takes every other pair.  Each exits at once on the other's pairs.
+// CU; bit 6 the same kernel for pairs of A/C/G/T/N chunks with an N (HASN: N masks beside
+// the packed bases).  The general kernel takes every other pair.  Each exits at once on the
+// others' pairs.
 template <int NB>
 struct ScanSmem {
   static constexpr int LOG = NB & 31;
-  static constexpr bool PURE = (NB & 32) != 0;
+  static constexpr bool HASN = (NB & 64) != 0;
+  static constexpr bool PURE = (NB & 96) != 0;
   static constexpr int N = 1 << LOG;
   // The segment scans' queue holds every run one lag can produce (npos / 2 + 1, npos <= N / 2);
   // the bitmap scan flushes its queue whenever it would overflow.
@@ -95,7 +102,9 @@ struct ScanSmem {
   static_assert(!COMPACT || (RB > 0 && (1 << RB) >= CE), "compact queue entries");
   static constexpr int QE = COMPACT ? 4 : 8;            // bytes per queue entry
   static constexpr int WPL = N / 64;                    // bitmap words per lag: npos <= N / 2
-  static constexpr int QUEUE = QPK + 4 * PKW;           // queue entries [QCAP]
+  static constexpr int TNK = QPK + 4 * PKW;             // uint32[PKW]: bit 2b set where base b of a word is N (HASN)
+  static constexpr int QNK = TNK + (HASN ? 4 * PKW : 0);
+  static constexpr int QUEUE = QNK + (HASN ? 4 * PKW : 0);  // queue entries [QCAP]
   static constexpr int SUMM = QUEUE + QE * QCAP;        // int2[SCAN_NT] (segments)
   static constexpr int BITMAP = SUMM;                   // uint32[CE][WPL] (BMRUNS)
   static constexpr int RLIST = BITMAP + 4 * CE * WPL;   // uint16[CE * WPL]: blocks with a rising edge (BMRUNS)
@@ -227,6 +236,27 @@ SXC_HD inline void packed_counts(const uint32_t* tpk, const uint32_t* qpk, int s
   }
 }
 
+// The same counts where bases may be N (bit 2b of the N mask words set): identical A/C/G/T
+// bases, positions with an N on either side, C/G bases and N bases of each side.
+SXC_HD inline void packed_counts_n(const uint32_t* tpk, const uint32_t* qpk, const uint32_t* tnk,
+                                   const uint32_t* qnk, int st, int sq, int len, int& match, int& nk, int& gct,
+                                   int& ntn, int& gcq, int& nqn) {
+  match = nk = gct = ntn = gcq = nqn = 0;
+  for (int off = 0; off < len; off += 32) {
+    const int m = (len - off < 32) ? len - off : 32;
+    const uint64_t keep = ((m == 32) ? ~0ull : ((1ull << (2 * m)) - 1)) & 0x5555555555555555ull;
+    const uint64_t tv = load32(tpk, st + off), qv = load32(qpk, sq + off);
+    const uint64_t tn = load32(tnk, st + off) & keep, qn = load32(qnk, sq + off) & keep;
+    const uint64_t x = tv ^ qv;
+    match += popc64(~(x | (x >> 1)) & keep & ~(tn | qn));
+    nk += popc64(tn | qn);
+    gct += popc64((tv ^ (tv >> 1)) & keep & ~tn);
+    gcq += popc64((qv ^ (qv >> 1)) & keep & ~qn);
+    ntn += popc64(tn);
+    nqn += popc64(qn);
+  }
+}
+
 // The tail after the sums (GCAdjustExpect, Sigma, CDF; analysis/AlignProbability.cc:11-40,
 // 107-126).  Returns false, without evaluating erf/exp, when z >= z_skip guarantees the
 // probability is below the keep threshold.
@@ -348,6 +378,57 @@ SXC_HD inline void scan_epoch_bitmap(Ex& ex, char* sm, const uint16_t* cands, in
     uint32_t* row = bm + j * WPL;
     const int ti = g.i0, qj = g.i0 + g.shift;
     const int k0 = na >> 5, klast = (nb - 1) >> 5;
+    uint16_t* rl = (uint16_t*)(sm + SS::RLIST);
+    if constexpr (SS::HASN) {
+      // D = 4 M + K - 76 over the window: M identical A/C/G/T bases (m bits), K positions
+      // with an N (k bits); the 2-bit groups of zm and zk hold m(n) + 1 - m(n - 46) and
+      // k(n) + 1 - k(n - 46), so D += 4 zm + zk - 5 per position.
+      const uint32_t* tnk = (const uint32_t*)(sm + SS::TNK);
+      const uint32_t* qnk = (const uint32_t*)(sm + SS::QNK);
+      auto mk_at = [&](int kb, uint64_t& m, uint64_t& kk) {
+        if (kb < 0) {
+          m = kk = 0;
+          return;
+        }
+        const uint64_t x = load32(tpk, ti + 32 * kb) ^ load32(qpk, qj + 32 * kb);
+        kk = (load32(tnk, ti + 32 * kb) | load32(qnk, qj + 32 * kb)) & 0x5555555555555555ull;
+        m = ~(x | (x >> 1)) & 0x5555555555555555ull & ~kk;
+      };
+      uint64_t m3, k3, m2, k2, m1, k1;
+      mk_at(k0 - 3, m3, k3);
+      mk_at(k0 - 2, m2, k2);
+      mk_at(k0 - 1, m1, k1);
+      int D = 4 * window_count(m1, m2, m3, 31) + window_count(k1, k2, k3, 31) - SXC_MIN_QUARTERS;  // at 32 k0 - 1
+      uint32_t prev = (na - 1 >= 46 && D >= 0) ? 1u : 0u;
+      for (int k = k0; k <= klast; ++k) {
+        uint64_t m0, kk0;
+        mk_at(k, m0, kk0);
+        const uint64_t mm = (m2 >> 36) | (m1 << 28), km = (k2 >> 36) | (k1 << 28);  // at n - 46
+        const uint32_t zml = (uint32_t)m0 + (~(uint32_t)mm & 0x55555555u);
+        const uint32_t zmh = (uint32_t)(m0 >> 32) + (~(uint32_t)(mm >> 32) & 0x55555555u);
+        const uint32_t zkl = (uint32_t)kk0 + (~(uint32_t)km & 0x55555555u);
+        const uint32_t zkh = (uint32_t)(kk0 >> 32) + (~(uint32_t)(km >> 32) & 0x55555555u);
+        uint32_t gbits = 0;
+        static_for<0, 32>([&](auto R) {
+          constexpr int r = decltype(R)::value;
+          const uint32_t zm = (r < 16) ? (zml >> (2 * r)) & 3u : (zmh >> (2 * (r - 16))) & 3u;
+          const uint32_t zk = (r < 16) ? (zkl >> (2 * r)) & 3u : (zkh >> (2 * (r - 16))) & 3u;
+          D += (int)(4 * zm + zk) - 5;
+          gbits = (gbits << 1) | ((uint32_t)D >> 31);
+        });
+        uint32_t f = ~rev32(gbits);
+        if (32 * k < 46) f &= (46 - 32 * k >= 32) ? 0u : ~((1u << (46 - 32 * k)) - 1);  // flags from n = 46
+        if (k == klast && nb - 32 * k < 32) f &= (1u << (nb - 32 * k)) - 1;
+        row[k] = f;
+        if (f & ~((f << 1) | prev)) rl[ex.atomic_add_shared(&misc[SM_NRB], 1)] = (uint16_t)(j * WPL + k);
+        prev = f >> 31;
+        m2 = m1;
+        m1 = m0;
+        k2 = k1;
+        k1 = kk0;
+      }
+      return;
+    }
     uint64_t e2 = match_block(tpk, qpk, ti, qj, k0 - 2);
     uint64_t e1 = match_block(tpk, qpk, ti, qj, k0 - 1);
     int C = window_count(e1, e2, match_block(tpk, qpk, ti, qj, k0 - 3), 31);  // C at 32 k0 - 1
@@ -633,7 +714,23 @@ SXC_HD inline void drain(Ex& ex, char* sm, const ScanArgs& a, const sxc_pair_dev
       }
       if (len < a.min_len) continue;  // FilterMatches: GetLength() < m_minLen
       double ident, prob = 0.;
-      if (SS::PURE || fast) {
+      if constexpr (SS::HASN) {
+        int cm, nk, ct, ntn, cq, nqn;
+        packed_counts_n((const uint32_t*)(sm + SS::TPK), (const uint32_t*)(sm + SS::QPK),
+                        (const uint32_t*)(sm + SS::TNK), (const uint32_t*)(sm + SS::QNK), start_t, start_q, len, cm,
+                        nk, ct, ntn, cq, nqn);
+        // DNA_Equal of N against any base or N, and N's GC weight, are quarter multiples, so
+        // the reference's running sums equal these products exactly
+        const double matches = (double)cm + a.tab->equal[SXC_CLASS_N][0] * (double)nk;
+        const double gct = (double)ct + a.tab->gc[SXC_CLASS_N] * (double)ntn;
+        const double gcq = (double)cq + a.tab->gc[SXC_CLASS_N] * (double)nqn;
+        if (a.ptable) {
+          prob = table_from_sums(matches, gct, gcq, len, a.ptable, a.table_prob, ident);
+        } else if ((nk == 0 && z_clearly_skipped(cm, ct, cq, len, a.z_skip)) ||
+                   !tail_from_sums(matches, gct, gcq, len, a.target_total, a.z_skip, prob, ident)) {
+          continue;
+        }
+      } else if (SS::PURE || fast) {
         int cm, ct, cq;
         packed_counts((const uint32_t*)(sm + SS::TPK), (const uint32_t*)(sm + SS::QPK), start_t, start_q, len, cm,
                       ct, cq);
@@ -696,8 +793,9 @@ SXC_HD inline void scan_body(Ex& ex, char* sm, const ScanArgs& a, int blk) {
   const int strand = ps % a.strands;
   const sxc_chunk tch = a.tchunks[pr.t_chunk];
   const sxc_chunk qch = a.qchunks[pr.q_chunk];
-  const bool fast = (tch.flags & 2) && (qch.flags & 2);  // both pure A/C/G/T
-  if (fast != SS::PURE) return;                           // the other kernel's pair
+  const bool fast = (tch.flags & 2) && (qch.flags & 2);   // both pure A/C/G/T
+  const bool acgtn = (tch.flags & 4) && (qch.flags & 4);  // both A/C/G/T/N
+  if (fast != (SS::PURE && !SS::HASN) || (!fast && acgtn) != SS::HASN) return;  // another kernel's pair
   const sxc_tables* tab = a.tab;
   unsigned char* tc = (unsigned char*)(sm + SS::TCODE);
   unsigned char* qc = (unsigned char*)(sm + SS::QCODE);
@@ -749,21 +847,28 @@ SXC_HD inline void scan_body(Ex& ex, char* sm, const ScanArgs& a, int blk) {
     uint32_t* qpk = (uint32_t*)(sm + SS::QPK);
     ex.each([&](int tid) {
       for (int w = tid; w < SS::PKW; w += SCAN_NT) {
-        uint32_t vt = 0, vq = 0;
+        uint32_t vt = 0, vq = 0, nt = 0, nq = 0;
         for (int b = 0; b < 16; ++b) {
           const int i = 16 * w + b;
           if (i < Lt) {
             const unsigned x = ts[i];
             vt |= (((x >> 1) ^ (x >> 2)) & 3u) << (2 * b);
+            nt |= (x == 'N' ? 1u : 0u) << (2 * b);
           }
           if (i < Lq) {
-            // DNAVector::ReverseComplement for strand 1 (analysis/DNAVector.cc:482-521)
+            // DNAVector::ReverseComplement for strand 1 (analysis/DNAVector.cc:482-521);
+            // the complement of N is N
             const unsigned x = strand ? qs[Lq - 1 - i] : qs[i];
             vq |= ((((x >> 1) ^ (x >> 2)) & 3u) ^ (strand ? 3u : 0u)) << (2 * b);
+            nq |= (x == 'N' ? 1u : 0u) << (2 * b);
           }
         }
         tpk[w] = vt;
         qpk[w] = vq;
+        if constexpr (SS::HASN) {
+          ((uint32_t*)(sm + SS::TNK))[w] = nt;
+          ((uint32_t*)(sm + SS::QNK))[w] = nq;
+        }
       }
     });
     ex.sync();

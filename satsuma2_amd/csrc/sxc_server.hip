@@ -443,13 +443,14 @@ int upload_chunks(std::vector<unsigned char>& bytes, std::vector<sxc_chunk>& tab
     sxc_chunk& t = table[(size_t)k];
     t.off = (int64_t)bytes.size();
     t.len = c.len;
-    t.flags = 2;
+    t.flags = 2 | 4;
     for (int i = 0; i < c.len; ++i) {
       const unsigned char b = (unsigned char)c.seq[i];
       if (b >= 128) return fail(SXC_ERR_ARG, "sequence byte >= 128");
       const int cl = sxc::class_of(b);
       if (sxc::class_has_thirds(cl)) t.flags |= 1;
       if (cl > 3) t.flags &= ~2;
+      if (cl > 3 && cl != SXC_CLASS_N) t.flags &= ~4;
       bytes.push_back(b);
     }
     while (bytes.size() % 16) bytes.push_back(0);

@@ -19,7 +19,8 @@
 typedef struct {
   int64_t off;
   int32_t len;
-  int32_t flags;  // bit 0: contains a codec class with 1/3 entries (B, V, H, D)
+  int32_t flags;  // bit 0: contains a codec class with 1/3 entries (B, V, H, D); bit 1: only
+                  // A, C, G, T; bit 2: only A, C, G, T, N
 } sxc_chunk;
 
 // One chunk pair of a batch (HomologyByXCorr::Align inputs plus the chunk metadata

@@ -73,3 +73,34 @@ def dense_run_pair(a=40, b=12, seed=3, length=4096):
         units.append(rng.choice(list(b"ACGT"), a).astype(np.uint8))
         units.append(blk)
     return np.concatenate(units)[:length].tobytes()
+
+
+def n_pairs(g):
+    """Chunk pairs of A/C/G/T/N from the golden uniform pairs (the homology planted in pair 0
+    and 2 included): one N, N runs on one or both sides, N at the chunk ends, N inside the
+    homologous region (runs that contain an N), an N every 50 bases; the last pair stays pure."""
+    out = []
+    for r in range(7):
+        t = bytearray(unpack2(g["t"][r], 4096))
+        q = bytearray(unpack2(g["q"][r], 4096))
+        if r == 0:
+            q[1234] = ord("N")
+        elif r == 1:
+            t[1000:1100] = b"N" * 100
+        elif r == 2:
+            t[2000:2300] = b"N" * 300
+            q[2100:2150] = b"N" * 50
+            for i in range(0, 4096, 97):
+                t[i] = ord("N")
+        elif r == 3:
+            t[0:7] = b"N" * 7
+            q[4090:4096] = b"N" * 6
+        elif r == 4:
+            for i in range(0, 4096, 50):
+                q[i] = ord("N")
+        elif r == 5:
+            for i in range(0, 4096, 61):
+                t[i] = ord("N")
+                q[(i + 17) % 4096] = ord("N")
+        out.append((bytes(t), bytes(q)))
+    return out

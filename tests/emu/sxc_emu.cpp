@@ -171,7 +171,8 @@ int run_batch(const sxc::HostBatch& hb, const sxc_params& prm, std::vector<sxc_r
     sc.table_prob = prm.table_prob;
   }
   // each kernel's LDS plan at its exact size, so the sanitizer bounds every access to it
-  std::vector<char> ssm_pure(sxc::ScanSmem<NB | 32>::BYTES), ssm(sxc::ScanSmem<NB>::BYTES);
+  std::vector<char> ssm_pure(sxc::ScanSmem<NB | 32>::BYTES), ssm_n(sxc::ScanSmem<NB | 64>::BYTES),
+      ssm(sxc::ScanSmem<NB>::BYTES);
   // SXC_EMU_SPLIT=k: k scan workgroups per (pair, strand), counts cleared first (as the pair
   // kernel does for a split scan), run in reverse order
   if (const char* v = getenv("SXC_EMU_SPLIT")) sc.split = atoi(v);
@@ -182,10 +183,12 @@ int run_batch(const sxc::HostBatch& hb, const sxc_params& prm, std::vector<sxc_r
       for (int ps = 0; ps < total; ++ps) stats[2 * ps + 1] = 0;
   }
   for (int blk = total * split - 1; blk >= 0; --blk) {
-    // both kernels, as launched on the device (each skips the other's pairs)
+    // the three kernels, as launched on the device (each skips the others' pairs)
     EmuExec<int> ex(sxc::SCAN_NT);
     memset(ssm_pure.data(), 0xCD, ssm_pure.size());
     sxc::scan_body<NB | 32>(ex, ssm_pure.data(), sc, blk);
+    memset(ssm_n.data(), 0xCD, ssm_n.size());
+    sxc::scan_body<NB | 64>(ex, ssm_n.data(), sc, blk);
     memset(ssm.data(), 0xCD, ssm.size());
     sxc::scan_body<NB>(ex, ssm.data(), sc, blk);
   }

@@ -13,7 +13,7 @@ import pytest
 
 import hostbuild
 import oracle
-from conftest import dense_run_pair, lcg_noise, unpack2
+from conftest import dense_run_pair, lcg_noise, n_pairs, unpack2
 
 COLS = ["query_id", "target_id", "query_size", "qstart", "tstart", "len", "reverse", "prob", "ident"]
 
@@ -75,6 +75,23 @@ def test_emulated_batch_uniform(emu_lib, golden):
         oxc = oracle.xcorr(oracle.encode(p["tseq"], 8192), oracle.encode(qs, 8192))
         assert np.array_equal(xc[ps].view(np.uint32), oxc.view(np.uint32))
         assert st[ps, 0] == len(oracle.findtop(oxc, 2.9 if p["fast"] else 1.8))
+
+
+def test_emulated_n_pairs(emu_lib, golden):
+    """Pairs of A/C/G/T/N chunks take the bitmap scan with N masks (a position with an N
+    scores 25): records and run counts equal the oracle's."""
+    pairs = [dict(tseq=t, qseq=q, t_record=r, q_record=r) for r, (t, q) in enumerate(n_pairs(golden("uniform_pairs.npz")))]
+    got, _, st = emu_batch(emu_lib, pairs)
+    ref = np.concatenate(oracle.align_batch([dict(p, q_record_len=4096) for p in pairs],
+                                            oracle.make_params(target_total=4096.0 * 64)))
+    assert len(ref) > 0
+    for c in COLS:
+        assert np.array_equal(got[c], ref[c]), c
+    for ps in range(2 * len(pairs)):
+        p = pairs[ps // 2]
+        qs = p["qseq"] if ps % 2 == 0 else oracle.reverse_complement(p["qseq"])
+        oxc = oracle.xcorr(oracle.encode(p["tseq"], 8192), oracle.encode(qs, 8192))
+        assert st[ps, 1] == len(oracle.matchup(qs, p["tseq"], oxc, 1.8)), ps
 
 
 def test_emulated_dense_runs(emu_lib):

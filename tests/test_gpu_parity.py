@@ -15,7 +15,7 @@ import numpy as np
 import pytest
 
 import oracle
-from conftest import chunk_records, dense_run_pair, lcg_noise, unpack2
+from conftest import chunk_records, dense_run_pair, lcg_noise, n_pairs, unpack2
 
 pytestmark = pytest.mark.gpu
 
@@ -167,6 +167,29 @@ def test_dense_runs(ctx):
     rb.run()
     assert_results_equal(rb.results(), ref, "dense runs")
     assert rb.stats()[1] == nruns > 1024
+    rb.close()
+
+
+def test_n_pairs(ctx, golden):
+    """Pairs of A/C/G/T/N chunks (the bitmap scan with N masks) and a pure pair in one batch:
+    records and the total run count equal the oracle's."""
+    from satsuma2_amd.api import ChunkPair, ResidentBatch, SlaveParams
+
+    P = n_pairs(golden("uniform_pairs.npz"))
+    ref = np.concatenate(oracle.align_batch(
+        [dict(tseq=t, qseq=q, t_record=r, q_record=r, q_record_len=4096) for r, (t, q) in enumerate(P)],
+        oracle.make_params(target_total=4096.0 * 64)))
+    nruns = 0
+    for t, q in P:
+        for strand in range(2):
+            qs = q if strand == 0 else oracle.reverse_complement(q)
+            oxc = oracle.xcorr(oracle.encode(t, 8192), oracle.encode(qs, 8192))
+            nruns += len(oracle.matchup(qs, t, oxc, 1.8))
+    rb = ResidentBatch(ctx, [ChunkPair(t, q, t_record=r, q_record=r, q_record_len=4096) for r, (t, q) in enumerate(P)],
+                       SlaveParams(target_total=4096.0 * 64))
+    rb.run()
+    assert_results_equal(rb.results(), ref, "N pairs")
+    assert rb.stats()[1] == nruns
     rb.close()
 
 
