@@ -522,9 +522,15 @@ SXC_HD inline void peaks(Ex& ex, char* sm, double cutoff) {
     if (tid >= NP) return;
     double e = 0.;
     if (NP > 8) {
-      for (int i = tid * SXC_ENV_SIZE; i < (tid + 1) * SXC_ENV_SIZE; ++i) {
-        const float x = Y[pad(i)];
-        e += (double)(x * x);
+      // the reference's sequential sum; the loads are issued 16 at a time ahead of it
+      static_assert(SXC_ENV_SIZE % 16 == 0, "");
+      for (int i0 = tid * SXC_ENV_SIZE; i0 < (tid + 1) * SXC_ENV_SIZE; i0 += 16) {
+        float sq[16];
+        static_for<0, 16>([&](auto K) {
+          const float x = Y[pad(i0 + decltype(K)::value)];
+          sq[decltype(K)::value] = x * x;
+        });
+        static_for<0, 16>([&](auto K) { e += (double)sq[decltype(K)::value]; });
       }
     }
     env[tid] = sqrt(e / (double)SXC_ENV_SIZE);
@@ -546,16 +552,28 @@ SXC_HD inline void peaks(Ex& ex, char* sm, double cutoff) {
     ex.template reg<Regs<NB>>(tid).acc[0] = __builtin_bit_cast(float, m);
   });
   ex.sync();
+  // exclusive prefix of the per-thread counts in two levels over blocks of 16 threads (the
+  // block sums use the entropy windows' area: the encodes are done)
+  constexpr int SB = 16, NBK = (G::NT + SB - 1) / SB;
+  static_assert(NBK <= SM::NWIN, "block sums fit the entropy area");
+  int* bsum = (int*)(sm + SM::ENT);
   ex.each([&](int tid) {
-    if (tid == 0) {
-      int s = 0;
-      for (int t = 0; t < G::NT; ++t) {
-        const int c = cnt[t];
-        cnt[t] = s;
-        s += c;
-      }
-      misc[MISC_NCAND] = s;
+    if (tid >= NBK) return;
+    int s = 0;
+    for (int t = SB * tid; t < SB * tid + SB && t < G::NT; ++t) s += cnt[t];
+    bsum[tid] = s;
+  });
+  ex.sync();
+  ex.each([&](int tid) {
+    if (tid >= NBK) return;
+    int s = 0;
+    for (int b = 0; b < tid; ++b) s += bsum[b];
+    for (int t = SB * tid; t < SB * tid + SB && t < G::NT; ++t) {
+      const int c = cnt[t];
+      cnt[t] = s;
+      s += c;
     }
+    if (tid == NBK - 1) misc[MISC_NCAND] = s;
   });
   ex.sync();
   uint16_t* cand = (uint16_t*)(sm + SM::CAND);
