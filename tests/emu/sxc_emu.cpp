@@ -86,7 +86,7 @@ int run_batch(const sxc::HostBatch& hb, const sxc_params& prm, std::vector<sxc_r
   sxc::SpecArgs sa{&T.tab, T.osc.data(), T.ent[NB].data(), hb.seq.data(), hb.tchunks.data(), which.data(),
                    (int)which.size(), spectra.data(), dst.data(), nullptr,
                    tails ? hb.ttail.data() : nullptr, tails ? hb.tails.data() : nullptr,
-                   tails ? hb.segs.data() : nullptr, tails ? src_sig.data() : nullptr};
+                   tails ? hb.segs.data() : nullptr, tails ? src_sig.data() : nullptr, 0};
   for (int k = 0; k < sa.n; ++k) {
     EmuExec<sxc::Regs<GEO>> ex(G::NT);
     memset(sm.data(), 0xCD, sm.size());
@@ -268,7 +268,7 @@ int emu_fft_forward(const float* in, float* out, int N) {
   auto go = [&](auto nbc) {
     constexpr int NB = decltype(nbc)::value;
     sxc::SpecArgs sa{&T.tab, T.osc.data(), T.ent[NB].data(), nullptr, &ck, &which, 1, spec.data(), &dst, sig.data(),
-                     nullptr, nullptr, nullptr, nullptr};
+                     nullptr, nullptr, nullptr, nullptr, 0};
     std::vector<char> sm(sxc::Smem<NB>::BYTES);
     EmuExec<sxc::Regs<NB>> ex(sxc::Geo<NB>::NT);
     sxc::spectra_body<NB>(ex, sm.data(), sa, 0);
