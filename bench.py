@@ -14,21 +14,26 @@ Serving (config of the serving loop): two synthetic 50-Mbp genomes sharing 2,000
 blocks, chunked as the slave chunks them (16,277 target chunks of 4096 bp every 3072 bp,
 12,208 query chunks), target spectra resident in HBM; battleship-shaped requests (seed
 lines of one target chunk x 256 query chunks, then the 24 x 24 matrix cells of hits) arrive
-as streams offering 25%, 50% and 75% of the batched throughput in chunk pairs per second
-(exponential gaps scaled by each request's pairs); each rate reports request latency p50/p99
-and chunk pairs per second.  The headline p50/p99 is at 50%.
+as streams offering 25%, 50%, 75%, 100% and 125% of the batched throughput in chunk pairs
+per second (exponential gaps scaled by each request's pairs); each rate reports request
+latency p50/p99 counted from the request's scheduled arrival, the server's own service
+latency, and chunk pairs per second; the highest completed rate is the saturation point.
+The headline p50/p99 is at 50%.
 
 Uneven (config of the uneven batch): 20,000 pairs with target and query lengths log-uniform
 in [256, 16384] bp, each at its own FFT length (1024 ... 32768), in one resident batch.
 
-With --large (config of the large served run, not part of the default run): two 200-Mbp
+Large served run (config of the large served run; --no-large skips it): two 200-Mbp
 genomes with GC drawn per 100-kbp window, tandem repeats over 10% of each, five
-transposon-like families of 1,000 copies and 5,000 syntenic blocks, served at 25% and 50% of
-the batched rate.
+transposon-like families of 1,000 copies and 5,000 syntenic blocks, served at 25%, 50% and
+100% of the batched rate.
 
 Printed: one JSON line with the batched throughput (`value`), the plain one-pair-per-call
-throughput beside it, the serving latencies, the dominant kernel's roofline figures from
-HIP events, and the reference CPU path timed on this host's cores on a bounded sample.
+throughput and the one-shot (host buffers in and out, PCIe included) batch rate beside it,
+the serving latencies, the dominant kernel's roofline figures from HIP events against the
+8 TB/s spec and against a device copy measured here, and the reference CPU path timed on
+this host's cores (all that are available, and one GPU's share of the host) on a bounded
+sample.
 """
 from __future__ import annotations
 
@@ -52,13 +57,14 @@ HBM_PEAK_GBS = 8000.0
 # materialised, target spectra resident, peak search fused): forward
 # 2*4*8192*4 + 1024 = 263,168 B; inverse 262,144 + 131,072 + 65,536 = 458,752 B.
 B_PAIR = 263_168 + 458_752
-# HBM bytes k_correlate<13> moves per chunk pair, from rocprofv3 FETCH_SIZE (x2.0) and
-# WRITE_SIZE (x1.0) on the 4096-pair launch (profiles/uniform_pairs_4096_kernels.md):
-# (271,555 * 2 + 4,916) KiB / 4096 pairs.
-TRAFFIC_PER_PAIR = (271_555 * 2 + 4_916) * 1024 / 4096
-SERVE_LOADS = (0.25, 0.5, 0.75)  # offered chunk pairs/s as fractions of the batched rate; 0.5 is the headline
+# HBM bytes the pair kernel moves per chunk pair, from rocprofv3 FETCH_SIZE and WRITE_SIZE
+# counter runs of this build on the 4096-pair launch, each scaled by what the counter reads
+# for a plain device copy of known size (profiles/uniform_pairs_4096_counters.md); None
+# until that profile is taken for the kernels as they stand.
+TRAFFIC_PER_PAIR = None
+SERVE_LOADS = (0.25, 0.5, 0.75, 1.0, 1.25)  # offered chunk pairs/s as fractions of the batched rate; 0.5: headline
 SEED_LINE = 256                  # query chunks per seed line
-LARGE_LOADS = (0.25, 0.5)        # offered loads of the large served run
+LARGE_LOADS = (0.25, 0.5, 1.0)   # offered loads of the large served run
 
 
 def kernel_name(ctx, N: int = 8192) -> str:
@@ -79,6 +85,43 @@ def make_workload(n_pairs: int, seed: int = 1):
     T, Q = synth.uniform_pairs(n_pairs, seed=seed)
     pairs = [ChunkPair(T[r], Q[r], t_record=r, q_record=r, q_record_len=4096) for r in range(n_pairs)]
     return T, Q, pairs
+
+
+def host_cpus():
+    """(CPUs of the host, CPUs this process may run on)."""
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count() or 1
+    return os.cpu_count() or 1, avail
+
+
+def copy_bandwidth(gib: float = 1.0, reps: int = 10):
+    """Device-to-device copy of `gib` GiB on this GPU (read + write bytes per second), the
+    measured ceiling the roofline fraction is also quoted against."""
+    try:
+        import torch
+    except ImportError:
+        return None
+    if not torch.cuda.is_available():
+        return None
+    n = int(gib * (1 << 30))
+    a = torch.empty(n, dtype=torch.uint8, device="cuda")
+    b = torch.empty_like(a)
+    a.fill_(1)
+    b.copy_(a)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        b.copy_(a)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    del a, b
+    torch.cuda.empty_cache()
+    return {"gb_per_s": round(2 * n / (ms * 1e-3) / 1e9, 1), "bytes": 2 * n, "ms": round(ms, 4),
+            "how": f"torch copy_ of {gib:g} GiB, device to device, mean of {reps}"}
 
 
 def cpu_baseline(T, Q, sample: int, threads: int):
@@ -141,7 +184,8 @@ def serve_bench(ctx, genome_bp: int, duration: float, capacity: float, loads, se
         runs.append(r)
     info = srv.stats()
     srv.close()
-    head = runs[min(1, len(runs) - 1)]
+    head = next((r for r in runs if r["offered_load"] == 0.5), runs[0])
+    sat = max(runs, key=lambda r: r["pairs_per_s"])
     kind = ("GC 0.38-0.45 per 100-kbp window, 10% tandem repeats, 5 transposon families x 1000 copies, "
             "5,000 syntenic blocks" if large else "2,000 syntenic blocks")
     return {"workload": f"two {genome_bp / 1e6:.0f}-Mbp synthetic genomes ({kind}), {len(tch)} target x "
@@ -149,6 +193,10 @@ def serve_bench(ctx, genome_bp: int, duration: float, capacity: float, loads, se
                         f"cells of hits)",
             "generate_s": round(gen_s, 1),
             "p50_ms": head["p50_ms"], "p99_ms": head["p99_ms"], "at_load": head["offered_load"],
+            "latency": "from each request's scheduled (Poisson) arrival to its completion; service_*: from "
+                       "submission, as the server measures it",
+            "saturation_pairs_per_s": sat["pairs_per_s"], "saturation_at_load": sat["offered_load"],
+            "saturation_p99_ms": sat["p99_ms"],
             "rates": runs, "create_s": round(create_s, 2), "resident_gb": round(info["resident_bytes"] / 1e9, 3),
             "slots": info["slots"], "max_pairs_per_launch": info["max_pairs_per_launch"],
             "launches": info["launches"]}
@@ -211,8 +259,8 @@ def main():
     ap.add_argument("--serve-bp", type=int, default=50_000_000)
     ap.add_argument("--serve-seconds", type=float, default=4.0)
     ap.add_argument("--no-serve", action="store_true")
-    ap.add_argument("--large", action="store_true",
-                    help="also serve the large whole-genome run (two 200-Mbp genomes with repeats; minutes)")
+    ap.add_argument("--no-large", action="store_true",
+                    help="skip the large served run (two 200-Mbp genomes with repeats; about a minute)")
     ap.add_argument("--large-bp", type=int, default=200_000_000)
     ap.add_argument("--no-autotune", action="store_true", help="default FFT kernel geometries (for profiling)")
     ap.add_argument("--uneven-pairs", type=int, default=20_000)
@@ -266,6 +314,18 @@ def main():
     assert len(rb.results()) == n_res
     rb.close()
 
+    # one-shot batch: host buffers in and out, PCIe transfers included (sxc_align_batch)
+    h = HomologyByXCorr(ctx, params)
+    h.align_batch(pairs[:64])
+    t0 = time.perf_counter()
+    one = h.align_batch(pairs)
+    one_s = time.perf_counter() - t0
+    tm1 = ctx.timing()
+    oneshot = {"pairs_per_s": round(args.pairs / one_s, 1), "ms": round(1e3 * one_s, 3),
+               "upload_ms": round(tm1.upload_ms, 3), "download_ms": round(tm1.download_ms, 3),
+               "results": int(len(one)),
+               "what": "sxc_align_batch on the same pairs: pack on the host, upload, run, download"}
+
     # plain drop-in: one chunk pair per call, host buffers in and out (as the slave calls Align)
     plain = float("nan")
     if args.plain_pairs > 0:
@@ -279,6 +339,7 @@ def main():
 
     k_ms = float(np.mean(cor_ms))
     achieved = B_PAIR * args.pairs / (k_ms * 1e-3) / 1e9
+    copy = copy_bandwidth()
     out = {
         "metric": METRIC,
         "value": round(value, 1),
@@ -299,6 +360,7 @@ def main():
         "eager_ms_per_step": round(eager_ms, 3),
         "plain_pairs_per_s": round(plain, 1) if plain == plain else None,
         "batched_over_plain": round(value / plain, 2) if plain == plain else None,
+        "oneshot": oneshot,
         "phase_ms": {"target_spectra": round(float(np.mean(spec_ms)), 3),
                      "correlate": round(float(np.mean(cor_ms)), 3), "scan": round(float(np.mean(scan_ms)), 3),
                      "finalize": round(float(np.mean(fin_ms)), 3)},
@@ -306,8 +368,11 @@ def main():
         "candidate_lags": cands,
         "diagonal_runs": runs,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": int(TRAFFIC_PER_PAIR * args.pairs),
-                     "kernel": kernel_name(ctx), "bytes_per_pair": B_PAIR},
+                     "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": int(TRAFFIC_PER_PAIR * args.pairs) if TRAFFIC_PER_PAIR else None,
+                     "kernel": kernel_name(ctx), "kernel_ms": round(k_ms, 4), "bytes_per_pair": B_PAIR,
+                     "copy_gbs": copy["gb_per_s"] if copy else None,
+                     "frac_vs_copy": round(achieved / copy["gb_per_s"], 4) if copy else None},
     }
     if tune:
         out["autotune"] = {"seconds": round(tune_s, 2), "classes": {
@@ -324,12 +389,23 @@ def main():
         out["serve_p50_ms"] = sv["p50_ms"]
         out["serve_p99_ms"] = sv["p99_ms"]
         out["serve"] = sv
-    if args.large:
+    if not args.no_large:
         out["serve_large"] = serve_bench(ctx, args.large_bp, args.serve_seconds, value, LARGE_LOADS, seed=5, large=True)
+    if copy:
+        out["hbm_copy"] = copy
     if not args.no_cpu:
-        threads = max(1, min(16, os.cpu_count() or 1))
-        print(f"[bench] cpu baseline on {threads} threads", file=sys.stderr, flush=True)
+        ncpu, avail = host_cpus()
+        threads = max(1, avail)
+        print(f"[bench] cpu baseline on {threads} threads (host: {ncpu} CPUs)", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(T, Q, min(args.cpu_sample, args.pairs), threads)
+        out["cpu_baseline"]["host_cpus"] = ncpu
+        share = max(1, min(avail, ncpu // 8))  # one of the node's eight GPUs' share of its host cores
+        if share != threads:
+            print(f"[bench] cpu baseline on {share} threads (one GPU's share)", file=sys.stderr, flush=True)
+            out["cpu_baseline_gpu_share"] = cpu_baseline(T, Q, min(args.cpu_sample // 2, args.pairs), share)
+        else:
+            out["cpu_baseline_gpu_share"] = dict(out["cpu_baseline"])
+        out["cpu_baseline_gpu_share"]["share"] = f"min(available CPUs, host CPUs / 8) = min({avail}, {ncpu} // 8)"
     ctx.close()
     print(json.dumps(out), flush=True)
 

@@ -116,6 +116,11 @@ int sxc_xcorr(sxc_ctx* ctx, const float* tsig, const float* qsig, int32_t N, flo
 int sxc_matchup(sxc_ctx* ctx, const char* q, int32_t lq, const char* t, int32_t lt, const float* xc, int32_t N,
                 double cutoff, sxc_seqmatch* out, int32_t cap, int32_t* n_out);
 
+/* SeqAnalyzer::FindTop(xc) with SetTopCutoff(cutoff) (analysis/CrossCorr.cc:878-944): the accepted
+ * lags, ascending (indices into xc; lag = index - N/2).  *n_out receives their number;
+ * SXC_ERR_CAPACITY if it exceeds cap. */
+int sxc_findtop(sxc_ctx* ctx, const float* xc, int32_t N, double cutoff, int32_t* out, int32_t cap, int32_t* n_out);
+
 /* GetMatchProbability(target, query, sT, sQ, len, targetSize) (analysis/AlignProbability.cc:42-127)
  * and PrintMatch's identity (analysis/CrossCorr.cc:967-1037) for n matches. */
 int sxc_match_probability(sxc_ctx* ctx, const char* t, int32_t lt, const char* q, int32_t lq,
@@ -172,6 +177,9 @@ int sxc_batch_set_graph(sxc_batch* batch, int32_t on);
 int sxc_batch_results(sxc_batch* batch, sxc_result* out, int64_t cap, int64_t* n_out);
 /* Accepted lags (FindTop) and diagonal runs (DoOne) of the last run, summed over the batch. */
 int sxc_batch_stats(sxc_batch* batch, int64_t* candidates, int64_t* runs);
+/* The lags the fused pair kernel accepted for one pair and strand in the last run: FindTop's
+ * list (analysis/CrossCorr.cc:878-944) that DoOne then scans, ascending indices into xc. */
+int sxc_batch_lags(sxc_batch* batch, int32_t pair, int32_t strand, int32_t* out, int32_t cap, int32_t* n_out);
 void sxc_batch_destroy(sxc_batch* batch);
 
 /* ---------------------------------------------------------------- serving loop
@@ -219,6 +227,8 @@ typedef struct {
   int32_t slots, max_pairs_per_launch;
   double device_ms;              /* summed device time of all launches */
   int64_t query_spectra_bytes;   /* resident query spectra (0: the pair kernel transforms queries) */
+  int64_t near_threshold;        /* records decided on the host because their device probability fell
+                                    within a few device-libm ulps of min_prob (see sxc_timing) */
 } sxc_server_info;
 
 /* prm->strands is ignored (both strands, as Align). */
@@ -285,6 +295,9 @@ typedef struct {
   double device_ms;     /* first kernel to last copy of the run, per length class, summed */
   int32_t graph;        /* 1 if the run was replayed from a captured graph */
   int32_t pad_;
+  int64_t near_threshold; /* records whose device probability fell within a few device-libm ulps
+                             of min_prob, decided again on the host with its libm (glibc erf/exp,
+                             analysis/AlignProbability.cc:122-124) */
 } sxc_timing;
 int sxc_last_timing(sxc_ctx* ctx, sxc_timing* out);
 

@@ -55,7 +55,8 @@ class Timing(C.Structure):
     _fields_ = [("total_ms", C.c_double), ("upload_ms", C.c_double), ("spectra_ms", C.c_double),
                 ("pair_ms", C.c_double), ("finalize_ms", C.c_double), ("download_ms", C.c_double),
                 ("candidates", C.c_int64), ("runs", C.c_int64), ("correlate_ms", C.c_double),
-                ("scan_ms", C.c_double), ("device_ms", C.c_double), ("graph", C.c_int32), ("pad_", C.c_int32)]
+                ("scan_ms", C.c_double), ("device_ms", C.c_double), ("graph", C.c_int32), ("pad_", C.c_int32),
+                ("near_threshold", C.c_int64)]
 
 
 class TailSeg(C.Structure):
@@ -86,7 +87,7 @@ class ServerInfo(C.Structure):
     _fields_ = [("requests_done", C.c_int64), ("pairs_done", C.c_int64), ("launches", C.c_int64),
                 ("results", C.c_int64), ("resident_bytes", C.c_int64), ("slot_bytes", C.c_int64),
                 ("slots", C.c_int32), ("max_pairs_per_launch", C.c_int32), ("device_ms", C.c_double),
-                ("query_spectra_bytes", C.c_int64)]
+                ("query_spectra_bytes", C.c_int64), ("near_threshold", C.c_int64)]
 
 
 class Geometry(C.Structure):
@@ -110,7 +111,7 @@ assert RESULT_DTYPE.itemsize == C.sizeof(Result) == 72
 EXPORTS = ["sxc_create", "sxc_destroy", "sxc_last_error", "sxc_version", "sxc_encode_signal", "sxc_fft_forward",
            "sxc_xcorr", "sxc_matchup", "sxc_match_probability", "sxc_align_pair", "sxc_align_batch",
            "sxc_batch_create", "sxc_batch_create_tails", "sxc_batch_run", "sxc_batch_set_graph", "sxc_batch_results",
-           "sxc_batch_stats",
+           "sxc_batch_stats", "sxc_batch_lags", "sxc_findtop",
            "sxc_batch_destroy", "sxc_last_timing", "sxc_server_create", "sxc_server_submit", "sxc_server_wait",
            "sxc_server_results", "sxc_server_stats", "sxc_server_destroy", "sxc_autotune", "sxc_set_geometry",
            "sxc_get_geometry"]
@@ -154,6 +155,8 @@ def load() -> C.CDLL:
         L.sxc_get_geometry.argtypes = [vp, i32, i32, C.POINTER(Geometry)]
         L.sxc_batch_results.argtypes = [vp, C.POINTER(Result), i64, C.POINTER(i64)]
         L.sxc_batch_stats.argtypes = [vp, C.POINTER(i64), C.POINTER(i64)]
+        L.sxc_batch_lags.argtypes = [vp, i32, i32, C.POINTER(i32), i32, C.POINTER(i32)]
+        L.sxc_findtop.argtypes = [vp, fp, i32, dbl, C.POINTER(i32), i32, C.POINTER(i32)]
         L.sxc_batch_destroy.argtypes = [vp]
         L.sxc_batch_destroy.restype = None
         L.sxc_last_timing.argtypes = [vp, C.POINTER(Timing)]

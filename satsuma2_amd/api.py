@@ -134,6 +134,15 @@ class Context:
             break
         return np.frombuffer(buf, np.int32, 3 * n.value).reshape(-1, 3).copy()
 
+    def findtop(self, xc: np.ndarray, cutoff: float) -> np.ndarray:
+        """SeqAnalyzer::FindTop (analysis/CrossCorr.cc:878-944): accepted lag indices."""
+        xc = np.ascontiguousarray(xc, np.float32)
+        out = np.zeros(xc.size, np.int32)
+        n = C.c_int32(0)
+        check(self.L.sxc_findtop(self._h, _fp(xc), xc.size, cutoff, out.ctypes.data_as(C.POINTER(C.c_int32)),
+                                 out.size, C.byref(n)))
+        return out[: n.value].copy()
+
     def match_probability(self, t: bytes, q: bytes, runs: np.ndarray, target_size: float):
         runs = np.ascontiguousarray(runs, np.int32).reshape(-1, 3)
         n = runs.shape[0]
@@ -266,6 +275,15 @@ class ResidentBatch:
         c, r = C.c_int64(0), C.c_int64(0)
         check(self.ctx.L.sxc_batch_stats(self._h, C.byref(c), C.byref(r)))
         return c.value, r.value
+
+    def lags(self, pair: int, strand: int) -> np.ndarray:
+        """The lags the pair kernel accepted (FindTop) for one pair and strand in the last run."""
+        cap = 1 << 15
+        out = np.zeros(cap, np.int32)
+        n = C.c_int32(0)
+        check(self.ctx.L.sxc_batch_lags(self._h, pair, strand, out.ctypes.data_as(C.POINTER(C.c_int32)), cap,
+                                        C.byref(n)))
+        return out[: n.value].copy()
 
     def close(self):
         if self._h:

@@ -70,7 +70,7 @@ int batch_stream(sxc_batch_dev& b) {
   if (b.s) return SXC_OK;
   HIPCHK(hipStreamCreateWithFlags(&b.s, hipStreamNonBlocking));
   for (hipEvent_t& e : b.ev) HIPCHK(hipEventCreate(&e));
-  HIPCHK(hipHostMalloc((void**)&b.h_ctl, 2 * sizeof(int64_t), hipHostMallocDefault));
+  HIPCHK(hipHostMalloc((void**)&b.h_ctl, 3 * sizeof(int64_t), hipHostMallocDefault));
   return SXC_OK;
 }
 
@@ -87,6 +87,10 @@ int batch_upload(sxc_ctx* ctx, sxc_batch_dev& b) {
   HIPCHK(b.pairs.ensure(hb.pairs.size() * sizeof(sxc_pair_dev)));
   HIPCHK(b.spectra.ensure(hb.tchunks.size() * 4 * (size_t)N * sizeof(float)));
   HIPCHK(b.which.ensure(hb.tchunks.size() * sizeof(int32_t)));
+  if (!hb.ent.empty()) {
+    HIPCHK(b.ent.ensure(hb.ent.size() * sizeof(float)));
+    HIPCHK(hipMemcpyAsync(b.ent.p, hb.ent.data(), hb.ent.size() * sizeof(float), hipMemcpyHostToDevice, s));
+  }
   if (!hb.seq.empty()) HIPCHK(hipMemcpyAsync(b.seq.p, hb.seq.data(), hb.seq.size(), hipMemcpyHostToDevice, s));
   if (!hb.tchunks.empty())
     HIPCHK(hipMemcpyAsync(b.tch.p, hb.tchunks.data(), hb.tchunks.size() * sizeof(sxc_chunk), hipMemcpyHostToDevice, s));
@@ -125,8 +129,11 @@ int batch_alloc(sxc_ctx* ctx, sxc_batch_dev& b) {
   HIPCHK(b.counts.ensure((size_t)total * sizeof(int32_t)));
   HIPCHK(b.stats.ensure((size_t)total * 2 * sizeof(int32_t)));
   HIPCHK(b.offsets.ensure((size_t)total * sizeof(int64_t)));
-  HIPCHK(b.total.ensure(sizeof(int64_t)));
+  HIPCHK(b.total.ensure(2 * sizeof(int64_t)));
   HIPCHK(b.maxc.ensure(sizeof(int32_t)));
+  const void* had = b.nflag.p;
+  HIPCHK(b.nflag.ensure(sizeof(int32_t)));
+  if (b.nflag.p != had) HIPCHK(hipMemsetAsync(b.nflag.p, 0, sizeof(int32_t), b.s));  // before any capture
   if (b.out_cap == 0) b.out_cap = std::max<int64_t>(1024, 2 * (int64_t)total);
   HIPCHK(b.final_.ensure((size_t)b.out_cap * sizeof(sxc_result_dev)));
   b.ptable = nullptr;
@@ -143,6 +150,7 @@ int launch_correlate(sxc_ctx* ctx, sxc_batch_dev& b) {
   a.tab = ctx->d_tab;
   a.osc = ctx->d_osc;
   a.ent_lut = ctx->d_ent[b.nb];
+  a.ent_rows = b.hb.ent.empty() ? nullptr : b.ent.as<float>();
   a.seq = b.seq.as<unsigned char>();
   a.tchunks = b.tch.as<sxc_chunk>();
   a.qchunks = b.qch.as<sxc_chunk>();
@@ -178,9 +186,8 @@ int launch_pairs(sxc_ctx* ctx, sxc_batch_dev& b) {
   a.strands = b.prm.strands;
   a.min_len = b.prm.min_len;
   a.q_chunk = b.prm.q_chunk;
-  a.min_prob = b.prm.min_prob;
-  a.target_total = b.prm.target_total;
-  a.z_skip = sxc::keep_z_bound(b.prm.min_prob, b.prm.target_total);
+  sxc::set_keep(a, b.prm.min_prob, b.prm.target_total);
+  a.nflag = b.nflag.as<int32_t>();
   a.mode = 0;
   a.out = b.slots.as<sxc_result_dev>();
   a.runs = nullptr;
@@ -211,6 +218,7 @@ int launch_target_spectra(sxc_ctx* ctx, sxc_batch_dev& b) {
   sa.tab = ctx->d_tab;
   sa.osc = ctx->d_osc;
   sa.ent_lut = ctx->d_ent[b.nb];
+  sa.ent_rows = b.hb.ent.empty() ? nullptr : b.ent.as<float>();
   sa.seq = b.seq.as<unsigned char>();
   sa.chunks = b.tch.as<sxc_chunk>();
   sa.which = b.which.as<int32_t>();
@@ -235,8 +243,9 @@ int batch_enqueue(sxc_ctx* ctx, sxc_batch_dev& b, bool events) {
   if (events) HIPCHK(hipEventRecord(b.ev[0], s));
   if (!b.hb.schunks.empty()) {
     // signals of the sequences named by signal tails
-    sxc::EncodeArgs ea{ctx->d_tab, ctx->d_ent[b.nb], b.seq.as<unsigned char>(), b.sch.as<sxc_chunk>(), 0,
-                       b.sstr.as<int32_t>(), b.src_sig.as<float>()};
+    sxc::EncodeArgs ea{ctx->d_tab, ctx->d_ent[b.nb], b.hb.ent.empty() ? nullptr : b.ent.as<float>(),
+                       b.seq.as<unsigned char>(), b.sch.as<sxc_chunk>(), 0, b.sstr.as<int32_t>(),
+                       b.src_sig.as<float>()};
     HIPCHK(sxc::launch_encode(b.nb, ea, (int)b.hb.schunks.size(), s));
   }
   int rc = launch_target_spectra(ctx, b);
@@ -249,12 +258,12 @@ int batch_enqueue(sxc_ctx* ctx, sxc_batch_dev& b, bool events) {
   if (rc) return rc;
   if (events) HIPCHK(hipEventRecord(b.ev[3], s));
   HIPCHK(sxc::launch_count_offsets(b.counts.as<int32_t>(), total, b.slot_cap, b.offsets.as<int64_t>(),
-                                   b.total.as<int64_t>(), b.maxc.as<int32_t>(), s));
+                                   b.total.as<int64_t>(), b.maxc.as<int32_t>(), s, b.nflag.as<int32_t>()));
   HIPCHK(sxc::launch_order_records(b.counts.as<int32_t>(), total, b.slot_cap, b.offsets.as<int64_t>(),
                                    b.slots.as<sxc_result_dev>(), b.keys.as<uint64_t>(),
                                    b.final_.as<sxc_result_dev>(), b.out_cap, s));
   HIPCHK(hipMemcpyAsync(&b.h_ctl[0], b.maxc.p, sizeof(int32_t), hipMemcpyDeviceToHost, s));
-  HIPCHK(hipMemcpyAsync(&b.h_ctl[1], b.total.p, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipMemcpyAsync(&b.h_ctl[1], b.total.p, 2 * sizeof(int64_t), hipMemcpyDeviceToHost, s));
   if (events) HIPCHK(hipEventRecord(b.ev[4], s));
   return SXC_OK;
 }
@@ -265,7 +274,7 @@ uint64_t graph_signature(const sxc_ctx* ctx, const sxc_batch_dev& b) {
   auto mix = [&](uint64_t v) { h = (h ^ v) * 1099511628211ull; };
   for (const DevBuf* d : {&b.seq, &b.tch, &b.qch, &b.pairs, &b.spectra, &b.which, &b.slots, &b.keys, &b.counts,
                           &b.offsets, &b.total, &b.maxc, &b.final_, &b.stats, &b.cands, &b.ncand, &b.sch, &b.sstr,
-                          &b.tails, &b.segs, &b.ttail, &b.src_sig})
+                          &b.tails, &b.segs, &b.ttail, &b.src_sig, &b.ent, &b.nflag})
     mix((uint64_t)(uintptr_t)d->p);
   mix((uint64_t)(uintptr_t)b.ptable);
   mix((uint64_t)(uintptr_t)b.h_ctl);
@@ -299,7 +308,7 @@ int batch_begin(sxc_ctx* ctx, sxc_batch_dev& b) {
   }
   rc = batch_alloc(ctx, b);
   if (rc) return rc;
-  b.h_ctl[0] = b.h_ctl[1] = 0;
+  b.h_ctl[0] = b.h_ctl[1] = b.h_ctl[2] = 0;
   if (b.use_graph && b.runs > 0) {
     const uint64_t sig = graph_signature(ctx, b);
     if (!b.gexec || b.gsig != sig) {
@@ -328,6 +337,42 @@ int batch_begin(sxc_ctx* ctx, sxc_batch_dev& b) {
     if (rc) return rc;
   }
   ++b.runs;
+  return SXC_OK;
+}
+
+// The scan flagged records whose device probability is within the keep band of min_prob:
+// decide them on the host with its libm (resolve_record), drop those below min_prob, and put
+// the ordered records and the per-(pair, strand) offsets back in place.  Flagged records are
+// rare (none in most runs), so this costs one copy each way when it happens.
+int batch_resolve(sxc_batch_dev& b) {
+  hipStream_t s = b.s;
+  const int total = (int)b.hb.pairs.size() * b.prm.strands;
+  const int64_t n = b.n_results;
+  std::vector<sxc_result_dev> rec((size_t)n);
+  std::vector<int64_t> off((size_t)total + 1);
+  if (n > 0) HIPCHK(hipMemcpyAsync(rec.data(), b.final_.p, (size_t)n * sizeof(sxc_result_dev), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipMemcpyAsync(off.data(), b.offsets.p, (size_t)total * sizeof(int64_t), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  off[(size_t)total] = n;
+  const sxc::ResolveCtx rc{b.hb.seq.data(), b.hb.tchunks.data(), b.hb.qchunks.data(), b.hb.pairs.data(),
+                           (int64_t)b.hb.pairs.size(), b.prm.q_chunk, b.prm.min_prob, b.prm.target_total};
+  int64_t kept = 0, decided = 0;
+  for (int ps = 0; ps < total; ++ps) {
+    const int64_t lo = off[(size_t)ps], hi = off[(size_t)ps + 1];
+    off[(size_t)ps] = kept;
+    for (int64_t i = lo; i < hi; ++i) {
+      decided += rec[(size_t)i].pad_[0] == SXC_REC_NEAR;
+      const int k = sxc::resolve_record(rec[(size_t)i], rc);
+      if (k < 0) return fail(SXC_ERR_HIP, "a record flagged near min_prob does not fit its chunk pair");
+      if (k) rec[(size_t)kept++] = rec[(size_t)i];
+    }
+  }
+  if (kept > 0)
+    HIPCHK(hipMemcpyAsync(b.final_.p, rec.data(), (size_t)kept * sizeof(sxc_result_dev), hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(b.offsets.p, off.data(), (size_t)total * sizeof(int64_t), hipMemcpyHostToDevice, s));
+  HIPCHK(hipStreamSynchronize(s));
+  b.n_results = kept;
+  b.resolved = decided;
   return SXC_OK;
 }
 
@@ -363,8 +408,8 @@ int batch_end(sxc_ctx* ctx, sxc_batch_dev& b) {
     rc = launch_pairs(ctx, b);
     if (rc) return rc;
     HIPCHK(sxc::launch_count_offsets(b.counts.as<int32_t>(), total, b.slot_cap, b.offsets.as<int64_t>(),
-                                     b.total.as<int64_t>(), b.maxc.as<int32_t>(), s));
-    HIPCHK(hipMemcpyAsync(&b.h_ctl[1], b.total.p, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+                                     b.total.as<int64_t>(), b.maxc.as<int32_t>(), s, b.nflag.as<int32_t>()));
+    HIPCHK(hipMemcpyAsync(&b.h_ctl[1], b.total.p, 2 * sizeof(int64_t), hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
     tot = b.h_ctl[1];
     order = true;
@@ -381,6 +426,8 @@ int batch_end(sxc_ctx* ctx, sxc_batch_dev& b) {
     HIPCHK(hipStreamSynchronize(s));
   }
   b.n_results = tot;
+  b.resolved = 0;
+  if (b.h_ctl[2] > 0) return batch_resolve(b);
   return SXC_OK;
 }
 
@@ -487,6 +534,7 @@ int groups_prepare(std::vector<Group>& gs, const sxc_pair* pairs, int32_t n, con
 // queued on their own streams before the first is waited for, so length classes overlap.
 int groups_run(sxc_ctx* ctx, std::vector<Group>& gs, sxc_timing& t, bool upload = false) {
   t.spectra_ms = t.correlate_ms = t.scan_ms = t.finalize_ms = 0;
+  t.near_threshold = 0;
   for (Group& g : gs) {
     const auto t0 = std::chrono::steady_clock::now();
     int rc = upload ? batch_upload(ctx, *g.b) : SXC_OK;
@@ -503,6 +551,7 @@ int groups_run(sxc_ctx* ctx, std::vector<Group>& gs, sxc_timing& t, bool upload 
     t.correlate_ms += g.b->correlate_ms;
     t.scan_ms += g.b->scan_ms;
     t.finalize_ms += g.b->finalize_ms;
+    t.near_threshold += g.b->resolved;
   }
   t.pair_ms = t.correlate_ms + t.scan_ms;
   return SXC_OK;
@@ -715,18 +764,9 @@ extern "C" {
 
 const char* sxc_last_error(void) { return sxc::g_err.c_str(); }
 
-static const sxc_tables& host_tables() {
-  static const sxc_tables t = [] {
-    sxc_tables x;
-    sxc::build_tables(&x);
-    return x;
-  }();
-  return t;
-}
-
 int sxc_reverse_complement(char* seq, int32_t len) {
   if (len < 0 || (len > 0 && !seq)) return fail(SXC_ERR_ARG, "null argument");
-  const sxc_tables& t = host_tables();
+  const sxc_tables& t = sxc::host_tables();
   for (int32_t i = 0, j = len - 1; i < (len + 1) / 2; ++i, --j) {
     const char a = (char)t.rc[(unsigned char)seq[i] & 127], b = (char)t.rc[(unsigned char)seq[j] & 127];
     seq[i] = b;
@@ -736,7 +776,7 @@ int sxc_reverse_complement(char* seq, int32_t len) {
 }
 
 double sxc_dna_equal(char a, char b) {
-  const sxc_tables& t = host_tables();
+  const sxc_tables& t = sxc::host_tables();
   return t.equal[t.cls[(unsigned char)a & 127]][t.cls[(unsigned char)b & 127]];
 }
 const char* sxc_version(void) { return "sxc 0.1 gfx950"; }
@@ -973,6 +1013,47 @@ int sxc_batch_stats(sxc_batch* bt, int64_t* candidates, int64_t* runs) {
   return SXC_OK;
 }
 
+int sxc_batch_lags(sxc_batch* bt, int32_t pair, int32_t strand, int32_t* out, int32_t cap, int32_t* n_out) {
+  if (!bt || !n_out || (cap > 0 && !out)) return fail(SXC_ERR_ARG, "null argument");
+  if (pair < 0 || pair >= bt->n) return fail(SXC_ERR_ARG, "no such pair");
+  std::lock_guard<std::mutex> g(bt->ctx->mu);
+  int rc = set_device(bt->ctx);
+  if (rc) return rc;
+  // the pair's length class and its index there
+  const Group* gr = nullptr;
+  int32_t idx = pair;
+  for (const Group& x : bt->groups) {
+    if (x.idx.empty()) {
+      gr = &x;
+      break;
+    }
+    auto it = std::lower_bound(x.idx.begin(), x.idx.end(), pair);
+    if (it != x.idx.end() && *it == pair) {
+      gr = &x;
+      idx = (int32_t)(it - x.idx.begin());
+      break;
+    }
+  }
+  if (!gr) return fail(SXC_ERR_ARG, "no such pair");
+  const sxc_batch_dev& b = *gr->b;
+  if (strand < 0 || strand >= b.prm.strands) return fail(SXC_ERR_ARG, "no such strand");
+  if (b.runs == 0) return fail(SXC_ERR_ARG, "the batch has not run");
+  const int64_t ps = (int64_t)idx * b.prm.strands + strand;
+  hipStream_t s = bt->ctx->stream;
+  int32_t n = 0;
+  HIPCHK(hipMemcpyAsync(&n, b.ncand.as<int32_t>() + ps, sizeof n, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  *n_out = n;
+  if (n > cap) return fail(SXC_ERR_CAPACITY, "output capacity too small");
+  std::vector<uint16_t> lags((size_t)n);
+  if (n > 0)
+    HIPCHK(hipMemcpyAsync(lags.data(), b.cands.as<uint16_t>() + ps * b.prm.N, (size_t)n * sizeof(uint16_t),
+                          hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  for (int32_t i = 0; i < n; ++i) out[i] = lags[(size_t)i];
+  return SXC_OK;
+}
+
 void sxc_batch_destroy(sxc_batch* bt) {
   if (!bt) return;
   {
@@ -1002,16 +1083,23 @@ int sxc_encode_signal(sxc_ctx* ctx, const char* seq, int32_t len, int32_t N, flo
     if (sxc::class_has_thirds(sxc::class_of(b))) ck.flags |= 1;
   }
   hipStream_t s = ctx->stream;
+  std::vector<float> erow;
+  if (ck.flags & 1) {  // row 0 of a one-row entropy table
+    erow.resize(SXC_ENT_ROW);
+    sxc::window_entropy_row((const unsigned char*)seq, len, N, erow.data());
+    HIPCHK(ctx->p3.ensure(erow.size() * sizeof(float)));
+    HIPCHK(hipMemcpyAsync(ctx->p3.p, erow.data(), erow.size() * sizeof(float), hipMemcpyHostToDevice, s));
+  }
   HIPCHK(ctx->p0.ensure((size_t)len + 16));
   HIPCHK(ctx->p1.ensure(sizeof ck));
   HIPCHK(ctx->p2.ensure((size_t)4 * N * sizeof(float)));
   if (len) HIPCHK(hipMemcpyAsync(ctx->p0.p, seq, len, hipMemcpyHostToDevice, s));
   HIPCHK(hipMemcpyAsync(ctx->p1.p, &ck, sizeof ck, hipMemcpyHostToDevice, s));
-  sxc::EncodeArgs a{ctx->d_tab, ctx->d_ent[nb], ctx->p0.as<unsigned char>(), ctx->p1.as<sxc_chunk>(), 0, nullptr,
-                    ctx->p2.as<float>()};
+  sxc::EncodeArgs a{ctx->d_tab, ctx->d_ent[nb], erow.empty() ? nullptr : ctx->p3.as<float>(),
+                    ctx->p0.as<unsigned char>(), ctx->p1.as<sxc_chunk>(), 0, nullptr, ctx->p2.as<float>()};
   HIPCHK(sxc::launch_encode(nb, a, 1, s));
   HIPCHK(hipMemcpyAsync(out, ctx->p2.p, (size_t)4 * N * sizeof(float), hipMemcpyDeviceToHost, s));
-  HIPCHK(hipStreamSynchronize(s));
+  HIPCHK(hipStreamSynchronize(s));  // erow is a host temporary
   return SXC_OK;
 }
 
@@ -1161,6 +1249,33 @@ int sxc_matchup(sxc_ctx* ctx, const char* q, int32_t lq, const char* t, int32_t 
   HIPCHK(sxc::launch_order_runs(ctx->p3.as<int32_t>(), ctx->p4.as<uint64_t>(), cnt, ctx->p6.as<int32_t>(), s));
   if (cnt) HIPCHK(hipMemcpyAsync(out, ctx->p6.p, (size_t)cnt * 3 * sizeof(int32_t), hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));
+  return SXC_OK;
+}
+
+int sxc_findtop(sxc_ctx* ctx, const float* xc, int32_t N, double cutoff, int32_t* out, int32_t cap, int32_t* n_out) {
+  if (!ctx || !xc || !n_out || (cap > 0 && !out)) return fail(SXC_ERR_ARG, "null argument");
+  int nb;
+  int rc = check_N(N, &nb);
+  if (rc) return rc;
+  std::lock_guard<std::mutex> g(ctx->mu);
+  rc = set_device(ctx);
+  if (rc) return rc;
+  hipStream_t s = ctx->stream;
+  HIPCHK(ctx->p2.ensure((size_t)N * sizeof(float) + (size_t)N * sizeof(uint16_t) + 2 * sizeof(int32_t)));
+  float* d_xc = ctx->p2.as<float>();
+  uint16_t* d_cands = (uint16_t*)(d_xc + N);
+  int32_t* d_n = (int32_t*)(d_cands + N);
+  HIPCHK(hipMemcpyAsync(d_xc, xc, (size_t)N * sizeof(float), hipMemcpyHostToDevice, s));
+  HIPCHK(sxc::launch_peaks(nb, d_xc, cutoff, d_cands, d_n, s));
+  int32_t n = 0;
+  HIPCHK(hipMemcpyAsync(&n, d_n, sizeof n, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  *n_out = n;
+  if (n > cap) return fail(SXC_ERR_CAPACITY, "output capacity too small");
+  std::vector<uint16_t> lags((size_t)n);
+  if (n > 0) HIPCHK(hipMemcpyAsync(lags.data(), d_cands, (size_t)n * sizeof(uint16_t), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  for (int32_t i = 0; i < n; ++i) out[i] = lags[(size_t)i];
   return SXC_OK;
 }
 

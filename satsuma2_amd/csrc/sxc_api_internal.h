@@ -59,6 +59,8 @@ using sxc::DevBuf;
 struct sxc_batch_dev {
   DevBuf seq, tch, qch, pairs, spectra, which, slots, keys, counts, offsets, total, maxc, final_, stats, cands, ncand;
   DevBuf sch, sstr, tails, segs, ttail, src_sig;  // signal tails
+  DevBuf ent;    // host-computed window entropies of chunks with B/D/H/V letters
+  DevBuf nflag;  // records the last scan flagged near min_prob (the counts kernel reads and clears it)
   sxc::HostBatch hb;
   sxc_params prm{};
   int nb = 0;
@@ -70,8 +72,10 @@ struct sxc_batch_dev {
   // sub-batches of different length classes run side by side.
   hipStream_t s = nullptr;
   hipEvent_t ev[5] = {};
-  int64_t* h_ctl = nullptr;  // [0] largest per-(pair, strand) count, [1] total records
+  int64_t* h_ctl = nullptr;  // [0] largest per-(pair, strand) count, [1] total records, [2] records
+                             // flagged near min_prob
   int64_t out_cap = 0;       // records final_ holds
+  int64_t resolved = 0;      // records of the last run decided on the host (near min_prob)
   const double* ptable = nullptr;
   // The run's launch sequence captured as a graph, replayed while the buffers it was
   // captured with (signature) stay in place.
@@ -84,7 +88,7 @@ struct sxc_batch_dev {
     if (s) (void)hipStreamSynchronize(s);
     if (gexec) (void)hipGraphExecDestroy(gexec), gexec = nullptr;
     for (DevBuf* b : {&seq, &tch, &qch, &pairs, &spectra, &which, &slots, &keys, &counts, &offsets, &total, &maxc,
-                      &final_, &stats, &cands, &ncand, &sch, &sstr, &tails, &segs, &ttail, &src_sig})
+                      &final_, &stats, &cands, &ncand, &sch, &sstr, &tails, &segs, &ttail, &src_sig, &ent, &nflag})
       b->release();
     for (hipEvent_t& e : ev)
       if (e) (void)hipEventDestroy(e), e = nullptr;

@@ -7,6 +7,8 @@
 //   qchunks  the same for distinct query chunks
 //   pairs    one sxc_pair_dev per chunk pair (indices into the chunk tables + the
 //            chunk metadata FilterMatches writes into t_result)
+//   ent      window entropies of the chunks with B/D/H/V letters, computed here with the
+//            host's log (SXC_ENT_ROW floats per such chunk; sxc_chunk flags bits 8-31)
 // Chunks are deduplicated by (pointer, length), so a target shared by many pairs, as in
 // the slave's align_target blocks (analysis/HomologyByXCorrSlave.cc:290-299), is
 // transformed once.
@@ -27,6 +29,7 @@ struct HostBatch {
   std::vector<unsigned char> seq;
   std::vector<sxc_chunk> tchunks, qchunks;
   std::vector<sxc_pair_dev> pairs;
+  std::vector<float> ent;
   int max_len = 0;
   // signal tails (sxc_batch_create_tails): source sequences, their strands, the tail and
   // segment tables, and the tail of each target chunk (-1: none)
@@ -59,6 +62,13 @@ inline std::string pack_batch(const sxc_pair* in, int n, int N, HostBatch& hb,
       if (cl > 3) c.flags &= ~2;
       if (cl > 3 && cl != SXC_CLASS_N) c.flags &= ~4;
       hb.seq.push_back(b);
+    }
+    if (c.flags & 1) {
+      const size_t row = hb.ent.size() / SXC_ENT_ROW;
+      if (row >= (1u << 24)) return "too many chunks with B, D, H or V letters";
+      c.flags |= (int32_t)(row << 8);
+      hb.ent.resize(hb.ent.size() + SXC_ENT_ROW);
+      window_entropy_row(hb.seq.data() + c.off, len, N, hb.ent.data() + row * SXC_ENT_ROW);
     }
     while (hb.seq.size() % 16) hb.seq.push_back(0);  // next chunk 16-byte aligned
     if (len > hb.max_len) hb.max_len = len;

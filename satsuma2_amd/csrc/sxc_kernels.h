@@ -20,9 +20,10 @@ hipError_t launch_xcorr(int nb, const XcorrArgs& a, int n, hipStream_t s);
 hipError_t launch_prob(const ProbArgs& a, hipStream_t s);
 hipError_t launch_order_runs(const int32_t* runs, const uint64_t* keys, int n, int32_t* out, hipStream_t s);
 // Exclusive scan of the per-(pair, strand) record counts (clipped at cap) into offsets, their
-// total and the largest unclipped count.
+// total and the largest unclipped count.  With nflag (a counter the scan raises for each record
+// it flags near min_prob), total[1] receives the counter and the counter is cleared.
 hipError_t launch_count_offsets(const int32_t* counts, int n, int cap, int64_t* offsets, int64_t* total,
-                                int32_t* max_count, hipStream_t s);
+                                int32_t* max_count, hipStream_t s, int32_t* nflag = nullptr);
 // Orders each (pair, strand)'s slot records and packs them at its offset; records at or past
 // out_cap are dropped.
 hipError_t launch_order_records(const int32_t* counts, int n, int cap, const int64_t* offsets,

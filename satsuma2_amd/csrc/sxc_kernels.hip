@@ -102,8 +102,10 @@ __global__ void k_prob(ProbArgs a) {
 }
 
 // Exclusive scan of per-(pair, strand) kept counts (clipped to cap) -> offsets, total.
+// With nflag: total[1] receives the scan's count of records flagged near min_prob, and the
+// counter is cleared for the next scan.
 __global__ __launch_bounds__(1024) void k_scan_counts(const int32_t* counts, int n, int cap, int64_t* offsets,
-                                                      int64_t* total, int32_t* max_count) {
+                                                      int64_t* total, int32_t* max_count, int32_t* nflag) {
   __shared__ int64_t part[1024];
   __shared__ int mx[1024];
   const int t = threadIdx.x;
@@ -130,6 +132,10 @@ __global__ __launch_bounds__(1024) void k_scan_counts(const int32_t* counts, int
     }
     *total = acc;
     *max_count = mm;
+    if (nflag) {
+      total[1] = *nflag;
+      *nflag = 0;
+    }
   }
   __syncthreads();
   int64_t acc = part[t];
@@ -277,9 +283,9 @@ hipError_t launch_prob(const ProbArgs& a, hipStream_t s) {
 }
 
 hipError_t launch_count_offsets(const int32_t* counts, int n, int cap, int64_t* offsets, int64_t* total,
-                                int32_t* max_count, hipStream_t s) {
+                                int32_t* max_count, hipStream_t s, int32_t* nflag) {
   if (n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(1024), 0, s, counts, n, cap, offsets, total, max_count);
+  hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(1024), 0, s, counts, n, cap, offsets, total, max_count, nflag);
   return hipGetLastError();
 }
 

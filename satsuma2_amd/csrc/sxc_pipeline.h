@@ -26,6 +26,8 @@ struct PairArgs {
   const sxc_tables* tab;
   const float* osc;          // oscillator twiddles for levels 13, 14
   const double* ent_lut;     // entropy of quarter-exact windows for this N, [k-1][q]
+  const float* ent_rows;     // host-computed window entropies of chunks with B/D/H/V letters
+                             // (sxc_chunk flags bits 8-31 index them)
   const unsigned char* seq;  // all chunk bytes
   const sxc_chunk* tchunks;
   const sxc_chunk* qchunks;
@@ -52,6 +54,7 @@ struct SpecArgs {
   const sxc_tables* tab;
   const float* osc;
   const double* ent_lut;
+  const float* ent_rows;
   const unsigned char* seq;
   const sxc_chunk* chunks;
   const int32_t* which;      // chunk indices to transform
@@ -69,6 +72,7 @@ struct SpecArgs {
 struct EncodeArgs {
   const sxc_tables* tab;
   const double* ent_lut;
+  const float* ent_rows;
   const unsigned char* seq;
   const sxc_chunk* chunks;
   int32_t strand;
@@ -151,10 +155,13 @@ struct Regs {
 
 // ---------------------------------------------------------------- helpers
 
-SXC_HD inline double ent_term(double p) {
-  // Ent (analysis/CrossCorr.cc:28-33)
-  if (p < 0.001) return 0;
-  return p * log(p) / 0.69314718056;
+// The window entropies of a chunk with a 1/3-valued letter (B, D, H, V) for one strand, or
+// null.  Such windows sum thirds in the reference's order and take glibc's log of the
+// result (Ent, analysis/CrossCorr.cc:28-33), which the device cannot reproduce bit for bit,
+// so the host computes them with its libm (sxc_host_exact.h) and the kernels read them.
+SXC_HD inline const float* ent_row(const float* rows, const sxc_chunk& c, int strand) {
+  if (!(c.flags & 1) || !rows) return nullptr;
+  return rows + (size_t)((uint32_t)c.flags >> 8) * SXC_ENT_ROW + (size_t)strand * (SXC_ENT_ROW / 2);
 }
 
 SXC_HD inline int score_of(const unsigned char* cls, const unsigned char* score, unsigned char tb, unsigned char qb) {
@@ -175,10 +182,12 @@ struct SeqView {
 };
 
 // Loads one chunk's bytes (reverse-complemented for strand 1) and computes the per-window
-// entropy weights and the four channel means.
+// entropy weights (CCSignal::ComputeEntropy, analysis/CrossCorr.cc:35-93) and the four
+// channel means.  `erow`: the host-computed window entropies when `thirds` (ent_row).
 template <int NB, class Ex>
 SXC_HD inline SeqView phase_prepare(Ex& ex, char* sm, const sxc_tables* tab, const double* ent_lut,
-                                    const unsigned char* src, int L, int thirds, int strand) {
+                                    const float* erow, const unsigned char* src, int L, int thirds,
+                                    int strand) {
   using SM = Smem<NB>;
   using G = Geo<NB>;
   float* lut = (float*)(sm + SM::LUT);
@@ -217,35 +226,24 @@ SXC_HD inline SeqView phase_prepare(Ex& ex, char* sm, const sxc_tables* tab, con
     if (L < 1024) return;  // entropy weight is 1 (analysis/CrossCorr.cc:39-44)
     const int nwin = (L + win - 1) / win;
     for (int w = tid; w < nwin; w += G::NT) {
+      if (thirds) {
+        ent[w] = erow[w];  // computed on the host with glibc's log
+        continue;
+      }
+      // quarter-exact window: the four sums are integers of quarters, Ent of each is a
+      // table entry computed on the host
       const int i = w * win;
       const int k = (L - i < win) ? (L - i) : win;
-      double s;
-      if (!thirds) {
-        int q[4] = {0, 0, 0, 0};
-        for (int j = i; j < i + k; ++j) {
-          const unsigned char* qq = quart + 4 * cls[qs[j]];
-          q[0] += qq[0];
-          q[1] += qq[1];
-          q[2] += qq[2];
-          q[3] += qq[3];
-        }
-        const double* row = ent_lut + (size_t)(k - 1) * (4 * win + 1);
-        s = row[q[0]] + row[q[1]] + row[q[2]] + row[q[3]];
-      } else {
-        double a = 0, c = 0, g = 0, t = 0;
-        for (int j = i; j < i + k; ++j) {
-          const double* cd = codec + 4 * cls[qs[j]];
-          a += cd[0];
-          c += cd[1];
-          g += cd[2];
-          t += cd[3];
-        }
-        a /= (double)k;
-        c /= (double)k;
-        g /= (double)k;
-        t /= (double)k;
-        s = ent_term(a) + ent_term(c) + ent_term(g) + ent_term(t);
+      int q[4] = {0, 0, 0, 0};
+      for (int j = i; j < i + k; ++j) {
+        const unsigned char* qq = quart + 4 * cls[qs[j]];
+        q[0] += qq[0];
+        q[1] += qq[1];
+        q[2] += qq[2];
+        q[3] += qq[3];
       }
+      const double* row = ent_lut + (size_t)(k - 1) * (4 * win + 1);
+      const double s = row[q[0]] + row[q[1]] + row[q[2]] + row[q[3]];
       float e = (float)(-s);
       if (e < 0.f) e = 0.f;
       ent[w] = e;
@@ -381,7 +379,8 @@ SXC_HD inline void encode_body(Ex& ex, char* sm, const EncodeArgs& a, int k) {
   using G = Geo<NB>;
   const sxc_chunk ck = a.chunks[k];
   const int strand = a.strands ? a.strands[k] : a.strand;
-  const SeqView qs = phase_prepare<NB>(ex, sm, a.tab, a.ent_lut, a.seq + ck.off, ck.len, ck.flags & 1, strand);
+  const SeqView qs = phase_prepare<NB>(ex, sm, a.tab, a.ent_lut, ent_row(a.ent_rows, ck, strand), a.seq + ck.off,
+                                       ck.len, ck.flags & 1, strand);
   const TailRef none{nullptr, 0, nullptr};
   ex.each([&](int tid) {
     for (int ch = 0; ch < 4; ++ch)
@@ -400,7 +399,8 @@ SXC_HD inline void spectra_body(Ex& ex, char* sm, const SpecArgs& a, int k) {
   const Twiddle tw{twiddle_table<NB>(sm, a.tab), a.osc};
   SeqView qs{nullptr, nullptr, ck.len};
   if (!a.signals) {
-    qs = phase_prepare<NB>(ex, sm, a.tab, a.ent_lut, a.seq + ck.off, ck.len, ck.flags & 1, a.strand);
+    qs = phase_prepare<NB>(ex, sm, a.tab, a.ent_lut, ent_row(a.ent_rows, ck, a.strand), a.seq + ck.off, ck.len,
+                           ck.flags & 1, a.strand);
   } else if constexpr (!SM::LEAN) {
     float* lut = (float*)(sm + SM::LUT);
     ex.each([&](int tid) {
@@ -467,6 +467,9 @@ SXC_HD inline void correlate(Ex& ex, char* sm, const sxc_tables* tab, const Twid
   const float mul = (float)(1.0 / (double)G::N);  // FFTReal::rescale (FFTReal.hpp:264)
   for (int ch = 0; ch < 4; ++ch) {
     if (qspec4) {
+      // the load below writes Y with another thread mapping than the previous channel's
+      // inverse stage A read it: every thread must have read before any thread writes
+      if (ch > 0) ex.sync();
       const float* qch = qspec4 + (size_t)ch * G::N;
       ex.each([&](int tid) {
         for (int p = tid; p < G::N; p += G::NT) Y[pad(p)] = qch[p];
@@ -489,7 +492,8 @@ SXC_HD inline void correlate(Ex& ex, char* sm, const sxc_tables* tab, const Twid
         r.acc[k] = r.acc[k] + x;
       });
     });
-    // the next channel's stage A writes the same positions from the same thread
+    // the next channel's forward stage A writes the same positions from the same thread (no
+    // barrier needed); loading a resident query spectrum does not (barrier above)
   }
   ex.sync();
   ex.each([&](int g) {
@@ -613,7 +617,8 @@ SXC_HD inline void correlate_body(Ex& ex, char* sm, const PairArgs& a, int ps) {
                   nullptr, TailRef{nullptr, 0, nullptr},
                   a.qspectra + ((size_t)pr.q_chunk * 2 + strand) * 4 * G::N);
   } else {
-    const SeqView qs = phase_prepare<NB>(ex, sm, a.tab, a.ent_lut, a.seq + qc.off, qc.len, qc.flags & 1, strand);
+    const SeqView qs = phase_prepare<NB>(ex, sm, a.tab, a.ent_lut, ent_row(a.ent_rows, qc, strand), a.seq + qc.off,
+                                         qc.len, qc.flags & 1, strand);
     correlate<NB>(ex, sm, a.tab, tw, qs, a.spectra + (size_t)pr.t_chunk * 4 * G::N, nullptr,
                   tail_ref<NB>(a.tails, a.segs, a.src_sig, pr.q_tail));
   }

@@ -23,6 +23,7 @@
 #include <math.h>
 
 #include "sxc_fft.h"
+#include "sxc_tables.h"
 
 namespace sxc {
 
@@ -41,6 +42,11 @@ struct ScanArgs {
   double min_prob;
   double target_total;
   double z_skip;             // runs with (m - x) / s / 1.414213562 >= z_skip have prob < min_prob
+  // keep band (keep_band): runs with prob >= keep_lo are written; those with prob < flag_hi
+  // are flagged SXC_REC_NEAR for the host to decide, and counted in *nflag (optional)
+  double keep_lo;
+  double flag_hi;
+  int32_t* nflag;
   int32_t mode;              // 0: filter runs into t_results; 1: write runs (MatchUp)
   sxc_result_dev* out;       // mode 0: cap records per (pair, strand)
   int32_t* runs;             // mode 1: cap x {start_target, start_query, len} per (pair, strand)
@@ -56,6 +62,15 @@ struct ScanArgs {
   // runs are added atomically.
   int32_t split;
 };
+
+// Host: the keep decision's parameters for min_prob and the target size -- the float
+// pre-filter's z bound and the keep band (sxc_tables.h).
+inline void set_keep(ScanArgs& a, double min_prob, double target_total) {
+  a.min_prob = min_prob;
+  a.target_total = target_total;
+  a.z_skip = keep_z_bound(min_prob, target_total);
+  keep_band(min_prob, target_total, &a.keep_lo, &a.flag_hi);
+}
 
 constexpr int SCAN_NT = 256;  // threads per scan workgroup
 // smallest count c of identical A/C/G/T bases with 100 * c > 1889 (19 of 46)
@@ -754,7 +769,16 @@ SXC_HD inline void drain(Ex& ex, char* sm, const ScanArgs& a, const sxc_pair_dev
           continue;
         }
       }
-      if (prob < a.min_prob) continue;
+      // FilterMatches keeps prob >= m_minProb.  The table's decision is exact; the device's
+      // erf/exp may differ from glibc's by a few ulp, so a probability within the keep band
+      // of min_prob is written flagged and decided on the host with glibc.
+      bool near = false;
+      if (a.ptable) {
+        if (prob < a.min_prob) continue;
+      } else {
+        if (prob < a.keep_lo) continue;
+        near = prob < a.flag_hi;
+      }
       const int slot = a.split > 1 ? ex.atomic_add_global(&a.counts[ps], 1) : ex.atomic_add_shared(&misc[SM_KEPT], 1);
       if (slot >= a.cap) continue;
       const int tstart = pr.t_chunk_start + start_t;
@@ -773,6 +797,15 @@ SXC_HD inline void drain(Ex& ex, char* sm, const ScanArgs& a, const sxc_pair_dev
       r.len = (uint64_t)(int64_t)len;
       r.reverse = (uint8_t)(strand == 1);
       for (int z = 0; z < 7; ++z) r.pad_[z] = 0;
+      if (near) {
+        const uint32_t pi = (uint32_t)(ps / a.strands);  // the pair, for the host's decision
+        r.pad_[0] = SXC_REC_NEAR;
+        r.pad_[1] = (uint8_t)pi;
+        r.pad_[2] = (uint8_t)(pi >> 8);
+        r.pad_[3] = (uint8_t)(pi >> 16);
+        r.pad_[4] = (uint8_t)(pi >> 24);
+        if (a.nflag) ex.atomic_add_global(a.nflag, 1);
+      }
       r.prob = prob;
       r.ident = ident;
       const size_t at = (size_t)ps * a.cap + slot;

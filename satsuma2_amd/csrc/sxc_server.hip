@@ -107,6 +107,12 @@ struct sxc_server {
   double records_per_pair = 0.;
   int cus = 256;  // compute units of the device
   DevBuf seq, tch, qch, spectra, qspectra;
+  DevBuf ent;  // window entropies of chunks with B/D/H/V letters (computed on the host)
+  // host copies of the chunk bytes and tables: records flagged near min_prob are decided
+  // on the host from them (sxc::resolve_record)
+  std::vector<unsigned char> h_seq;
+  std::vector<sxc_chunk> h_tch, h_qch;
+  std::vector<float> h_ent;
   const double* ptable = nullptr;  // -prob_table thresholds (owned by the context)
   std::vector<ChunkMeta> tmeta, qmeta;
   int64_t resident_bytes = 0, slot_bytes = 0, qspec_bytes = 0;
@@ -121,7 +127,7 @@ struct sxc_server {
   int fatal = 0;
   std::string fatal_msg;
   // statistics
-  int64_t st_requests = 0, st_pairs = 0, st_launches = 0, st_results = 0;
+  int64_t st_requests = 0, st_pairs = 0, st_launches = 0, st_results = 0, st_near = 0;
   double st_device_ms = 0;
 };
 
@@ -199,9 +205,8 @@ int launch_scan_slot(sxc_server* sv, Slot& sl, int split, bool order) {
   a.strands = 2;
   a.min_len = sv->prm.min_len;
   a.q_chunk = sv->prm.q_chunk;
-  a.min_prob = sv->prm.min_prob;
-  a.target_total = sv->prm.target_total;
-  a.z_skip = sxc::keep_z_bound(sv->prm.min_prob, sv->prm.target_total);
+  sxc::set_keep(a, sv->prm.min_prob, sv->prm.target_total);
+  a.nflag = nullptr;  // the host looks at every record of the slot anyway (slot_complete)
   a.mode = 0;
   a.out = sl.outs.as<sxc_result_dev>();
   a.runs = nullptr;
@@ -280,6 +285,7 @@ int slot_fill(sxc_server* sv, Slot& sl) {
   a.tab = sv->ctx->d_tab;
   a.osc = sv->ctx->d_osc;
   a.ent_lut = sv->ctx->d_ent[sv->nb];
+  a.ent_rows = sv->h_ent.empty() ? nullptr : sv->ent.as<float>();
   a.seq = sv->seq.as<unsigned char>();
   a.tchunks = sv->tch.as<sxc_chunk>();
   a.qchunks = sv->qch.as<sxc_chunk>();
@@ -359,13 +365,29 @@ void slot_complete(sxc_server* sv, Slot& sl, hipEvent_t last) {
   if (hipEventElapsedTime(&ms, sl.ev_start, last) == hipSuccess) sv->st_device_ms += ms;
   const int total = 2 * sl.npairs;
   const auto now = Clock::now();
+  const sxc::ResolveCtx rc{sv->h_seq.data(), sv->h_tch.data(), sv->h_qch.data(), sl.h_pairs, sl.npairs,
+                           sv->prm.q_chunk, sv->prm.min_prob, sv->prm.target_total};
   for (const Piece& pc : sl.pieces) {
     const int64_t b = sl.h_offsets[2 * pc.at];
     const int64_t e = (2 * (pc.at + pc.count) < total) ? sl.h_offsets[2 * (pc.at + pc.count)] : sl.h_offsets[total];
     Request* r = pc.req;
-    r->parts[pc.first].assign(sl.h_final + b, sl.h_final + e);
+    std::vector<sxc_result>& part = r->parts[pc.first];
+    part.clear();
+    part.reserve((size_t)(e - b));
+    for (int64_t i = b; i < e; ++i) {
+      // records flagged near min_prob are decided here with the host's libm
+      sxc_result_dev rec;
+      memcpy(&rec, &sl.h_final[i], sizeof rec);
+      sv->st_near += rec.pad_[0] == SXC_REC_NEAR;
+      const int keep = sxc::resolve_record(rec, rc);
+      if (keep < 0 && !r->error) r->error = SXC_ERR_HIP;
+      if (keep > 0) {
+        part.emplace_back();
+        memcpy(&part.back(), &rec, sizeof rec);
+      }
+    }
     r->done_pairs += pc.count;
-    sv->st_results += e - b;
+    sv->st_results += (int64_t)part.size();
     if (r->done_pairs == r->total_pairs) {
       for (auto& kv : r->parts) r->results.insert(r->results.end(), kv.second.begin(), kv.second.end());
       r->parts.clear();
@@ -432,8 +454,8 @@ void dispatch_loop(sxc_server* sv) {
   }
 }
 
-int upload_chunks(std::vector<unsigned char>& bytes, std::vector<sxc_chunk>& table, std::vector<ChunkMeta>& meta,
-                  const sxc_chunk_desc* in, int32_t n, int max_len) {
+int upload_chunks(std::vector<unsigned char>& bytes, std::vector<float>& ent, std::vector<sxc_chunk>& table,
+                  std::vector<ChunkMeta>& meta, const sxc_chunk_desc* in, int32_t n, int max_len, int N) {
   table.resize((size_t)n);
   meta.resize((size_t)n);
   for (int32_t k = 0; k < n; ++k) {
@@ -453,6 +475,13 @@ int upload_chunks(std::vector<unsigned char>& bytes, std::vector<sxc_chunk>& tab
       if (cl > 3 && cl != SXC_CLASS_N) t.flags &= ~4;
       bytes.push_back(b);
     }
+    if (t.flags & 1) {  // window entropies with the host's log
+      const size_t row = ent.size() / SXC_ENT_ROW;
+      if (row >= (1u << 24)) return fail(SXC_ERR_ARG, "too many chunks with B, D, H or V letters");
+      t.flags |= (int32_t)(row << 8);
+      ent.resize(ent.size() + SXC_ENT_ROW);
+      sxc::window_entropy_row(bytes.data() + t.off, c.len, N, ent.data() + row * SXC_ENT_ROW);
+    }
     while (bytes.size() % 16) bytes.push_back(0);
     meta[(size_t)k] = ChunkMeta{c.start, c.record, c.record_len};
   }
@@ -462,11 +491,12 @@ int upload_chunks(std::vector<unsigned char>& bytes, std::vector<sxc_chunk>& tab
 int server_init(sxc_server* sv, const sxc_chunk_desc* targets, int32_t nt, const sxc_chunk_desc* queries,
                 int32_t nq, const sxc_server_config* cfg) {
   const int N = sv->prm.N;
-  std::vector<unsigned char> bytes;
-  std::vector<sxc_chunk> ttab, qtab;
+  std::vector<unsigned char>& bytes = sv->h_seq;
+  std::vector<sxc_chunk>& ttab = sv->h_tch;
+  std::vector<sxc_chunk>& qtab = sv->h_qch;
   // targets at most N/2 and queries at most N: every lag of every pair fits the transform
-  int rc = upload_chunks(bytes, ttab, sv->tmeta, targets, nt, N / 2);
-  if (!rc) rc = upload_chunks(bytes, qtab, sv->qmeta, queries, nq, N);
+  int rc = upload_chunks(bytes, sv->h_ent, ttab, sv->tmeta, targets, nt, N / 2, N);
+  if (!rc) rc = upload_chunks(bytes, sv->h_ent, qtab, sv->qmeta, queries, nq, N, N);
   if (rc) return rc;
   bytes.resize(bytes.size() + 16, 0);
   sv->P = cfg && cfg->max_pairs_per_launch > 0 ? cfg->max_pairs_per_launch : 2048;
@@ -476,7 +506,7 @@ int server_init(sxc_server* sv, const sxc_chunk_desc* targets, int32_t nt, const
   HIPCHK(hipMemGetInfo(&free_b, &total_b));
   const int64_t budget = cfg && cfg->hbm_budget > 0 ? cfg->hbm_budget : (int64_t)(free_b / 4 * 3);
   sv->resident_bytes = (int64_t)bytes.size() + (int64_t)(ttab.size() + qtab.size()) * (int64_t)sizeof(sxc_chunk) +
-                       (int64_t)nt * 4 * N * (int64_t)sizeof(float);
+                       (int64_t)nt * 4 * N * (int64_t)sizeof(float) + (int64_t)(sv->h_ent.size() * sizeof(float));
   sv->slot_bytes = slot_bytes_for(sv->P, N, 64);
   // both strands' spectra of every query chunk, if they leave room for at least one slot
   const int64_t qspec_bytes = (int64_t)nq * 2 * 4 * N * (int64_t)sizeof(float);
@@ -501,6 +531,10 @@ int server_init(sxc_server* sv, const sxc_chunk_desc* targets, int32_t nt, const
   HIPCHK(sv->qch.ensure(std::max<size_t>(1, qtab.size()) * sizeof(sxc_chunk)));
   HIPCHK(sv->spectra.ensure(std::max<size_t>(1, ttab.size()) * 4 * (size_t)N * sizeof(float)));
   HIPCHK(hipMemcpyAsync(sv->seq.p, bytes.data(), bytes.size(), hipMemcpyHostToDevice, s));
+  if (!sv->h_ent.empty()) {
+    HIPCHK(sv->ent.ensure(sv->h_ent.size() * sizeof(float)));
+    HIPCHK(hipMemcpyAsync(sv->ent.p, sv->h_ent.data(), sv->h_ent.size() * sizeof(float), hipMemcpyHostToDevice, s));
+  }
   if (!ttab.empty())
     HIPCHK(hipMemcpyAsync(sv->tch.p, ttab.data(), ttab.size() * sizeof(sxc_chunk), hipMemcpyHostToDevice, s));
   if (!qtab.empty())
@@ -519,6 +553,7 @@ int server_init(sxc_server* sv, const sxc_chunk_desc* targets, int32_t nt, const
     sa.tab = sv->ctx->d_tab;
     sa.osc = sv->ctx->d_osc;
     sa.ent_lut = sv->ctx->d_ent[sv->nb];
+    sa.ent_rows = sv->h_ent.empty() ? nullptr : sv->ent.as<float>();
     sa.seq = sv->seq.as<unsigned char>();
     sa.chunks = sv->tch.as<sxc_chunk>();
     sa.which = which.as<int32_t>();
@@ -549,6 +584,7 @@ int server_init(sxc_server* sv, const sxc_chunk_desc* targets, int32_t nt, const
         sa.tab = sv->ctx->d_tab;
         sa.osc = sv->ctx->d_osc;
         sa.ent_lut = sv->ctx->d_ent[sv->nb];
+        sa.ent_rows = sv->h_ent.empty() ? nullptr : sv->ent.as<float>();
         sa.seq = sv->seq.as<unsigned char>();
         sa.chunks = sv->qch.as<sxc_chunk>();
         sa.which = which.as<int32_t>();
@@ -576,7 +612,7 @@ void server_release(sxc_server* sv) {
     if (sl.s) (void)hipStreamSynchronize(sl.s);
     slot_free(sl);
   }
-  for (DevBuf* b : {&sv->seq, &sv->tch, &sv->qch, &sv->spectra, &sv->qspectra}) b->release();
+  for (DevBuf* b : {&sv->seq, &sv->tch, &sv->qch, &sv->spectra, &sv->qspectra, &sv->ent}) b->release();
 }
 
 }  // namespace
@@ -681,6 +717,7 @@ int sxc_server_stats(sxc_server* sv, sxc_server_info* out) {
   out->pairs_done = sv->st_pairs;
   out->launches = sv->st_launches;
   out->results = sv->st_results;
+  out->near_threshold = sv->st_near;
   out->resident_bytes = sv->resident_bytes;
   out->slot_bytes = sv->slot_bytes;
   out->slots = (int32_t)sv->slots.size();

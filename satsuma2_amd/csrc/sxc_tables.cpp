@@ -183,4 +183,114 @@ void build_prob_table(double target_size, double* t) {
   for (auto& x : th) x.join();
 }
 
+const sxc_tables& host_tables() {
+  static const sxc_tables t = [] {
+    sxc_tables x;
+    build_tables(&x);
+    return x;
+  }();
+  return t;
+}
+
+void window_entropy_row(const unsigned char* seq, int L, int N, float* row) {
+  // CCSignal::ComputeEntropy (analysis/CrossCorr.cc:35-93) for both strands of one chunk:
+  // sequential double sums of the codec values per window, Ent with the host's log
+  const sxc_tables& t = host_tables();
+  const int win = N / 512;
+  for (int strand = 0; strand < 2; ++strand) {
+    float* out = row + strand * (SXC_ENT_ROW / 2);
+    for (int w = 0; w < SXC_ENT_ROW / 2; ++w) out[w] = 0.f;
+    if (L < 1024 || win < 1) continue;  // weight 1 everywhere (:39-44): the row is not read
+    for (int i = 0, w = 0; i < L && w < SXC_ENT_ROW / 2; i += win, ++w) {
+      double a = 0, c = 0, g = 0, u = 0;
+      int k = 0;
+      for (int j = i; j < i + win && j < L; ++j) {
+        const unsigned char b = strand ? t.rc[seq[L - 1 - j] & 127] : (unsigned char)(seq[j] & 127);
+        const double* cd = t.codec[t.cls[b]];
+        a += cd[0];
+        c += cd[1];
+        g += cd[2];
+        u += cd[3];
+        ++k;
+      }
+      a /= (double)k;
+      c /= (double)k;
+      g /= (double)k;
+      u /= (double)k;
+      const double e = ent_ref(a) + ent_ref(c) + ent_ref(g) + ent_ref(u);
+      float v = (float)(-e);
+      if (v < 0.f) v = 0.f;
+      out[w] = v;
+    }
+  }
+}
+
+void keep_band(double min_prob, double target_size, double* keep_lo, double* flag_hi) {
+  // prob = exp(-T * cdf), cdf = 0.5 * (1 + erf(z)).  The device and the host evaluate z with
+  // the same correctly rounded IEEE operations; only erf and exp differ, each by a few ulp of
+  // its result: |erf_dev - erf_host| <= k * 2^-53 (|erf| < 1) moves cdf by half that and
+  // prob by prob * T * k * 2^-54; exp adds k * 2^-53 of prob.  With k = 16 on each side the
+  // band below covers both with a factor of 2 to spare.
+  if (!(min_prob > 0.) || !(target_size >= 0.)) {
+    *keep_lo = *flag_hi = min_prob;
+    return;
+  }
+  const double rel = (16. * target_size + 64.) * ldexp(1., -52);
+  *keep_lo = min_prob * (1. - rel);
+  *flag_hi = min_prob * (1. + rel);
+}
+
+double host_match_probability(const unsigned char* t, const unsigned char* q, int qlen, int rc, int start_t,
+                              int start_q, int len, double target_size, double* ident) {
+  // GetMatchProbabilityEx (analysis/AlignProbability.cc:62-127) with GCAdjustExpect, Sigma
+  // and CDF (:11-40): sums in base order, then the host's sqrt, erf and exp
+  const sxc_tables& tb = host_tables();
+  double matches = 0, gct = 0, gcq = 0;
+  for (int i = 0; i < len; ++i) {
+    const int j = start_q + i;
+    const unsigned char qb = rc ? tb.rc[q[qlen - 1 - j] & 127] : (unsigned char)(q[j] & 127);
+    const int ca = tb.cls[t[start_t + i] & 127], cb = tb.cls[qb];
+    matches += tb.equal[ca][cb];
+    gct += tb.gc[ca];
+    gcq += tb.gc[cb];
+  }
+  *ident = matches / (double)len;
+  const double gc_target = gct / (double)len;
+  const double at_target = 1. - gc_target;
+  double r = gcq * gc_target;
+  r += ((double)len - gcq) * at_target;
+  const double p = r / (double)len / 2.;
+  const double s = sqrt(p * (1. - p) * (double)len);
+  const double m = p * (double)len;
+  const double x = (double)len * *ident;
+  const double cdf = 0.5 * (1. + erf((m - x) / s / 1.414213562));
+  const double expect = cdf * target_size;
+  return exp(-expect);
+}
+
+int resolve_record(sxc_result_dev& r, const ResolveCtx& c) {
+  if (r.pad_[0] != SXC_REC_NEAR) return 1;
+  uint32_t pi = 0;
+  memcpy(&pi, &r.pad_[1], 4);
+  if ((int64_t)pi >= c.npairs) return -1;
+  const sxc_pair_dev& pr = c.pairs[pi];
+  const sxc_chunk& tc = c.tch[pr.t_chunk];
+  const sxc_chunk& qc = c.qch[pr.q_chunk];
+  const int64_t len = (int64_t)r.len;
+  const int64_t st = (int64_t)r.tstart - pr.t_chunk_start;
+  // invert RCQuery (analysis/HomologyByXCorrSlave.cc:56-60) on the reverse strand
+  const int64_t rc_chunk = c.q_chunk > 0 ? c.q_chunk : qc.len;
+  const int64_t sq = r.reverse ? (int64_t)r.qstart - pr.q_record_len + pr.q_chunk_start + rc_chunk
+                               : (int64_t)r.qstart - pr.q_chunk_start;
+  if (len <= 0 || st < 0 || st + len > tc.len || sq < 0 || sq + len > qc.len) return -1;
+  double ident = 0.;
+  const double prob = host_match_probability(c.seq + tc.off, c.seq + qc.off, qc.len, r.reverse ? 1 : 0, (int)st,
+                                             (int)sq, (int)len, c.target_total, &ident);
+  memset(r.pad_, 0, sizeof r.pad_);
+  if (prob < c.min_prob) return 0;  // FilterMatches: prob < m_minProb
+  r.prob = prob;
+  r.ident = ident;
+  return 1;
+}
+
 }  // namespace sxc

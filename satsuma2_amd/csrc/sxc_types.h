@@ -20,8 +20,13 @@ typedef struct {
   int64_t off;
   int32_t len;
   int32_t flags;  // bit 0: contains a codec class with 1/3 entries (B, V, H, D); bit 1: only
-                  // A, C, G, T; bit 2: only A, C, G, T, N
+                  // A, C, G, T; bit 2: only A, C, G, T, N; bits 8-31 (chunks with bit 0):
+                  // row of the host-computed window entropy table (SXC_ENT_ROW floats per
+                  // row: SXC_ENT_ROW / 2 windows of the forward strand, then of the reverse
+                  // complement)
 } sxc_chunk;
+
+#define SXC_ENT_ROW 1024  // floats per entropy row: 2 strands x at most 512 windows
 
 // One chunk pair of a batch (HomologyByXCorr::Align inputs plus the chunk metadata
 // FilterMatches uses, analysis/HomologyByXCorrSlave.cc:168-253).
@@ -50,7 +55,11 @@ typedef struct {
   int32_t src;
 } sxc_tail_seg_dev;
 
-// Reference t_result (analysis/WorkQueue.h:23-33), 72 bytes on LP64.
+// Reference t_result (analysis/WorkQueue.h:23-33), 72 bytes on LP64.  Inside the library,
+// a record whose device probability lies within the keep band of min_prob (keep_band) carries
+// pad_[0] = SXC_REC_NEAR and its pair's index in the launch in pad_[1..4]; the host decides
+// it and clears the pad before any record leaves the library.
+#define SXC_REC_NEAR 1
 typedef struct {
   uint64_t query_id;
   uint64_t target_id;

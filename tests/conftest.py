@@ -104,3 +104,26 @@ def n_pairs(g):
                 q[(i + 17) % 4096] = ord("N")
         out.append((bytes(t), bytes(q)))
     return out
+
+
+def threshold_size(t, q, st, sq, length, min_prob=0.99):
+    """The largest target size T (a double) at which the oracle (glibc erf/exp) still keeps
+    the run: prob(T) >= min_prob > prob(nextafter(T))."""
+    import oracle
+
+    lo, hi = 1.0, 1.0
+    while oracle.match_probability(t, q, st, sq, length, hi) >= min_prob:
+        lo, hi = hi, hi * 2.0
+        if hi > 1e300:  # kept at every size (prob = 1)
+            return None
+    if oracle.match_probability(t, q, st, sq, length, lo) < min_prob:
+        return None
+    for _ in range(200):  # bisection on doubles
+        mid = 0.5 * (lo + hi)
+        if mid in (lo, hi):
+            break
+        if oracle.match_probability(t, q, st, sq, length, mid) >= min_prob:
+            lo = mid
+        else:
+            hi = mid
+    return lo

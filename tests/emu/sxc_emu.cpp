@@ -20,14 +20,42 @@
 
 namespace {
 
+// SXC_EMU_ORDER=reverse | shuffle runs the threads of every phase in that order instead of
+// ascending: a kernel whose phase reads what another thread of the same phase writes (a
+// missing barrier) then computes something else.
+int emu_order_mode() {
+  static const int m = [] {
+    const char* v = getenv("SXC_EMU_ORDER");
+    if (!v) return 0;
+    return strcmp(v, "reverse") == 0 ? 1 : strcmp(v, "shuffle") == 0 ? 2 : 0;
+  }();
+  return m;
+}
+
 template <class R>
 struct EmuExec {
   int nt;
   std::vector<R> regs;
+  uint64_t rng = 0x9e3779b97f4a7c15ull;
   explicit EmuExec(int n) : nt(n), regs(n) {}
   template <class F>
   void each(F&& f) {
-    for (int t = 0; t < nt; ++t) f(t);
+    const int mode = emu_order_mode();
+    if (mode == 0) {
+      for (int t = 0; t < nt; ++t) f(t);
+    } else if (mode == 1) {
+      for (int t = nt - 1; t >= 0; --t) f(t);
+    } else {
+      std::vector<int> ord(nt);
+      for (int t = 0; t < nt; ++t) ord[t] = t;
+      for (int t = nt - 1; t > 0; --t) {
+        rng ^= rng << 13;
+        rng ^= rng >> 7;
+        rng ^= rng << 17;
+        std::swap(ord[t], ord[(int)(rng % (uint64_t)(t + 1))]);
+      }
+      for (int t : ord) f(t);
+    }
   }
   void sync() {}
   template <class T>
@@ -56,6 +84,8 @@ struct Tables {
   }
 };
 
+int64_t g_emu_near = 0;  // records the last emulated scan flagged near min_prob
+
 const Tables& tables() {
   static Tables t;
   return t;
@@ -75,7 +105,8 @@ int run_batch(const sxc::HostBatch& hb, const sxc_params& prm, std::vector<sxc_r
   std::vector<char> sm(SM::BYTES);
   // signals of the sequences named by signal tails
   std::vector<float> src_sig((size_t)hb.schunks.size() * 4 * G::N);
-  sxc::EncodeArgs ea{&T.tab, T.ent[NB].data(), hb.seq.data(), hb.schunks.data(), 0, hb.sstrand.data(),
+  const float* erows = hb.ent.empty() ? nullptr : hb.ent.data();
+  sxc::EncodeArgs ea{&T.tab, T.ent[NB].data(), erows, hb.seq.data(), hb.schunks.data(), 0, hb.sstrand.data(),
                      src_sig.data()};
   for (int k = 0; k < (int)hb.schunks.size(); ++k) {
     EmuExec<sxc::Regs<GEO>> ex(G::NT);
@@ -83,7 +114,7 @@ int run_batch(const sxc::HostBatch& hb, const sxc_params& prm, std::vector<sxc_r
     sxc::encode_body<GEO>(ex, sm.data(), ea, k);
   }
   const bool tails = !hb.tails.empty();
-  sxc::SpecArgs sa{&T.tab, T.osc.data(), T.ent[NB].data(), hb.seq.data(), hb.tchunks.data(), which.data(),
+  sxc::SpecArgs sa{&T.tab, T.osc.data(), T.ent[NB].data(), erows, hb.seq.data(), hb.tchunks.data(), which.data(),
                    (int)which.size(), spectra.data(), dst.data(), nullptr,
                    tails ? hb.ttail.data() : nullptr, tails ? hb.tails.data() : nullptr,
                    tails ? hb.segs.data() : nullptr, tails ? src_sig.data() : nullptr, 0};
@@ -101,6 +132,7 @@ int run_batch(const sxc::HostBatch& hb, const sxc_params& prm, std::vector<sxc_r
   pa.tab = &T.tab;
   pa.osc = T.osc.data();
   pa.ent_lut = T.ent[NB].data();
+  pa.ent_rows = erows;
   pa.seq = hb.seq.data();
   pa.tchunks = hb.tchunks.data();
   pa.qchunks = hb.qchunks.data();
@@ -129,7 +161,7 @@ int run_batch(const sxc::HostBatch& hb, const sxc_params& prm, std::vector<sxc_r
       for (int strand = 0; strand < 2; ++strand) {
         qw[0] = q;
         qd[0] = 2 * q + strand;
-        sxc::SpecArgs qa{&T.tab, T.osc.data(), T.ent[NB].data(), hb.seq.data(), hb.qchunks.data(), qw.data(), 1,
+        sxc::SpecArgs qa{&T.tab, T.osc.data(), T.ent[NB].data(), erows, hb.seq.data(), hb.qchunks.data(), qw.data(), 1,
                          qspec.data(), qd.data(), nullptr, nullptr, nullptr, nullptr, nullptr, strand};
         EmuExec<sxc::Regs<GEO>> ex(G::NT);
         memset(sm.data(), 0xCD, sm.size());
@@ -154,9 +186,9 @@ int run_batch(const sxc::HostBatch& hb, const sxc_params& prm, std::vector<sxc_r
   sc.strands = prm.strands;
   sc.min_len = prm.min_len;
   sc.q_chunk = prm.q_chunk;
-  sc.min_prob = prm.min_prob;
-  sc.target_total = prm.target_total;
-  sc.z_skip = sxc::keep_z_bound(prm.min_prob, prm.target_total);
+  sxc::set_keep(sc, prm.min_prob, prm.target_total);
+  int32_t nflag = 0;
+  sc.nflag = &nflag;
   sc.mode = 0;
   sc.out = slots.data();
   sc.keys = keys.data();
@@ -193,14 +225,23 @@ int run_batch(const sxc::HostBatch& hb, const sxc_params& prm, std::vector<sxc_r
     sxc::scan_body<NB>(ex, ssm.data(), sc, blk);
   }
   res.clear();
+  const sxc::ResolveCtx rc{hb.seq.data(), hb.tchunks.data(), hb.qchunks.data(), hb.pairs.data(),
+                           (int64_t)hb.pairs.size(), prm.q_chunk, prm.min_prob, prm.target_total};
   for (int ps = 0; ps < total; ++ps) {
     const int n = std::min(counts[ps], cap);
     std::vector<int> idx(n);
     for (int i = 0; i < n; ++i) idx[i] = i;
     std::sort(idx.begin(), idx.end(), [&](int a, int b) { return keys[(size_t)ps * cap + a] < keys[(size_t)ps * cap + b]; });
-    for (int i : idx) res.push_back(slots[(size_t)ps * cap + i]);
+    for (int i : idx) {
+      // records flagged near min_prob are decided on the host, as the library does
+      sxc_result_dev r = slots[(size_t)ps * cap + i];
+      const int keep = sxc::resolve_record(r, rc);
+      if (keep < 0) return -4;
+      if (keep) res.push_back(r);
+    }
     if (counts[ps] > cap) return -3;
   }
+  g_emu_near = nflag;
   return 0;
 }
 
@@ -258,6 +299,19 @@ int emu_align_batch(const sxc_pair* pairs, int n, const sxc_params* prm, sxc_res
 // The library's ProbTable thresholds (sxc_tables.cpp), SXC_PT_ROWS x SXC_PT_LEN doubles.
 void emu_prob_table(double target_size, double* out) { sxc::build_prob_table(target_size, out); }
 
+// Records the last emulated batch flagged near min_prob (decided on the host).
+int64_t emu_last_near(void) { return g_emu_near; }
+
+// The host's window entropies of a chunk (sxc::window_entropy_row): SXC_ENT_ROW floats.
+void emu_window_entropy(const char* seq, int len, int N, float* row) {
+  sxc::window_entropy_row((const unsigned char*)seq, len, N, row);
+}
+
+// The keep band around min_prob (sxc::keep_band).
+void emu_keep_band(double min_prob, double target_size, double* lo, double* hi) {
+  sxc::keep_band(min_prob, target_size, lo, hi);
+}
+
 // FFT of one signal through stage A + rounds (spectra_body with a given signal).
 int emu_fft_forward(const float* in, float* out, int N) {
   const Tables& T = tables();
@@ -267,7 +321,7 @@ int emu_fft_forward(const float* in, float* out, int N) {
   memcpy(sig.data(), in, sizeof(float) * N);
   auto go = [&](auto nbc) {
     constexpr int NB = decltype(nbc)::value;
-    sxc::SpecArgs sa{&T.tab, T.osc.data(), T.ent[NB].data(), nullptr, &ck, &which, 1, spec.data(), &dst, sig.data(),
+    sxc::SpecArgs sa{&T.tab, T.osc.data(), T.ent[NB].data(), nullptr, nullptr, &ck, &which, 1, spec.data(), &dst, sig.data(),
                      nullptr, nullptr, nullptr, nullptr, 0};
     std::vector<char> sm(sxc::Smem<NB>::BYTES);
     EmuExec<sxc::Regs<NB>> ex(sxc::Geo<NB>::NT);

@@ -18,6 +18,10 @@ Fixtures:
   cli_records.npz    12 + 12 records of uneven lengths through the standalone CLI under four
                      flag sets (match files, MultiMatches v3)
   prob_table.npz     ProbTable thresholds (digest + rows) and the slave path with -prob_table
+  bdhv_pairs.npz     chunks dense in B/D/H/V (entropy windows of thirds, glibc log) at N = 2048 ..
+                     32768: signals, xc, FindTop lags and MatchUp runs
+
+  python tests/golden/make_golden.py bdhv   (one generator only)
 """
 from __future__ import annotations
 
@@ -233,6 +237,45 @@ def gen_uneven(tmp, n=32):
     np.savez_compressed(os.path.join(HERE, "uneven_pairs.npz"), **d)
 
 
+def bdhv_cases(seed=21):
+    """Chunks dense in B, D, H and V (codec rows of thirds) mixed with A/C/G/T, the other IUPAC
+    codes and N, with a planted homology: every entropy window sums thirds."""
+    rng = np.random.default_rng(seed)
+    thirds = np.frombuffer(b"BDHV", np.uint8)
+    other = np.frombuffer(b"KMRYSWNX", np.uint8)
+    cases = []
+    for k, (lt, lq, N) in enumerate([(4096, 4096, 8192), (4096, 4096, 8192), (3000, 4096, 8192), (1024, 1500, 4096),
+                                     (1100, 900, 2048), (8192, 8192, 16384), (16384, 9000, 32768)]):
+        t = synth.random_seq(rng, lt, 0.45)
+        q = synth.random_seq(rng, lq, 0.45)
+        L = min(lt, lq) // 2
+        a, b = int(rng.integers(0, lt - L)), int(rng.integers(0, lq - L))
+        q[b:b + L] = synth.mutate(rng, t[a:a + L], 0.9)
+        for s, dens in ((t, 0.3 if k % 2 == 0 else 0.05), (q, 0.2)):
+            pos = rng.random(s.size) < dens
+            s[pos] = thirds[rng.integers(0, 4, size=int(pos.sum()))]
+            pos = rng.random(s.size) < 0.03
+            s[pos] = other[rng.integers(0, other.size, size=int(pos.sum()))]
+        cases.append((f"bdhv{k}", t.tobytes(), q.tobytes(), N))
+    return cases
+
+
+def gen_bdhv(tmp):
+    d = {}
+    names = []
+    for name, t, q, N in bdhv_cases():
+        names.append(name)
+        sig, st = detail(tmp, t, q, N, 1.8, 4096.0 * 64)
+        d[f"{name}_t"] = np.frombuffer(t, np.uint8)
+        d[f"{name}_q"] = np.frombuffer(q, np.uint8)
+        d[f"{name}_N"] = np.array([N], np.int32)
+        d[f"{name}_sigsha"] = np.array([sha(sig[i]) for i in range(3)])
+        for s, (xc, top, m) in enumerate(st):
+            digest_strand(f"{name}_s{s}_", d, xc, top, m)
+    d["names"] = np.array(names)
+    np.savez_compressed(os.path.join(HERE, "bdhv_pairs.npz"), **d)
+
+
 def read_results(path):
     b = open(path, "rb").read()
     n = int(np.frombuffer(b, np.uint64, 1)[0])
@@ -377,7 +420,12 @@ def gen_probtable(tmp):
 
 def main():
     subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "ref", f"REF={REF}"], check=True)
+    only = sys.argv[1:]  # generator names (e.g. bdhv); none: all
     with tempfile.TemporaryDirectory() as tmp:
+        if only:
+            for name in only:
+                globals()["gen_" + name](tmp)
+            return
         gen_fft(tmp)
         gen_single(tmp)
         gen_edges(tmp)
@@ -386,6 +434,7 @@ def main():
         gen_samples(tmp)
         gen_cli(tmp)
         gen_probtable(tmp)
+        gen_bdhv(tmp)
     for f in sorted(os.listdir(HERE)):
         if f.endswith(".npz"):
             print(f, os.path.getsize(os.path.join(HERE, f)))

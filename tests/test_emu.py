@@ -13,7 +13,7 @@ import pytest
 
 import hostbuild
 import oracle
-from conftest import dense_run_pair, lcg_noise, n_pairs, unpack2
+from conftest import dense_run_pair, lcg_noise, n_pairs, threshold_size, unpack2
 
 COLS = ["query_id", "target_id", "query_size", "qstart", "tstart", "len", "reverse", "prob", "ident"]
 
@@ -237,3 +237,77 @@ def test_tail_float_prefilter_is_one_sided(emu_lib):
     bad = emu_lib.emu_check_z_filter(C.c_int64(1_000_000), C.c_uint64(7), C.byref(fired))
     assert bad == 0
     assert fired.value > 900_000
+
+
+def test_emulated_bdhv_pairs(emu_lib, golden):
+    """Chunks dense in B/D/H/V: the kernels read the window entropies the host computed with
+    its log (sxc::window_entropy_row); signals (through the xc bits) and records equal the
+    oracle's at every N of the fixture up to 8192."""
+    g = golden("bdhv_pairs.npz")
+    for name in g["names"]:
+        t, q, N = g[f"{name}_t"].tobytes(), g[f"{name}_q"].tobytes(), int(g[f"{name}_N"][0])
+        if N > 8192:
+            continue
+        pairs = [dict(tseq=t, qseq=q, t_record=0, q_record=0)]
+        got, xc, _ = emu_batch(emu_lib, pairs, N=N, want_xc=True)
+        ref = oracle.align_batch([dict(pairs[0], q_record_len=len(q))],
+                                 oracle.make_params(N=N, target_total=4096.0 * 64))[0]
+        assert len(ref) > 0
+        for c in COLS:
+            assert np.array_equal(got[c], ref[c]), (name, c)
+        for s, qs in ((0, q), (1, oracle.reverse_complement(q))):
+            oxc = oracle.xcorr(oracle.encode(t, N), oracle.encode(qs, N))
+            assert np.array_equal(xc[s].view(np.uint32), oxc.view(np.uint32)), (name, s)
+
+
+def test_window_entropy_rows(emu_lib, golden):
+    """The host's window entropies of a B/D/H/V chunk, both strands, turned into the signal by
+    the reference's SeqToPCM formula, give the oracle's encoded signal bits."""
+    g = golden("bdhv_pairs.npz")
+    name = str(g["names"][0])
+    seq, N = g[f"{name}_t"].tobytes(), int(g[f"{name}_N"][0])
+    row = np.zeros(1024, np.float32)
+    emu_lib.emu_window_entropy(seq, len(seq), N, row.ctypes.data_as(C.c_void_p))
+    win = N // 512
+    codec = {b"A": (1, 0, 0, 0), b"C": (0, 1, 0, 0), b"G": (0, 0, 1, 0), b"T": (0, 0, 0, 1),
+             b"K": (0, 0, .5, .5), b"M": (.5, .5, 0, 0), b"R": (.5, 0, .5, 0), b"Y": (0, .5, 0, .5),
+             b"S": (0, .5, .5, 0), b"W": (.5, 0, 0, .5), b"B": (0, 1 / 3, 1 / 3, 1 / 3), b"V": (1 / 3, 1 / 3, 1 / 3, 0),
+             b"H": (1 / 3, 1 / 3, 0, 1 / 3), b"D": (1 / 3, 0, 1 / 3, 1 / 3), b"N": (.25,) * 4, b"X": (.25,) * 4}
+    for strand, s in ((0, seq), (1, oracle.reverse_complement(seq))):
+        vals = np.array([codec.get(bytes([c]), (0, 0, 0, 0)) for c in s], np.float64)
+        ent = np.repeat(row[strand * 512: strand * 512 + (len(s) + win - 1) // win], win)[: len(s)]
+        sig = np.zeros((4, N), np.float32)
+        for ch in range(4):
+            off = 0.0
+            for v in vals[:, ch]:  # the reference's sequential sum
+                off += v
+            off /= len(s)
+            sig[ch, : len(s)] = (ent.astype(np.float64) * (vals[:, ch] - off)).astype(np.float32)
+        assert np.array_equal(sig.view(np.uint32), oracle.encode(s, N).view(np.uint32)), strand
+
+
+def test_emulated_keep_band(emu_lib, golden):
+    """Target sizes that put one run's reference probability on either side of min_prob by
+    one double: the scan flags the run, the host decides it with its libm, and the records
+    equal the oracle's, probabilities bit for bit."""
+    emu_lib.emu_last_near.restype = C.c_int64
+    g = golden("single_pair.npz")
+    t, q = g["t"].tobytes(), g["q"].tobytes()
+    m = g["m0"]
+    r = next(r for r in m[m["len"] >= 40]
+             if (lambda T: T is not None and 1e3 < T < 1e8)(threshold_size(t, q, int(r["st"]), int(r["sq"]),
+                                                                           int(r["len"]))))
+    T = threshold_size(t, q, int(r["st"]), int(r["sq"]), int(r["len"]))
+    for size in (T, np.nextafter(T, np.inf)):
+        got, _, _ = emu_batch(emu_lib, [dict(tseq=t, qseq=q)], strands=1, target_total=size)
+        assert emu_lib.emu_last_near() >= 1
+        ref = oracle.align_batch([dict(tseq=t, qseq=q)], oracle.make_params(strands=1, target_total=size))[0]
+        for c in COLS:
+            assert np.array_equal(got[c], ref[c]), (size, c)
+        sel = (ref["tstart"] == r["st"]) & (ref["qstart"] == r["sq"]) & (ref["len"] == r["len"])
+        assert sel.sum() == (1 if size == T else 0)
+        assert not got["pad_"].any()
+    lo, hi = C.c_double(0), C.c_double(0)
+    emu_lib.emu_keep_band(C.c_double(0.99), C.c_double(16777216.0), C.byref(lo), C.byref(hi))
+    # (16 T + 64) 2^-52 either side: 6e-8 at the 4096-pair workload's size
+    assert lo.value < 0.99 < hi.value and (hi.value - lo.value) / 0.99 < 1.3e-7

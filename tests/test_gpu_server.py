@@ -1,18 +1,24 @@
 """The serving loop (sxc_server_*) on the GPU.
 
 Requests of t_pair blocks against resident genomes must return exactly the records the
-batch path (sxc_align_batch, itself checked against the reference slave) produces for the
-same chunk pairs -- alone, and in a mix where launches carry pairs of many requests and a
-request spans several launches."""
+reference slave's align_target produces for the same chunk pairs
+(analysis/HomologyByXCorrSlave.cc:270-300; the oracle restates it, pinned against the
+reference's own slave records) -- alone, and in a mix where launches carry pairs of many
+requests and a request spans several launches."""
+import collections
+import os
 import threading
 
 import numpy as np
 import pytest
 
+import oracle
 from satsuma2_amd import synth
 from satsuma2_amd._lib import SxcError
 
 pytestmark = pytest.mark.gpu
+
+THREADS = max(1, min(16, os.cpu_count() or 1))
 
 
 def _chunks(seq: bytes, size: int, step: int, record: int):
@@ -72,49 +78,82 @@ def _requests(tch, qch, rng, n=24):
     return reqs
 
 
-def _batch_records(ctx, params, tch, qch, blocks):
-    from satsuma2_amd.api import ChunkPair, HomologyByXCorr
-    h = HomologyByXCorr(ctx, params)
-    pairs = [ChunkPair(tch[ti][0], qch[qi][0], tch[ti][1], qch[qi][1], tch[ti][2], qch[qi][2], qch[qi][3], fast=f)
+def oracle_records(tch, qch, blocks, target_total, threads=THREADS):
+    """align_target over the blocks' chunk pairs (query chunks outer, target chunks inner),
+    both strands, by the oracle."""
+    pairs = [dict(tseq=tch[ti][0], qseq=qch[qi][0], t_chunk_start=tch[ti][1], q_chunk_start=qch[qi][1],
+                  t_record=tch[ti][2], q_record=qch[qi][2], q_record_len=qch[qi][3], fast=f)
              for tf, tt, qf, qt, f in blocks for qi in range(qf, qt + 1) for ti in range(tf, tt + 1)]
-    return h.align_batch(pairs)
+    if not pairs:
+        return np.zeros(0, oracle.RESULT_DTYPE)
+    recs = oracle.align_batch(pairs, oracle.make_params(target_total=target_total), threads=threads,
+                              cap_per_pair=2048)
+    assert max(len(r) for r in recs) < 2048
+    return np.concatenate(recs)
 
 
-def test_server_alone_equals_batch_and_mix(genomes, served):
+def record_diff(got, ref):
+    """A short description of how two record lists differ (for assertion messages)."""
+    key = lambda r: (int(r["reverse"]), int(r["tstart"]), int(r["qstart"]), int(r["len"]))
+    g, r = [key(x) for x in got], [key(x) for x in ref]
+    cg, cr = collections.Counter(g), collections.Counter(r)  # overlapping chunks repeat records
+    miss, extra = sorted((cr - cg).elements()), sorted((cg - cr).elements())
+    first = next((i for i in range(min(len(g), len(r))) if g[i] != r[i]), None)
+    return (f"{len(got)} records vs {len(ref)}; missing {miss[:5]} ({len(miss)}), extra {extra[:5]} "
+            f"({len(extra)}); first difference at {first}: got {g[first] if first is not None else None}, "
+            f"want {r[first] if first is not None else None}")
+
+
+def assert_records(got, ref, what):
+    same = len(got) == len(ref)
+    if same:
+        for f in ("query_id", "target_id", "query_size", "qstart", "tstart", "len", "reverse", "ident"):
+            same &= bool(np.array_equal(got[f], ref[f]))
+    assert same, f"{what}: {record_diff(got, ref)}"
+    if len(ref):
+        rel = np.abs(got["prob"] - ref["prob"]) / np.maximum(np.abs(ref["prob"]), 1e-300)
+        assert rel.max() <= 1e-11, f"{what}: prob rel err {rel.max()}"
+
+
+def test_server_alone_and_mix_equal_reference(genomes, served):
     T, Q, tch, qch = genomes
     ctx, params, srv = served
     reqs = _requests(tch, qch, np.random.default_rng(3))
-    alone = []
-    for blocks in reqs:
-        t = srv.submit(blocks)
-        alone.append(srv.results(t))
-    assert sum(len(a) for a in alone) > 0
-    assert len(alone[-2]) > 64  # the repeat pair overflowed the first slot capacity
-    for blocks, a in zip(reqs, alone):
-        assert a.tobytes() == _batch_records(ctx, params, tch, qch, blocks).tobytes()
-    # the mix: every request at once, in shuffled order, from two submitting threads
-    order = np.random.default_rng(5).permutation(len(reqs))
-    tickets = {}
-    lock = threading.Lock()
+    ref = [oracle_records(tch, qch, blocks, params.target_total) for blocks in reqs]
+    assert sum(len(a) for a in ref) > 0
+    assert len(ref[-2]) > 64  # the repeat pair overflows the first slot capacity
+    for i, blocks in enumerate(reqs):
+        assert_records(srv.results(srv.submit(blocks)), ref[i], f"request {i} alone")
+    # the mix: every request at once, in shuffled order, from two submitting threads; twice
+    for rnd in range(2):
+        order = np.random.default_rng(5 + rnd).permutation(len(reqs))
+        tickets = {}
+        lock = threading.Lock()
 
-    def submit(ids):
-        for i in ids:
-            t = srv.submit(reqs[i])
-            with lock:
-                tickets[i] = t
+        def submit(ids):
+            for i in ids:
+                t = srv.submit(reqs[i])
+                with lock:
+                    tickets[i] = t
 
-    th = [threading.Thread(target=submit, args=(order[k::2],)) for k in range(2)]
-    for x in th:
-        x.start()
-    for x in th:
-        x.join()
-    for i in range(len(reqs)):
-        assert srv.results(tickets[i]).tobytes() == alone[i].tobytes(), i
+        th = [threading.Thread(target=submit, args=(order[k::2],)) for k in range(2)]
+        for x in th:
+            x.start()
+        for x in th:
+            x.join()
+        bad = []
+        for i in range(len(reqs)):
+            got = srv.results(tickets[i])
+            try:
+                assert_records(got, ref[i], f"mix {rnd} request {i}")
+            except AssertionError as e:
+                bad.append(str(e))
+        assert not bad, "\n".join(bad)
     st = srv.stats()
     assert st["slots"] == 3 and st["max_pairs_per_launch"] == 256
     # served pairs read both strands' query spectra from HBM instead of transforming queries
     assert st["query_spectra_bytes"] == len(qch) * 2 * 4 * 8192 * 4
-    assert st["pairs_done"] >= 2 * sum((b[1] - b[0] + 1) * (b[3] - b[2] + 1) for r in reqs for b in r)
+    assert st["pairs_done"] >= 3 * sum((b[1] - b[0] + 1) * (b[3] - b[2] + 1) for r in reqs for b in r)
 
 
 def test_server_without_query_spectra(genomes):
@@ -128,9 +167,9 @@ def test_server_without_query_spectra(genomes):
                  query_spectra=False)
     assert srv.stats()["query_spectra_bytes"] == 0
     reqs = _requests(tch, qch, np.random.default_rng(11), n=6)
-    for blocks in reqs:
-        got = srv.results(srv.submit(blocks))
-        assert got.tobytes() == _batch_records(ctx, params, tch, qch, blocks).tobytes()
+    for i, blocks in enumerate(reqs):
+        assert_records(srv.results(srv.submit(blocks)), oracle_records(tch, qch, blocks, params.target_total),
+                       f"request {i}")
     srv.close()
     ctx.close()
 

@@ -286,6 +286,19 @@ def test_edge_pairs_golden(golden):
                 _check_digest(g, f"{name}_{cut}_s{s}_", t, qs, xc, cut, 4096.0 * 64)
 
 
+def test_bdhv_pairs_golden(golden):
+    """Chunks dense in B/D/H/V: every entropy window sums thirds and takes glibc's log."""
+    g = golden("bdhv_pairs.npz")
+    for name in g["names"]:
+        t, q, N = g[f"{name}_t"].tobytes(), g[f"{name}_q"].tobytes(), int(g[f"{name}_N"][0])
+        qrc = oracle.reverse_complement(q)
+        sigs = [oracle.encode(s, N) for s in (t, q, qrc)]
+        assert [sha(s) for s in sigs] == list(g[f"{name}_sigsha"]), name
+        for s, qs in ((0, q), (1, qrc)):
+            xc = oracle.xcorr(sigs[0], sigs[1 + s])
+            _check_digest(g, f"{name}_s{s}_", t, qs, xc, 1.8, 4096.0 * 64)
+
+
 def test_uneven_pairs_golden(golden):
     g = golden("uneven_pairs.npz")
     for k in range(int(g["n"][0])):
