@@ -132,6 +132,15 @@ int sxc_match_probability(sxc_ctx* ctx, const char* t, int32_t lt, const char* q
 int sxc_align_pair(sxc_ctx* ctx, const sxc_pair* pair, const sxc_params* prm, sxc_result* out, int32_t cap,
                    int32_t* n_out);
 
+/* sxc_align_pair without device allocation.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  Every device buffer it uses lives in a static
+ * per-GPU array of 4 MiB that the library holds until the process ends; calls on one GPU are
+ * serialised.  Same results as sxc_align_pair; limits: at most 4096 records kept per strand
+ * (SXC_ERR_CAPACITY beyond that), no -prob_table. */
+int sxc_align_pair_static(sxc_ctx* ctx, const sxc_pair* pair, const sxc_params* prm, sxc_result* out,
+                          int32_t cap, int32_t* n_out);
+
 /* Many chunk pairs in one fused launch (align_target over a block of t_pairs,
  * :270-300).  Results grouped by pair in input order, each pair as sxc_align_pair. */
 int sxc_align_batch(sxc_ctx* ctx, const sxc_pair* pairs, int32_t n, const sxc_params* prm, sxc_result* out,

@@ -109,7 +109,8 @@ RESULT_DTYPE = np.dtype([("query_id", "<u8"), ("target_id", "<u8"), ("query_size
 assert RESULT_DTYPE.itemsize == C.sizeof(Result) == 72
 
 EXPORTS = ["sxc_create", "sxc_destroy", "sxc_last_error", "sxc_version", "sxc_encode_signal", "sxc_fft_forward",
-           "sxc_xcorr", "sxc_matchup", "sxc_match_probability", "sxc_align_pair", "sxc_align_batch",
+           "sxc_xcorr", "sxc_matchup", "sxc_match_probability", "sxc_align_pair", "sxc_align_pair_static",
+           "sxc_align_batch",
            "sxc_batch_create", "sxc_batch_create_tails", "sxc_batch_run", "sxc_batch_set_graph", "sxc_batch_results",
            "sxc_batch_stats", "sxc_batch_lags", "sxc_findtop",
            "sxc_batch_destroy", "sxc_last_timing", "sxc_server_create", "sxc_server_submit", "sxc_server_wait",
@@ -145,6 +146,8 @@ def load() -> C.CDLL:
         L.sxc_match_probability.argtypes = [vp, C.c_char_p, i32, C.c_char_p, i32, C.POINTER(SeqMatch), i32, dbl, dp,
                                             dp]
         L.sxc_align_pair.argtypes = [vp, C.POINTER(Pair), C.POINTER(Params), C.POINTER(Result), i32, C.POINTER(i32)]
+        L.sxc_align_pair_static.argtypes = [vp, C.POINTER(Pair), C.POINTER(Params), C.POINTER(Result), i32,
+                                            C.POINTER(i32)]
         L.sxc_align_batch.argtypes = [vp, C.POINTER(Pair), i32, C.POINTER(Params), C.POINTER(Result), i64,
                                       C.POINTER(i64)]
         L.sxc_batch_create.argtypes = [vp, C.POINTER(Pair), i32, C.POINTER(Params), C.POINTER(vp)]

@@ -26,7 +26,7 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall", "-Wno-unus
 DEVICE = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 SOURCES = ["sxc_kernels.hip", "sxc_kernels_r2.hip", "sxc_kernels_s32.hip", "sxc_kernels_occ.hip", "sxc_kernels_lean.hip",
            "sxc_api.hip",
-           "sxc_server.hip", "sxc_tables.cpp"]
+           "sxc_server.hip", "sxc_static.hip", "sxc_tables.cpp"]
 # Host programs linked against the library (plain C++, no device code).
 PROGRAMS = {
     "HomologyByXCorrSlave": ["host/slave_main.cpp", "host/sxc_seqio.cpp"],

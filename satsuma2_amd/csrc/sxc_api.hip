@@ -790,6 +790,11 @@ int sxc_create(sxc_ctx** out, int device) {
   sxc_ctx* c = new sxc_ctx();
   c->device = device;
   HIPCHK(hipSetDevice(device));
+  const int src = sxc::static_init(device);
+  if (src) {
+    delete c;
+    return src;
+  }
   HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
   for (auto& e : c->ev) HIPCHK(hipEventCreate(&e));
   sxc_tables tab;

@@ -120,4 +120,7 @@ namespace sxc {
 // The ProbTable thresholds for `target_size` in HBM, built on the host on first use.
 // Call with ctx->mu held.
 int ctx_prob_table(sxc_ctx* ctx, double target_size, const double** out);
+// Looks up the device address of the static scratch array of sxc_align_pair_static on the
+// current device, once per device (sxc_static.hip).
+int static_init(int device);
 }  // namespace sxc

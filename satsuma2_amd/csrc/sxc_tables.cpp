@@ -284,8 +284,10 @@ int resolve_record(sxc_result_dev& r, const ResolveCtx& c) {
                                : (int64_t)r.qstart - pr.q_chunk_start;
   if (len <= 0 || st < 0 || st + len > tc.len || sq < 0 || sq + len > qc.len) return -1;
   double ident = 0.;
-  const double prob = host_match_probability(c.seq + tc.off, c.seq + qc.off, qc.len, r.reverse ? 1 : 0, (int)st,
-                                             (int)sq, (int)len, c.target_total, &ident);
+  const unsigned char* tb = c.tbytes ? c.tbytes : c.seq + tc.off;
+  const unsigned char* qb = c.qbytes ? c.qbytes : c.seq + qc.off;
+  const double prob = host_match_probability(tb, qb, qc.len, r.reverse ? 1 : 0, (int)st, (int)sq, (int)len,
+                                             c.target_total, &ident);
   memset(r.pad_, 0, sizeof r.pad_);
   if (prob < c.min_prob) return 0;  // FilterMatches: prob < m_minProb
   r.prob = prob;

@@ -46,6 +46,10 @@ struct ResolveCtx {
   int q_chunk;
   double min_prob;
   double target_total;
+  // optional: the target and query chunk bytes themselves (one pair per launch), instead of
+  // seq + the chunk tables' offsets
+  const unsigned char* tbytes;
+  const unsigned char* qbytes;
 };
 // A record without the flag: 1 (kept as is).  A flagged record: its probability and identity
 // recomputed on the host, the flag cleared; 1 if kept (prob >= min_prob), 0 if dropped; -1 if
