@@ -1082,11 +1082,9 @@ int sxc_encode_signal(sxc_ctx* ctx, const char* seq, int32_t len, int32_t N, flo
   rc = set_device(ctx);
   if (rc) return rc;
   sxc_chunk ck{0, len, 0};
-  for (int i = 0; i < len; ++i) {
-    const unsigned char b = (unsigned char)seq[i];
-    if (b >= 128) return fail(SXC_ERR_ARG, "sequence byte >= 128");
-    if (sxc::class_has_thirds(sxc::class_of(b))) ck.flags |= 1;
-  }
+  const int fl = sxc::chunk_flags((const unsigned char*)seq, len);
+  if (fl < 0) return fail(SXC_ERR_ARG, "sequence byte >= 128");
+  ck.flags = fl & 1;  // the encoder only asks for B/D/H/V letters
   hipStream_t s = ctx->stream;
   std::vector<float> erow;
   if (ck.flags & 1) {  // row 0 of a one-row entropy table

@@ -13,6 +13,8 @@
 // the slave's align_target blocks (analysis/HomologyByXCorrSlave.cc:290-299), is
 // transformed once.
 #pragma once
+#include <string.h>
+
 #include <map>
 #include <string>
 #include <tuple>
@@ -53,16 +55,11 @@ inline std::string pack_batch(const sxc_pair* in, int n, int N, HostBatch& hb,
     if (len > N) return "sequence longer than the FFT length N";
     c.off = (int64_t)hb.seq.size();
     c.len = len;
-    c.flags = 2 | 4;  // bit 1: only A, C, G, T (scores are 100 or 0); bit 2: only A, C, G, T, N
-    for (int i = 0; i < len; ++i) {
-      const unsigned char b = (unsigned char)s[i];
-      if (b >= 128) return "sequence byte >= 128";
-      const int cl = class_of(b);
-      if (class_has_thirds(cl)) c.flags |= 1;
-      if (cl > 3) c.flags &= ~2;
-      if (cl > 3 && cl != SXC_CLASS_N) c.flags &= ~4;
-      hb.seq.push_back(b);
-    }
+    // bit 1: only A, C, G, T (scores are 100 or 0); bit 2: only A, C, G, T, N
+    c.flags = chunk_flags((const unsigned char*)s, len);
+    if (c.flags < 0) return "sequence byte >= 128";
+    hb.seq.resize(hb.seq.size() + (size_t)len);
+    if (len) memcpy(hb.seq.data() + c.off, s, (size_t)len);
     if (c.flags & 1) {
       const size_t row = hb.ent.size() / SXC_ENT_ROW;
       if (row >= (1u << 24)) return "too many chunks with B, D, H or V letters";

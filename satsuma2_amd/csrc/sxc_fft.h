@@ -175,6 +175,18 @@ SXC_HD inline void level_contig(float* v, const Twiddle& tw) {
 
 // ---------------------------------------------------------------- stage A
 
+SXC_HD constexpr unsigned bitrev_c(unsigned x, int bits) {
+  unsigned r = 0;
+  for (int b = 0; b < bits; ++b) r |= ((x >> b) & 1u) << (bits - 1 - b);
+  return r;
+}
+
+// The part of stageA_pos(g, r) that depends on r alone (its low bits come from g).
+template <int NB>
+SXC_HD constexpr int stageA_hi(int r) {
+  return (int)(bitrev_c((unsigned)r, Geo<NB>::LSA) << (Geo<NB>::LOG - Geo<NB>::LSA));
+}
+
 // Time position read by slot r of stage-A group g (pass 1+2 reads x[br[4k+m]]).
 template <int NB>
 SXC_HD inline int stageA_pos(int g, int r) {
@@ -341,21 +353,49 @@ struct RoundGeo {
   static constexpr int S = 1 << (K + 2);
 };
 
+// LDS indices of a unit's positions split into a per-thread base and a compile-time constant,
+// so every slot is one LDS access with an immediate offset.  A regular unit j (1 <= j < h_m/2)
+// of the big block starting at `off` (a multiple of 32) touches off + A*h_m + s*j, s = +-1:
+//   pad(off + A*h_m + j) = pad(off) + pj + cp(A),   pad(off + A*h_m - j) = pad(off) - mj + cm(A)
+// with, for h_m >= 32 (h_m a multiple of 32): pj = pad(j), mj = j + ceil(j/32),
+// cp(A) = cm(A) = 33*A*h_m/32; for h_m < 32 (h_m divides 32, j < h_m): pj = mj = j,
+// cp(A) = pad(A*h_m), cm(A) = A*h_m + floor((A*h_m - 1)/32) (the pad word of the block
+// holding A*h_m - 1).
+template <int HM>
+SXC_HD constexpr int pad_cp(int A) {
+  return HM >= 32 ? 33 * A * (HM / 32) : A * HM + ((A * HM) >> 5);
+}
+template <int HM>
+SXC_HD constexpr int pad_cm(int A) {
+  return HM >= 32 ? 33 * A * (HM / 32) : A * HM + ((A * HM - 1) >> 5);
+}
+template <int HM>
+SXC_HD inline int pad_pj(int j) {
+  return HM >= 32 ? j + (j >> 5) : j;
+}
+template <int HM>
+SXC_HD inline int pad_mj(int j) {
+  return HM >= 32 ? j + ((j + 31) >> 5) : j;
+}
+
 // Runs unit u of a round on the padded LDS array Y.  Units [0, NREG) are regular,
 // [NREG, UNITS) special, so only one wavefront per round mixes the two code paths.
 template <int NB, int K, int M, bool INV>
 SXC_HD inline void round_unit(float* Y, int u, const Twiddle& tw) {
   using RG = RoundGeo<NB, K, M>;
+  constexpr int HM = RG::HM;
+  static_assert(RG::BIG % 32 == 0, "big blocks start on a pad boundary");
   float v[RG::S];
   if (u < RG::NREG) {
     const int b = u / (RG::UPB - 1);
     const int j = 1 + u % (RG::UPB - 1);
-    const int off = b * RG::BIG;
+    const int p0 = pad(b * RG::BIG);
+    float* yp = Y + (p0 + pad_pj<HM>(j));  // slots with s = +1
+    float* ym = Y + (p0 - pad_mj<HM>(j));  // slots with s = -1
     static_for<0, RG::S>([&](auto I) {
       constexpr int q = decltype(I)::value;
       constexpr int A = (q + 1) >> 1;
-      constexpr int sg = (q & 1) ? -1 : 1;
-      v[q] = Y[pad(off + A * RG::HM + sg * j)];
+      if constexpr (q & 1) v[q] = ym[pad_cm<HM>(A)]; else v[q] = yp[pad_cp<HM>(A)];
     });
     if constexpr (!INV) {
       static_for<0, K>([&](auto L) { unit_level_regular<K, M, decltype(L)::value, false>(v, j, tw); });
@@ -365,14 +405,14 @@ SXC_HD inline void round_unit(float* Y, int u, const Twiddle& tw) {
     static_for<0, RG::S>([&](auto I) {
       constexpr int q = decltype(I)::value;
       constexpr int A = (q + 1) >> 1;
-      constexpr int sg = (q & 1) ? -1 : 1;
-      Y[pad(off + A * RG::HM + sg * j)] = v[q];
+      if constexpr (q & 1) ym[pad_cm<HM>(A)] = v[q]; else yp[pad_cp<HM>(A)] = v[q];
     });
   } else {
-    const int off = (u - RG::NREG) * RG::BIG;
+    // positions off + q*h_m/2: pad(off) + pad(q*h_m/2), off a multiple of 32
+    float* y0 = Y + pad((u - RG::NREG) * RG::BIG);
     static_for<0, RG::S>([&](auto I) {
       constexpr int q = decltype(I)::value;
-      v[q] = Y[pad(off + q * (RG::HM / 2))];
+      v[q] = y0[pad(q * (HM / 2))];
     });
     if constexpr (!INV) {
       static_for<0, K>([&](auto L) { unit_level_special<K, M, decltype(L)::value, false>(v, tw); });
@@ -381,7 +421,94 @@ SXC_HD inline void round_unit(float* Y, int u, const Twiddle& tw) {
     }
     static_for<0, RG::S>([&](auto I) {
       constexpr int q = decltype(I)::value;
-      Y[pad(off + q * (RG::HM / 2))] = v[q];
+      y0[pad(q * (HM / 2))] = v[q];
+    });
+  }
+}
+
+// The last round's big block is the whole transform (M + K = log2 N), so a unit holds both
+// halves of each of its bins: slot (A, s) and slot (A + 2^K, s) are bins i and i + N/2, the
+// real and the stored imaginary part (FFTReal's packed layout).  These run the round's
+// forward levels (when FWD), the conjugate product with the target spectrum T of
+// CrossCorrelation::DoOne (analysis/CrossCorr.cc:477-492; ComplexMult :20-26) on the unit's
+// bins, and the same round's inverse levels, without leaving registers.  Bins N/2 - 1 and
+// N/2 (the imaginary part of bin 0) take T's values unchanged, as the reference leaves them.
+template <int NB, int K, int M, bool FWD>
+SXC_HD inline void round_unit_product(float* Y, int u, const Twiddle& tw, const float* T) {
+  using RG = RoundGeo<NB, K, M>;
+  constexpr int HM = RG::HM, S = RG::S, HALF = 1 << K;
+  constexpr int H = Geo<NB>::H;
+  static_assert(M + K == Geo<NB>::LOG && RG::NBIG == 1, "the last round spans the transform");
+  float v[S];
+  if (u < RG::NREG) {
+    const int j = 1 + u;
+    const int p0 = 0;
+    float* yp = Y + (p0 + pad_pj<HM>(j));
+    float* ym = Y + (p0 - pad_mj<HM>(j));
+    static_for<0, S>([&](auto I) {
+      constexpr int q = decltype(I)::value;
+      constexpr int A = (q + 1) >> 1;
+      if constexpr (q & 1) v[q] = ym[pad_cm<HM>(A)]; else v[q] = yp[pad_cp<HM>(A)];
+    });
+    if constexpr (FWD) static_for<0, K>([&](auto L) { unit_level_regular<K, M, decltype(L)::value, false>(v, j, tw); });
+    const float* tp = T + j;
+    const float* tm = T - j;
+    static_for<0, S>([&](auto I) {
+      constexpr int q = decltype(I)::value;
+      constexpr int A = (q + 1) >> 1;
+      constexpr int sg = (q & 1) ? -1 : 1;
+      // real slots: A in [0, 2^K) for s = +1, [1, 2^K] for s = -1; bin i = A h_m + s j
+      if constexpr ((sg > 0 && A < HALF) || (sg < 0 && A >= 1 && A <= HALF)) {
+        constexpr int qi = slot(A + HALF, sg);
+        const float r1 = sg > 0 ? tp[A * HM] : tm[A * HM];
+        const float i1 = sg > 0 ? tp[A * HM + H] : tm[A * HM + H];
+        bool quirk = false;
+        if constexpr (sg < 0 && A == HALF) quirk = j == 1;  // bin N/2 - 1
+        if (quirk) {
+          v[q] = r1;
+          v[qi] = i1;
+        } else {
+          const float r2 = v[q], i2 = -v[qi];
+          const float fr = r1 * r2 - i1 * i2;
+          const float fi = i1 * r2 + i2 * r1;
+          v[q] = fr;
+          v[qi] = -fi;
+        }
+      }
+    });
+    static_for<0, K>([&](auto L) { unit_level_regular<K, M, K - 1 - decltype(L)::value, true>(v, j, tw); });
+    static_for<0, S>([&](auto I) {
+      constexpr int q = decltype(I)::value;
+      constexpr int A = (q + 1) >> 1;
+      if constexpr (q & 1) ym[pad_cm<HM>(A)] = v[q]; else yp[pad_cp<HM>(A)] = v[q];
+    });
+  } else {
+    // bins q h_m/2, q < 2^(K+1), with their imaginary parts at q + 2^(K+1)
+    static_for<0, S>([&](auto I) {
+      constexpr int q = decltype(I)::value;
+      v[q] = Y[pad(q * (HM / 2))];
+    });
+    if constexpr (FWD) static_for<0, K>([&](auto L) { unit_level_special<K, M, decltype(L)::value, false>(v, tw); });
+    constexpr int QH = 2 * HALF;
+    static_for<0, QH>([&](auto I) {
+      constexpr int q = decltype(I)::value;
+      constexpr int i = q * (HM / 2);
+      if constexpr (q == 0) {
+        v[0] = T[0] * v[0];
+        v[QH] = T[H];
+      } else {
+        const float r1 = T[i], i1 = T[i + H];
+        const float r2 = v[q], i2 = -v[q + QH];
+        const float fr = r1 * r2 - i1 * i2;
+        const float fi = i1 * r2 + i2 * r1;
+        v[q] = fr;
+        v[q + QH] = -fi;
+      }
+    });
+    static_for<0, K>([&](auto L) { unit_level_special<K, M, K - 1 - decltype(L)::value, true>(v, tw); });
+    static_for<0, S>([&](auto I) {
+      constexpr int q = decltype(I)::value;
+      Y[pad(q * (HM / 2))] = v[q];
     });
   }
 }

@@ -158,7 +158,7 @@ struct Regs {
 // The window entropies of a chunk with a 1/3-valued letter (B, D, H, V) for one strand, or
 // null.  Such windows sum thirds in the reference's order and take glibc's log of the
 // result (Ent, analysis/CrossCorr.cc:28-33), which the device cannot reproduce bit for bit,
-// so the host computes them with its libm (sxc_host_exact.h) and the kernels read them.
+// so the host computes them with its libm (window_entropy_row) and the kernels read them.
 SXC_HD inline const float* ent_row(const float* rows, const sxc_chunk& c, int strand) {
   if (!(c.flags & 1) || !rows) return nullptr;
   return rows + (size_t)((uint32_t)c.flags >> 8) * SXC_ENT_ROW + (size_t)strand * (SXC_ENT_ROW / 2);
@@ -324,17 +324,31 @@ SXC_HD inline float encode_at(const char* sm, const SeqView& qs, int p, int ch, 
 // Forward FFT of one channel into the padded LDS array: stage A (encode fused into
 // the bit-reversed load, or loaded from a given signal) then the rounds.
 template <int NB, int R, class Ex>
-SXC_HD inline void fwd_rounds(Ex& ex, float* Y, const Twiddle& tw) {
+SXC_HD inline void fwd_rounds(Ex& ex, float* Y, const Twiddle& tw, int upto = Schedule<NB>::NR) {
   using S = Schedule<NB>;
   if constexpr (R < S::NR) {
+    if (R >= upto) return;
     constexpr int K = S::k(R), M = S::m(R);
     using RG = RoundGeo<NB, K, M>;
     ex.each([&](int tid) {
       for (int u = tid; u < RG::UNITS; u += Geo<NB>::NT) round_unit<NB, K, M, false>(Y, u, tw);
     });
     ex.sync();
-    fwd_rounds<NB, R + 1>(ex, Y, tw);
+    fwd_rounds<NB, R + 1>(ex, Y, tw, upto);
   }
+}
+
+// The last round with the product (round_unit_product): forward levels when FWD, product with
+// T, inverse levels; the inverse transform then continues with round NR - 2.
+template <int NB, bool FWD, class Ex>
+SXC_HD inline void product_round(Ex& ex, float* Y, const Twiddle& tw, const float* T) {
+  using S = Schedule<NB>;
+  constexpr int R = S::NR - 1, K = S::k(R), M = S::m(R);
+  using RG = RoundGeo<NB, K, M>;
+  ex.each([&](int tid) {
+    for (int u = tid; u < RG::UNITS; u += Geo<NB>::NT) round_unit_product<NB, K, M, FWD>(Y, u, tw, T);
+  });
+  ex.sync();
 }
 
 template <int NB, int R, class Ex>
@@ -353,7 +367,8 @@ SXC_HD inline void inv_rounds(Ex& ex, float* Y, const Twiddle& tw) {
 
 template <int NB, class Ex>
 SXC_HD inline void forward_channel(Ex& ex, char* sm, const sxc_tables* tab, const Twiddle& tw, const SeqView& qs,
-                                   int ch, const float* given, const TailRef& tail) {
+                                   int ch, const float* given, const TailRef& tail,
+                                   int rounds = Schedule<NB>::NR) {
   using SM = Smem<NB>;
   using G = Geo<NB>;
   float* Y = (float*)(sm + SM::Y);
@@ -366,10 +381,11 @@ SXC_HD inline void forward_channel(Ex& ex, char* sm, const sxc_tables* tab, cons
       v[r] = given ? given[p] : encode_at<NB>(sm, qs, p, ch, tail);
     });
     stageA_forward<G::SA>(v, tw);
-    static_for<0, G::SA>([&](auto I) { Y[pad(G::SA * g + decltype(I)::value)] = v[decltype(I)::value]; });
+    float* ys = Y + pad(G::SA * g);  // pad(SA*g + r) = pad(SA*g) + r for r < SA (SA divides 32 or 32 | SA)
+    static_for<0, G::SA>([&](auto I) { ys[decltype(I)::value] = v[decltype(I)::value]; });
   });
   ex.sync();
-  fwd_rounds<NB, 0>(ex, Y, tw);
+  fwd_rounds<NB, 0>(ex, Y, tw, rounds);
 }
 
 // One workgroup per chunk: CCSignal::SetSequence(chunk, N) (reverse-complemented for
@@ -416,40 +432,17 @@ SXC_HD inline void spectra_body(Ex& ex, char* sm, const SpecArgs& a, int k) {
     const float* given = a.signals ? a.signals + ((size_t)k * 4 + ch) * G::N : nullptr;
     forward_channel<NB>(ex, sm, a.tab, tw, qs, ch, given, tail);
     ex.each([&](int tid) {
-      for (int p = tid; p < G::N; p += G::NT) out[(size_t)ch * G::N + p] = Y[pad(p)];
+      const float* yt = Y + pad(tid);  // pad(tid + k*NT) = pad(tid) + 33*k*NT/32 (NT a multiple of 32)
+      static_for<0, G::N / G::NT>([&](auto K) {
+        constexpr int k = decltype(K)::value;
+        out[(size_t)ch * G::N + tid + k * G::NT] = yt[33 * k * (G::NT / 32)];
+      });
     });
     ex.sync();
   }
 }
 
 // ---------------------------------------------------------------- correlate
-
-// CrossCorrelation::DoOne product on bins 0 .. H-2; bins H-1 and H keep the target
-// spectrum (analysis/CrossCorr.cc:480-492).  Y holds the query spectrum, T the target's.
-template <int NB, class Ex>
-SXC_HD inline void product(Ex& ex, float* Y, const float* T) {
-  using G = Geo<NB>;
-  constexpr int H = G::H;
-  ex.each([&](int tid) {
-    for (int i = tid; i < H; i += G::NT) {
-      if (i == 0) {
-        Y[pad(0)] = T[0] * Y[pad(0)];
-        Y[pad(H)] = T[H];
-      } else if (i < H - 1) {
-        const float r1 = T[i], i1 = T[i + H];
-        const float r2 = Y[pad(i)], i2 = -Y[pad(i + H)];
-        const float fr = r1 * r2 - i1 * i2;  // ComplexMult (analysis/CrossCorr.cc:20-26)
-        const float fi = i1 * r2 + i2 * r1;
-        Y[pad(i)] = fr;
-        Y[pad(i + H)] = -fi;
-      } else {
-        Y[pad(H - 1)] = T[H - 1];
-        Y[pad(2 * H - 1)] = T[2 * H - 1];
-      }
-    }
-  });
-  ex.sync();
-}
 
 // xc = (((0 + A) + C) + G) + T of rescaled, fftshifted channel correlations, left in
 // natural order in Y.  Per-thread accumulators stay in registers across channels.
@@ -472,19 +465,28 @@ SXC_HD inline void correlate(Ex& ex, char* sm, const sxc_tables* tab, const Twid
       if (ch > 0) ex.sync();
       const float* qch = qspec4 + (size_t)ch * G::N;
       ex.each([&](int tid) {
-        for (int p = tid; p < G::N; p += G::NT) Y[pad(p)] = qch[p];
+        float* yt = Y + pad(tid);
+        static_for<0, G::N / G::NT>([&](auto K) {
+          constexpr int k = decltype(K)::value;
+          yt[33 * k * (G::NT / 32)] = qch[tid + k * G::NT];
+        });
       });
       ex.sync();
+      // the resident spectrum is final: product and inverse from the last round on
+      product_round<NB, false>(ex, Y, tw, T4 + (size_t)ch * G::N);
     } else {
-      forward_channel<NB>(ex, sm, tab, tw, qs, ch, qsig4 ? qsig4 + (size_t)ch * G::N : nullptr, tail);
+      // all forward rounds but the last, which runs with the product and the first inverse round
+      forward_channel<NB>(ex, sm, tab, tw, qs, ch, qsig4 ? qsig4 + (size_t)ch * G::N : nullptr, tail,
+                          Schedule<NB>::NR - 1);
+      product_round<NB, true>(ex, Y, tw, T4 + (size_t)ch * G::N);
     }
-    product<NB>(ex, Y, T4 + (size_t)ch * G::N);
-    inv_rounds<NB, Schedule<NB>::NR - 1>(ex, Y, tw);
+    inv_rounds<NB, Schedule<NB>::NR - 2>(ex, Y, tw);
     ex.each([&](int g) {
       if (g >= G::GROUPS) return;
       Regs<NB>& r = ex.template reg<Regs<NB>>(g);
       float v[G::SA], o[G::SA];
-      static_for<0, G::SA>([&](auto I) { v[decltype(I)::value] = Y[pad(G::SA * g + decltype(I)::value)]; });
+      const float* ys = Y + pad(G::SA * g);
+      static_for<0, G::SA>([&](auto I) { v[decltype(I)::value] = ys[decltype(I)::value]; });
       stageA_inverse<G::SA>(v, o, tw);
       static_for<0, G::SA>([&](auto I) {
         constexpr int k = decltype(I)::value;
@@ -499,10 +501,19 @@ SXC_HD inline void correlate(Ex& ex, char* sm, const sxc_tables* tab, const Twid
   ex.each([&](int g) {
     if (g >= G::GROUPS) return;
     Regs<NB>& r = ex.template reg<Regs<NB>>(g);
+    // fftshift by N/2 (analysis/CrossCorr.cc:500-505): slot k of group g goes to
+    // stageA_pos(g, k) ^ N/2 = X_k | bitrev(g), X_k = stageA_hi(k) ^ N/2 a multiple of N/SA;
+    // when N/SA >= 32, pad(X_k | rg) = pad(rg) + 33 X_k / 32
+    constexpr int LOW = G::LOG - G::LSA;
+    const int rg = (int)bitrev((unsigned)g, LOW);
+    float* yg = Y + pad(rg);
     static_for<0, G::SA>([&](auto I) {
       constexpr int k = decltype(I)::value;
-      const int p = stageA_pos<NB>(g, k);
-      Y[pad(p ^ G::H)] = r.acc[k];  // fftshift by N/2 (analysis/CrossCorr.cc:500-505)
+      if constexpr (LOW >= 5) {
+        yg[33 * ((stageA_hi<NB>(k) ^ G::H) >> 5)] = r.acc[k];
+      } else {
+        Y[pad(stageA_pos<NB>(g, k) ^ G::H)] = r.acc[k];
+      }
     });
   });
   ex.sync();
@@ -528,10 +539,11 @@ SXC_HD inline void peaks(Ex& ex, char* sm, double cutoff) {
     if (NP > 8) {
       // the reference's sequential sum; the loads are issued 16 at a time ahead of it
       static_assert(SXC_ENV_SIZE % 16 == 0, "");
-      for (int i0 = tid * SXC_ENV_SIZE; i0 < (tid + 1) * SXC_ENV_SIZE; i0 += 16) {
+      const float* yb = Y + pad(tid * SXC_ENV_SIZE);  // + pad(i) for i < 256 (a multiple of 32 before)
+      for (int i0 = 0; i0 < SXC_ENV_SIZE; i0 += 16) {
         float sq[16];
         static_for<0, 16>([&](auto K) {
-          const float x = Y[pad(i0 + decltype(K)::value)];
+          const float x = yb[pad(i0 + decltype(K)::value)];
           sq[decltype(K)::value] = x * x;
         });
         static_for<0, 16>([&](auto K) { e += (double)sq[decltype(K)::value]; });
@@ -545,13 +557,17 @@ SXC_HD inline void peaks(Ex& ex, char* sm, double cutoff) {
   ex.each([&](int tid) {
     unsigned m = 0;
     int c = 0;
-    for (int k = 0; k < CS; ++k) {
-      const int i = tid * CS + k;
-      if ((double)Y[pad(i)] > env[i / SXC_ENV_SIZE] * cutoff + 1.) {
+    // the CS elements of this thread lie in one envelope block and one 32-float pad row
+    static_assert(SXC_ENV_SIZE % CS == 0 && (32 % CS == 0), "");
+    const float* yt = Y + pad(tid * CS);
+    const double thr = env[(tid * CS) / SXC_ENV_SIZE] * cutoff + 1.;
+    static_for<0, CS>([&](auto K) {
+      constexpr int k = decltype(K)::value;
+      if ((double)yt[k] > thr) {
         m |= 1u << k;
         ++c;
       }
-    }
+    });
     cnt[tid] = c;
     ex.template reg<Regs<NB>>(tid).acc[0] = __builtin_bit_cast(float, m);
   });

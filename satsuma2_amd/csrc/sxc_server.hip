@@ -465,16 +465,10 @@ int upload_chunks(std::vector<unsigned char>& bytes, std::vector<float>& ent, st
     sxc_chunk& t = table[(size_t)k];
     t.off = (int64_t)bytes.size();
     t.len = c.len;
-    t.flags = 2 | 4;
-    for (int i = 0; i < c.len; ++i) {
-      const unsigned char b = (unsigned char)c.seq[i];
-      if (b >= 128) return fail(SXC_ERR_ARG, "sequence byte >= 128");
-      const int cl = sxc::class_of(b);
-      if (sxc::class_has_thirds(cl)) t.flags |= 1;
-      if (cl > 3) t.flags &= ~2;
-      if (cl > 3 && cl != SXC_CLASS_N) t.flags &= ~4;
-      bytes.push_back(b);
-    }
+    t.flags = sxc::chunk_flags((const unsigned char*)c.seq, c.len);
+    if (t.flags < 0) return fail(SXC_ERR_ARG, "sequence byte >= 128");
+    bytes.resize(bytes.size() + (size_t)c.len);
+    if (c.len) memcpy(bytes.data() + t.off, c.seq, (size_t)c.len);
     if (t.flags & 1) {  // window entropies with the host's log
       const size_t row = ent.size() / SXC_ENT_ROW;
       if (row >= (1u << 24)) return fail(SXC_ERR_ARG, "too many chunks with B, D, H or V letters");

@@ -80,15 +80,8 @@ struct Layout {
 int chunk_of(const char* s, int len, int N, int row, float* erow, sxc_chunk* c) {
   c->off = 0;
   c->len = len;
-  c->flags = 2 | 4;
-  for (int i = 0; i < len; ++i) {
-    const unsigned char b = (unsigned char)s[i];
-    if (b >= 128) return fail(SXC_ERR_ARG, "sequence byte >= 128");
-    const int cl = sxc::class_of(b);
-    if (sxc::class_has_thirds(cl)) c->flags |= 1;
-    if (cl > 3) c->flags &= ~2;
-    if (cl > 3 && cl != SXC_CLASS_N) c->flags &= ~4;
-  }
+  c->flags = sxc::chunk_flags((const unsigned char*)s, len);
+  if (c->flags < 0) return fail(SXC_ERR_ARG, "sequence byte >= 128");
   if (c->flags & 1) {
     c->flags |= row << 8;
     sxc::window_entropy_row((const unsigned char*)s, len, N, erow);

@@ -15,6 +15,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <array>
 #include <atomic>
 #include <thread>
 #include <vector>
@@ -57,6 +58,27 @@ int class_of(int byte) {
 }
 
 bool class_has_thirds(int c) { return c == 10 || c == 11 || c == 12 || c == 13; }
+
+int chunk_flags(const unsigned char* s, int len) {
+  // per byte: bit 0 a 1/3-valued letter, bit 1 not A/C/G/T, bit 2 neither A/C/G/T nor N,
+  // bit 3 not a valid byte (>= 128)
+  static const auto kind = [] {
+    std::array<unsigned char, 256> k{};
+    for (int b = 0; b < 256; ++b) {
+      if (b >= 128) {
+        k[b] = 8;
+        continue;
+      }
+      const int cl = class_of(b);
+      k[b] = (unsigned char)((class_has_thirds(cl) ? 1 : 0) | (cl > 3 ? 2 : 0) | (cl > 3 && cl != SXC_CLASS_N ? 4 : 0));
+    }
+    return k;
+  }();
+  unsigned any = 0;
+  for (int i = 0; i < len; ++i) any |= kind[s[i]];
+  if (any & 8) return -1;
+  return (int)((any & 1) | (~any & 6));
+}
 
 void build_tables(sxc_tables* t) {
   memset(t, 0, sizeof *t);

@@ -10,6 +10,9 @@ int ent_lut_len(int nb);     // doubles in the entropy table for N = 2^nb
 void build_ent_lut(int nb, double* out);
 int class_of(int byte);
 bool class_has_thirds(int c);
+// sxc_chunk flags bits 0-2 of a chunk's bytes (bit 0: a B/D/H/V letter; bit 1: only A/C/G/T;
+// bit 2: only A/C/G/T/N), or -1 if a byte is >= 128.
+int chunk_flags(const unsigned char* s, int len);
 // z above which a run's probability is certainly below min_prob (see sxc_scan.h tail).
 double keep_z_bound(double min_prob, double target_size);
 // ProbTable thresholds for a target size: SXC_PT_ROWS x SXC_PT_LEN doubles (row 0 unused).
