@@ -18,8 +18,8 @@
 // (FFTReal.hpp:661-906) runs the same units with levels in reverse order.
 //
 // Twiddles: FFTReal's per-level LUT (float)cos(i*PI/2^L) equals the level-12 table at
-// stride 2^(12-L) (same double argument), so one 2048-entry table serves every level
-// <= 12.  Levels > 12 use FFTReal's float oscillator (OscSinCos.hpp:88-104); its values
+// stride 2^(12-L) (same double argument); the host lays the levels out back to back so a
+// unit's lanes read consecutive entries of one level.  Levels > 12 use FFTReal's float oscillator (OscSinCos.hpp:88-104); its values
 // depend only on the butterfly index and are tabulated by the host.
 //
 // All arithmetic here must be compiled without FP contraction (no FMA): FFTReal is built
@@ -47,9 +47,17 @@ SXC_HD inline void static_for(F&& f) {
   }
 }
 
-// LDS index of FFT position p: one pad word per 32 floats keeps the 32-float rows of
-// stage A on distinct banks.
-SXC_HD constexpr int pad(int p) { return p + (p >> 5); }
+// LDS index of FFT position p: one pad word per row of PAD_ROW floats keeps the 32-float rows
+// of stage A on distinct banks.
+#ifndef SXC_PAD_SHIFT
+#define SXC_PAD_SHIFT 5
+#endif
+constexpr int PAD_SHIFT = SXC_PAD_SHIFT;
+constexpr int PAD_ROW = 1 << PAD_SHIFT;
+SXC_HD constexpr int pad(int p) { return p + (p >> PAD_SHIFT); }
+// pad(p + k*PAD_ROW*m) = pad(p) + k*(PAD_ROW + 1)*m: the index step of `d` positions, d a
+// multiple of PAD_ROW
+SXC_HD constexpr int pad_step(int d) { return d + d / PAD_ROW; }
 // Stage-A group size (positions per thread in the first levels) and the largest number
 // of levels one LDS round carries.
 #ifndef SXC_STAGE_A
@@ -88,7 +96,7 @@ struct Geo {
   static constexpr int KM = geo_km(NB);                    // levels per LDS round at most
   static constexpr int LSA = SA == 32 ? 5 : 4;             // log2(SA)
   static constexpr int NT = (N / SA < 64) ? 64 : N / SA;   // threads per workgroup
-  static constexpr int YLEN = N + N / 32;                  // padded LDS floats
+  static constexpr int YLEN = N + N / PAD_ROW;            // padded LDS floats
   static constexpr int GROUPS = N / SA;                    // stage-A units
   static_assert(NT <= 1024, "a workgroup has at most 1024 threads");
 };
@@ -103,13 +111,13 @@ SXC_HD inline unsigned bitrev(unsigned x, int bits) {
 #define SXC_SQRT2_2 ((float)(1.41421356237309514547462185873883 * 0.5))
 
 struct Twiddle {
-  const float* lut;  // level-12 cosine table (LDS copy)
+  const float* lut;  // per-level cosine tables (SXC_LUT_OFF), the LDS copy or the constant table
   const float* osc;  // {cos, sin} after i oscillator steps: level 13 at [0, 8192), level 14 at [8192, 24576)
   template <int LEVEL>
   SXC_HD inline void get(int beta, int hbeta, float& c, float& s) const {
     if constexpr (LEVEL <= 12) {
-      c = lut[beta << (12 - LEVEL)];
-      s = lut[hbeta << (12 - LEVEL)];
+      c = lut[SXC_LUT_OFF(LEVEL) + beta];
+      s = lut[SXC_LUT_OFF(LEVEL) + hbeta];
     } else {
       const float* t = osc + (LEVEL == 13 ? 0 : 8192) + 2 * beta;
       c = t[0];
@@ -355,27 +363,28 @@ struct RoundGeo {
 
 // LDS indices of a unit's positions split into a per-thread base and a compile-time constant,
 // so every slot is one LDS access with an immediate offset.  A regular unit j (1 <= j < h_m/2)
-// of the big block starting at `off` (a multiple of 32) touches off + A*h_m + s*j, s = +-1:
+// of the big block starting at `off` (a multiple of the pad row R) touches off + A*h_m + s*j,
+// s = +-1:
 //   pad(off + A*h_m + j) = pad(off) + pj + cp(A),   pad(off + A*h_m - j) = pad(off) - mj + cm(A)
-// with, for h_m >= 32 (h_m a multiple of 32): pj = pad(j), mj = j + ceil(j/32),
-// cp(A) = cm(A) = 33*A*h_m/32; for h_m < 32 (h_m divides 32, j < h_m): pj = mj = j,
-// cp(A) = pad(A*h_m), cm(A) = A*h_m + floor((A*h_m - 1)/32) (the pad word of the block
-// holding A*h_m - 1).
+// with, for h_m >= R (h_m a multiple of R): pj = pad(j), mj = j + ceil(j/R),
+// cp(A) = cm(A) = pad_step(A*h_m); for h_m < R (h_m divides R, j < h_m): pj = mj = j,
+// cp(A) = pad(A*h_m), cm(A) = A*h_m + floor((A*h_m - 1)/R) (the pad word of the row holding
+// A*h_m - 1).
 template <int HM>
 SXC_HD constexpr int pad_cp(int A) {
-  return HM >= 32 ? 33 * A * (HM / 32) : A * HM + ((A * HM) >> 5);
+  return HM >= PAD_ROW ? pad_step(A * HM) : A * HM + ((A * HM) >> PAD_SHIFT);
 }
 template <int HM>
 SXC_HD constexpr int pad_cm(int A) {
-  return HM >= 32 ? 33 * A * (HM / 32) : A * HM + ((A * HM - 1) >> 5);
+  return HM >= PAD_ROW ? pad_step(A * HM) : A * HM + ((A * HM - 1) >> PAD_SHIFT);
 }
 template <int HM>
 SXC_HD inline int pad_pj(int j) {
-  return HM >= 32 ? j + (j >> 5) : j;
+  return HM >= PAD_ROW ? j + (j >> PAD_SHIFT) : j;
 }
 template <int HM>
 SXC_HD inline int pad_mj(int j) {
-  return HM >= 32 ? j + ((j + 31) >> 5) : j;
+  return HM >= PAD_ROW ? j + ((j + PAD_ROW - 1) >> PAD_SHIFT) : j;
 }
 
 // Runs unit u of a round on the padded LDS array Y.  Units [0, NREG) are regular,
@@ -384,7 +393,7 @@ template <int NB, int K, int M, bool INV>
 SXC_HD inline void round_unit(float* Y, int u, const Twiddle& tw) {
   using RG = RoundGeo<NB, K, M>;
   constexpr int HM = RG::HM;
-  static_assert(RG::BIG % 32 == 0, "big blocks start on a pad boundary");
+  static_assert(RG::BIG % PAD_ROW == 0, "big blocks start on a pad boundary");
   float v[RG::S];
   if (u < RG::NREG) {
     const int b = u / (RG::UPB - 1);

@@ -63,11 +63,11 @@ __global__ __launch_bounds__(Geo<NB>::NT) void k_xcorr(XcorrArgs a) {
   DevExec<Regs<NB>> ex{r};
   using SM = Smem<NB>;
   float* lut = (float*)(sm + SM::LUT);
-  for (int i = threadIdx.x; i < 2048; i += Geo<NB>::NT) lut[i] = a.tab->lut12[i];
+  for (int i = threadIdx.x; i < SXC_LUT_LEN; i += Geo<NB>::NT) lut[i] = a.tab->lut[i];
   __syncthreads();
   const Twiddle tw{lut, a.osc};
   const int k = blockIdx.x;
-  correlate<NB>(ex, sm, a.tab, tw, SeqView{nullptr, nullptr, 0}, a.spectra + (size_t)k * 4 * Geo<NB>::N,
+  correlate<NB>(ex, sm, a.tab, tw, SeqView{nullptr, nullptr, 0, 0}, a.spectra + (size_t)k * 4 * Geo<NB>::N,
                 a.qsig + (size_t)k * 4 * Geo<NB>::N, TailRef{nullptr, 0, nullptr});
   const float* Y = (const float*)(sm + SM::Y);
   for (int p = threadIdx.x; p < Geo<NB>::N; p += Geo<NB>::NT) a.xc[(size_t)k * Geo<NB>::N + p] = Y[pad(p)];

@@ -111,22 +111,23 @@ struct Smem {
   static constexpr int Y = 0;                                  // float[YLEN]: FFT / xc / scan area
   // lean plan: twiddles and chunk bytes are read through the caches, not staged here
   static constexpr bool LEAN = geo_lean(NB) != 0;
-  static constexpr int LUT = Y + 4 * G::YLEN;                  // float[2048] twiddles
+  static constexpr int LUT = Y + 4 * G::YLEN;                  // float[SXC_LUT_LEN] twiddles
   // the peaks phase's per-thread counts reuse the twiddle area (the transforms are done)
   static constexpr int CNT = LUT;                              // int32[NT]
-  static_assert(4 * NT <= 4 * 2048, "counts fit the twiddle area");
-  static constexpr int ENT = LUT + (LEAN ? 4 * NT : 4 * 2048);  // float[NWIN]
+  static_assert(NT <= SXC_LUT_LEN, "counts fit the twiddle area");
+  static constexpr int ENT = LUT + (LEAN ? 4 * NT : 4 * SXC_LUT_LEN);  // float[NWIN]
   static constexpr int ENV = ENT + 4 * NWIN;                   // double[max(N/256, 1)]
   static constexpr int NENV = (N / 256) > 0 ? N / 256 : 1;
   static constexpr int OFF = ENV + 8 * NENV;                   // double[4] channel means
   static constexpr int DSUM = OFF + 32;                        // double[4] sequential sums
   static constexpr int MISC = DSUM + 32;                       // int32[16]
-  static constexpr int CODEC = MISC + 64;                      // double[18][4]
+  static constexpr int CODEC = MISC + 64;                      // double[18][4]; DNA_X - off[X] after prepare
   static constexpr int CLS = CODEC + 8 * 4 * SXC_NCLASS;       // uchar[128] byte -> class
   static constexpr int QUART = CLS + 128;                      // uchar[18][4]
-  static constexpr int RCT = QUART + 4 * SXC_NCLASS;           // uchar[128] reverse complement
-  // strand-applied chunk bytes: an LDS copy up to N = 16384; at 32768 the plan would not fit
-  // the 160 KB of a workgroup, and the bytes are read from global memory instead
+  static constexpr int RCT = QUART + 4 * SXC_NCLASS;           // uchar[128] byte -> class of its complement
+  // codec classes of the strand-applied chunk bytes: an LDS copy up to N = 16384; at 32768
+  // the plan would not fit the 160 KB of a workgroup, and the bytes are read from global
+  // memory (through the class tables) instead
   static constexpr bool SEQ_IN_LDS = G::LOG <= 14 && !LEAN;
   static constexpr int QSEQ = (RCT + 128 + 15) & ~15;          // uchar[N] when SEQ_IN_LDS
   static constexpr int BYTES = (QSEQ + (SEQ_IN_LDS ? N : 0) + 15) & ~15;
@@ -139,7 +140,7 @@ struct Smem {
 template <int NB>
 SXC_HD inline const float* twiddle_table(const char* sm, const sxc_tables* tab) {
   if constexpr (Smem<NB>::LEAN) {
-    return tab->lut12;
+    return tab->lut;
   } else {
     return (const float*)(sm + Smem<NB>::LUT);
   }
@@ -171,14 +172,16 @@ SXC_HD inline int score_of(const unsigned char* cls, const unsigned char* score,
   return score[cls[tb] * SXC_NCLASS + cls[qb]];
 }
 
-// The bytes being encoded with the strand applied: the LDS copy, or (rc != null) the chunk's
-// bytes in global memory read reverse-complemented (DNAVector::ReverseComplement,
-// analysis/DNAVector.cc:482-521).
+// The codec classes of the bytes being encoded, with the strand applied: the LDS copy (which
+// holds classes), or the chunk's bytes in global memory mapped through a 128-entry table
+// (byte -> class; for the reverse strand byte -> class of its complement, read from the end:
+// DNAVector::ReverseComplement, analysis/DNAVector.cc:482-521).
 struct SeqView {
   const unsigned char* p;
-  const unsigned char* rc;
+  const unsigned char* map;  // null: p holds classes
   int L;
-  SXC_HD unsigned char operator[](int i) const { return rc ? rc[p[L - 1 - i] & 127] : p[i]; }
+  int rev;
+  SXC_HD unsigned char operator[](int i) const { return map ? map[p[rev ? L - 1 - i : i] & 127] : p[i]; }
 };
 
 // Loads one chunk's bytes (reverse-complemented for strand 1) and computes the per-window
@@ -192,32 +195,33 @@ SXC_HD inline SeqView phase_prepare(Ex& ex, char* sm, const sxc_tables* tab, con
   using G = Geo<NB>;
   float* lut = (float*)(sm + SM::LUT);
   unsigned char* cls = (unsigned char*)(sm + SM::CLS);
-  unsigned char* rct = (unsigned char*)(sm + SM::RCT);
+  unsigned char* rct = (unsigned char*)(sm + SM::RCT);  // byte -> class of its complement
   unsigned char* quart = (unsigned char*)(sm + SM::QUART);
   double* codec = (double*)(sm + SM::CODEC);
   int* misc = (int*)(sm + SM::MISC);
   ex.each([&](int tid) {
     if constexpr (!SM::LEAN)
-      for (int i = tid; i < 2048; i += G::NT) lut[i] = tab->lut12[i];
+      for (int i = tid; i < SXC_LUT_LEN; i += G::NT) lut[i] = tab->lut[i];
     for (int i = tid; i < 128; i += G::NT) cls[i] = tab->cls[i];
     for (int i = tid; i < 4 * SXC_NCLASS; i += G::NT) {
       quart[i] = (&tab->quarters[0][0])[i];
       codec[i] = (&tab->codec[0][0])[i];
     }
-    for (int i = tid; i < 128; i += G::NT) rct[i] = tab->rc[i];
+    for (int i = tid; i < 128; i += G::NT) rct[i] = tab->cls[tab->rc[i] & 127];  // class of the complement
     if constexpr (SM::SEQ_IN_LDS) {
-      // DNAVector::ReverseComplement (analysis/DNAVector.cc:482-521)
+      // classes of the strand's bytes (DNAVector::ReverseComplement, analysis/DNAVector.cc:482-521)
       unsigned char* q = (unsigned char*)(sm + SM::QSEQ);
-      for (int i = tid; i < L; i += G::NT) q[i] = strand ? tab->rc[src[L - 1 - i] & 127] : src[i];
+      for (int i = tid; i < L; i += G::NT)
+        q[i] = strand ? tab->cls[tab->rc[src[L - 1 - i] & 127] & 127] : tab->cls[src[i] & 127];
     }
     if (tid < 16) misc[tid] = 0;
   });
   ex.sync();
   SeqView qs;
   if constexpr (SM::SEQ_IN_LDS) {
-    qs = SeqView{(const unsigned char*)(sm + SM::QSEQ), nullptr, L};
+    qs = SeqView{(const unsigned char*)(sm + SM::QSEQ), nullptr, L, 0};
   } else {
-    qs = SeqView{src, strand ? rct : nullptr, L};
+    qs = SeqView{src, strand ? rct : cls, L, strand};
   }
   float* ent = (float*)(sm + SM::ENT);
   double* dsum = (double*)(sm + SM::DSUM);
@@ -236,7 +240,7 @@ SXC_HD inline SeqView phase_prepare(Ex& ex, char* sm, const sxc_tables* tab, con
       const int k = (L - i < win) ? (L - i) : win;
       int q[4] = {0, 0, 0, 0};
       for (int j = i; j < i + k; ++j) {
-        const unsigned char* qq = quart + 4 * cls[qs[j]];
+        const unsigned char* qq = quart + 4 * qs[j];
         q[0] += qq[0];
         q[1] += qq[1];
         q[2] += qq[2];
@@ -255,7 +259,7 @@ SXC_HD inline SeqView phase_prepare(Ex& ex, char* sm, const sxc_tables* tab, con
     if (!thirds) {
       int q[4] = {0, 0, 0, 0};
       for (int i = tid; i < L; i += G::NT) {
-        const unsigned char* qq = quart + 4 * cls[qs[i]];
+        const unsigned char* qq = quart + 4 * qs[i];
         q[0] += qq[0];
         q[1] += qq[1];
         q[2] += qq[2];
@@ -265,7 +269,7 @@ SXC_HD inline SeqView phase_prepare(Ex& ex, char* sm, const sxc_tables* tab, con
         if (q[ch]) ex.atomic_add_shared(&misc[MISC_Q4 + ch], q[ch]);
     } else if (tid < 4) {
       double s = 0;
-      for (int i = 0; i < L; ++i) s += codec[4 * cls[qs[i]] + tid];
+      for (int i = 0; i < L; ++i) s += codec[4 * qs[i] + tid];
       dsum[tid] = s;
     }
   });
@@ -276,6 +280,12 @@ SXC_HD inline SeqView phase_prepare(Ex& ex, char* sm, const sxc_tables* tab, con
       const double sum = thirds ? dsum[tid] : (double)misc[MISC_Q4 + tid] * 0.25;
       off[tid] = sum / (double)L;
     }
+  });
+  ex.sync();
+  // the encoder's (DNA_X(b) - off) per class and channel: the same double subtraction
+  // SeqToPCM evaluates per base (analysis/CrossCorr.cc:127), done once per class
+  ex.each([&](int tid) {
+    for (int i = tid; i < 4 * SXC_NCLASS; i += G::NT) codec[i] = codec[i] - off[i & 3];
   });
   ex.sync();
   return qs;
@@ -314,11 +324,9 @@ SXC_HD inline float encode_at(const char* sm, const SeqView& qs, int p, int ch, 
   using SM = Smem<NB>;
   const int L = qs.L;
   if (p >= L) return tail.n ? tail_at<NB>(tail, p, ch) : 0.f;
-  const unsigned char* cls = (const unsigned char*)(sm + SM::CLS);
-  const double* codec = (const double*)(sm + SM::CODEC);
+  const double* diff = (const double*)(sm + SM::CODEC);  // DNA_X(class) - off[X] (phase_prepare)
   const float e = (L < 1024) ? 1.f : ((const float*)(sm + SM::ENT))[p / (Geo<NB>::N / 512)];
-  const double off = ((const double*)(sm + SM::OFF))[ch];
-  return (float)((double)e * (codec[4 * cls[qs[p]] + ch] - off));
+  return (float)((double)e * diff[4 * qs[p] + ch]);
 }
 
 // Forward FFT of one channel into the padded LDS array: stage A (encode fused into
@@ -381,7 +389,7 @@ SXC_HD inline void forward_channel(Ex& ex, char* sm, const sxc_tables* tab, cons
       v[r] = given ? given[p] : encode_at<NB>(sm, qs, p, ch, tail);
     });
     stageA_forward<G::SA>(v, tw);
-    float* ys = Y + pad(G::SA * g);  // pad(SA*g + r) = pad(SA*g) + r for r < SA (SA divides 32 or 32 | SA)
+    float* ys = Y + pad(G::SA * g);  // pad(SA*g + r) = pad(SA*g) + r for r < SA (SA divides the pad row)
     static_for<0, G::SA>([&](auto I) { ys[decltype(I)::value] = v[decltype(I)::value]; });
   });
   ex.sync();
@@ -413,14 +421,14 @@ SXC_HD inline void spectra_body(Ex& ex, char* sm, const SpecArgs& a, int k) {
   using G = Geo<NB>;
   const sxc_chunk ck = a.chunks[a.which[k]];
   const Twiddle tw{twiddle_table<NB>(sm, a.tab), a.osc};
-  SeqView qs{nullptr, nullptr, ck.len};
+  SeqView qs{nullptr, nullptr, ck.len, 0};
   if (!a.signals) {
     qs = phase_prepare<NB>(ex, sm, a.tab, a.ent_lut, ent_row(a.ent_rows, ck, a.strand), a.seq + ck.off, ck.len,
                            ck.flags & 1, a.strand);
   } else if constexpr (!SM::LEAN) {
     float* lut = (float*)(sm + SM::LUT);
     ex.each([&](int tid) {
-      for (int i = tid; i < 2048; i += G::NT) lut[i] = a.tab->lut12[i];
+      for (int i = tid; i < SXC_LUT_LEN; i += G::NT) lut[i] = a.tab->lut[i];
     });
     ex.sync();
   }
@@ -432,10 +440,11 @@ SXC_HD inline void spectra_body(Ex& ex, char* sm, const SpecArgs& a, int k) {
     const float* given = a.signals ? a.signals + ((size_t)k * 4 + ch) * G::N : nullptr;
     forward_channel<NB>(ex, sm, a.tab, tw, qs, ch, given, tail);
     ex.each([&](int tid) {
-      const float* yt = Y + pad(tid);  // pad(tid + k*NT) = pad(tid) + 33*k*NT/32 (NT a multiple of 32)
+      static_assert(G::NT % PAD_ROW == 0, "");
+      const float* yt = Y + pad(tid);  // pad(tid + k*NT) = pad(tid) + pad_step(k*NT)
       static_for<0, G::N / G::NT>([&](auto K) {
         constexpr int k = decltype(K)::value;
-        out[(size_t)ch * G::N + tid + k * G::NT] = yt[33 * k * (G::NT / 32)];
+        out[(size_t)ch * G::N + tid + k * G::NT] = yt[pad_step(k * G::NT)];
       });
     });
     ex.sync();
@@ -468,7 +477,7 @@ SXC_HD inline void correlate(Ex& ex, char* sm, const sxc_tables* tab, const Twid
         float* yt = Y + pad(tid);
         static_for<0, G::N / G::NT>([&](auto K) {
           constexpr int k = decltype(K)::value;
-          yt[33 * k * (G::NT / 32)] = qch[tid + k * G::NT];
+          yt[pad_step(k * G::NT)] = qch[tid + k * G::NT];
         });
       });
       ex.sync();
@@ -503,14 +512,14 @@ SXC_HD inline void correlate(Ex& ex, char* sm, const sxc_tables* tab, const Twid
     Regs<NB>& r = ex.template reg<Regs<NB>>(g);
     // fftshift by N/2 (analysis/CrossCorr.cc:500-505): slot k of group g goes to
     // stageA_pos(g, k) ^ N/2 = X_k | bitrev(g), X_k = stageA_hi(k) ^ N/2 a multiple of N/SA;
-    // when N/SA >= 32, pad(X_k | rg) = pad(rg) + 33 X_k / 32
+    // when N/SA is a multiple of the pad row, pad(X_k | rg) = pad(rg) + pad_step(X_k)
     constexpr int LOW = G::LOG - G::LSA;
     const int rg = (int)bitrev((unsigned)g, LOW);
     float* yg = Y + pad(rg);
     static_for<0, G::SA>([&](auto I) {
       constexpr int k = decltype(I)::value;
-      if constexpr (LOW >= 5) {
-        yg[33 * ((stageA_hi<NB>(k) ^ G::H) >> 5)] = r.acc[k];
+      if constexpr (LOW >= PAD_SHIFT) {
+        yg[pad_step(stageA_hi<NB>(k) ^ G::H)] = r.acc[k];
       } else {
         Y[pad(stageA_pos<NB>(g, k) ^ G::H)] = r.acc[k];
       }
@@ -558,7 +567,7 @@ SXC_HD inline void peaks(Ex& ex, char* sm, double cutoff) {
     unsigned m = 0;
     int c = 0;
     // the CS elements of this thread lie in one envelope block and one 32-float pad row
-    static_assert(SXC_ENV_SIZE % CS == 0 && (32 % CS == 0), "");
+    static_assert(SXC_ENV_SIZE % CS == 0 && (PAD_ROW % CS == 0), "");
     const float* yt = Y + pad(tid * CS);
     const double thr = env[(tid * CS) / SXC_ENV_SIZE] * cutoff + 1.;
     static_for<0, CS>([&](auto K) {
@@ -625,11 +634,11 @@ SXC_HD inline void correlate_body(Ex& ex, char* sm, const PairArgs& a, int ps) {
     int* misc0 = (int*)(sm + SM::MISC);
     ex.each([&](int tid) {
       if constexpr (!SM::LEAN)
-        for (int i = tid; i < 2048; i += G::NT) lut[i] = a.tab->lut12[i];
+        for (int i = tid; i < SXC_LUT_LEN; i += G::NT) lut[i] = a.tab->lut[i];
       if (tid < 16) misc0[tid] = 0;
     });
     ex.sync();
-    correlate<NB>(ex, sm, a.tab, tw, SeqView{nullptr, nullptr, qc.len}, a.spectra + (size_t)pr.t_chunk * 4 * G::N,
+    correlate<NB>(ex, sm, a.tab, tw, SeqView{nullptr, nullptr, qc.len, 0}, a.spectra + (size_t)pr.t_chunk * 4 * G::N,
                   nullptr, TailRef{nullptr, 0, nullptr},
                   a.qspectra + ((size_t)pr.q_chunk * 2 + strand) * 4 * G::N);
   } else {

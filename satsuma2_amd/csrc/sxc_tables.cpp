@@ -2,7 +2,8 @@
 //
 // Every value is computed with the same double expression the reference evaluates
 // (glibc libm on the host), so the device only ever looks values up:
-//   * FFTReal twiddles: (float)cos(i * (PI / 4096)) for the level-12 LUT
+//   * FFTReal twiddles: (float)cos(i * (PI / 4096)) for the level-12 LUT and its strides as
+//     the lower levels' LUTs
 //     (extern/RealFFT/FFTReal.hpp:352-374) and the float oscillator sequence for levels
 //     13, 14 (OscSinCos.hpp:61-104, reset per block, FFTReal.hpp:626, 639-641);
 //   * DNACodec rows, reverse complement and DNA_Equal (analysis/DNAVector.cc:13-57, 390-403);
@@ -84,7 +85,12 @@ void build_tables(sxc_tables* t) {
   memset(t, 0, sizeof *t);
   const double PI = 3.1415926535897932384626433832795;  // extern/RealFFT/def.h:42
   const double mul = PI / (double)(2048 << 1);
-  for (int i = 0; i < 2048; ++i) t->lut12[i] = (float)cos((double)i * mul);
+  float* lut12 = t->lut + SXC_LUT_OFF(12);
+  for (int i = 0; i < 2048; ++i) lut12[i] = (float)cos((double)i * mul);
+  // FFTReal's level-L table holds (float)cos(i * PI / 2^L), computed from the same double
+  // argument as level 12's entry i << (12 - L): the same floats
+  for (int lv = 3; lv < 12; ++lv)
+    for (int i = 0; i < (1 << (lv - 1)); ++i) t->lut[SXC_LUT_OFF(lv) + i] = lut12[i << (12 - lv)];
   double v[4];
   char rc;
   for (int c = 0; c < SXC_NCLASS; ++c) {

@@ -73,10 +73,16 @@ typedef struct {
   double ident;
 } sxc_result_dev;
 
+// Twiddle tables: levels 3 .. 12 back to back, level L holding 2^(L-1) entries (a unit's lanes
+// then read consecutive words of one level, not words 2^(12-L) apart in one table).
+#define SXC_LUT_OFF(L) ((1 << ((L) - 1)) - 4)
+#define SXC_LUT_LEN SXC_LUT_OFF(13)
+
 // Constant tables built by the host library at context creation.
 typedef struct {
-  float lut12[2048];            // (float)cos(i * PI / 4096): FFTReal's level-12 LUT, every lower
-                                // level is a stride of it (FFTReal.hpp:352-374)
+  float lut[SXC_LUT_LEN];        // FFTReal's per-level cosine LUTs (FFTReal.hpp:352-374): level L
+                                // (3..12) at SXC_LUT_OFF(L), (float)cos(i * PI / 2^L) for
+                                // i < 2^(L-1); every level is a stride of level 12's
   unsigned char cls[128];       // byte -> codec class
   unsigned char rc[128];        // DNACodec::m_rc (DNAVector.cc:13-57)
   unsigned char score[SXC_NCLASS][SXC_NCLASS];  // (int)(100 * DNA_Equal + 0.5) per class pair
