@@ -133,7 +133,7 @@ struct ScanSmem {
   static constexpr int BYTES = MISC + 64 + 4 * CE;
 };
 
-enum { SM_QN = 0, SM_KEPT = 1, SM_RUNS = 2, SM_NCAND = 3, SM_LT = 4, SM_LQ = 5, SM_FAST = 6, SM_NRB = 7, SM_NPOS = 16 };
+enum { SM_QN = 0, SM_KEPT = 1, SM_RUNS = 2, SM_NCAND = 3, SM_LT = 4, SM_LQ = 5, SM_NRB = 7, SM_NPOS = 16 };
 
 // Base code: class id (0..16) of a codec letter, or 128 | byte for a byte whose codec
 // row is zero.  Two bases are identical bytes iff their codes are equal.
@@ -221,6 +221,68 @@ SXC_HD inline uint32_t rev32(uint32_t x) {
   x = ((x >> 8) & 0x00ff00ffu) | ((x & 0x00ff00ffu) << 8);
   return (x >> 16) | (x << 16);
 #endif
+}
+
+// Bytes of hi:lo chosen by the selector bytes of sel (0-3: lo's bytes, 4-7: hi's; v_perm_b32).
+SXC_HD inline uint32_t byte_perm(uint32_t hi, uint32_t lo, uint32_t sel) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_amdgcn_perm(hi, lo, sel);
+#else
+  const uint64_t v = ((uint64_t)hi << 32) | lo;
+  uint32_t r = 0;
+  for (int b = 0; b < 4; ++b) r |= (uint32_t)((v >> (8 * ((sel >> (8 * b)) & 7))) & 0xffu) << (8 * b);
+  return r;
+#endif
+}
+
+// Hides x's value from the optimizer (no instruction).
+SXC_HD inline void opaque(uint32_t& x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  asm("" : "+v"(x));
+#else
+  (void)x;
+#endif
+}
+
+// a where mask is set, b elsewhere (v_bfi_b32)
+SXC_HD inline uint32_t bit_select(uint32_t mask, uint32_t a, uint32_t b) { return (a & mask) | (b & ~mask); }
+
+// Window flags of 16 consecutive positions, four at a time in the bytes of a word.  Group p
+// (bits 2p, 2p + 1) of z holds m(n) + 1 - m(n - 46) in {0, 1, 2} for the half's position p;
+// byte i covers positions 4i .. 4i + 3 and carries u = C - 19 + 128, C the window count.  As
+// 0 <= C <= 46, every u below lies in [109, 155]: no add carries and no subtraction borrows
+// across a byte, and bit 7 of u is the flag C >= 19.  ub holds u of the position before the
+// half in every byte and is advanced to the half's last position.  The flag of position
+// 4i + r is returned at bit 8i + 4 + r (bits 8i .. 8i + 3 are not defined).
+SXC_HD inline uint32_t half_flags(uint32_t z, uint32_t& ub) {
+  const uint32_t A = z & 0x33333333u;               // per byte: z0 | z2 << 4
+  const uint32_t P = A + ((z >> 2) & 0x33333333u);  // z0 + z1 | (z2 + z3) << 4
+  const uint32_t P01 = P & 0x0F0F0F0Fu;
+  const uint32_t s = P01 + ((P >> 4) & 0x0F0F0F0Fu);  // the byte's four z, <= 8
+  // the earlier bytes' s (<= 24): shift-adds, kept from being folded into a quarter-rate multiply
+  uint32_t x = s + (s << 8);
+  opaque(x);
+  x = x + (x << 16);
+  opaque(x);
+  x <<= 8;
+  const uint32_t u0 = ub + x - 0x0C080400u;  // u at position 4i - 1
+  const uint32_t u1 = u0 + (A & 0x03030303u) - 0x01010101u;
+  const uint32_t u2 = u0 + P01 - 0x02020202u;
+  const uint32_t u3 = u2 + ((A >> 4) & 0x03030303u) - 0x01010101u;
+  const uint32_t u4 = u0 + s - 0x04040404u;
+  ub = byte_perm(u4, u4, 0x03030303u);
+  return bit_select(0x80808080u, u4, bit_select(0x40404040u, u3 >> 1, bit_select(0x20202020u, u2 >> 2, u1 >> 3)));
+}
+
+// Window flags of a 32-position block (bit r: position r) from its z groups (positions 0-15
+// in zl, 16-31 in zh); ub as in half_flags.
+SXC_HD inline uint32_t window_flags(uint32_t zl, uint32_t zh, uint32_t& ub) {
+  const uint32_t n0 = half_flags(zl, ub);
+  const uint32_t n1 = half_flags(zh, ub);
+  // bytes 0 and 2 of c gather the half's positions 0-7 and 8-15
+  const uint32_t c0 = bit_select(0x0F0F0F0Fu, n0 >> 4, n0 >> 8);
+  const uint32_t c1 = bit_select(0x0F0F0F0Fu, n1 >> 4, n1 >> 8);
+  return byte_perm(c1, c0, 0x06040200u);
 }
 
 // matches in the window [n - 45, n], n = 32k + r
@@ -446,8 +508,11 @@ SXC_HD inline void scan_epoch_bitmap(Ex& ex, char* sm, const uint16_t* cands, in
     }
     uint64_t e2 = match_block(tpk, qpk, ti, qj, k0 - 2);
     uint64_t e1 = match_block(tpk, qpk, ti, qj, k0 - 1);
-    int C = window_count(e1, e2, match_block(tpk, qpk, ti, qj, k0 - 3), 31);  // C at 32 k0 - 1
-    uint32_t prev = (na - 1 >= 46 && C >= SXC_MIN_MATCHES) ? 1u : 0u;    // flag at 32 k0 - 1
+    const int C = window_count(e1, e2, match_block(tpk, qpk, ti, qj, k0 - 3), 31);  // C at 32 k0 - 1
+    uint32_t prev = (na - 1 >= 46 && C >= SXC_MIN_MATCHES) ? 1u : 0u;          // flag at 32 k0 - 1
+    uint32_t ub = byte_perm(0u, (uint32_t)(C + 128 - SXC_MIN_MATCHES), 0u);   // half_flags' u, every byte
+    // flags from n = 46: block 0 keeps none, block 1 its bits from 14, later blocks all
+    uint32_t keep = k0 == 0 ? 0u : (k0 == 1 ? 0xFFFFC000u : ~0u);
     uint16_t* rlist = (uint16_t*)(sm + SS::RLIST);
     // The block's 32 bases of each side are words w, w + 1, w + 2 shifted right by the
     // diagonal's bit offset; word w + 2 of one block is word w of the next.
@@ -456,6 +521,7 @@ SXC_HD inline void scan_epoch_bitmap(Ex& ex, char* sm, const uint16_t* cands, in
     const uint32_t* qw = qpk + (qj >> 4) + 2 * k0;
     uint32_t t0 = tw[0], q0 = qw[0];
     uint32_t e1l = (uint32_t)e1, e1h = (uint32_t)(e1 >> 32), e2h = (uint32_t)(e2 >> 32);
+    uint32_t f = 0;
     for (int k = k0; k <= klast; ++k, tw += 2, qw += 2) {
       const uint32_t t1 = tw[1], t2 = tw[2], q1 = qw[1], q2 = qw[2];
       const uint32_t xl = funnel32(t1, t0, tb) ^ funnel32(q1, q0, qb);
@@ -466,30 +532,21 @@ SXC_HD inline void scan_epoch_bitmap(Ex& ex, char* sm, const uint16_t* cands, in
       // m(n - 46) for the block: offsets r < 14 come from block k-2, r >= 14 from block k-1
       const uint32_t ml = funnel32(e1l, e2h, 4), mh = funnel32(e1h, e1l, 4);
       // 2-bit group r of z holds m(n) + 1 - m(n - 46), in {0, 1, 2}: no carry crosses a
-      // group, so each 32-bit half is one add.  D = C - 19 is negative exactly where the
-      // window is below the threshold; its sign bits shift into g, position 0 first.
+      // group, so each 32-bit half is one add
       const uint32_t zl = e0l + (~ml & 0x55555555u);
       const uint32_t zh = e0h + (~mh & 0x55555555u);
-      int D = C - SXC_MIN_MATCHES;
-      uint32_t gbits = 0;
-      static_for<0, 32>([&](auto R) {
-        constexpr int r = decltype(R)::value;
-        const uint32_t zr = (r < 16) ? (zl >> (2 * r)) & 3u : (zh >> (2 * (r - 16))) & 3u;
-        D += (int)zr - 1;
-        gbits = (gbits << 1) | ((uint32_t)D >> 31);
-      });
-      C = D + SXC_MIN_MATCHES;
-      uint32_t f = ~rev32(gbits);
-      if (32 * k < 46) f &= (46 - 32 * k >= 32) ? 0u : ~((1u << (46 - 32 * k)) - 1);  // flags from n = 46
-      if (k == klast && nb - 32 * k < 32) f &= (1u << (nb - 32 * k)) - 1;
+      f = window_flags(zl, zh, ub) & keep;
+      keep = 0xFFFFC000u | (keep >> 18);
       row[k] = f;
-      // (the same increment on every lane: one atomic per wave)
+      // (the same increment on every lane: one atomic per wave).  A rise past nb in the
+      // lag's last block only adds a list entry whose word (cut below) yields no run.
       if (f & ~((f << 1) | prev)) rlist[ex.atomic_add_shared(&misc[SM_NRB], 1)] = (uint16_t)(j * WPL + k);
       prev = f >> 31;
       e2h = e1h;
       e1l = e0l;
       e1h = e0h;
     }
+    if (nb - 32 * klast < 32) row[klast] = f & ((1u << (nb - 32 * klast)) - 1);  // the lag ends at npos
   });
   ex.sync();
   int* queue = (int*)(sm + SS::QUEUE);
@@ -563,7 +620,6 @@ SXC_HD inline void scan_epoch_segments(Ex& ex, char* sm, const uint16_t* cands, 
   int* misc = (int*)(sm + SS::MISC);
   int* summ = (int*)(sm + SS::SUMM);
   const int Lt = misc[SM_LT], Lq = misc[SM_LQ];
-  const bool fast = misc[SM_FAST] != 0;
   auto push = [&](int rank, int ns, int ne) { push_run<NB>(ex, sm, c0, rank, ns, ne); };
   ex.each([&](int tid) {
     const int c = c0 + tid / gsz;
@@ -571,76 +627,13 @@ SXC_HD inline void scan_epoch_segments(Ex& ex, char* sm, const uint16_t* cands, 
     int first_end = INF, open = -1;
     if (c < c1) {
       const Lag g = lag_geometry(cands[c], H, Lq, Lt);
-      // pure A/C/G/T segments start on 32-position blocks: no block is scanned by two threads
-      const int S = fast ? (((g.npos + gsz - 1) / gsz + 31) & ~31) : (g.npos + gsz - 1) / gsz;
+      const int S = (g.npos + gsz - 1) / gsz;
       const int na = l * S;
       const int nb = (na + S < g.npos) ? na + S : g.npos;
-      if (na < nb && fast) {
-        // Pure A/C/G/T: score = 100 * [match] and a window sum > 1889 <=> >= 19 matches.
-        // C(n) = C(n - 1) + m(n) - m(n - 46) with m(< 0) = 0 equals the reference's running
-        // `match` (analysis/CrossCorr.cc:700-705).  Every position of a 32-block is updated
-        // branch-free; flags become a 32-bit mask and run boundaries are its transitions.
-        const uint32_t* tpk = (const uint32_t*)(sm + SS::TPK);
-        const uint32_t* qpk = (const uint32_t*)(sm + SS::QPK);
-        const int ti = g.i0, qj = g.i0 + g.shift;
-        const int k0 = na >> 5, klast = (nb - 1) >> 5;
-        uint64_t e3 = match_block(tpk, qpk, ti, qj, k0 - 3);
-        uint64_t e2 = match_block(tpk, qpk, ti, qj, k0 - 2);
-        uint64_t e1 = match_block(tpk, qpk, ti, qj, k0 - 1);
-        int C = window_count(e1, e2, e3, 31);  // C at position 32 k0 - 1
-        bool prev = (na - 1 >= 46) && (na & 31) == 0 && C >= SXC_MIN_MATCHES;
-        for (int k = k0; k <= klast; ++k) {
-          const uint64_t e0 = match_block(tpk, qpk, ti, qj, k);
-          // m(n - 46) for the block: offsets r < 14 come from block k-2, r >= 14 from block k-1
-          const uint64_t minus = (e2 >> 36) | (e1 << 28);
-          // 2-bit group r of z holds m(n) + 1 - m(n - 46), in {0, 1, 2}: no carry crosses a
-          // group, so each 32-bit half is one add.  D = C - 19 is negative exactly where the
-          // window is below the threshold; its sign bits shift into g, position 0 first.
-          const uint32_t zl = (uint32_t)e0 + (~(uint32_t)minus & 0x55555555u);
-          const uint32_t zh = (uint32_t)(e0 >> 32) + (~(uint32_t)(minus >> 32) & 0x55555555u);
-          int D = C - SXC_MIN_MATCHES;
-          uint32_t g = 0;
-          static_for<0, 32>([&](auto R) {
-            constexpr int r = decltype(R)::value;
-            const uint32_t zr = (r < 16) ? (zl >> (2 * r)) & 3u : (zh >> (2 * (r - 16))) & 3u;
-            D += (int)zr - 1;
-            g = (g << 1) | ((uint32_t)D >> 31);
-          });
-          C = D + SXC_MIN_MATCHES;
-          uint32_t f = ~rev32(g);
-          if (32 * k < 46) f &= (46 - 32 * k >= 32) ? 0u : ~((1u << (46 - 32 * k)) - 1);  // flags from n = 46
-          const int r0 = (k == k0) ? na - 32 * k : 0;
-          const int r1 = (k == klast) ? nb - 32 * k : 32;
-          if (k == k0 && r0 > 0) prev = ((f >> (r0 - 1)) & 1) != 0;
-          const uint32_t range = (r1 == 32 ? ~0u : ((1u << r1) - 1)) & ~((1u << r0) - 1);
-          f &= range;
-          uint32_t trans = (f ^ ((f << 1) | ((prev ? 1u : 0u) << r0))) & range;
-          while (trans) {
-            const int r = __builtin_ctz(trans);
-            trans &= trans - 1;
-            const int n = 32 * k + r;
-            if ((f >> r) & 1) {
-              open = n;
-            } else if (open >= 0) {
-              push(c, open, n);
-              open = -1;
-            } else if (first_end == INF) {
-              first_end = n;
-            }
-          }
-          prev = ((f >> (r1 - 1)) & 1) != 0;
-          e3 = e2;
-          e2 = e1;
-          e1 = e0;
-        }
-        (void)e3;
-      } else if (!SS::PURE && na < nb) {
+      if (na < nb) {
         const unsigned char* tp = tc + g.i0;
         const unsigned char* qp = qc + g.i0 + g.shift;
-        auto sc = [&](int n) -> int {
-          const unsigned a = tp[n], b = qp[n];
-          return fast ? (a == b ? 100 : 0) : code_score(score, a, b);
-        };
+        auto sc = [&](int n) -> int { return code_score(score, tp[n], qp[n]); };
         int W = 0;
         for (int m = (na - 46 > 0 ? na - 46 : 0); m < na; ++m) W += sc(m);
         bool prev = (na - 1 >= 46) && W > SXC_WIN_THRESHOLD;
@@ -696,7 +689,6 @@ SXC_HD inline void drain(Ex& ex, char* sm, const ScanArgs& a, const sxc_pair_dev
   const double* gcv = (const double*)(sm + SS::GC);
   int* misc = (int*)(sm + SS::MISC);
   const int* queue = (const int*)(sm + SS::QUEUE);
-  const bool fast = misc[SM_FAST] != 0;
   const int strand = ps % a.strands;
   ex.each([&](int tid) {
     for (int k = tid; k < n; k += SCAN_NT) {
@@ -745,7 +737,7 @@ SXC_HD inline void drain(Ex& ex, char* sm, const ScanArgs& a, const sxc_pair_dev
                    !tail_from_sums(matches, gct, gcq, len, a.target_total, a.z_skip, prob, ident)) {
           continue;
         }
-      } else if (SS::PURE || fast) {
+      } else if constexpr (SS::PURE) {
         int cm, ct, cq;
         packed_counts((const uint32_t*)(sm + SS::TPK), (const uint32_t*)(sm + SS::QPK), start_t, start_q, len, cm,
                       ct, cq);
@@ -868,7 +860,6 @@ SXC_HD inline void scan_body(Ex& ex, char* sm, const ScanArgs& a, int blk) {
       misc[SM_NCAND] = a.ncand[ps];
       misc[SM_LT] = tch.len;
       misc[SM_LQ] = qch.len;
-      misc[SM_FAST] = fast ? 1 : 0;
     }
   });
   ex.sync();

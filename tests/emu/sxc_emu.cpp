@@ -382,6 +382,47 @@ int64_t emu_check_z_filter(int64_t n, uint64_t seed, int64_t* fired) {
   return bad;
 }
 
+// window_flags (the byte-parallel window test of the pure A/C/G/T scan) against the window
+// counted position by position: n diagonals of 32 * blocks positions whose match bits have a
+// density drawn per diagonal from 0 to 1 (so counts reach 0 and 46); returns the number of
+// wrong flag bits and, in *flagged, how many flags were set.
+int64_t emu_check_window_flags(int64_t n, int blocks, uint64_t seed, int64_t* flagged) {
+  uint64_t x = seed * 2654435761ull + 1;
+  auto next = [&]() {
+    x ^= x << 13;
+    x ^= x >> 7;
+    x ^= x << 17;
+    return x;
+  };
+  int64_t bad = 0, set = 0;
+  std::vector<int> m(32 * blocks);
+  for (int64_t i = 0; i < n; ++i) {
+    const uint64_t density = next() % 65;  // out of 64
+    for (auto& v : m) v = (next() % 64) < density ? 1 : 0;
+    uint32_t ub = sxc::byte_perm(0u, (uint32_t)(128 - sxc::SXC_MIN_MATCHES), 0u);  // C = 0 before position 0
+    int C = 0;
+    for (int k = 0; k < blocks; ++k) {
+      uint32_t zw[2] = {0, 0};
+      for (int r = 0; r < 32; ++r) {
+        const int p = 32 * k + r;
+        const int z = m[p] + 1 - (p >= 46 ? m[p - 46] : 0);
+        zw[r >> 4] |= (uint32_t)z << (2 * (r & 15));
+      }
+      const uint32_t f = sxc::window_flags(zw[0], zw[1], ub);
+      for (int r = 0; r < 32; ++r) {
+        const int p = 32 * k + r;
+        C += m[p] - (p >= 46 ? m[p - 46] : 0);
+        const uint32_t want = C >= sxc::SXC_MIN_MATCHES ? 1u : 0u;
+        bad += ((f >> r) & 1u) != want;
+        set += want;
+      }
+      bad += ub != sxc::byte_perm(0u, (uint32_t)(C + 128 - sxc::SXC_MIN_MATCHES), 0u);
+    }
+  }
+  *flagged = set;
+  return bad;
+}
+
 }  // extern "C"
 
 

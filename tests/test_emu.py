@@ -239,6 +239,18 @@ def test_tail_float_prefilter_is_one_sided(emu_lib):
     assert fired.value > 900_000
 
 
+def test_window_flags_bytewise(emu_lib):
+    """The pure A/C/G/T scan tests the 46-wide window four positions per byte: over 4000
+    diagonals of 256 positions with match densities from 0 to 1 every flag and every carried
+    count equals the window counted position by position, and flags are neither all set nor
+    all clear."""
+    emu_lib.emu_check_window_flags.restype = C.c_int64
+    flagged = C.c_int64(0)
+    bad = emu_lib.emu_check_window_flags(C.c_int64(4000), C.c_int(8), C.c_uint64(11), C.byref(flagged))
+    assert bad == 0
+    assert 0.2 * 4000 * 256 < flagged.value < 0.8 * 4000 * 256
+
+
 def test_emulated_bdhv_pairs(emu_lib, golden):
     """Chunks dense in B/D/H/V: the kernels read the window entropies the host computed with
     its log (sxc::window_entropy_row); signals (through the xc bits) and records equal the
