@@ -26,7 +26,9 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall", "-Wno-unus
 DEVICE = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 SOURCES = ["sxc_kernels.hip", "sxc_kernels_r2.hip", "sxc_kernels_s32.hip", "sxc_kernels_occ.hip", "sxc_kernels_lean.hip",
            "sxc_api.hip",
-           "sxc_server.hip", "sxc_static.hip", "sxc_tables.cpp"]
+           "sxc_server.hip", "sxc_static.hip", "sxc_tables.cpp",
+           # the master's side of the search (host C++): match sets, chaining, grid search
+           "sxc_master_api.cpp", "host/sxc_master.cpp", "host/sxc_seqio.cpp"]
 # Host programs linked against the library (plain C++, no device code).
 PROGRAMS = {
     "HomologyByXCorrSlave": ["host/slave_main.cpp", "host/sxc_seqio.cpp"],
@@ -38,7 +40,7 @@ CXX = os.environ.get("CXX", "g++")
 
 
 def _compile(src: str) -> str:
-    obj = os.path.join(OBJDIR, src + ".o")
+    obj = os.path.join(OBJDIR, src.replace("/", "_") + ".o")
     cmd = [HIPCC] + COMMON + (DEVICE if src.endswith(".hip") else []) + [
         "-c", os.path.join(CSRC, src), "-o", obj + ".tmp"]
     if src.endswith(".cpp"):

@@ -1,0 +1,553 @@
+// sxc_master.cpp -- match sets, chaining, interpolation, coverage and the grid search of
+// Satsuma2's master (declarations and references in sxc_master.h).
+#include "sxc_master.h"
+
+#include <math.h>
+
+#include <algorithm>
+#include <utility>
+
+namespace sxc_master {
+
+bool match_less(const Match& a, const Match& b) {
+  if (a.target != b.target) return a.target < b.target;
+  if (a.query != b.query) return a.query < b.query;
+  if (a.rc != b.rc) return !a.rc;
+  if (a.start_target == b.start_target) return a.length < b.length;
+  return a.start_target < b.start_target;
+}
+
+void MatchSet::sort() { std::sort(m.begin(), m.end(), match_less); }
+
+// Neighbouring matches (in the set's order) on the same target and strand whose target and
+// query starts both differ by less than 4 merge into one: the merged match keeps its first
+// member's target start and spans the members' query extents.  The query ids are not
+// compared.  The group still open when the list ends is not emitted, so a set of one match
+// collapses to none (analysis/SequenceMatch.cc:418-469).
+void MatchSet::collapse() {
+  std::vector<Match> out;
+  if (m.empty()) return;
+  auto near = [](int a, int b) { return a - b < 4 && b - a < 4; };  // Laps (:408-416)
+  Match cur = m[0];
+  for (size_t i = 1; i < m.size(); ++i) {
+    const Match& a = m[i - 1];
+    const Match& b = m[i];
+    if (a.target == b.target && a.rc == b.rc && near(a.start_target, b.start_target) &&
+        near(a.start_query, b.start_query)) {
+      if (b.start_query < cur.start_query) {
+        cur.length = cur.start_query + cur.length - b.start_query;
+        cur.start_query = b.start_query;
+      } else {
+        cur.length = b.start_query + b.length - cur.start_query;
+      }
+    } else {
+      out.push_back(cur);
+      cur = b;
+    }
+  }
+  m.swap(out);
+}
+
+void MatchSet::length_filter(uint64_t min_length) {
+  std::vector<Match> out;
+  out.reserve(m.size());
+  for (const Match& x : m)
+    if ((uint64_t)(int64_t)x.length >= min_length) out.push_back(x);
+  m.swap(out);
+}
+
+int64_t MatchSet::collect(const Result* r, int64_t n, bool from_file) {
+  // collect_new_matches reuses one SingleMatch for every result and only adds to its
+  // m_matches, so each match carries the running sum of ident * len over the results
+  // collected before it in the same call.
+  double acc = 0.;
+  m.reserve(m.size() + (size_t)n);
+  for (int64_t i = 0; i < n; ++i) {
+    Result x = r[i];
+    if (from_file && x.reverse) x.qstart = x.query_size - x.len - x.qstart;
+    Match y;
+    y.query = (int32_t)x.query_id;
+    y.target = (int32_t)x.target_id;
+    y.query_len = (int32_t)x.query_size;
+    y.start_query = (int32_t)x.qstart;
+    y.start_target = (int32_t)x.tstart;
+    y.length = (int32_t)x.len;
+    y.rc = x.reverse;
+    y.prob = x.prob;
+    y.ident = x.ident;
+    acc += x.ident * (double)x.len;
+    y.matches = acc;
+    m.push_back(y);
+  }
+  return n;
+}
+
+// ---- chaining (analysis/MatchDynProg.cc)
+
+namespace {
+
+constexpr double kUnreached = 999999999999999.;  // PRETTY_INFINITE (:12)
+constexpr double kRcPenalty = 20.;               // RC_PENALTY (:14)
+constexpr double kMaxPenalty = 40;               // MAX_PENALTY (:15)
+constexpr double kOtherQueryPenalty = 150;       // GetHighMaxPenalty (:33-36)
+
+// MatchPenalty (:22-27)
+double match_penalty(double prob) { return -200 * log(prob); }
+
+// TransPenalty (:38-86): the cost of following a match at (t1, q1) with one at (t2, q2).
+double trans_penalty(int t1, int q1, bool rc1, int t2, int q2, bool rc2) {
+  if (rc1 != rc2) return kRcPenalty;
+  double expect = t2 - t1;
+  double observe = q2 - q1;
+  double sigma = 0;
+  if (expect > 1000000) sigma = sqrt(expect) / 10. - 100;
+  if (rc1 && rc2) observe = -observe;
+  double diff = observe - expect;
+  if (diff < 0.) diff = -diff;
+  const double flat = diff == 1 ? 0. : 0.01;
+  if (expect < 300.) expect = 300;
+  double p = diff / expect + sigma + flat;
+  if (p > kMaxPenalty) p = kMaxPenalty;
+  return p;
+}
+
+// GetRepeatScore (:88-106): weight of a match by how many matches cover its midpoint.
+double repeat_score(int v) {
+  if (v <= 1) return 10.;
+  if (v == 2) return 5.;
+  if (v == 3) return 2.5;
+  if (v == 4) return 1.2;
+  if (v >= 25) return 0.000001;
+  if (v >= 10) return 0.001;
+  return 0.5;
+}
+
+// SingleMatchDP (:109-163)
+struct Node {
+  Match m;
+  double score = kUnreached;
+  double pen = -1.;
+  int back = -1;
+  double rep = 100.;
+  double total() {
+    if (pen < 0.) pen = match_penalty(m.prob);
+    return score + pen;
+  }
+  void offer(double s, int from) {
+    if (s < score) {
+      score = s;
+      back = from;
+    }
+  }
+};
+
+// MatchDynProg::Chain (:194-243): matches by target start; each one offers every later
+// match within the look-ahead its score plus the transition cost and the repeat weights of
+// the matches it skips; the chain is traced back from the last match and returned sorted.
+void chain_target(std::vector<Node>& v, std::vector<Match>* out) {
+  constexpr int kLookAheadDist = 250000;  // m_laDist (:173)
+  constexpr int kLookAheadCount = 2000;   // la_limit (:200)
+  std::sort(v.begin(), v.end(), [](const Node& a, const Node& b) { return a.m.start_target < b.m.start_target; });
+  if (v.empty()) return;
+  v[0].offer(0., -1);
+  const int n = (int)v.size();
+  for (int i = 0; i < n; ++i) {
+    Node& one = v[(size_t)i];
+    double skip = 0.;
+    int fed = 0;
+    for (int j = i + 1; j < n; ++j) {
+      Node& two = v[(size_t)j];
+      if (two.m.start_target - one.m.start_target > kLookAheadDist && fed > 10) break;
+      if (j - i > kLookAheadCount) break;
+      ++fed;
+      double trans = kOtherQueryPenalty;
+      if (one.m.query == two.m.query)
+        trans = trans_penalty(one.m.start_target, one.m.start_query, one.m.rc, two.m.start_target, two.m.start_query,
+                              two.m.rc);
+      trans += skip;
+      skip += two.rep;
+      two.offer(one.total() + trans, i);
+    }
+  }
+  const size_t first = out->size();
+  for (int i = n - 1; i >= 0; i = v[(size_t)i].back) out->push_back(v[(size_t)i].m);
+  std::sort(out->begin() + (std::ptrdiff_t)first, out->end(), match_less);
+}
+
+}  // namespace
+
+void chain(const MatchSet& in, MatchSet* out) {
+  const int nt = (int)in.target_size.size(), nq = (int)in.query_size.size();
+  out->target_size = in.target_size;
+  out->query_size = in.query_size;
+  out->m.clear();
+  // per-base coverage counts, capped at 100, with a match overlapping its predecessor on
+  // the same target and query counted only past it (:444-485)
+  std::vector<std::vector<signed char>> tcov((size_t)nt), qcov((size_t)nq);
+  for (int i = 0; i < nt; ++i) tcov[(size_t)i].assign((size_t)std::max(0, in.target_size[(size_t)i]), 0);
+  for (int i = 0; i < nq; ++i) qcov[(size_t)i].assign((size_t)std::max(0, in.query_size[(size_t)i]), 0);
+  Match last;
+  for (const Match& x : in.m) {
+    if (x.query < 0) continue;
+    int st = x.start_target, sq = x.start_query;
+    if (x.target == last.target && x.query == last.query && x.start_target <= last.start_target + last.length &&
+        x.start_query <= last.start_query + last.length && x.start_target > last.start_target &&
+        x.start_query > last.start_query) {
+      st = last.start_target + last.length;
+      sq = last.start_query + last.length;
+    }
+    last = x;
+    std::vector<signed char>& t = tcov[(size_t)x.target];
+    std::vector<signed char>& q = qcov[(size_t)x.query];
+    for (int j = st; j < x.start_target + x.length; ++j)
+      if (j > 0 && j < (int)t.size() && t[(size_t)j] < 100) ++t[(size_t)j];
+    for (int j = sq; j < x.start_query + x.length; ++j)
+      if (j > 0 && j < (int)q.size() && q[(size_t)j] < 100) ++q[(size_t)j];
+  }
+  // each target's matches are taken from its first to its last index (:489-505)
+  const int n = (int)in.m.size();
+  std::vector<int> first((size_t)nt, n + 1), lastI((size_t)nt, -1);
+  for (int i = 0; i < n; ++i) {
+    const int id = in.m[(size_t)i].target;
+    if (i < first[(size_t)id]) first[(size_t)id] = i;
+    if (i > lastI[(size_t)id]) lastI[(size_t)id] = i;
+  }
+  auto cov_at = [](const std::vector<signed char>& c, int p) -> int {
+    // a midpoint outside the sequence reads as uncovered (the reference reads past its array)
+    return (p >= 0 && p < (int)c.size()) ? c[(size_t)p] : 0;
+  };
+  std::vector<Node> nodes;
+  std::vector<Match> result;
+  for (int j = 0; j < nt; ++j) {
+    int f = first[(size_t)j];
+    const int l = lastI[(size_t)j];
+    if (l == -1) f = 0;
+    nodes.clear();
+    const std::vector<signed char>& t = tcov[(size_t)j];
+    for (int i = f; i <= l; ++i) {
+      const Match& x = in.m[(size_t)i];
+      const std::vector<signed char>& q = qcov[(size_t)x.query];
+      double r = repeat_score(cov_at(t, x.start_target + x.length / 2));
+      const double r2 = repeat_score(cov_at(q, x.start_query + x.length / 2));
+      if (r2 < r) r = r2;
+      if (r > 0.0001) {
+        Node nd;
+        nd.m = x;
+        nd.rep = r;
+        nodes.push_back(nd);
+      }
+    }
+    result.clear();
+    chain_target(nodes, &result);
+    out->m.insert(out->m.end(), result.begin(), result.end());
+  }
+}
+
+void chain_dups(const MatchSet& in, MatchSet* out) {
+  MatchSet first;
+  chain(in, &first);
+  first.sort();
+  // per target, the query with the most chained matches; every match of the target to
+  // another query goes into a second chaining round (:263-341)
+  MatchSet rest;
+  rest.target_size = in.target_size;
+  rest.query_size = in.query_size;
+  const int nt = (int)in.target_size.size(), nq = (int)in.query_size.size();
+  std::vector<int> hits((size_t)nq);
+  for (int j = 0; j < nt; ++j) {
+    std::fill(hits.begin(), hits.end(), 0);
+    for (const Match& x : first.m)
+      if (x.target == j && x.query >= 0) ++hits[(size_t)x.query];
+    int best = -1, most = 0;
+    for (int i = 0; i < nq; ++i)
+      if (hits[(size_t)i] > most) {
+        most = hits[(size_t)i];
+        best = i;
+      }
+    for (const Match& x : in.m)
+      if (x.target == j && x.query != best) rest.m.push_back(x);
+  }
+  chain(rest, out);
+  out->m.insert(out->m.end(), first.m.begin(), first.m.end());
+  out->sort();
+}
+
+// ---- interpolation (analysis/SatsumaSynteny2.cc:18-146)
+
+void interpolate(MatchSet* chained) {
+  constexpr int kMinRun = 40;       // m_min
+  constexpr int kAdded = 2;         // m_num
+  constexpr int kSpace = 100000;    // m_space
+  std::vector<Match>& v = chained->m;
+  const int n = (int)v.size();
+  // mean angle of the steps between consecutive matches first..last (Slope, :30-49)
+  auto slope = [&](int first, int last, double& sx, double& sy) {
+    double eps = 0, div = 0;
+    for (int i = first + 1; i <= last; ++i) {
+      const double x = v[(size_t)i].start_target - v[(size_t)(i - 1)].start_target;
+      const double y = v[(size_t)i].start_query - v[(size_t)(i - 1)].start_query;
+      double s = y / x;
+      s = atan(s);
+      eps += s;
+      div += 1.;
+    }
+    eps /= div;
+    sy = sin(eps);
+    sx = cos(eps);
+  };
+  int last = 0, count = 0;
+  // a run that is still open when the list ends is not extended
+  for (int i = 1; i < n; ++i) {
+    const Match& a = v[(size_t)i];
+    const Match& b = v[(size_t)(i - 1)];
+    const int x1 = a.start_target, y1 = a.start_query, x2 = b.start_target, y2 = b.start_query;
+    if (a.target == b.target && a.query == b.query && a.rc == b.rc && x1 - x2 < kSpace / 2 && y1 - y2 < kSpace / 2 &&
+        y1 > y2) {
+      if (x1 - x2 > b.length && y1 - y2 > b.length) ++count;
+      continue;
+    }
+    if (count >= kMinRun) {
+      double sx = 0., sy = 0;
+      slope(last, i - 1, sx, sy);
+      const Match head = v[(size_t)last];
+      const Match tail = v[(size_t)(i - 1)];
+      auto add = [&](const Match& at, int x, int y) {
+        Match d;
+        d.prob = 0.999;
+        d.ident = 0.999;
+        d.query = at.query;
+        d.target = at.target;
+        d.query_len = chained->query_size[(size_t)at.query];
+        d.start_query = y;
+        d.start_target = x;
+        d.length = 50 + i - last;
+        d.rc = at.rc;
+        v.push_back(d);
+      };
+      for (int j = 1; j <= kAdded; ++j)
+        add(head, head.start_target - (int)((double)kSpace * (double)j * sx),
+            head.start_query - (int)((double)kSpace * (double)j * sy));
+      for (int j = 1; j <= kAdded; ++j)
+        add(tail, tail.start_target + (int)((double)kSpace * (double)j * sx),
+            tail.start_query + (int)((double)kSpace * (double)j * sy));
+    }
+    last = i;
+    count = 0;
+  }
+}
+
+// ---- coverage (analysis/SatsumaSynteny2.cc:151-214)
+
+void CoverageTracker::update(const MatchSet& s, uint64_t* added, uint64_t* removed) {
+  constexpr int64_t kRecordStride = 10000000000L;  // POSITION_CHR_CNST (:16)
+  std::vector<Seg> cur(s.m.size());
+  for (size_t i = 0; i < s.m.size(); ++i) {
+    cur[i].start = (uint64_t)(s.m[i].target * kRecordStride + s.m[i].start_target);
+    cur[i].end = cur[i].start + (uint64_t)(int64_t)s.m[i].length;
+  }
+  std::sort(cur.begin(), cur.end(), [](const Seg& a, const Seg& b) { return a.start < b.start; });
+  // a sweep over the boundaries of both lists; each step measures the span from the last
+  // boundary to the next by which of the two current segments contain it
+  const std::vector<Seg>& old = prev_;
+  size_t i = 0, j = 0;
+  uint64_t lastp = 0, next = 0;
+  uint64_t cover[4] = {0, 0, 0, 0};
+  while (i < old.size() || j < cur.size()) {
+    if (i < old.size()) next = old[i].end;
+    if (j < cur.size() && cur[j].end > next) next = cur[j].end;
+    if (i < old.size()) {
+      if (old[i].start < next && old[i].start > lastp)
+        next = old[i].start;
+      else if (old[i].end < next && old[i].end > lastp)
+        next = old[i].end;
+    }
+    if (j < cur.size()) {
+      if (cur[j].start < next && cur[j].start > lastp)
+        next = cur[j].start;
+      else if (cur[j].end < next && cur[j].end > lastp)
+        next = cur[j].end;
+    }
+    int who = 0;
+    if (i < old.size() && lastp >= old[i].start && next <= old[i].end) who += 1;
+    if (j < cur.size() && lastp >= cur[j].start && next <= cur[j].end) who += 2;
+    cover[who] += next - lastp;
+    lastp = next;
+    while (i < old.size() && lastp >= old[i].end) ++i;
+    while (j < cur.size() && lastp >= cur[j].end) ++j;
+  }
+  prev_.swap(cur);
+  *added = cover[2];
+  *removed = cover[1];
+}
+
+// ---- grid search (analysis/GridSearch.cc)
+
+Grid::Grid(int size, int blocks, const std::vector<Record>& target, const std::vector<Record>& query)
+    : size_(size), blocks_(blocks) {
+  // ChunkManager::ChunkIt (analysis/SeqChunk.cc:72-165): chunks every size - overlap bases
+  // of records of at least 6 bases; a target chunk of N/X only is remembered as such.
+  auto chunk = [](const std::vector<Record>& recs, int sz, int overlap, std::vector<int>* rec, std::vector<int>* st,
+                  std::vector<bool>* empty) {
+    const int step = sz - overlap;
+    for (size_t i = 0; i < recs.size(); ++i) {
+      const int l = (int)recs[i].len;
+      if (l < 6) continue;
+      for (int j = 0; j < 1 + l / step; ++j) {
+        const int start = j * step;
+        int end = (j + 1) * step + overlap;
+        if (end >= l) end = l;
+        rec->push_back((int)i);
+        st->push_back(start);
+        if (empty) {
+          int ns = 0;
+          for (int k = start; k < end; ++k) ns += (recs[i].seq[k] == 'N' || recs[i].seq[k] == 'X');
+          empty->push_back(ns >= end - start);
+        }
+      }
+    }
+  };
+  chunk(target, size, size / 4, &t_chunk_rec_, &t_chunk_start_, &all_n_);
+  chunk(query, size, 0, &q_chunk_rec_, &q_chunk_start_, nullptr);
+  // each record's first chunk is the first chunk carrying its name (GridSearch::SetUp,
+  // :77-107): records sharing a name share their first chunk
+  auto firsts = [](const std::vector<Record>& recs, const std::vector<int>& rec, std::vector<int>* out) {
+    out->assign(recs.size(), -1);
+    for (size_t i = 0; i < recs.size(); ++i)
+      for (size_t c = 0; c < rec.size(); ++c)
+        if (recs[(size_t)rec[c]].name == recs[i].name) {
+          (*out)[i] = (int)c;
+          break;
+        }
+  };
+  firsts(target, t_chunk_rec_, &t_first_);
+  firsts(query, q_chunk_rec_, &q_first_);
+  // SearchMatrix::Setup (analysis/GridSearch.h:99-128)
+  mx_ = target_chunks() / blocks + 1;
+  my_ = query_chunks() / blocks + 1;
+  state_.assign((size_t)mx_ * my_, UNKNOWN);
+  count_.assign((size_t)mx_ * my_, 0);
+  vertical_.assign((size_t)query_chunks(), 0);
+  horizontal_.assign((size_t)target_chunks(), 0);
+}
+
+// ConsiderTargets (:123-186) with CoordsToBlocks (:410-458).  The reference's repeat
+// tracker only records here; its answer is never used, so it is left out.
+void Grid::consider(int tid, int st, int et, int qid, int sq, int eq, double ident) {
+  // a record too short for chunks has no block to weight (the reference indexes before
+  // its arrays there)
+  if (t_first_[(size_t)tid] < 0 || q_first_[(size_t)qid] < 0) return;
+  const int tmid = (st + et) / 2, qmid = (sq + eq) / 2;
+  const int qblock = q_first_[(size_t)qid] + qmid / size_;
+  const int tstep = size_ - size_ / 4;  // target chunks overlap by a quarter
+  const int off = tmid / tstep;
+  int tblock = t_first_[(size_t)tid] + off;
+  if (off > 0 && tmid - tstep * off < size_ / 4) --tblock;  // the lower of two overlapping chunks
+  const int x = tblock / blocks_, y = qblock / blocks_;
+  vertical_[(size_t)qblock] += et - st;
+  horizontal_[(size_t)tblock] += et - st;
+  const int plus = state_[(size_t)index(x, y)] != UNKNOWN ? 1 : 0;
+  for (int i = x - plus; i <= x + plus; ++i)
+    for (int j = y - plus; j <= y + plus; ++j) {
+      if (i < 0 || j < 0 || i >= mx_ || j >= my_) continue;
+      const int w = (int)(100. * ident * (double)(et - st));
+      int& c = count_[(size_t)index(i, j)];
+      if (w > c) c = w;
+    }
+}
+
+void Grid::update(const MatchSet& chained) {
+  // ClearTargetWeights (GridSearch.h:376-385): cell weights and query occupancy cleared,
+  // target occupancy kept as a mark
+  std::fill(count_.begin(), count_.end(), 0);
+  std::fill(vertical_.begin(), vertical_.end(), 0);
+  for (int& h : horizontal_)
+    if (h > 0) h = 1;
+  for (const Match& m : chained.m) {
+    const int x1 = m.start_target, x2 = m.start_target + m.length;
+    int y1 = m.start_query, y2 = m.start_query + m.length;
+    if (m.rc) {
+      const int s = chained.query_size[(size_t)m.query];
+      const int t = y1;
+      y1 = s - y2;
+      y2 = s - t;
+    }
+    if (x1 < 0 || x2 < 0 || y1 < 0 || y2 < 0) continue;
+    if (x2 >= chained.target_size[(size_t)m.target] || y1 >= chained.query_size[(size_t)m.query] ||
+        y2 >= chained.query_size[(size_t)m.query])
+      continue;
+    consider(m.target, x1, x2, m.query, y1, y2, m.ident);
+  }
+}
+
+// GridTarget ordering (analysis/GridSearch.h:258-273): heavier first, then by cell.
+static bool target_less(const GridTarget& a, const GridTarget& b) {
+  if (a.count != b.count) return -a.count < -b.count;
+  if (a.x != b.x) return a.x < b.x;
+  return a.y < b.y;
+}
+
+int Grid::collect(std::vector<GridTarget>* out, int n) {
+  for (int i = 0; i < mx_; ++i)
+    for (int j = 0; j < my_; ++j) {
+      const size_t k = (size_t)index(i, j);
+      if (state_[k] != UNKNOWN || count_[k] == 0) continue;
+      const int tl = std::min((i + 1) * blocks_ - 1, target_chunks() - 1);
+      const int ql = std::min((j + 1) * blocks_ - 1, query_chunks() - 1);
+      out->push_back(GridTarget{i * blocks_, tl, j * blocks_, ql, i, j, -1, count_[k], false});
+    }
+  std::sort(out->begin(), out->end(), target_less);
+  const int candidates = (int)out->size();
+  if (n < (int)out->size()) out->resize((size_t)n);
+  for (const GridTarget& t : *out) state_[(size_t)index(t.x, t.y)] = SUBMITTED;
+  if ((int)out->size() < n) seeds(out, n - (int)out->size());
+  return candidates;
+}
+
+// CollectSeeds (:189-315): lines through untouched target columns.  Every run of >= 5
+// unused target chunks (none of N only) gives its middle chunk; down the query axis, every
+// stretch that starts on an unused query chunk and runs until 16 used chunks in a row
+// (more than 2 chunks long) becomes one line, weighted by the run's length.  The heaviest
+// lines are taken, the cut extended to keep every line of the last target chunk taken.
+void Grid::seeds(std::vector<GridTarget>* out, int n) {
+  if (n == 0) return;
+  constexpr int kSkipMax = 16;
+  std::vector<GridTarget> lines;
+  const int nh = (int)horizontal_.size(), nv = (int)vertical_.size();
+  for (int i = 0; i < nh; ++i) {
+    if (horizontal_[(size_t)i] > 0 || all_n_[(size_t)i]) continue;
+    int j = i;
+    while (j < nh && horizontal_[(size_t)j] <= 0 && !all_n_[(size_t)j]) ++j;
+    const int len = j - i - 1;
+    if (len < 4) continue;
+    const int x = i + len / 2;
+    i = j;
+    for (int y = 0; y < nv; ++y) {
+      if (vertical_[(size_t)y] > 0) continue;
+      int skip = 0, k = y;
+      for (; k < nv; ++k) {
+        if (vertical_[(size_t)k] > 0) {
+          if (++skip >= kSkipMax) break;
+        } else {
+          skip = 0;
+        }
+      }
+      const int y1 = y, y2 = k - 1;
+      if (y2 - y1 < 2) continue;
+      lines.push_back(GridTarget{x, x, y1, y2, x, y1, y2, len, true});
+      y = k;
+    }
+  }
+  std::sort(lines.begin(), lines.end(), target_less);
+  if (n < (int)lines.size()) {
+    const int lastx = lines[(size_t)(n - 1)].target_first;
+    size_t i = (size_t)n;
+    while (i < lines.size() && lines[i].target_first == lastx) ++i;
+    lines.resize(i);
+  }
+  for (const GridTarget& t : lines) {
+    for (int j = t.target_first; j <= t.target_last; ++j) horizontal_[(size_t)j] = 1;
+    out->push_back(t);
+  }
+}
+
+}  // namespace sxc_master

@@ -20,6 +20,9 @@ Fixtures:
   prob_table.npz     ProbTable thresholds (digest + rows) and the slave path with -prob_table
   bdhv_pairs.npz     chunks dense in B/D/H/V (entropy windows of thirds, glibc log) at N = 2048 ..
                      32768: signals, xc, FindTop lags and MatchUp runs
+  master_cases.npz   the master's match-set steps (Sort, Collapse, LengthFilter, RunMatchDynProg,
+                     RunMatchDynProgMult, SyntenyInterpolator) on random match sets, through
+                     oracle/_ref/ref_master
 
   python tests/golden/make_golden.py bdhv   (one generator only)
 """
@@ -418,6 +421,82 @@ def gen_probtable(tmp):
                         rows=t[PT_ROWS], row_idx=np.array(PT_ROWS, np.int32), **out)
 
 
+MASTER = os.path.join(ROOT, "oracle", "_ref", "ref_master")
+M56 = np.dtype([("target_id", "<i4"), ("query_id", "<i4"), ("query_len", "<i4"), ("start_target", "<i4"),
+                ("start_query", "<i4"), ("length", "<i4"), ("rc", "<i4"), ("reserved", "<i4"),
+                ("matches", "<f8"), ("prob", "<f8"), ("ident", "<f8")])
+
+
+def master_set(seed, nt=2, nq=3, runs=8, noise=150):
+    """Collinear runs (two long enough to interpolate), near copies and noise."""
+    rng = np.random.default_rng(seed)
+    ts = rng.integers(300_000, 900_000, nt).astype(np.int32)
+    qs = rng.integers(300_000, 900_000, nq).astype(np.int32)
+    rows = []
+    for k in range(runs):
+        t, q, rc = int(rng.integers(nt)), int(rng.integers(nq)), int(rng.integers(2))
+        n = 60 if k < 2 else int(rng.integers(5, 30))
+        step = int(rng.integers(400, 3000))
+        st = int(rng.integers(0, max(1, ts[t] - n * step - 500)))
+        sq = int(rng.integers(0, max(1, qs[q] - n * step - 500)))
+        for i in range(n):
+            ln = int(rng.integers(50, 250))
+            rows.append((t, q, qs[q], st + i * step, sq + i * step, ln, rc))
+            if rng.random() < 0.2:
+                rows.append((t, q, qs[q], st + i * step + 2, sq + i * step + 1, ln + 10, rc))
+    for _ in range(noise):
+        t, q = int(rng.integers(nt)), int(rng.integers(nq))
+        ln = int(rng.integers(40, 300))
+        rows.append((t, q, qs[q], int(rng.integers(0, ts[t] - ln)), int(rng.integers(0, qs[q] - ln)), ln,
+                     int(rng.integers(2))))
+    m = np.zeros(len(rows), M56)
+    for i, r in enumerate(rows):
+        (m[i]["target_id"], m[i]["query_id"], m[i]["query_len"], m[i]["start_target"], m[i]["start_query"],
+         m[i]["length"], m[i]["rc"]) = r
+    m["prob"] = rng.uniform(0.99, 1.0, len(m))
+    m["ident"] = rng.uniform(0.6, 1.0, len(m))
+    m["matches"] = m["ident"] * m["length"]
+    rng.shuffle(m)
+    return ts, qs, m
+
+
+def master_run(tmp, op, ts, qs, m, *extra):
+    inp, out = os.path.join(tmp, "m.in"), os.path.join(tmp, "m.out")
+    with open(inp, "wb") as f:
+        f.write(np.array([len(ts), len(qs)], np.int32).tobytes() + ts.tobytes() + qs.tobytes()
+                + np.array([len(m)], np.int64).tobytes() + m.tobytes())
+    args = [MASTER, op, inp, *[str(e) for e in extra], out]
+    subprocess.run(args, check=True, stdout=subprocess.DEVNULL)
+    b = open(out, "rb").read()
+    nt, nq = np.frombuffer(b, np.int32, 2)
+    off = 8 + 4 * int(nt + nq)
+    n = int(np.frombuffer(b, np.int64, 1, off)[0])
+    return np.frombuffer(b, M56, n, off + 8).copy()
+
+
+def gen_master(tmp):
+    out, cases = {}, []
+
+    def case(name, op, ts, qs, m, res, *extra):
+        cases.append(name)
+        out.update({f"{name}_op": np.array(op), f"{name}_ts": ts, f"{name}_qs": qs, f"{name}_in": m,
+                    f"{name}_out": res})
+        if extra:
+            out[f"{name}_arg"] = np.array(extra)
+
+    for seed in (101, 102):
+        ts, qs, m = master_set(seed)
+        srt = master_run(tmp, "sort", ts, qs, m)
+        case(f"sort{seed}", "sort", ts, qs, m, srt)
+        col = master_run(tmp, "collapse", ts, qs, srt)
+        case(f"collapse{seed}", "collapse", ts, qs, srt, col)
+        for op in ("chain", "chainmult"):
+            case(f"{op}{seed}", op, ts, qs, col, master_run(tmp, op, ts, qs, col))
+        ch = master_run(tmp, "chain", ts, qs, col)
+        case(f"interpolate{seed}", "interpolate", ts, qs, ch, master_run(tmp, "interpolate", ts, qs, ch))
+    np.savez_compressed(os.path.join(HERE, "master_cases.npz"), cases=np.array(cases), **out)
+
+
 def main():
     subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "ref", f"REF={REF}"], check=True)
     only = sys.argv[1:]  # generator names (e.g. bdhv); none: all
@@ -435,6 +514,7 @@ def main():
         gen_cli(tmp)
         gen_probtable(tmp)
         gen_bdhv(tmp)
+        gen_master(tmp)
     for f in sorted(os.listdir(HERE)):
         if f.endswith(".npz"):
             print(f, os.path.getsize(os.path.join(HERE, f)))
