@@ -26,7 +26,7 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall", "-Wno-unus
 DEVICE = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 SOURCES = ["sxc_kernels.hip", "sxc_kernels_r2.hip", "sxc_kernels_s32.hip", "sxc_kernels_occ.hip", "sxc_kernels_lean.hip",
            "sxc_api.hip",
-           "sxc_server.hip", "sxc_static.hip", "sxc_tables.cpp",
+           "sxc_server.hip", "sxc_static.hip", "sxc_tables.cpp", "sxc_kmatch.hip",
            # the master's side of the search (host C++): match sets, chaining, grid search
            "sxc_master_api.cpp", "host/sxc_master.cpp", "host/sxc_seqio.cpp"]
 # Host programs linked against the library (plain C++, no device code).
@@ -35,6 +35,8 @@ PROGRAMS = {
     "HomologyByXCorr": ["host/xcorr_main.cpp", "host/sxc_seqio.cpp"],
     # a caller of the reference-compatible C++ classes (include/satsuma_compat/)
     "compat_align": ["host/compat_align.cpp"],
+    # kmatch/KMatch.cc's program with its k-mer tables on the GPU
+    "KMatch": ["host/kmatch_main.cpp"],
 }
 CXX = os.environ.get("CXX", "g++")
 

@@ -1,27 +1,27 @@
-"""Request streams for the serving loop, shaped like Satsuma2's battleship search.
+"""Request streams for the serving loop, shaped like Satsuma2's search.
 
-The master's grid search (analysis/GridSearch.cc:188-411, driven from
-analysis/SatsumaSynteny2.cc:484-599) asks for two kinds of t_pair blocks:
-  * seed lines (CollectSeeds): one target chunk against a run of query chunks, with the
-    fast cutoff;
-  * neighbourhoods (CollectTargets): the cells of its search matrix next to earlier hits,
-    blocks of target x query chunks with the normal cutoff.
-`Battleship` produces both from a pair of chunked genomes: seeds at random until hits come
-back, then the 24 x 24-chunk matrix cell of every hit not asked for yet.
+`GridSearchProducer` is the master's search (analysis/SatsumaSynteny2.cc:484-599) over two
+genomes: the restated GridSearch, match-set steps and chaining of satsuma2_amd.master
+(include/sxc_master.h) run on a thread of their own, as the reference's master does beside
+its slaves; each collected GridTarget (a block of target x query chunks, seed lines with
+the fast cutoff) is one request, and the records of completed requests feed the next
+iteration.  `Battleship` is a simple stand-in with the same two request shapes (seed lines
+at random, then the 24 x 24-chunk cell of every hit), kept for tests that need a request
+stream independent of the search's state.
 `run_stream` offers requests as a Poisson stream (a rate of requests, or of chunk pairs with
 gaps scaled by each request's pairs) to a `Server` and reports per-request latency
 percentiles and the chunk pairs per second completed.  Latency is counted from each
 request's scheduled arrival (completion minus the arrival time the Poisson schedule gave
-it), so a generator that falls behind an overloaded server does not hide the queueing
-(coordinated omission); the server's own submission-to-completion time is reported beside
-it as the service latency.
+it), so a generator that falls behind an overloaded server, or waits for the master's next
+targets, does not hide the queueing (coordinated omission); the server's own
+submission-to-completion time is reported beside it as the service latency.
 """
 from __future__ import annotations
 
 import collections
 import threading
 import time
-from typing import List, Sequence, Tuple
+from typing import List, Optional, Sequence, Tuple
 
 import numpy as np
 
@@ -74,6 +74,101 @@ class Battleship:
             if self.queue:
                 return self.queue.popleft()
         return self.seed()
+
+
+class GridSearchProducer:
+    """The master's search loop as a request source.  A background thread runs
+    SearchIteration (grid update from the chained matches, CollectTargets) whenever results
+    arrived or fewer than `queue` targets are pending, asking for the difference, as the
+    master tops up its work queue (analysis/SatsumaSynteny2.cc:544-555).  When the grid has
+    nothing left, the search starts over on the same genomes (`restarts`), so a stream can
+    run for any duration.  `seeds` (RESULT_DTYPE, e.g. KMatch's) start the match set as seed
+    files do (analysis/SatsumaSynteny2.cc:466-477)."""
+
+    def __init__(self, target: bytes, query: bytes, chunk: int = 4096, blocks: int = 24, queue: int = 128,
+                 seeds: Optional[np.ndarray] = None, min_seed_length: int = 24, dups: bool = False):
+        self.T, self.Q, self.chunk, self.blocks, self.queue = target, query, chunk, blocks, queue
+        self.seeds, self.min_seed_length, self.dups = seeds, min_seed_length, dups
+        self.pending: collections.deque = collections.deque()
+        self.inbox: List[np.ndarray] = []
+        self.cv = threading.Condition()
+        self.closed = False
+        self.iterations = 0
+        self.iteration_ms: List[float] = []
+        self.restarts = 0
+        self.targets = 0
+        self.seed_lines = 0
+        self._start()
+        self.thread = threading.Thread(target=self._loop, daemon=True)
+        self.thread.start()
+
+    def _start(self):
+        from .master import Grid, SearchIteration
+        self.grid = Grid.from_records(self.chunk, self.blocks, [("target", self.T)], [("query", self.Q)])
+        self.it = SearchIteration(self.grid, [len(self.T)], [len(self.Q)], dups=self.dups)
+        if self.seeds is not None and len(self.seeds):
+            self.it.seed(self.seeds, self.min_seed_length)
+
+    def _loop(self):
+        while True:
+            with self.cv:
+                while not self.closed and not self.inbox and len(self.pending) >= self.queue:
+                    self.cv.wait()
+                if self.closed:
+                    return
+                res = np.concatenate(self.inbox) if self.inbox else np.zeros(0, _lib_result_dtype())
+                self.inbox = []
+                want = max(0, self.queue - len(self.pending))
+            t0 = time.perf_counter()
+            targets, _, _ = self.it.step(res, want)
+            if len(targets) == 0 and not len(res):
+                with self.cv:
+                    idle = not self.pending
+                if idle:
+                    self._start()
+                    self.restarts += 1
+                    targets, _, _ = self.it.step(np.zeros(0, _lib_result_dtype()), want)
+            self.iteration_ms.append(1e3 * (time.perf_counter() - t0))
+            self.iterations += 1
+            with self.cv:
+                for t in targets.tolist():
+                    tf, tl, qf, ql, _x, _y, _y2, _count, fast = t
+                    self.pending.append([(tf, tl, qf, ql, fast)])
+                    self.seed_lines += fast
+                self.targets += len(targets)
+                self.cv.notify_all()
+                if not len(targets) and not self.inbox:
+                    # nothing new until results come back
+                    self.cv.wait(timeout=0.05)
+
+    def next(self) -> List[Block]:
+        with self.cv:
+            while not self.pending and not self.closed:
+                self.cv.notify_all()
+                self.cv.wait(timeout=0.1)
+            return self.pending.popleft() if self.pending else []
+
+    def on_results(self, recs: np.ndarray):
+        with self.cv:
+            self.inbox.append(np.asarray(recs))
+            self.cv.notify_all()
+
+    def close(self):
+        with self.cv:
+            self.closed = True
+            self.cv.notify_all()
+        self.thread.join()
+
+    def stats(self) -> dict:
+        ms = np.array(self.iteration_ms) if self.iteration_ms else np.zeros(1)
+        return {"iterations": self.iterations, "targets": self.targets, "seed_lines": self.seed_lines,
+                "restarts": self.restarts, "iteration_ms_p50": round(float(np.percentile(ms, 50)), 2),
+                "iteration_ms_max": round(float(ms.max()), 2)}
+
+
+def _lib_result_dtype():
+    from ._lib import RESULT_DTYPE
+    return RESULT_DTYPE
 
 
 def _pairs(blocks: Sequence[Block]) -> int:

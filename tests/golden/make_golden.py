@@ -20,6 +20,9 @@ Fixtures:
   prob_table.npz     ProbTable thresholds (digest + rows) and the slave path with -prob_table
   bdhv_pairs.npz     chunks dense in B/D/H/V (entropy windows of thirds, glibc log) at N = 2048 ..
                      32768: signals, xc, FindTop lags and MatchUp runs
+  kmatch_cases.npz   KMatch (kmatch/KMatch.cc, oracle/_ref/KMatch) on a FASTA pair with repeats,
+                     soft-masked and N stretches: its t_result files at several K, lengths,
+                     jumps and max_freq
   master_cases.npz   the master's match-set steps (Sort, Collapse, LengthFilter, RunMatchDynProg,
                      RunMatchDynProgMult, SyntenyInterpolator) on random match sets, through
                      oracle/_ref/ref_master
@@ -497,6 +500,24 @@ def gen_master(tmp):
     np.savez_compressed(os.path.join(HERE, "master_cases.npz"), cases=np.array(cases), **out)
 
 
+def gen_kmatch(tmp):
+    sys.path.insert(0, os.path.dirname(HERE))
+    from kmatch_cases import fasta_text, genome_pair
+    t, q = genome_pair(4, n_records=2, size=30_000)
+    tf, qf, of = os.path.join(tmp, "t.fa"), os.path.join(tmp, "q.fa"), os.path.join(tmp, "o.bin")
+    tt, qt = fasta_text(t, 60, False, "t"), fasta_text(q, 70, True, "q")
+    open(tf, "wb").write(tt)
+    open(qf, "wb").write(qt)
+    args = [(11, 11, 10, 1), (21, 21, 20, 1), (31, 31, 30, 1), (13, 40, 5, 1), (15, 30, 14, 3)]
+    outs = {}
+    for i, (K, ml, jump, freq) in enumerate(args):
+        subprocess.run([os.path.join(ROOT, "oracle", "_ref", "KMatch"), qf, tf, str(K), of, str(ml), str(jump),
+                        str(freq)], check=True, stdout=subprocess.DEVNULL)
+        outs[f"out{i}"] = np.frombuffer(open(of, "rb").read(), np.uint8)
+    np.savez_compressed(os.path.join(HERE, "kmatch_cases.npz"), query_fasta=np.frombuffer(qt, np.uint8),
+                        target_fasta=np.frombuffer(tt, np.uint8), args=np.array(args, np.int32), **outs)
+
+
 def main():
     subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "ref", f"REF={REF}"], check=True)
     only = sys.argv[1:]  # generator names (e.g. bdhv); none: all
@@ -515,6 +536,7 @@ def main():
         gen_probtable(tmp)
         gen_bdhv(tmp)
         gen_master(tmp)
+        gen_kmatch(tmp)
     for f in sorted(os.listdir(HERE)):
         if f.endswith(".npz"):
             print(f, os.path.getsize(os.path.join(HERE, f)))
