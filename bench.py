@@ -12,21 +12,27 @@ replayed from the HIP graph the library captures on the second run.  The per-ker
 
 Serving (config of the serving loop): two synthetic 50-Mbp genomes sharing 2,000 syntenic
 blocks, chunked as the slave chunks them (16,277 target chunks of 4096 bp every 3072 bp,
-12,208 query chunks), target spectra resident in HBM; battleship-shaped requests (seed
-lines of one target chunk x 256 query chunks, then the 24 x 24 matrix cells of hits) arrive
-as streams offering 25%, 50%, 75%, 100%, 150% and 200% of the batched throughput in chunk
-pairs per second (exponential gaps scaled by each request's pairs); each rate reports request
-latency p50/p99 counted from the request's scheduled arrival, the server's own service
-latency, and chunk pairs per second; the highest completed rate is the saturation point.
-The headline p50/p99 is at 50%.
+12,208 query chunks), target spectra resident in HBM.  The requests are SatsumaSynteny2's
+own: KMatch seeds on the GPU (K = 11 .. 31), then the master's search loop (the restated
+grid search, match-set steps and chaining of satsuma2_amd.master) with the server as its
+slaves, in synchronous rounds for a time budget (`search`: rounds, chunk pairs per second
+through the whole loop, master and alignment seconds).  The requests it made (24 x 24-chunk
+cells and seed lines) are then replayed as streams offering 25%, 50%, 75%, 100%, 150% and
+200% of the batched throughput in chunk pairs per second (exponential gaps scaled by each
+request's pairs); each rate reports request latency p50/p99 counted from the request's
+scheduled arrival, the server's own service latency, and chunk pairs per second; the highest
+completed rate is the saturation point.  The headline p50/p99 is at 50%.
 
 Uneven (config of the uneven batch): 20,000 pairs with target and query lengths log-uniform
 in [256, 16384] bp, each at its own FFT length (1024 ... 32768), in one resident batch.
 
 Large served run (config of the large served run; --no-large skips it): two 200-Mbp
 genomes with GC drawn per 100-kbp window, tandem repeats over 10% of each, five
-transposon-like families of 1,000 copies and 5,000 syntenic blocks, served at 25%, 50% and
-100% of the batched rate.
+transposon-like families of 1,000 copies and 5,000 syntenic blocks: the same seeds and search
+loop, its requests served at 25%, 50% and 100% of the batched rate.
+
+KMatch: the GPU KMatch program beside the reference's (compiled from kmatch/KMatch.cc) on
+two 10-Mbp genomes at K = 21, same files in and out.
 
 Printed: one JSON line with the batched throughput (`value`), the plain one-pair-per-call
 throughput and the one-shot (host buffers in and out, PCIe included) batch rate beside it,
@@ -65,7 +71,6 @@ TRAFFIC_PER_PAIR = None
 # offered chunk pairs/s as fractions of the batched rate (0.5: the headline); the serving loop
 # keeps query spectra resident, so its saturation lies above the batched rate
 SERVE_LOADS = (0.25, 0.5, 0.75, 1.0, 1.5, 2.0)
-SEED_LINE = 256                  # query chunks per seed line
 LARGE_LOADS = (0.25, 0.5, 1.0)   # offered loads of the large served run
 
 
@@ -162,10 +167,16 @@ def cpu_baseline(T, Q, sample: int, threads: int):
             "sample": f"{sample} pairs of the same workload, both strands, oracle restatement"}
 
 
-def serve_bench(ctx, genome_bp: int, duration: float, capacity: float, loads, seed: int = 4, large: bool = False):
+def serve_bench(ctx, genome_bp: int, duration: float, capacity: float, loads, seed: int = 4, large: bool = False,
+                search_s: float = 15.0):
+    """The served path under SatsumaSynteny2's own request stream: KMatch seeds (GPU,
+    K = 11 .. 31), then the master's search loop (restated grid search and chaining) with the
+    server as its slaves, in synchronous rounds under a time budget; the requests it made are
+    then replayed as a Poisson stream at each offered load for latency percentiles."""
     from satsuma2_amd import synth
     from satsuma2_amd.api import Server, SlaveParams
-    from satsuma2_amd.serve_load import Battleship, genome_chunks, run_stream
+    from satsuma2_amd.kmatch import satsuma_seeds
+    from satsuma2_amd.serve_load import Replay, genome_chunks, run_search, run_stream
 
     print(f"[bench] serving: generating two {genome_bp / 1e6:.0f}-Mbp genomes", file=sys.stderr, flush=True)
     t0 = time.perf_counter()
@@ -177,16 +188,20 @@ def serve_bench(ctx, genome_bp: int, duration: float, capacity: float, loads, se
     gen_s = time.perf_counter() - t0
     tch = genome_chunks(T, 4096, 3072)
     qch = genome_chunks(Q, 4096, 4096)
+    print("[bench] serving: KMatch seeds, K = 11 .. 31", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    seeds, kst = satsuma_seeds(ctx, [Q], [T])
+    seeds_s = time.perf_counter() - t0
     t0 = time.perf_counter()
     srv = Server(ctx, SlaveParams(target_total=float(len(T))), tch, qch)
     create_s = time.perf_counter() - t0
-    info = srv.stats()
+    print(f"[bench] serving: search loop ({search_s:.0f} s budget)", file=sys.stderr, flush=True)
+    reqs, search = run_search(srv, T, Q, seeds, budget_s=search_s)
     runs = []
     for k, load in enumerate(loads):
         print(f"[bench] serving: {load * capacity:.0f} chunk pairs/s offered ({load:.0%} of the batched rate) "
               f"for {duration:.0f} s", file=sys.stderr, flush=True)
-        prod = Battleship(len(tch), len(qch), np.random.default_rng(seed + k), line=SEED_LINE)
-        r = run_stream(srv, prod, 0.0, duration, seed=seed + k, pairs_rate=load * capacity)
+        r = run_stream(srv, Replay(reqs), 0.0, duration, seed=seed + k, pairs_rate=load * capacity)
         r["offered_load"] = load
         runs.append(r)
     info = srv.stats()
@@ -195,18 +210,60 @@ def serve_bench(ctx, genome_bp: int, duration: float, capacity: float, loads, se
     sat = max(runs, key=lambda r: r["pairs_per_s"])
     kind = ("GC 0.38-0.45 per 100-kbp window, 10% tandem repeats, 5 transposon families x 1000 copies, "
             "5,000 syntenic blocks" if large else "2,000 syntenic blocks")
+    npairs = [sum((b[1] - b[0] + 1) * (b[3] - b[2] + 1) for b in rq) for rq in reqs]
     return {"workload": f"two {genome_bp / 1e6:.0f}-Mbp synthetic genomes ({kind}), {len(tch)} target x "
-                        f"{len(qch)} query chunks, battleship requests (seed lines of {SEED_LINE} pairs, 24x24 "
-                        f"cells of hits)",
+                        f"{len(qch)} query chunks; requests: the targets SatsumaSynteny2's search loop collects "
+                        "(24x24-chunk cells, seed lines) from KMatch seeds, replayed as a Poisson stream",
             "generate_s": round(gen_s, 1),
             "p50_ms": head["p50_ms"], "p99_ms": head["p99_ms"], "at_load": head["offered_load"],
             "latency": "from each request's scheduled (Poisson) arrival to its completion; service_*: from "
                        "submission, as the server measures it",
             "saturation_pairs_per_s": sat["pairs_per_s"], "saturation_at_load": sat["offered_load"],
             "saturation_p99_ms": sat["p99_ms"],
+            "requests": {"recorded": len(reqs), "pairs_p50": int(np.percentile(npairs, 50)) if npairs else 0,
+                         "pairs_max": int(max(npairs)) if npairs else 0,
+                         "seed_lines": int(sum(b[4] for rq in reqs for b in rq))},
+            "kmatch": {"seconds": round(seeds_s, 2), "records": int(len(seeds)),
+                       "device_ms": round(kst["device_ms"], 1), "sweep_ms": round(kst["sweep_ms"], 1)},
+            "search": search,
             "rates": runs, "create_s": round(create_s, 2), "resident_gb": round(info["resident_bytes"] / 1e9, 3),
             "slots": info["slots"], "max_pairs_per_launch": info["max_pairs_per_launch"],
             "launches": info["launches"]}
+
+
+def kmatch_bench(ctx, genome_bp: int = 10_000_000, K: int = 21, seed: int = 6):
+    """KMatch on one genome pair at SatsumaSynteny2's arguments for one K: the GPU program
+    (satsuma2_amd/lib/KMatch) beside the reference program compiled from kmatch/KMatch.cc
+    (oracle/_ref/KMatch, its two loader threads), same FASTA files, same output."""
+    import subprocess
+    import tempfile
+
+    from satsuma2_amd import synth
+
+    ref = os.path.join(ROOT, "oracle", "_ref", "KMatch")
+    mine = os.path.join(ROOT, "satsuma2_amd", "lib", "KMatch")
+    T, Q, _ = synth.syntenic_genomes(length=genome_bp, n_blocks=max(1, genome_bp // 25_000), seed=seed)
+    out = {"workload": f"two {genome_bp / 1e6:.0f}-Mbp synthetic genomes, K = {K}, min_length {K}, jump {K - 1}, "
+                       "max_freq 1 (FASTA files in, t_result file out)"}
+    with tempfile.TemporaryDirectory() as d:
+        tf, qf = os.path.join(d, "t.fa"), os.path.join(d, "q.fa")
+        for path, seq, name in ((tf, T, b">t\n"), (qf, Q, b">q\n")):
+            with open(path, "wb") as f:
+                f.write(name)
+                for i in range(0, len(seq), 80):
+                    f.write(seq[i:i + 80] + b"\n")
+        for key, prog in (("gpu_s", mine), ("reference_cpu_s", ref)):
+            if not os.path.exists(prog):
+                continue
+            o = os.path.join(d, key + ".bin")
+            t0 = time.perf_counter()
+            subprocess.run([prog, qf, tf, str(K), o, str(K), str(K - 1), "1"], check=True, stdout=subprocess.DEVNULL)
+            out[key] = round(time.perf_counter() - t0, 3)
+            out[key.replace("_s", "_records")] = os.path.getsize(o) // 72
+    if "gpu_s" in out and "reference_cpu_s" in out:
+        out["speedup"] = round(out["reference_cpu_s"] / out["gpu_s"], 1)
+        out["reference_cores"] = 2
+    return out
 
 
 def uneven_bench(ctx, n_pairs: int, steps: int, seed: int = 3):
@@ -397,7 +454,11 @@ def main():
         out["serve_p99_ms"] = sv["p99_ms"]
         out["serve"] = sv
     if not args.no_large:
-        out["serve_large"] = serve_bench(ctx, args.large_bp, args.serve_seconds, value, LARGE_LOADS, seed=5, large=True)
+        out["serve_large"] = serve_bench(ctx, args.large_bp, args.serve_seconds, value, LARGE_LOADS, seed=5, large=True,
+                                         search_s=12.0)
+    if not args.no_serve:
+        print("[bench] kmatch: GPU program beside the reference", file=sys.stderr, flush=True)
+        out["kmatch"] = kmatch_bench(ctx)
     if copy:
         out["hbm_copy"] = copy
     if not args.no_cpu:

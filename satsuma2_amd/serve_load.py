@@ -166,16 +166,108 @@ class GridSearchProducer:
                 "iteration_ms_max": round(float(ms.max()), 2)}
 
 
+def run_search(server, target: bytes, query: bytes, seeds: Optional[np.ndarray] = None, budget_s: float = 20.0,
+               max_rounds: int = 1000, slaves: int = 1, threads: int = 16, per_block: int = 4, min_matches: int = 20,
+               dups: bool = False, chunk: int = 4096, blocks: int = 24) -> Tuple[List[List[Block]], dict]:
+    """SatsumaSynteny2's search loop (analysis/SatsumaSynteny2.cc:484-599) with `server` as
+    its slaves, in synchronous rounds: the master collects targets, all of them are aligned,
+    and their records are the next round's new matches (one schedule the reference's
+    asynchronous queue can take).  The queue size starts at slaves * threads * 8 and grows by
+    slaves * threads while the queue runs low (:544-555); the loop ends when no target is
+    left, after 10 rounds in a row with fewer than per_block * min_matches new matches per
+    slave request (:572-585; a slave request carries 2 * threads targets, WorkQueue.cc:95-110),
+    or at the time budget.  Returns the requests in order and the run's figures."""
+    from ._lib import RESULT_DTYPE
+    from .master import Grid, SearchIteration
+    grid = Grid.from_records(chunk, blocks, [("target", target)], [("query", query)])
+    it = SearchIteration(grid, [len(target)], [len(query)], dups=dups)
+    t_start = time.perf_counter()
+    if seeds is not None and len(seeds):
+        it.seed(seeds)
+    seed_s = time.perf_counter() - t_start
+    queue = slaves * threads * 8
+    min_per_slave = per_block * min_matches
+    exit_counter = extra = 10
+    seen_match = False
+    requests: List[List[Block]] = []
+    new = np.zeros(0, RESULT_DTYPE)
+    master_ms, align_ms, per_round = [], [], []
+    pairs = records = 0
+    end = "targets exhausted"
+    for r in range(max_rounds):
+        t0 = time.perf_counter()
+        want = queue
+        if want > 0.75 * queue:
+            queue += slaves * threads
+            want = queue
+        targets, cand, cov = it.step(new, want)
+        t1 = time.perf_counter()
+        master_ms.append(1e3 * (t1 - t0))
+        if not len(targets):
+            break
+        reqs = [[(int(t["target_first"]), int(t["target_last"]), int(t["query_first"]), int(t["query_last"]),
+                  int(t["fast"]))] for t in targets]
+        tickets = [server.submit(b) for b in reqs]
+        got = []
+        for tk in tickets:
+            server.wait(tk)
+            got.append(server.results(tk))
+        new = np.concatenate(got) if got else np.zeros(0, RESULT_DTYPE)
+        align_ms.append(1e3 * (time.perf_counter() - t1))
+        requests.extend(reqs)
+        npairs = _pairs([b for rq in reqs for b in rq])
+        pairs += npairs
+        records += len(new)
+        slave_msgs = max(1, -(-len(reqs) // (2 * threads)))
+        per_round.append({"targets": len(reqs), "pairs": npairs, "new_matches": len(new),
+                          "matches": len(it.matches), "master_ms": round(master_ms[-1], 1),
+                          "align_ms": round(align_ms[-1], 1)})
+        if len(new):
+            seen_match = True
+        if seen_match:
+            if len(new) / slave_msgs < min_per_slave:
+                exit_counter -= 1
+                if not exit_counter:
+                    end = "converged"
+                    break
+            else:
+                exit_counter = extra
+        if time.perf_counter() - t_start > budget_s:
+            end = "time budget"
+            break
+    wall = time.perf_counter() - t_start
+    return requests, {"rounds": len(per_round), "end": end, "seconds": round(wall, 2), "seed_s": round(seed_s, 2),
+                      "chunk_pairs": pairs, "pairs_per_s": round(pairs / max(wall, 1e-9), 1), "records": records,
+                      "matches": len(it.matches), "master_s": round(sum(master_ms) / 1e3, 2),
+                      "align_s": round(sum(align_ms) / 1e3, 2), "rounds_detail": per_round[:12]}
+
+
 def _lib_result_dtype():
     from ._lib import RESULT_DTYPE
     return RESULT_DTYPE
+
+
+class Replay:
+    """Recorded requests (e.g. run_search's) offered again in order, cycling."""
+
+    def __init__(self, requests: Sequence[List[Block]]):
+        self.requests = list(requests) or [[(0, 0, 0, 0, 0)]]
+        self.i = 0
+
+    def next(self) -> List[Block]:
+        r = self.requests[self.i % len(self.requests)]
+        self.i += 1
+        return r
+
+    def on_results(self, recs: np.ndarray):
+        pass
 
 
 def _pairs(blocks: Sequence[Block]) -> int:
     return sum((b[1] - b[0] + 1) * (b[3] - b[2] + 1) for b in blocks)
 
 
-def run_stream(server, producer: Battleship, rate: float, duration: float, seed: int = 0,
+def run_stream(server, producer, rate: float, duration: float, seed: int = 0,
                pairs_rate: float = 0.0) -> dict:
     """Offers requests for `duration` seconds: a Poisson stream of `rate` requests/s, or, with
     pairs_rate > 0, requests spaced by exponential gaps whose mean is the request's chunk

@@ -225,3 +225,34 @@ def test_large_run_model_served(ctx):
         assert_same(got[i], ref, f"request {i} {blocks}")
         total += len(ref)
     assert total > 100
+
+
+def test_search_loop_served(ctx):
+    """SatsumaSynteny2's request stream end to end at 2 Mbp: KMatch seeds on the GPU
+    (K = 11 .. 31), the master's search loop (restated grid search and chaining) with the
+    server as its slaves for a few rounds; the records of the first requests of each kind
+    (cells, seed lines) equal the oracle's."""
+    from satsuma2_amd.api import Server, SlaveParams
+    from satsuma2_amd.kmatch import satsuma_seeds
+    from satsuma2_amd.serve_load import genome_chunks, run_search
+    from test_gpu_server import oracle_records
+
+    T, Q, _ = synth.syntenic_genomes(length=2_000_000, n_blocks=80, seed=12)
+    tch = genome_chunks(T, 4096, 3072)
+    qch = genome_chunks(Q, 4096, 4096)
+    seeds, st = satsuma_seeds(ctx, [Q], [T])
+    assert len(seeds) > 100 and st["blocks"] == len(seeds)
+    size = float(len(T))
+    srv = Server(ctx, SlaveParams(target_total=size), tch, qch, max_pairs_per_launch=512)
+    reqs, stats = run_search(srv, T, Q, seeds, budget_s=20.0, max_rounds=4, threads=2)
+    assert stats["rounds"] >= 2 and stats["records"] > 0, stats
+    cells = [r for r in reqs if r[0][4] == 0][:3]
+    lines = [r for r in reqs if r[0][4] == 1][:2]
+    assert cells
+    tl = [(c.seq, c.start, c.record, c.record_len if c.record_len >= 0 else len(c.seq)) for c in tch]
+    ql = [(c.seq, c.start, c.record, c.record_len if c.record_len >= 0 else len(c.seq)) for c in qch]
+    for i, blocks in enumerate(cells + lines):
+        print(f"[search] request {i} {blocks}", flush=True)
+        got = srv.results(srv.submit(blocks))
+        assert_same(got, oracle_records(tl, ql, blocks, size), f"request {i} {blocks}")
+    srv.close()
