@@ -64,10 +64,12 @@ HBM_PEAK_GBS = 8000.0
 # 2*4*8192*4 + 1024 = 263,168 B; inverse 262,144 + 131,072 + 65,536 = 458,752 B.
 B_PAIR = 263_168 + 458_752
 # HBM bytes the pair kernel moves per chunk pair, from rocprofv3 FETCH_SIZE and WRITE_SIZE
-# counter runs of this build on the 4096-pair launch, each scaled by what the counter reads
-# for a plain device copy of known size (profiles/uniform_pairs_4096_counters.md); None
-# until that profile is taken for the kernels as they stand.
-TRAFFIC_PER_PAIR = None
+# counter runs on the 4096-pair launch, each scaled by what the counter reads for a plain
+# device copy of known size: (271,522 KiB x 2.0 + 4,916 KiB) / 4096 pairs
+# (profiles/uniform_pairs_4096_kernels.md, the build of commit 0ac9833; kernels changed
+# since need a new counter run before this figure holds for them).
+TRAFFIC_PER_PAIR = 136_990
+TRAFFIC_SOURCE = "profiles/uniform_pairs_4096_kernels.md (FETCH_SIZE x 2.0 + WRITE_SIZE, commit 0ac9833)"
 # offered chunk pairs/s as fractions of the batched rate (0.5: the headline); the serving loop
 # keeps query spectra resident, so its saturation lies above the batched rate
 SERVE_LOADS = (0.25, 0.5, 0.75, 1.0, 1.5, 2.0)
@@ -329,7 +331,13 @@ def main():
     ap.add_argument("--no-autotune", action="store_true", help="default FFT kernel geometries (for profiling)")
     ap.add_argument("--uneven-pairs", type=int, default=20_000)
     ap.add_argument("--uneven-steps", type=int, default=5)
+    ap.add_argument("--copy-probe", type=float, default=0.0,
+                    help="only the device copy of this many GiB (the HBM counters' calibration), then exit")
     args = ap.parse_args()
+    if args.copy_probe > 0:
+        sys.argv = [sys.argv[0], str(args.copy_probe), "2"]
+        exec(_COPY_PROBE, {})
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (chunk pairs shard by target chunk "
               f"across GPUs with no collective, see DESIGN.md)", file=sys.stderr)
@@ -434,6 +442,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": int(TRAFFIC_PER_PAIR * args.pairs) if TRAFFIC_PER_PAIR else None,
+                     "traffic_source": TRAFFIC_SOURCE if TRAFFIC_PER_PAIR else None,
                      "kernel": kernel_name(ctx), "kernel_ms": round(k_ms, 4), "bytes_per_pair": B_PAIR,
                      "copy_gbs": copy["gb_per_s"] if copy else None,
                      "frac_vs_copy": round(achieved / copy["gb_per_s"], 4) if copy else None},
