@@ -152,6 +152,12 @@ enum { MISC_L = 0, MISC_THIRDS = 1, MISC_NCAND = 2, MISC_KEPT = 3, MISC_Q4 = 4 /
 template <int NB>
 struct Regs {
   float acc[Geo<NB>::SA];  // correlation accumulated over channels, stage-A output order
+  // codec classes of the thread's stage-A positions (byte r % 4 of word r / 4), read from the
+  // chunk once for the four channels (class_cache).  Up to N = 8192: at larger N the eight
+  // extra live registers cost the pair kernel occupancy or spills, and each channel reads
+  // the chunk again.
+  static constexpr bool CACHE = Geo<NB>::LOG <= 13;
+  uint32_t cls[CACHE ? Geo<NB>::SA / 4 : 1];
 };
 
 // ---------------------------------------------------------------- helpers
@@ -318,19 +324,54 @@ SXC_HD inline float tail_at(const TailRef& t, int p, int ch) {
   return t.sig[((size_t)t.seg[lo].src * 4 + ch) * (size_t)Geo<NB>::N + p];
 }
 
-// Signal value of channel ch at time position p (CCSignal::SeqToPCM, :118-131).
+// Signal value of channel ch at time position p of a chunk of length L whose base at p has
+// codec class cls (CCSignal::SeqToPCM, :118-131).
 template <int NB>
-SXC_HD inline float encode_at(const char* sm, const SeqView& qs, int p, int ch, const TailRef& tail) {
+SXC_HD inline float encode_class(const char* sm, int L, unsigned cls, int p, int ch, const TailRef& tail) {
   using SM = Smem<NB>;
-  const int L = qs.L;
   if (p >= L) return tail.n ? tail_at<NB>(tail, p, ch) : 0.f;
   const double* diff = (const double*)(sm + SM::CODEC);  // DNA_X(class) - off[X] (phase_prepare)
   const float e = (L < 1024) ? 1.f : ((const float*)(sm + SM::ENT))[p / (Geo<NB>::N / 512)];
-  return (float)((double)e * diff[4 * qs[p] + ch]);
+  return (float)((double)e * diff[4 * cls + ch]);
+}
+
+template <int NB>
+SXC_HD inline float encode_at(const char* sm, const SeqView& qs, int p, int ch, const TailRef& tail) {
+  return encode_class<NB>(sm, qs.L, p < qs.L ? qs[p] : 0u, p, ch, tail);
+}
+
+// x, copied where it is used: the compiler can neither hoist nor share the copy between
+// loop iterations (no instruction), so one packed word stays live instead of its fields.
+SXC_HD inline uint32_t fresh(uint32_t x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  asm volatile("" : "+v"(x));
+#endif
+  return x;
+}
+
+// Fills each thread's class cache (Regs::cls) for its stage-A positions: the chunk's bytes
+// and the class tables are read once per transform set instead of once per channel.
+template <int NB, class Ex>
+SXC_HD inline void class_cache(Ex& ex, const SeqView& qs) {
+  using G = Geo<NB>;
+  if constexpr (!Regs<NB>::CACHE) return;
+  ex.each([&](int g) {
+    if (g >= G::GROUPS) return;
+    Regs<NB>& rg = ex.template reg<Regs<NB>>(g);
+    static_for<0, G::SA / 4>([&](auto W) {
+      uint32_t w = 0;
+      static_for<0, 4>([&](auto B) {
+        const int p = stageA_pos<NB>(g, 4 * decltype(W)::value + decltype(B)::value);
+        if (p < qs.L) w |= (uint32_t)qs[p] << (8 * decltype(B)::value);
+      });
+      rg.cls[decltype(W)::value] = w;
+    });
+  });
 }
 
 // Forward FFT of one channel into the padded LDS array: stage A (encode fused into
-// the bit-reversed load, or loaded from a given signal) then the rounds.
+// the bit-reversed load, from the class cache where class_cache fills one, or loaded from a
+// given signal) then the rounds.
 template <int NB, int R, class Ex>
 SXC_HD inline void fwd_rounds(Ex& ex, float* Y, const Twiddle& tw, int upto = Schedule<NB>::NR) {
   using S = Schedule<NB>;
@@ -382,12 +423,24 @@ SXC_HD inline void forward_channel(Ex& ex, char* sm, const sxc_tables* tab, cons
   float* Y = (float*)(sm + SM::Y);
   ex.each([&](int g) {
     if (g >= G::GROUPS) return;
+    const Regs<NB>& rg = ex.template reg<Regs<NB>>(g);
     float v[G::SA];
-    static_for<0, G::SA>([&](auto I) {
-      constexpr int r = decltype(I)::value;
-      const int p = stageA_pos<NB>(g, r);
-      v[r] = given ? given[p] : encode_at<NB>(sm, qs, p, ch, tail);
-    });
+    if constexpr (Regs<NB>::CACHE) {
+      static_for<0, G::SA / 4>([&](auto W) {
+        const uint32_t w = fresh(rg.cls[decltype(W)::value]);
+        static_for<0, 4>([&](auto B) {
+          constexpr int r = 4 * decltype(W)::value + decltype(B)::value;
+          const int p = stageA_pos<NB>(g, r);
+          v[r] = given ? given[p] : encode_class<NB>(sm, qs.L, (w >> (8 * (r % 4))) & 255u, p, ch, tail);
+        });
+      });
+    } else {
+      static_for<0, G::SA>([&](auto I) {
+        constexpr int r = decltype(I)::value;
+        const int p = stageA_pos<NB>(g, r);
+        v[r] = given ? given[p] : encode_at<NB>(sm, qs, p, ch, tail);
+      });
+    }
     stageA_forward<G::SA>(v, tw);
     float* ys = Y + pad(G::SA * g);  // pad(SA*g + r) = pad(SA*g) + r for r < SA (SA divides the pad row)
     static_for<0, G::SA>([&](auto I) { ys[decltype(I)::value] = v[decltype(I)::value]; });
@@ -436,6 +489,7 @@ SXC_HD inline void spectra_body(Ex& ex, char* sm, const SpecArgs& a, int k) {
   float* out = a.spectra + (size_t)a.dst[k] * 4 * G::N;
   const TailRef tail =
       tail_ref<NB>(a.tails, a.segs, a.src_sig, a.chunk_tail ? a.chunk_tail[a.which[k]] : -1);
+  if (!a.signals) class_cache<NB>(ex, qs);
   for (int ch = 0; ch < 4; ++ch) {
     const float* given = a.signals ? a.signals + ((size_t)k * 4 + ch) * G::N : nullptr;
     forward_channel<NB>(ex, sm, a.tab, tw, qs, ch, given, tail);
@@ -467,6 +521,7 @@ SXC_HD inline void correlate(Ex& ex, char* sm, const sxc_tables* tab, const Twid
     static_for<0, G::SA>([&](auto I) { r.acc[decltype(I)::value] = 0.f; });
   });
   const float mul = (float)(1.0 / (double)G::N);  // FFTReal::rescale (FFTReal.hpp:264)
+  if (!qspec4 && !qsig4) class_cache<NB>(ex, qs);
   for (int ch = 0; ch < 4; ++ch) {
     if (qspec4) {
       // the load below writes Y with another thread mapping than the previous channel's
