@@ -122,57 +122,131 @@ double repeat_score(int v) {
   return 0.5;
 }
 
-// SingleMatchDP (:109-163)
-struct Node {
-  Match m;
-  double score = kUnreached;
-  double pen = -1.;
-  int back = -1;
-  double rep = 100.;
-  double total() {
-    if (pen < 0.) pen = match_penalty(m.prob);
-    return score + pen;
-  }
-  void offer(double s, int from) {
-    if (s < score) {
-      score = s;
-      back = from;
-    }
-  }
-};
-
-// MatchDynProg::Chain (:194-243): matches by target start; each one offers every later
-// match within the look-ahead its score plus the transition cost and the repeat weights of
-// the matches it skips; the chain is traced back from the last match and returned sorted.
-void chain_target(std::vector<Node>& v, std::vector<Match>* out) {
+// MatchDynProg::Chain (:194-243) and SingleMatchDP (:109-163): matches by target start; each
+// one offers every later match within the look-ahead its score plus its match penalty, the
+// transition cost and the repeat weights of the matches it skips; the chain is traced back
+// from the last match and returned sorted.
+//
+// The reference pushes each source's offers forward.  Here each destination pulls them, which
+// gives the same scores and back pointers bit for bit:
+//  * source i reaches destination j iff j - i <= 2000 and (j - i <= 11 or their target starts
+//    differ by at most 250 kbp): its push loop breaks at the first j that fails this, and with
+//    the starts sorted the test fails for every later j as well;
+//  * the offer from i to j is total(i) + (trans(i, j) + skip(i, j)), skip(i, j) the repeat
+//    weights of i + 1 .. j - 1 added in that order; each source keeps its running sum and
+//    adds each destination's weight after that destination took its offers, as the push loop
+//    does after each offer;
+//  * a destination keeps the first smallest offer in source order (the push loop's strict
+//    comparison), starting from "unreached";
+//  * total(i) is final once every earlier source offered to i, i.e. when i is pulled.
+// Most sources cannot win: an offer is at least total(i) + skip(i, j), the transition cost
+// being non-negative (and FP addition monotone), so only sources whose bound does not exceed
+// the best offer of the nearest few sources are evaluated, in source order.
+void chain_target(std::vector<Match>& v, std::vector<double>& rep_of, std::vector<Match>* out) {
   constexpr int kLookAheadDist = 250000;  // m_laDist (:173)
   constexpr int kLookAheadCount = 2000;   // la_limit (:200)
-  std::sort(v.begin(), v.end(), [](const Node& a, const Node& b) { return a.m.start_target < b.m.start_target; });
-  if (v.empty()) return;
-  v[0].offer(0., -1);
+  constexpr int kNear = 4;                // sources evaluated for the bound
   const int n = (int)v.size();
-  for (int i = 0; i < n; ++i) {
-    Node& one = v[(size_t)i];
-    double skip = 0.;
-    int fed = 0;
-    for (int j = i + 1; j < n; ++j) {
-      Node& two = v[(size_t)j];
-      if (two.m.start_target - one.m.start_target > kLookAheadDist && fed > 10) break;
-      if (j - i > kLookAheadCount) break;
-      ++fed;
-      double trans = kOtherQueryPenalty;
-      if (one.m.query == two.m.query)
-        trans = trans_penalty(one.m.start_target, one.m.start_query, one.m.rc, two.m.start_target, two.m.start_query,
-                              two.m.rc);
-      trans += skip;
-      skip += two.rep;
-      two.offer(one.total() + trans, i);
+  if (n == 0) return;
+  // the reference sorts its match vector by target start with std::sort; sorting the same
+  // records the same way gives the same order among equal starts
+  {
+    struct Rec {
+      Match m;
+      double rep;
+    };
+    std::vector<Rec> r((size_t)n);
+    for (int i = 0; i < n; ++i) r[(size_t)i] = Rec{v[(size_t)i], rep_of[(size_t)i]};
+    std::sort(r.begin(), r.end(), [](const Rec& a, const Rec& b) { return a.m.start_target < b.m.start_target; });
+    for (int i = 0; i < n; ++i) {
+      v[(size_t)i] = r[(size_t)i].m;
+      rep_of[(size_t)i] = r[(size_t)i].rep;
     }
   }
+  std::vector<int> back((size_t)n, -1);
+  std::vector<double> total((size_t)n), skip((size_t)n, 0.);
+  total[0] = 0. + match_penalty(v[0].prob);  // the first match starts at score 0
+  int lo = 0;
+  for (int j = 1; j < n; ++j) {
+    const Match& two = v[(size_t)j];
+    while (two.start_target - v[(size_t)lo].start_target > kLookAheadDist) ++lo;
+    int w0 = std::min(lo, j - 11);
+    if (w0 < j - kLookAheadCount) w0 = j - kLookAheadCount;
+    if (w0 < 0) w0 = 0;
+    auto offer = [&](int i) {
+      const Match& one = v[(size_t)i];
+      double trans = kOtherQueryPenalty;
+      if (one.query == two.query)
+        trans = trans_penalty(one.start_target, one.start_query, one.rc, two.start_target, two.start_query, two.rc);
+      return total[(size_t)i] + (trans + skip[(size_t)i]);
+    };
+    double bound = kUnreached;
+    for (int i = std::max(w0, j - kNear); i < j; ++i) {
+      const double c = offer(i);
+      if (c < bound) bound = c;
+    }
+    // sources in source order, 16 at a time: the bound test (vectorized) first, the offers of
+    // those that pass, then each source's running skip takes this destination's weight (a
+    // source before w0 reaches no later destination either, so its skip is left as it is)
+    double best = kUnreached;
+    int bi = -1;
+    const double* ti = total.data();
+    double* ks = skip.data();
+    const double rj = rep_of[(size_t)j];
+    auto take = [&](int i) {
+      if (ti[i] + ks[i] <= bound) {
+        const double c = offer(i);
+        if (c < best) {
+          best = c;
+          bi = i;
+        }
+      }
+    };
+    int i = w0;
+    for (; i + 16 <= j; i += 16) {
+      bool any = false;
+      for (int k = 0; k < 16; ++k) any |= ti[i + k] + ks[i + k] <= bound;
+      if (any)
+        for (int k = 0; k < 16; ++k) take(i + k);
+      for (int k = 0; k < 16; ++k) ks[i + k] += rj;
+    }
+    for (; i < j; ++i) {
+      take(i);
+      ks[i] += rj;
+    }
+    back[(size_t)j] = bi;
+    total[(size_t)j] = best + match_penalty(two.prob);
+  }
   const size_t first = out->size();
-  for (int i = n - 1; i >= 0; i = v[(size_t)i].back) out->push_back(v[(size_t)i].m);
+  for (int i = n - 1; i >= 0; i = back[(size_t)i]) out->push_back(v[(size_t)i]);
   std::sort(out->begin() + (std::ptrdiff_t)first, out->end(), match_less);
 }
+
+// Capped per-base coverage of one sequence by half-open intervals of bases.
+struct Cover {
+  std::vector<int> lo, hi;  // interval starts and ends, sorted separately by finish()
+  int size = 0;
+  void add(int a, int b) {
+    if (a < 1) a = 1;
+    if (b > size) b = size;
+    if (a < b) {
+      lo.push_back(a);
+      hi.push_back(b);
+    }
+  }
+  void finish() {
+    std::sort(lo.begin(), lo.end());
+    std::sort(hi.begin(), hi.end());
+  }
+  // intervals holding p (started at or before p, not ended at or before p), at most 100; a
+  // position outside the sequence reads as uncovered (the reference reads past its array)
+  int at(int p) const {
+    if (p < 0 || p >= size) return 0;
+    const long c = (std::upper_bound(lo.begin(), lo.end(), p) - lo.begin()) -
+                   (std::upper_bound(hi.begin(), hi.end(), p) - hi.begin());
+    return c < 100 ? (int)c : 100;
+  }
+};
 
 }  // namespace
 
@@ -182,10 +256,12 @@ void chain(const MatchSet& in, MatchSet* out) {
   out->query_size = in.query_size;
   out->m.clear();
   // per-base coverage counts, capped at 100, with a match overlapping its predecessor on
-  // the same target and query counted only past it (:444-485)
-  std::vector<std::vector<signed char>> tcov((size_t)nt), qcov((size_t)nq);
-  for (int i = 0; i < nt; ++i) tcov[(size_t)i].assign((size_t)std::max(0, in.target_size[(size_t)i]), 0);
-  for (int i = 0; i < nq; ++i) qcov[(size_t)i].assign((size_t)std::max(0, in.query_size[(size_t)i]), 0);
+  // the same target and query counted only past it (:444-485).  Only the counts at match
+  // midpoints are read, so each sequence keeps its covered intervals' sorted ends instead of
+  // a count per base: the count at p is the number of intervals holding it, capped.
+  std::vector<Cover> tcov((size_t)nt), qcov((size_t)nq);
+  for (int i = 0; i < nt; ++i) tcov[(size_t)i].size = std::max(0, in.target_size[(size_t)i]);
+  for (int i = 0; i < nq; ++i) qcov[(size_t)i].size = std::max(0, in.query_size[(size_t)i]);
   Match last;
   for (const Match& x : in.m) {
     if (x.query < 0) continue;
@@ -197,13 +273,12 @@ void chain(const MatchSet& in, MatchSet* out) {
       sq = last.start_query + last.length;
     }
     last = x;
-    std::vector<signed char>& t = tcov[(size_t)x.target];
-    std::vector<signed char>& q = qcov[(size_t)x.query];
-    for (int j = st; j < x.start_target + x.length; ++j)
-      if (j > 0 && j < (int)t.size() && t[(size_t)j] < 100) ++t[(size_t)j];
-    for (int j = sq; j < x.start_query + x.length; ++j)
-      if (j > 0 && j < (int)q.size() && q[(size_t)j] < 100) ++q[(size_t)j];
+    // bases 1 .. size - 1 are counted (the reference skips base 0)
+    tcov[(size_t)x.target].add(st, x.start_target + x.length);
+    qcov[(size_t)x.query].add(sq, x.start_query + x.length);
   }
+  for (Cover& c : tcov) c.finish();
+  for (Cover& c : qcov) c.finish();
   // each target's matches are taken from its first to its last index (:489-505)
   const int n = (int)in.m.size();
   std::vector<int> first((size_t)nt, n + 1), lastI((size_t)nt, -1);
@@ -212,33 +287,29 @@ void chain(const MatchSet& in, MatchSet* out) {
     if (i < first[(size_t)id]) first[(size_t)id] = i;
     if (i > lastI[(size_t)id]) lastI[(size_t)id] = i;
   }
-  auto cov_at = [](const std::vector<signed char>& c, int p) -> int {
-    // a midpoint outside the sequence reads as uncovered (the reference reads past its array)
-    return (p >= 0 && p < (int)c.size()) ? c[(size_t)p] : 0;
-  };
-  std::vector<Node> nodes;
+  std::vector<Match> nodes;
+  std::vector<double> reps;
   std::vector<Match> result;
   for (int j = 0; j < nt; ++j) {
     int f = first[(size_t)j];
     const int l = lastI[(size_t)j];
     if (l == -1) f = 0;
     nodes.clear();
-    const std::vector<signed char>& t = tcov[(size_t)j];
+    reps.clear();
+    const Cover& t = tcov[(size_t)j];
     for (int i = f; i <= l; ++i) {
       const Match& x = in.m[(size_t)i];
-      const std::vector<signed char>& q = qcov[(size_t)x.query];
-      double r = repeat_score(cov_at(t, x.start_target + x.length / 2));
-      const double r2 = repeat_score(cov_at(q, x.start_query + x.length / 2));
+      const Cover& q = qcov[(size_t)x.query];
+      double r = repeat_score(t.at(x.start_target + x.length / 2));
+      const double r2 = repeat_score(q.at(x.start_query + x.length / 2));
       if (r2 < r) r = r2;
       if (r > 0.0001) {
-        Node nd;
-        nd.m = x;
-        nd.rep = r;
-        nodes.push_back(nd);
+        nodes.push_back(x);
+        reps.push_back(r);
       }
     }
     result.clear();
-    chain_target(nodes, &result);
+    chain_target(nodes, reps, &result);
     out->m.insert(out->m.end(), result.begin(), result.end());
   }
 }

@@ -354,19 +354,20 @@ SXC_HD inline uint32_t fresh(uint32_t x) {
 template <int NB, class Ex>
 SXC_HD inline void class_cache(Ex& ex, const SeqView& qs) {
   using G = Geo<NB>;
-  if constexpr (!Regs<NB>::CACHE) return;
-  ex.each([&](int g) {
-    if (g >= G::GROUPS) return;
-    Regs<NB>& rg = ex.template reg<Regs<NB>>(g);
-    static_for<0, G::SA / 4>([&](auto W) {
-      uint32_t w = 0;
-      static_for<0, 4>([&](auto B) {
-        const int p = stageA_pos<NB>(g, 4 * decltype(W)::value + decltype(B)::value);
-        if (p < qs.L) w |= (uint32_t)qs[p] << (8 * decltype(B)::value);
+  if constexpr (Regs<NB>::CACHE) {
+    ex.each([&](int g) {
+      if (g >= G::GROUPS) return;
+      Regs<NB>& rg = ex.template reg<Regs<NB>>(g);
+      static_for<0, G::SA / 4>([&](auto W) {
+        uint32_t w = 0;
+        static_for<0, 4>([&](auto B) {
+          const int p = stageA_pos<NB>(g, 4 * decltype(W)::value + decltype(B)::value);
+          if (p < qs.L) w |= (uint32_t)qs[p] << (8 * decltype(B)::value);
+        });
+        rg.cls[decltype(W)::value] = w;
       });
-      rg.cls[decltype(W)::value] = w;
     });
-  });
+  }
 }
 
 // Forward FFT of one channel into the padded LDS array: stage A (encode fused into
