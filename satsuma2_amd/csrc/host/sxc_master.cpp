@@ -133,15 +133,56 @@ double repeat_score(int v) {
 //    differ by at most 250 kbp): its push loop breaks at the first j that fails this, and with
 //    the starts sorted the test fails for every later j as well;
 //  * the offer from i to j is total(i) + (trans(i, j) + skip(i, j)), skip(i, j) the repeat
-//    weights of i + 1 .. j - 1 added in that order; each source keeps its running sum and
-//    adds each destination's weight after that destination took its offers, as the push loop
-//    does after each offer;
+//    weights of i + 1 .. j - 1 added to 0 in that order, as the push loop's running sum adds
+//    them after each offer;
 //  * a destination keeps the first smallest offer in source order (the push loop's strict
 //    comparison), starting from "unreached";
 //  * total(i) is final once every earlier source offered to i, i.e. when i is pulled.
 // Most sources cannot win: an offer is at least total(i) + skip(i, j), the transition cost
 // being non-negative (and FP addition monotone), so only sources whose bound does not exceed
-// the best offer of the nearest few sources are evaluated, in source order.
+// the best offer B of the nearest few sources can, and only those are evaluated, in source
+// order.  With S the prefix sums of the repeat weights, that bound is C(i) + S(j - 1),
+// C(i) = total(i) - S(i): a min-tree over C finds the sources with C(i) <= B - S(j - 1)
+// (widened by a margin far above the rounding of these sums), and a source's running skip
+// sum is brought up to j, by the same additions in the same order, only when it is found.
+namespace {
+
+// Min-tree over the sources' C values, reporting in ascending order every index of a range
+// whose value is at most a threshold.
+class MinTree {
+ public:
+  explicit MinTree(int n) : p_(1) {
+    while (p_ < n) p_ <<= 1;
+    mn_.assign((size_t)(2 * p_), HUGE_VAL);
+  }
+  void set(int i, double x) {
+    size_t v = (size_t)(i + p_);
+    mn_[v] = x;
+    for (v >>= 1; v >= 1; v >>= 1) mn_[v] = std::min(mn_[2 * v], mn_[2 * v + 1]);
+  }
+  template <class F>
+  void report(int lo, int hi, double t, F&& f) const {
+    visit(1, 0, p_, lo, hi, t, f);
+  }
+
+ private:
+  template <class F>
+  void visit(size_t v, int a, int b, int lo, int hi, double t, F& f) const {
+    if (b <= lo || a >= hi || !(mn_[v] <= t)) return;
+    if (b - a == 1) {
+      f(a);
+      return;
+    }
+    const int mid = (a + b) >> 1;
+    visit(2 * v, a, mid, lo, hi, t, f);
+    visit(2 * v + 1, mid, b, lo, hi, t, f);
+  }
+  int p_;
+  std::vector<double> mn_;
+};
+
+}  // namespace
+
 void chain_target(std::vector<Match>& v, std::vector<double>& rep_of, std::vector<Match>* out) {
   constexpr int kLookAheadDist = 250000;  // m_laDist (:173)
   constexpr int kLookAheadCount = 2000;   // la_limit (:200)
@@ -163,9 +204,21 @@ void chain_target(std::vector<Match>& v, std::vector<double>& rep_of, std::vecto
       rep_of[(size_t)i] = r[(size_t)i].rep;
     }
   }
-  std::vector<int> back((size_t)n, -1);
-  std::vector<double> total((size_t)n), skip((size_t)n, 0.);
+  std::vector<int> back((size_t)n, -1), next((size_t)n);
+  std::vector<double> total((size_t)n), skip((size_t)n, 0.), pre((size_t)n);
+  double acc = 0.;
+  for (int i = 0; i < n; ++i) {
+    acc += rep_of[(size_t)i];
+    pre[(size_t)i] = acc;
+    next[(size_t)i] = i + 1;  // skip(i) holds the weights of i + 1 .. next(i) - 1
+  }
+  MinTree tree(n);
+  auto settle = [&](int i) {
+    const double c = total[(size_t)i] - pre[(size_t)i];
+    tree.set(i, c == c ? c : HUGE_VAL);  // a NaN total never wins
+  };
   total[0] = 0. + match_penalty(v[0].prob);  // the first match starts at score 0
+  settle(0);
   int lo = 0;
   for (int j = 1; j < n; ++j) {
     const Match& two = v[(size_t)j];
@@ -174,48 +227,33 @@ void chain_target(std::vector<Match>& v, std::vector<double>& rep_of, std::vecto
     if (w0 < j - kLookAheadCount) w0 = j - kLookAheadCount;
     if (w0 < 0) w0 = 0;
     auto offer = [&](int i) {
+      double& k = skip[(size_t)i];
+      for (int& m = next[(size_t)i]; m < j; ++m) k += rep_of[(size_t)m];
       const Match& one = v[(size_t)i];
       double trans = kOtherQueryPenalty;
       if (one.query == two.query)
         trans = trans_penalty(one.start_target, one.start_query, one.rc, two.start_target, two.start_query, two.rc);
-      return total[(size_t)i] + (trans + skip[(size_t)i]);
+      return total[(size_t)i] + (trans + k);
     };
-    double bound = kUnreached;
+    double bound = kUnreached;  // no offer at or above "unreached" is taken
     for (int i = std::max(w0, j - kNear); i < j; ++i) {
       const double c = offer(i);
       if (c < bound) bound = c;
     }
-    // sources in source order, 16 at a time: the bound test (vectorized) first, the offers of
-    // those that pass, then each source's running skip takes this destination's weight (a
-    // source before w0 reaches no later destination either, so its skip is left as it is)
+    const double sj = pre[(size_t)j - 1];
+    const double margin = 1e-7 * (fabs(bound) + 2. * fabs(sj)) + 1e-9;
     double best = kUnreached;
     int bi = -1;
-    const double* ti = total.data();
-    double* ks = skip.data();
-    const double rj = rep_of[(size_t)j];
-    auto take = [&](int i) {
-      if (ti[i] + ks[i] <= bound) {
-        const double c = offer(i);
-        if (c < best) {
-          best = c;
-          bi = i;
-        }
+    tree.report(w0, j, bound - sj + margin, [&](int i) {
+      const double c = offer(i);
+      if (c < best) {
+        best = c;
+        bi = i;
       }
-    };
-    int i = w0;
-    for (; i + 16 <= j; i += 16) {
-      bool any = false;
-      for (int k = 0; k < 16; ++k) any |= ti[i + k] + ks[i + k] <= bound;
-      if (any)
-        for (int k = 0; k < 16; ++k) take(i + k);
-      for (int k = 0; k < 16; ++k) ks[i + k] += rj;
-    }
-    for (; i < j; ++i) {
-      take(i);
-      ks[i] += rj;
-    }
+    });
     back[(size_t)j] = bi;
     total[(size_t)j] = best + match_penalty(two.prob);
+    settle(j);
   }
   const size_t first = out->size();
   for (int i = n - 1; i >= 0; i = back[(size_t)i]) out->push_back(v[(size_t)i]);
@@ -238,13 +276,23 @@ struct Cover {
     std::sort(lo.begin(), lo.end());
     std::sort(hi.begin(), hi.end());
   }
-  // intervals holding p (started at or before p, not ended at or before p), at most 100; a
-  // position outside the sequence reads as uncovered (the reference reads past its array)
-  int at(int p) const {
-    if (p < 0 || p >= size) return 0;
-    const long c = (std::upper_bound(lo.begin(), lo.end(), p) - lo.begin()) -
-                   (std::upper_bound(hi.begin(), hi.end(), p) - hi.begin());
-    return c < 100 ? (int)c : 100;
+  // For each (position p, slot k) of q: the intervals holding p (started at or before p, not
+  // ended at or before p), at most 100, into cnt[k]; a position outside the sequence reads
+  // as uncovered (the reference reads past its array).  One sweep over q sorted by position.
+  void lookup(std::vector<std::pair<int, int>>& q, std::vector<int>& cnt) const {
+    std::sort(q.begin(), q.end());
+    size_t a = 0, b = 0;
+    for (const auto& e : q) {
+      const int p = e.first;
+      if (p < 0 || p >= size) {
+        cnt[(size_t)e.second] = 0;
+        continue;
+      }
+      while (a < lo.size() && lo[a] <= p) ++a;
+      while (b < hi.size() && hi[b] <= p) ++b;
+      const long c = (long)a - (long)b;
+      cnt[(size_t)e.second] = c < 100 ? (int)c : 100;
+    }
   }
 };
 
@@ -287,6 +335,17 @@ void chain(const MatchSet& in, MatchSet* out) {
     if (i < first[(size_t)id]) first[(size_t)id] = i;
     if (i > lastI[(size_t)id]) lastI[(size_t)id] = i;
   }
+  // query coverage at each match's query midpoint
+  std::vector<int> cq((size_t)n), ct;
+  {
+    std::vector<std::vector<std::pair<int, int>>> qq((size_t)nq);
+    for (int i = 0; i < n; ++i) {
+      const Match& x = in.m[(size_t)i];
+      qq[(size_t)x.query].emplace_back(x.start_query + x.length / 2, i);
+    }
+    for (int k = 0; k < nq; ++k) qcov[(size_t)k].lookup(qq[(size_t)k], cq);
+  }
+  std::vector<std::pair<int, int>> tq;
   std::vector<Match> nodes;
   std::vector<double> reps;
   std::vector<Match> result;
@@ -296,12 +355,18 @@ void chain(const MatchSet& in, MatchSet* out) {
     if (l == -1) f = 0;
     nodes.clear();
     reps.clear();
-    const Cover& t = tcov[(size_t)j];
+    // target j's coverage at the target midpoints of the matches in its index range
+    tq.clear();
     for (int i = f; i <= l; ++i) {
       const Match& x = in.m[(size_t)i];
-      const Cover& q = qcov[(size_t)x.query];
-      double r = repeat_score(t.at(x.start_target + x.length / 2));
-      const double r2 = repeat_score(q.at(x.start_query + x.length / 2));
+      tq.emplace_back(x.start_target + x.length / 2, i - f);
+    }
+    ct.assign(tq.size(), 0);
+    tcov[(size_t)j].lookup(tq, ct);
+    for (int i = f; i <= l; ++i) {
+      const Match& x = in.m[(size_t)i];
+      double r = repeat_score(ct[(size_t)(i - f)]);
+      const double r2 = repeat_score(cq[(size_t)i]);
       if (r2 < r) r = r2;
       if (r > 0.0001) {
         nodes.push_back(x);

@@ -228,6 +228,55 @@ def test_chaining_matches_reference(tmp_path, op, seed):
     assert np.array_equal(wts, ts) and np.array_equal(wqs, qs)
 
 
+def limits_set(seed: int):
+    """A set that reaches the chaining's look-ahead limits: more than 2,000 matches within
+    250 kbp (the count limit), a repeat-dense stretch whose matches weigh almost nothing (skip
+    sums stay tiny, so far sources keep competing), and sparse matches more than 1 Mbp apart
+    (the distance break after 11 sources, and TransPenalty's square-root term)."""
+    rng = np.random.default_rng(seed)
+    tsizes = np.array([6_000_000], np.int32)
+    qsizes = np.array([6_000_000, 3_000_000], np.int32)
+    rows = []
+    pos = 100
+    for i in range(9000):  # dense collinear run with jitter, both queries, both strands
+        q = int(rng.integers(2)) if rng.random() < 0.1 else 0
+        rc = 1 if rng.random() < 0.05 else 0
+        ln = int(rng.integers(40, 200))
+        rows.append((0, q, qsizes[q], pos, min(int(qsizes[q]) - 300, max(0, pos + int(rng.integers(-20, 21)))), ln,
+                     rc, float(rng.uniform(0.98, 1.0)), 0.9))
+        pos += int(rng.integers(20, 160))
+    base = pos + 5000
+    for i in range(3000):  # a repeat: many copies of one region, piled on one spot
+        st = base + int(rng.integers(0, 400))
+        rows.append((0, 1, qsizes[1], st, int(rng.integers(0, qsizes[1] - 400)), 300, int(rng.integers(2)),
+                     0.999, 0.8))
+    pos = base + 600_000
+    while pos < tsizes[0] - 500:  # sparse: gaps of 0.3 - 1.6 Mbp
+        rows.append((0, 0, qsizes[0], pos, min(int(qsizes[0]) - 300, pos // 2), 150, 0, 0.995, 0.85))
+        pos += int(rng.integers(300_000, 1_600_000))
+    m = np.zeros(len(rows), MATCH_DTYPE)
+    for i, r in enumerate(rows):
+        m[i]["target_id"], m[i]["query_id"], m[i]["query_len"], m[i]["start_target"], m[i]["start_query"] = r[:5]
+        m[i]["length"], m[i]["rc"], m[i]["prob"], m[i]["ident"] = r[5:]
+        m[i]["matches"] = r[8] * r[5]
+    rng.shuffle(m)
+    return tsizes, qsizes, MatchSet(tsizes, qsizes).add(m).sort().collapse().matches()
+
+
+@need_ref
+@pytest.mark.parametrize("op", ["chain", "chainmult"])
+def test_chaining_at_look_ahead_limits_matches_reference(tmp_path, op):
+    ts, qs, m = limits_set(31)
+    st = np.sort(m["start_target"])
+    assert len(m) > 12_000 and np.max(np.searchsorted(st, st + 250_000) - np.arange(len(st))) > 2000
+    assert np.max(np.diff(st)) > 1_000_000
+    p = tmp_path / "in.bin"
+    p.write_bytes(set_bytes(ts, qs, m))
+    _, _, want, _ = parse_set(run_ref(tmp_path, op, p))
+    got = mine(op, ts, qs, m)
+    assert len(want) > 1000 and same(got, want)
+
+
 @need_ref
 def test_interpolation_adds_guesses(tmp_path):
     """A chained set with a run of >= 40 collinear matches gets two guesses on each side."""
