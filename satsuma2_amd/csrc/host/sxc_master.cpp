@@ -5,6 +5,8 @@
 #include <math.h>
 
 #include <algorithm>
+#include <atomic>
+#include <thread>
 #include <utility>
 
 namespace sxc_master {
@@ -336,7 +338,7 @@ void chain(const MatchSet& in, MatchSet* out) {
     if (i > lastI[(size_t)id]) lastI[(size_t)id] = i;
   }
   // query coverage at each match's query midpoint
-  std::vector<int> cq((size_t)n), ct;
+  std::vector<int> cq((size_t)n);
   {
     std::vector<std::vector<std::pair<int, int>>> qq((size_t)nq);
     for (int i = 0; i < n; ++i) {
@@ -345,38 +347,49 @@ void chain(const MatchSet& in, MatchSet* out) {
     }
     for (int k = 0; k < nq; ++k) qcov[(size_t)k].lookup(qq[(size_t)k], cq);
   }
-  std::vector<std::pair<int, int>> tq;
-  std::vector<Match> nodes;
-  std::vector<double> reps;
-  std::vector<Match> result;
-  for (int j = 0; j < nt; ++j) {
-    int f = first[(size_t)j];
-    const int l = lastI[(size_t)j];
-    if (l == -1) f = 0;
-    nodes.clear();
-    reps.clear();
-    // target j's coverage at the target midpoints of the matches in its index range
-    tq.clear();
-    for (int i = f; i <= l; ++i) {
-      const Match& x = in.m[(size_t)i];
-      tq.emplace_back(x.start_target + x.length / 2, i - f);
-    }
-    ct.assign(tq.size(), 0);
-    tcov[(size_t)j].lookup(tq, ct);
-    for (int i = f; i <= l; ++i) {
-      const Match& x = in.m[(size_t)i];
-      double r = repeat_score(ct[(size_t)(i - f)]);
-      const double r2 = repeat_score(cq[(size_t)i]);
-      if (r2 < r) r = r2;
-      if (r > 0.0001) {
-        nodes.push_back(x);
-        reps.push_back(r);
+  // targets are chained independently: on up to 16 host threads, each taking the next
+  // target, results concatenated in target order
+  std::vector<std::vector<Match>> results((size_t)nt);
+  std::atomic<int> next_target{0};
+  auto worker = [&]() {
+    std::vector<std::pair<int, int>> tq;
+    std::vector<int> ct;
+    std::vector<Match> nodes;
+    std::vector<double> reps;
+    for (int j = next_target++; j < nt; j = next_target++) {
+      int f = first[(size_t)j];
+      const int l = lastI[(size_t)j];
+      if (l == -1) f = 0;
+      nodes.clear();
+      reps.clear();
+      // target j's coverage at the target midpoints of the matches in its index range
+      tq.clear();
+      for (int i = f; i <= l; ++i) {
+        const Match& x = in.m[(size_t)i];
+        tq.emplace_back(x.start_target + x.length / 2, i - f);
       }
+      ct.assign(tq.size(), 0);
+      tcov[(size_t)j].lookup(tq, ct);
+      for (int i = f; i <= l; ++i) {
+        const Match& x = in.m[(size_t)i];
+        double r = repeat_score(ct[(size_t)(i - f)]);
+        const double r2 = repeat_score(cq[(size_t)i]);
+        if (r2 < r) r = r2;
+        if (r > 0.0001) {
+          nodes.push_back(x);
+          reps.push_back(r);
+        }
+      }
+      chain_target(nodes, reps, &results[(size_t)j]);
     }
-    result.clear();
-    chain_target(nodes, reps, &result);
-    out->m.insert(out->m.end(), result.begin(), result.end());
-  }
+  };
+  int nthreads = (int)std::min<unsigned>(16u, std::max(1u, std::thread::hardware_concurrency()));
+  if (nthreads > nt) nthreads = nt;
+  std::vector<std::thread> pool;
+  for (int k = 1; k < nthreads; ++k) pool.emplace_back(worker);
+  worker();
+  for (std::thread& t : pool) t.join();
+  for (const std::vector<Match>& r : results) out->m.insert(out->m.end(), r.begin(), r.end());
 }
 
 void chain_dups(const MatchSet& in, MatchSet* out) {
