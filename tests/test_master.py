@@ -228,6 +228,20 @@ def test_chaining_matches_reference(tmp_path, op, seed):
     assert np.array_equal(wts, ts) and np.array_equal(wqs, qs)
 
 
+@need_ref
+@pytest.mark.parametrize("op", ["chain", "chainmult"])
+def test_chaining_many_targets_matches_reference(tmp_path, op):
+    """Targets are chained on several host threads; the result keeps the reference's target
+    order."""
+    ts, qs, m = random_set(12, nt=24, nq=5, runs=60, noise=600)
+    m = MatchSet(ts, qs).add(m).sort().collapse().matches()
+    assert len(np.unique(m["target_id"])) == 24
+    p = tmp_path / "in.bin"
+    p.write_bytes(set_bytes(ts, qs, m))
+    _, _, want, _ = parse_set(run_ref(tmp_path, op, p))
+    assert len(want) > 0 and same(mine(op, ts, qs, m), want)
+
+
 def limits_set(seed: int):
     """A set that reaches the chaining's look-ahead limits: more than 2,000 matches within
     250 kbp (the count limit), a repeat-dense stretch whose matches weigh almost nothing (skip
