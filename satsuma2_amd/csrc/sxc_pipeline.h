@@ -153,12 +153,17 @@ template <int NB>
 struct Regs {
   float acc[Geo<NB>::SA];  // correlation accumulated over channels, stage-A output order
   // codec classes of the thread's stage-A positions (byte r % 4 of word r / 4), read from the
-  // chunk once for the four channels (class_cache).  Up to N = 8192: at larger N the eight
-  // extra live registers cost the pair kernel occupancy or spills, and each channel reads
-  // the chunk again.
-  static constexpr bool CACHE = Geo<NB>::LOG <= 13;
-  uint32_t cls[CACHE ? Geo<NB>::SA / 4 : 1];
+  // chunk once for the four channels (class_cache)
+  uint32_t cls[Geo<NB>::SA / 4];
 };
+
+// Whether the pair kernel keeps the class cache: up to N = 8192.  At larger N its eight live
+// registers cost the pair kernel occupancy or spills, and each channel reads the chunk
+// again; the target-spectra kernel, with registers to spare, always keeps it.
+template <int NB>
+constexpr bool pair_class_cache() {
+  return Geo<NB>::LOG <= 13;
+}
 
 // ---------------------------------------------------------------- helpers
 
@@ -354,25 +359,23 @@ SXC_HD inline uint32_t fresh(uint32_t x) {
 template <int NB, class Ex>
 SXC_HD inline void class_cache(Ex& ex, const SeqView& qs) {
   using G = Geo<NB>;
-  if constexpr (Regs<NB>::CACHE) {
-    ex.each([&](int g) {
-      if (g >= G::GROUPS) return;
-      Regs<NB>& rg = ex.template reg<Regs<NB>>(g);
-      static_for<0, G::SA / 4>([&](auto W) {
-        uint32_t w = 0;
-        static_for<0, 4>([&](auto B) {
-          const int p = stageA_pos<NB>(g, 4 * decltype(W)::value + decltype(B)::value);
-          if (p < qs.L) w |= (uint32_t)qs[p] << (8 * decltype(B)::value);
-        });
-        rg.cls[decltype(W)::value] = w;
+  ex.each([&](int g) {
+    if (g >= G::GROUPS) return;
+    Regs<NB>& rg = ex.template reg<Regs<NB>>(g);
+    static_for<0, G::SA / 4>([&](auto W) {
+      uint32_t w = 0;
+      static_for<0, 4>([&](auto B) {
+        const int p = stageA_pos<NB>(g, 4 * decltype(W)::value + decltype(B)::value);
+        if (p < qs.L) w |= (uint32_t)qs[p] << (8 * decltype(B)::value);
       });
+      rg.cls[decltype(W)::value] = w;
     });
-  }
+  });
 }
 
 // Forward FFT of one channel into the padded LDS array: stage A (encode fused into
-// the bit-reversed load, from the class cache where class_cache fills one, or loaded from a
-// given signal) then the rounds.
+// the bit-reversed load, from the class cache when CACHED, or loaded from a given signal)
+// then the rounds.
 template <int NB, int R, class Ex>
 SXC_HD inline void fwd_rounds(Ex& ex, float* Y, const Twiddle& tw, int upto = Schedule<NB>::NR) {
   using S = Schedule<NB>;
@@ -415,7 +418,7 @@ SXC_HD inline void inv_rounds(Ex& ex, float* Y, const Twiddle& tw) {
   }
 }
 
-template <int NB, class Ex>
+template <int NB, bool CACHED, class Ex>
 SXC_HD inline void forward_channel(Ex& ex, char* sm, const sxc_tables* tab, const Twiddle& tw, const SeqView& qs,
                                    int ch, const float* given, const TailRef& tail,
                                    int rounds = Schedule<NB>::NR) {
@@ -426,7 +429,7 @@ SXC_HD inline void forward_channel(Ex& ex, char* sm, const sxc_tables* tab, cons
     if (g >= G::GROUPS) return;
     const Regs<NB>& rg = ex.template reg<Regs<NB>>(g);
     float v[G::SA];
-    if constexpr (Regs<NB>::CACHE) {
+    if constexpr (CACHED) {
       static_for<0, G::SA / 4>([&](auto W) {
         const uint32_t w = fresh(rg.cls[decltype(W)::value]);
         static_for<0, 4>([&](auto B) {
@@ -493,7 +496,7 @@ SXC_HD inline void spectra_body(Ex& ex, char* sm, const SpecArgs& a, int k) {
   if (!a.signals) class_cache<NB>(ex, qs);
   for (int ch = 0; ch < 4; ++ch) {
     const float* given = a.signals ? a.signals + ((size_t)k * 4 + ch) * G::N : nullptr;
-    forward_channel<NB>(ex, sm, a.tab, tw, qs, ch, given, tail);
+    forward_channel<NB, true>(ex, sm, a.tab, tw, qs, ch, given, tail);
     ex.each([&](int tid) {
       static_assert(G::NT % PAD_ROW == 0, "");
       const float* yt = Y + pad(tid);  // pad(tid + k*NT) = pad(tid) + pad_step(k*NT)
@@ -522,7 +525,9 @@ SXC_HD inline void correlate(Ex& ex, char* sm, const sxc_tables* tab, const Twid
     static_for<0, G::SA>([&](auto I) { r.acc[decltype(I)::value] = 0.f; });
   });
   const float mul = (float)(1.0 / (double)G::N);  // FFTReal::rescale (FFTReal.hpp:264)
-  if (!qspec4 && !qsig4) class_cache<NB>(ex, qs);
+  if constexpr (pair_class_cache<NB>()) {
+    if (!qspec4 && !qsig4) class_cache<NB>(ex, qs);
+  }
   for (int ch = 0; ch < 4; ++ch) {
     if (qspec4) {
       // the load below writes Y with another thread mapping than the previous channel's
@@ -541,8 +546,9 @@ SXC_HD inline void correlate(Ex& ex, char* sm, const sxc_tables* tab, const Twid
       product_round<NB, false>(ex, Y, tw, T4 + (size_t)ch * G::N);
     } else {
       // all forward rounds but the last, which runs with the product and the first inverse round
-      forward_channel<NB>(ex, sm, tab, tw, qs, ch, qsig4 ? qsig4 + (size_t)ch * G::N : nullptr, tail,
-                          Schedule<NB>::NR - 1);
+      forward_channel<NB, pair_class_cache<NB>()>(ex, sm, tab, tw, qs, ch,
+                                                  qsig4 ? qsig4 + (size_t)ch * G::N : nullptr, tail,
+                                                  Schedule<NB>::NR - 1);
       product_round<NB, true>(ex, Y, tw, T4 + (size_t)ch * G::N);
     }
     inv_rounds<NB, Schedule<NB>::NR - 2>(ex, Y, tw);
