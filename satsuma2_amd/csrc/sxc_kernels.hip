@@ -14,18 +14,19 @@ namespace sxc {
 struct NoRegs {};
 
 // The pure-A/C/G/T scan kernel gets a register budget for as many of its 4-wave
-// workgroups per CU as its LDS plan admits, up to 6 waves per SIMD (measured on the 4096-pair
-// workload at N = 8192: 4 workgroups 2.28 ms, 5: 2.16 ms, 6: 2.09 ms, 7: 2.11 ms; the spills
-// the budget costs are cheaper than the latency the extra waves hide).
+// workgroups per CU as its LDS plan admits, up to 8 waves per SIMD (measured on the 4096-pair
+// workload at N = 8192 with byte-parallel flags: 5 waves 1.05 ms, 6: 1.00 ms, 7: 0.94 ms,
+// 8: 0.93 ms; the spills the budget costs are cheaper than the latency the extra waves hide).
 template <int NB>
 constexpr int scan_waves() {
   if constexpr (!ScanSmem<NB>::PURE) {
     return 1;
   } else {
     constexpr int wg = 163840 / ScanSmem<NB>::BYTES;
-    return wg < 1 ? 1 : (wg > 6 ? 6 : wg);
+    return wg < 1 ? 1 : (wg > 8 ? 8 : wg);
   }
 }
+static_assert(scan_waves<13 | 32>() == 8, "the pure scan's LDS plan at N = 8192 fits 8 workgroups per CU");
 
 template <int NB>
 __global__ __launch_bounds__(SCAN_NT) __attribute__((amdgpu_waves_per_eu(scan_waves<NB>()))) void k_scan(ScanArgs a) {

@@ -94,8 +94,9 @@ struct ScanSmem {
   static constexpr bool PURE = (NB & 96) != 0;
   static constexpr int N = 1 << LOG;
   // The segment scans' queue holds every run one lag can produce (npos / 2 + 1, npos <= N / 2);
-  // the bitmap scan flushes its queue whenever it would overflow.
-  static constexpr int QCAP = (PURE ? 1024 : N / 4 + 64);
+  // the bitmap scan flushes its queue whenever it would overflow (896 entries keep its LDS
+  // plan at N = 8192 within 20 KB, 8 workgroups per CU).
+  static constexpr int QCAP = (PURE ? 896 : N / 4 + 64);
   static constexpr int PKW = N / 16 + 4;                // 2-bit packed words (+3 words of read-ahead)
   static constexpr int TCODE = 0;                       // uchar[N] (general kernel)
   static constexpr int QCODE = TCODE + (PURE ? 0 : N);  // uchar[N] (general kernel)
