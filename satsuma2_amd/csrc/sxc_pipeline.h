@@ -123,7 +123,8 @@ struct Smem {
   static constexpr int MISC = DSUM + 32;                       // int32[16]
   static constexpr int CODEC = MISC + 64;                      // double[18][4]; DNA_X - off[X] after prepare
   static constexpr int CLS = CODEC + 8 * 4 * SXC_NCLASS;       // uchar[128] byte -> class
-  static constexpr int QUART = CLS + 128;                      // uchar[18][4]
+  static constexpr int QUART = CLS + 128;                      // uchar[18][4], read a row at a time
+  static_assert(QUART % 4 == 0, "quarter rows are read as words");
   static constexpr int RCT = QUART + 4 * SXC_NCLASS;           // uchar[128] byte -> class of its complement
   // codec classes of the strand-applied chunk bytes: an LDS copy up to N = 16384; at 32768
   // the plan would not fit the 160 KB of a workgroup, and the bytes are read from global
@@ -237,9 +238,24 @@ SXC_HD inline SeqView phase_prepare(Ex& ex, char* sm, const sxc_tables* tab, con
   float* ent = (float*)(sm + SM::ENT);
   double* dsum = (double*)(sm + SM::DSUM);
   const int win = G::N / 512;
+  // a class's four quarter counts (each <= 4) as the bytes of one word: 16 of them add
+  // without a carry between bytes
+  auto qpack = [&](unsigned c) { return *(const uint32_t*)(quart + 4 * c); };
+  auto quarter_sums = [&](int i0, int i1, int* q) {
+    for (int j0 = i0; j0 < i1; j0 += 16) {
+      const int j1 = (j0 + 16 < i1) ? j0 + 16 : i1;
+      uint32_t acc = 0;
+      for (int j = j0; j < j1; ++j) acc += qpack(qs[j]);
+      q[0] += (int)(acc & 255u);
+      q[1] += (int)((acc >> 8) & 255u);
+      q[2] += (int)((acc >> 16) & 255u);
+      q[3] += (int)(acc >> 24);
+    }
+  };
   ex.each([&](int tid) {
     if (L < 1024) return;  // entropy weight is 1 (analysis/CrossCorr.cc:39-44)
     const int nwin = (L + win - 1) / win;
+    int tq[4] = {0, 0, 0, 0};  // this thread's windows' sums: its share of the channel sums
     for (int w = tid; w < nwin; w += G::NT) {
       if (thirds) {
         ent[w] = erow[w];  // computed on the host with glibc's log
@@ -250,31 +266,31 @@ SXC_HD inline SeqView phase_prepare(Ex& ex, char* sm, const sxc_tables* tab, con
       const int i = w * win;
       const int k = (L - i < win) ? (L - i) : win;
       int q[4] = {0, 0, 0, 0};
-      for (int j = i; j < i + k; ++j) {
-        const unsigned char* qq = quart + 4 * qs[j];
-        q[0] += qq[0];
-        q[1] += qq[1];
-        q[2] += qq[2];
-        q[3] += qq[3];
-      }
+      quarter_sums(i, i + k, q);
       const double* row = ent_lut + (size_t)(k - 1) * (4 * win + 1);
       const double s = row[q[0]] + row[q[1]] + row[q[2]] + row[q[3]];
       float e = (float)(-s);
       if (e < 0.f) e = 0.f;
       ent[w] = e;
+      for (int ch = 0; ch < 4; ++ch) tq[ch] += q[ch];
     }
+    // the windows partition the chunk, so their integer sums add up to the channel sums
+    if (!thirds)
+      for (int ch = 0; ch < 4; ++ch)
+        if (tq[ch]) ex.atomic_add_shared(&misc[MISC_Q4 + ch], tq[ch]);
   });
   // channel sums: exact in quarters unless a 1/3-valued letter is present, in which case
   // the reference's sequential double order is replayed (analysis/CrossCorr.cc:108-116)
   ex.each([&](int tid) {
     if (!thirds) {
+      if (L >= 1024) return;  // summed with the entropy windows above
       int q[4] = {0, 0, 0, 0};
       for (int i = tid; i < L; i += G::NT) {
-        const unsigned char* qq = quart + 4 * qs[i];
-        q[0] += qq[0];
-        q[1] += qq[1];
-        q[2] += qq[2];
-        q[3] += qq[3];
+        const uint32_t x = qpack(qs[i]);
+        q[0] += (int)(x & 255u);
+        q[1] += (int)((x >> 8) & 255u);
+        q[2] += (int)((x >> 16) & 255u);
+        q[3] += (int)(x >> 24);
       }
       for (int ch = 0; ch < 4; ++ch)
         if (q[ch]) ex.atomic_add_shared(&misc[MISC_Q4 + ch], q[ch]);
