@@ -220,6 +220,16 @@ SXC_HD inline SeqView phase_prepare(Ex& ex, char* sm, const sxc_tables* tab, con
       codec[i] = (&tab->codec[0][0])[i];
     }
     for (int i = tid; i < 128; i += G::NT) rct[i] = tab->cls[tab->rc[i] & 127];  // class of the complement
+    if constexpr (!SM::SEQ_IN_LDS) {
+      // byte -> the four quarter counts of its class on this strand, packed in a word; held in
+      // the FFT area, which the transforms take over only after this phase
+      uint32_t* qtab = (uint32_t*)(sm + SM::Y);
+      for (int i = tid; i < 128; i += G::NT) {
+        const unsigned c = strand ? tab->cls[tab->rc[i] & 127] : tab->cls[i];
+        const unsigned char* r = tab->quarters[c];
+        qtab[i] = (uint32_t)r[0] | ((uint32_t)r[1] << 8) | ((uint32_t)r[2] << 16) | ((uint32_t)r[3] << 24);
+      }
+    }
     if constexpr (SM::SEQ_IN_LDS) {
       // classes of the strand's bytes (DNAVector::ReverseComplement, analysis/DNAVector.cc:482-521)
       unsigned char* q = (unsigned char*)(sm + SM::QSEQ);
@@ -241,11 +251,34 @@ SXC_HD inline SeqView phase_prepare(Ex& ex, char* sm, const sxc_tables* tab, con
   // a class's four quarter counts (each <= 4) as the bytes of one word: 16 of them add
   // without a carry between bytes
   auto qpack = [&](unsigned c) { return *(const uint32_t*)(quart + 4 * c); };
+  const uint32_t* qtab = (const uint32_t*)(sm + SM::Y);
   auto quarter_sums = [&](int i0, int i1, int* q) {
     for (int j0 = i0; j0 < i1; j0 += 16) {
       const int j1 = (j0 + 16 < i1) ? j0 + 16 : i1;
       uint32_t acc = 0;
-      for (int j = j0; j < j1; ++j) acc += qpack(qs[j]);
+      if constexpr (!SM::SEQ_IN_LDS) {
+        // the positions' source bytes [a, b) (reversed on strand 1; the sum is order-free),
+        // read as aligned words: chunks start 16-byte aligned and are followed by 16 bytes
+        const int a = strand ? L - j1 : j0, b = strand ? L - j0 : j1;
+        if (((a | b) & 3) == 0) {
+          for (int x = a; x < b; x += 4) {
+            const uint32_t wv = *(const uint32_t*)(src + x);
+            acc += qtab[wv & 127u] + qtab[(wv >> 8) & 127u] + qtab[(wv >> 16) & 127u] + qtab[(wv >> 24) & 127u];
+          }
+        } else {
+          for (int x = a & ~3; x < b; x += 4) {
+            const uint32_t wv = *(const uint32_t*)(src + x);
+            static_for<0, 4>([&](auto T) {
+              constexpr int t = decltype(T)::value;
+              const int y = x + t;
+              const uint32_t v = qtab[(wv >> (8 * t)) & 127u];
+              acc += (y >= a && y < b) ? v : 0u;
+            });
+          }
+        }
+      } else {
+        for (int j = j0; j < j1; ++j) acc += qpack(qs[j]);
+      }
       q[0] += (int)(acc & 255u);
       q[1] += (int)((acc >> 8) & 255u);
       q[2] += (int)((acc >> 16) & 255u);
