@@ -219,16 +219,12 @@ SXC_HD inline SeqView phase_prepare(Ex& ex, char* sm, const sxc_tables* tab, con
       quart[i] = (&tab->quarters[0][0])[i];
       codec[i] = (&tab->codec[0][0])[i];
     }
-    for (int i = tid; i < 128; i += G::NT) rct[i] = tab->cls[tab->rc[i] & 127];  // class of the complement
+    for (int i = tid; i < 128; i += G::NT) rct[i] = tab->rcls[i];  // class of the complement
     if constexpr (!SM::SEQ_IN_LDS) {
       // byte -> the four quarter counts of its class on this strand, packed in a word; held in
       // the FFT area, which the transforms take over only after this phase
       uint32_t* qtab = (uint32_t*)(sm + SM::Y);
-      for (int i = tid; i < 128; i += G::NT) {
-        const unsigned c = strand ? tab->cls[tab->rc[i] & 127] : tab->cls[i];
-        const unsigned char* r = tab->quarters[c];
-        qtab[i] = (uint32_t)r[0] | ((uint32_t)r[1] << 8) | ((uint32_t)r[2] << 16) | ((uint32_t)r[3] << 24);
-      }
+      for (int i = tid; i < 128; i += G::NT) qtab[i] = tab->qpack[strand ? 1 : 0][i];
     }
     if constexpr (SM::SEQ_IN_LDS) {
       // classes of the strand's bytes (DNAVector::ReverseComplement, analysis/DNAVector.cc:482-521)

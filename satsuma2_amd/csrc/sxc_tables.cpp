@@ -119,6 +119,13 @@ void build_tables(sxc_tables* t) {
     codec_row((char)byte, v, &rc);
     t->rc[byte] = (unsigned char)rc;
   }
+  for (int byte = 0; byte < 128; ++byte) {
+    t->rcls[byte] = t->cls[t->rc[byte] & 127];
+    for (int strand = 0; strand < 2; ++strand) {
+      const unsigned char* q = t->quarters[strand ? t->rcls[byte] : t->cls[byte]];
+      t->qpack[strand][byte] = (uint32_t)q[0] | ((uint32_t)q[1] << 8) | ((uint32_t)q[2] << 16) | ((uint32_t)q[3] << 24);
+    }
+  }
 }
 
 void build_osc(float* osc) {

@@ -90,4 +90,7 @@ typedef struct {
   double codec[SXC_NCLASS][4];  // DNA_A/C/G/T per class
   double equal[SXC_NCLASS][SXC_NCLASS];  // DNA_Equal per class pair (DNAVector.cc:390-403)
   double gc[SXC_NCLASS];        // DNA_C + DNA_G per class
+  unsigned char rcls[128];      // byte -> codec class of its complement: cls[rc[byte] & 127]
+  uint32_t qpack[2][128];       // strand, byte -> quarters of its class (strand 0) or of its
+                                // complement's (strand 1), channel k in byte k
 } sxc_tables;
