@@ -65,10 +65,10 @@ HBM_PEAK_GBS = 8000.0
 B_PAIR = 263_168 + 458_752
 # HBM bytes the pair kernel moves per chunk pair, from rocprofv3 FETCH_SIZE and WRITE_SIZE
 # counter runs on the 4096-pair launch, each scaled by what the counter reads for a plain
-# device copy of known size: (271,781 KiB x 2.0 + 4,916 KiB) / 4096 pairs
+# device copy of known size: (271,704 KiB x 2.0 + 4,916 KiB) / 4096 pairs
 # (profiles/uniform_pairs_4096_kernels.md, the build named there; kernels changed since need
 # a new counter run before this figure holds for them).
-TRAFFIC_PER_PAIR = 137_119
+TRAFFIC_PER_PAIR = 137_081
 TRAFFIC_SOURCE = "profiles/uniform_pairs_4096_kernels.md (FETCH_SIZE x 2.0 + WRITE_SIZE)"
 # offered chunk pairs/s as fractions of the batched rate (0.5: the headline); the serving loop
 # keeps query spectra resident, so its saturation lies above the batched rate
