@@ -212,18 +212,6 @@ SXC_HD inline uint32_t funnel32(uint32_t hi, uint32_t lo, int s) {
 #endif
 }
 
-SXC_HD inline uint32_t rev32(uint32_t x) {
-#if defined(__clang__)
-  return __builtin_bitreverse32(x);
-#else
-  x = ((x >> 1) & 0x55555555u) | ((x & 0x55555555u) << 1);
-  x = ((x >> 2) & 0x33333333u) | ((x & 0x33333333u) << 2);
-  x = ((x >> 4) & 0x0f0f0f0fu) | ((x & 0x0f0f0f0fu) << 4);
-  x = ((x >> 8) & 0x00ff00ffu) | ((x & 0x00ff00ffu) << 8);
-  return (x >> 16) | (x << 16);
-#endif
-}
-
 // Bytes of hi:lo chosen by the selector bytes of sel (0-3: lo's bytes, 4-7: hi's; v_perm_b32).
 SXC_HD inline uint32_t byte_perm(uint32_t hi, uint32_t lo, uint32_t sel) {
 #if defined(__HIP_DEVICE_COMPILE__)
@@ -281,6 +269,49 @@ SXC_HD inline uint32_t window_flags(uint32_t zl, uint32_t zh, uint32_t& ub) {
   const uint32_t n0 = half_flags(zl, ub);
   const uint32_t n1 = half_flags(zh, ub);
   // bytes 0 and 2 of c gather the half's positions 0-7 and 8-15
+  const uint32_t c0 = bit_select(0x0F0F0F0Fu, n0 >> 4, n0 >> 8);
+  const uint32_t c1 = bit_select(0x0F0F0F0Fu, n1 >> 4, n1 >> 8);
+  return byte_perm(c1, c0, 0x06040200u);
+}
+
+// half_flags' four u words (byte i of u[t]: u at position 4i + t) for any count that changes
+// by z - 1 per position and stays within [0, 255] at every position; ub as in half_flags.
+SXC_HD inline void half_values(uint32_t z, uint32_t& ub, uint32_t* u) {
+  const uint32_t A = z & 0x33333333u;
+  const uint32_t P = A + ((z >> 2) & 0x33333333u);
+  const uint32_t P01 = P & 0x0F0F0F0Fu;
+  const uint32_t s = P01 + ((P >> 4) & 0x0F0F0F0Fu);
+  uint32_t x = s + (s << 8);
+  opaque(x);
+  x = x + (x << 16);
+  opaque(x);
+  x <<= 8;
+  const uint32_t u0 = ub + x - 0x0C080400u;
+  u[0] = u0 + (A & 0x03030303u) - 0x01010101u;
+  u[1] = u0 + P01 - 0x02020202u;
+  u[2] = u[1] + ((A >> 4) & 0x03030303u) - 0x01010101u;
+  u[3] = u0 + s - 0x04040404u;
+  ub = byte_perm(u[3], u[3], 0x03030303u);
+}
+
+// Window flags of a 32-position block of an A/C/G/T/N pair (bit r: position r).  The window
+// holds M identical A/C/G/T bases and K positions with an N (M + K <= 46); its score
+// 100 M + 25 K exceeds 1889 iff 4 M + K >= 76.  zm* and zk* hold m(n) + 1 - m(n - 46) and
+// k(n) + 1 - k(n - 46) per position as z of half_flags (positions 0-15 in *l, 16-31 in *h);
+// ubm carries M and ubk K + 128 - 76 in every byte, advanced as ub of half_flags.  Per byte
+// 4 M + (K + 52) <= 3 M + 98 <= 236 (4 M < 256 keeps M's bits in its byte), so bit 7 of the
+// combined byte is the flag.
+SXC_HD inline uint32_t window_flags_mk(uint32_t zml, uint32_t zmh, uint32_t zkl, uint32_t zkh, uint32_t& ubm,
+                                       uint32_t& ubk) {
+  auto half = [&](uint32_t zm, uint32_t zk) {
+    uint32_t um[4], uk[4], c[4];
+    half_values(zm, ubm, um);
+    half_values(zk, ubk, uk);
+    for (int t = 0; t < 4; ++t) c[t] = (um[t] << 2) + uk[t];
+    return bit_select(0x80808080u, c[3], bit_select(0x40404040u, c[2] >> 1, bit_select(0x20202020u, c[1] >> 2, c[0] >> 3)));
+  };
+  const uint32_t n0 = half(zml, zkl);
+  const uint32_t n1 = half(zmh, zkh);
   const uint32_t c0 = bit_select(0x0F0F0F0Fu, n0 >> 4, n0 >> 8);
   const uint32_t c1 = bit_select(0x0F0F0F0Fu, n1 >> 4, n1 >> 8);
   return byte_perm(c1, c0, 0x06040200u);
@@ -458,9 +489,9 @@ SXC_HD inline void scan_epoch_bitmap(Ex& ex, char* sm, const uint16_t* cands, in
     const int k0 = na >> 5, klast = (nb - 1) >> 5;
     uint16_t* rl = (uint16_t*)(sm + SS::RLIST);
     if constexpr (SS::HASN) {
-      // D = 4 M + K - 76 over the window: M identical A/C/G/T bases (m bits), K positions
-      // with an N (k bits); the 2-bit groups of zm and zk hold m(n) + 1 - m(n - 46) and
-      // k(n) + 1 - k(n - 46), so D += 4 zm + zk - 5 per position.
+      // the window test 4 M + K >= 76 over M identical A/C/G/T bases (m bits) and K positions
+      // with an N (k bits), byte-parallel from the 2-bit groups m(n) + 1 - m(n - 46) and
+      // k(n) + 1 - k(n - 46) (window_flags_mk)
       const uint32_t* tnk = (const uint32_t*)(sm + SS::TNK);
       const uint32_t* qnk = (const uint32_t*)(sm + SS::QNK);
       auto mk_at = [&](int kb, uint64_t& m, uint64_t& kk) {
@@ -476,8 +507,10 @@ SXC_HD inline void scan_epoch_bitmap(Ex& ex, char* sm, const uint16_t* cands, in
       mk_at(k0 - 3, m3, k3);
       mk_at(k0 - 2, m2, k2);
       mk_at(k0 - 1, m1, k1);
-      int D = 4 * window_count(m1, m2, m3, 31) + window_count(k1, k2, k3, 31) - SXC_MIN_QUARTERS;  // at 32 k0 - 1
-      uint32_t prev = (na - 1 >= 46 && D >= 0) ? 1u : 0u;
+      const int M0 = window_count(m1, m2, m3, 31), K0 = window_count(k1, k2, k3, 31);  // at 32 k0 - 1
+      uint32_t prev = (na - 1 >= 46 && 4 * M0 + K0 >= SXC_MIN_QUARTERS) ? 1u : 0u;
+      uint32_t ubm = byte_perm(0u, (uint32_t)M0, 0u);
+      uint32_t ubk = byte_perm(0u, (uint32_t)(K0 + 128 - SXC_MIN_QUARTERS), 0u);
       for (int k = k0; k <= klast; ++k) {
         uint64_t m0, kk0;
         mk_at(k, m0, kk0);
@@ -486,15 +519,7 @@ SXC_HD inline void scan_epoch_bitmap(Ex& ex, char* sm, const uint16_t* cands, in
         const uint32_t zmh = (uint32_t)(m0 >> 32) + (~(uint32_t)(mm >> 32) & 0x55555555u);
         const uint32_t zkl = (uint32_t)kk0 + (~(uint32_t)km & 0x55555555u);
         const uint32_t zkh = (uint32_t)(kk0 >> 32) + (~(uint32_t)(km >> 32) & 0x55555555u);
-        uint32_t gbits = 0;
-        static_for<0, 32>([&](auto R) {
-          constexpr int r = decltype(R)::value;
-          const uint32_t zm = (r < 16) ? (zml >> (2 * r)) & 3u : (zmh >> (2 * (r - 16))) & 3u;
-          const uint32_t zk = (r < 16) ? (zkl >> (2 * r)) & 3u : (zkh >> (2 * (r - 16))) & 3u;
-          D += (int)(4 * zm + zk) - 5;
-          gbits = (gbits << 1) | ((uint32_t)D >> 31);
-        });
-        uint32_t f = ~rev32(gbits);
+        uint32_t f = window_flags_mk(zml, zmh, zkl, zkh, ubm, ubk);
         if (32 * k < 46) f &= (46 - 32 * k >= 32) ? 0u : ~((1u << (46 - 32 * k)) - 1);  // flags from n = 46
         if (k == klast && nb - 32 * k < 32) f &= (1u << (nb - 32 * k)) - 1;
         row[k] = f;

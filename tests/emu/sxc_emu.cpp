@@ -423,6 +423,52 @@ int64_t emu_check_window_flags(int64_t n, int blocks, uint64_t seed, int64_t* fl
   return bad;
 }
 
+// window_flags_mk (the byte-parallel window test of the A/C/G/T/N scan) against the window
+// scored position by position: n diagonals whose positions are identical A/C/G/T bases, N
+// positions or neither, with densities drawn per diagonal; returns the number of wrong flag
+// bits and carried values and, in *flagged, how many flags were set.
+int64_t emu_check_window_flags_mk(int64_t n, int blocks, uint64_t seed, int64_t* flagged) {
+  uint64_t x = seed * 2654435761ull + 1;
+  auto next = [&]() {
+    x ^= x << 13;
+    x ^= x >> 7;
+    x ^= x << 17;
+    return x;
+  };
+  int64_t bad = 0, set = 0;
+  std::vector<int> m(32 * blocks), kk(32 * blocks);
+  for (int64_t i = 0; i < n; ++i) {
+    const uint64_t dm = next() % 65, dk = next() % 65;  // out of 64
+    for (int p = 0; p < 32 * blocks; ++p) {
+      kk[p] = (next() % 64) < dk ? 1 : 0;
+      m[p] = !kk[p] && (next() % 64) < dm ? 1 : 0;  // a position with an N is never a match
+    }
+    uint32_t ubm = 0, ubk = sxc::byte_perm(0u, (uint32_t)(128 - sxc::SXC_MIN_QUARTERS), 0u);
+    int M = 0, K = 0;
+    for (int b = 0; b < blocks; ++b) {
+      uint32_t zm[2] = {0, 0}, zk[2] = {0, 0};
+      for (int r = 0; r < 32; ++r) {
+        const int p = 32 * b + r;
+        zm[r >> 4] |= (uint32_t)(m[p] + 1 - (p >= 46 ? m[p - 46] : 0)) << (2 * (r & 15));
+        zk[r >> 4] |= (uint32_t)(kk[p] + 1 - (p >= 46 ? kk[p - 46] : 0)) << (2 * (r & 15));
+      }
+      const uint32_t f = sxc::window_flags_mk(zm[0], zm[1], zk[0], zk[1], ubm, ubk);
+      for (int r = 0; r < 32; ++r) {
+        const int p = 32 * b + r;
+        M += m[p] - (p >= 46 ? m[p - 46] : 0);
+        K += kk[p] - (p >= 46 ? kk[p - 46] : 0);
+        const uint32_t want = 100 * M + 25 * K > SXC_WIN_THRESHOLD ? 1u : 0u;
+        bad += ((f >> r) & 1u) != want;
+        set += want;
+      }
+      bad += ubm != sxc::byte_perm(0u, (uint32_t)M, 0u);
+      bad += ubk != sxc::byte_perm(0u, (uint32_t)(K + 128 - sxc::SXC_MIN_QUARTERS), 0u);
+    }
+  }
+  *flagged = set;
+  return bad;
+}
+
 }  // extern "C"
 
 

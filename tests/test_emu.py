@@ -251,6 +251,18 @@ def test_window_flags_bytewise(emu_lib):
     assert 0.2 * 4000 * 256 < flagged.value < 0.8 * 4000 * 256
 
 
+def test_window_flags_mk_bytewise(emu_lib):
+    """The A/C/G/T/N scan tests the window (100 per identical A/C/G/T base, 25 per position
+    with an N, above 1889) four positions per byte: over 4000 diagonals of 256 positions with
+    match and N densities from 0 to 1 every flag and every carried value equals the window
+    scored position by position, and flags are neither all set nor all clear."""
+    emu_lib.emu_check_window_flags_mk.restype = C.c_int64
+    flagged = C.c_int64(0)
+    bad = emu_lib.emu_check_window_flags_mk(C.c_int64(4000), C.c_int(8), C.c_uint64(13), C.byref(flagged))
+    assert bad == 0
+    assert 0.1 * 4000 * 256 < flagged.value < 0.9 * 4000 * 256
+
+
 def test_emulated_bdhv_pairs(emu_lib, golden):
     """Chunks dense in B/D/H/V: the kernels read the window entropies the host computed with
     its log (sxc::window_entropy_row); signals (through the xc bits) and records equal the
