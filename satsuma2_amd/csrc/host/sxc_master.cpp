@@ -262,6 +262,25 @@ void chain_target(std::vector<Match>& v, std::vector<double>& rep_of, std::vecto
   std::sort(out->begin() + (std::ptrdiff_t)first, out->end(), match_less);
 }
 
+// Sorts v by key(x) (unsigned 32-bit): LSD radix in three passes of 11 bits, std::sort for
+// short vectors.  Equal keys come out in any order (the callers count).
+template <class T, class Key>
+void radix_sort(std::vector<T>& v, Key key) {
+  if (v.size() < 4096) {
+    std::sort(v.begin(), v.end(), [&](const T& a, const T& b) { return key(a) < key(b); });
+    return;
+  }
+  std::vector<T> tmp(v.size());
+  std::vector<size_t> count(2049);
+  for (int shift = 0; shift < 33; shift += 11) {
+    std::fill(count.begin(), count.end(), 0);
+    for (const T& x : v) ++count[((key(x) >> shift) & 2047u) + 1];
+    for (int i = 0; i < 2048; ++i) count[(size_t)i + 1] += count[(size_t)i];
+    for (const T& x : v) tmp[count[(key(x) >> shift) & 2047u]++] = x;
+    v.swap(tmp);
+  }
+}
+
 // Capped per-base coverage of one sequence by half-open intervals of bases.
 struct Cover {
   std::vector<int> lo, hi;  // interval starts and ends, sorted separately by finish()
@@ -275,14 +294,15 @@ struct Cover {
     }
   }
   void finish() {
-    std::sort(lo.begin(), lo.end());
-    std::sort(hi.begin(), hi.end());
+    radix_sort(lo, [](int x) { return (uint32_t)x; });  // >= 1
+    radix_sort(hi, [](int x) { return (uint32_t)x; });
   }
   // For each (position p, slot k) of q: the intervals holding p (started at or before p, not
   // ended at or before p), at most 100, into cnt[k]; a position outside the sequence reads
   // as uncovered (the reference reads past its array).  One sweep over q sorted by position.
   void lookup(std::vector<std::pair<int, int>>& q, std::vector<int>& cnt) const {
-    std::sort(q.begin(), q.end());
+    // by position, signed order
+    radix_sort(q, [](const std::pair<int, int>& e) { return (uint32_t)e.first ^ 0x80000000u; });
     size_t a = 0, b = 0;
     for (const auto& e : q) {
       const int p = e.first;
