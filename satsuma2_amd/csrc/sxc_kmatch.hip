@@ -23,7 +23,6 @@
 
 #include <chrono>
 #include <fstream>
-#include <list>
 #include <memory>
 #include <rocprim/device/device_radix_sort.hpp>
 #include <rocprim/device/device_scan.hpp>
@@ -336,42 +335,45 @@ void sweep(const std::vector<uint64_t>& q, const std::vector<int64_t>& t, const 
     bool reverse;
     int64_t length;
   };
-  std::list<Block> active;
+  // the open blocks in creation order, kept in a vector (compacted in place each step)
+  std::vector<Block> active;
   const uint64_t n = q.size();
   for (uint64_t i = 1; i <= n; ++i) {
     const bool last = i == n;
     bool used = false;
     int64_t qd = 0;
-    for (auto b = active.begin(); b != active.end();) {
+    size_t w = 0;
+    for (size_t r = 0; r < active.size(); ++r) {
+      Block b = active[r];
       if (!last) {
-        qd = (int64_t)q[i] - b->q_start;
-        const int64_t td = b->reverse ? b->t_start - t[i] : t[i] - b->t_start;
-        if (qd == td && qd - b->length <= p.max_jump && b->reverse == (rev[i] != 0)) {
-          b->length = qd;
+        qd = (int64_t)q[i] - b.q_start;
+        const int64_t td = b.reverse ? b.t_start - t[i] : t[i] - b.t_start;
+        if (qd == td && qd - b.length <= p.max_jump && b.reverse == (rev[i] != 0)) {
+          b.length = qd;
           used = true;
         }
       }
-      if (last || qd - b->length > p.max_jump) {
-        if (b->length + p.K >= p.min_length) {
-          sxc_result r{};
-          r.query_id = (uint64_t)(b->q_start / kRecordStride - 1);
-          r.target_id = (uint64_t)(b->t_start / kRecordStride - 1);
+      if (last || qd - b.length > p.max_jump) {
+        if (b.length + p.K >= p.min_length) {
+          sxc_result res{};
+          res.query_id = (uint64_t)(b.q_start / kRecordStride - 1);
+          res.target_id = (uint64_t)(b.t_start / kRecordStride - 1);
           // query_seqs[query_id] (a record skipped as empty shifts this lookup, as there)
-          r.query_size = r.query_id < qs.lengths.size() ? qs.lengths[r.query_id] : 0;
-          r.qstart = (uint64_t)(b->q_start % kRecordStride - 1);
-          r.tstart = b->reverse ? (uint64_t)((b->t_start - b->length) % kRecordStride - 1)
-                                : (uint64_t)(b->t_start % kRecordStride - 1);
-          r.len = (uint64_t)(b->length + p.K);
-          r.reverse = b->reverse ? 1 : 0;
-          r.prob = 1;
-          r.ident = 1;
-          out->push_back(r);
+          res.query_size = res.query_id < qs.lengths.size() ? qs.lengths[res.query_id] : 0;
+          res.qstart = (uint64_t)(b.q_start % kRecordStride - 1);
+          res.tstart = b.reverse ? (uint64_t)((b.t_start - b.length) % kRecordStride - 1)
+                                 : (uint64_t)(b.t_start % kRecordStride - 1);
+          res.len = (uint64_t)(b.length + p.K);
+          res.reverse = b.reverse ? 1 : 0;
+          res.prob = 1;
+          res.ident = 1;
+          out->push_back(res);
         }
-        b = active.erase(b);
       } else {
-        ++b;
+        active[w++] = b;
       }
     }
+    active.resize(w);
     if (!used && !last) active.push_back(Block{(int64_t)q[i], t[i], rev[i] != 0, 0});
   }
   (void)ts;
