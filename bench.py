@@ -340,7 +340,8 @@ def main():
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (chunk pairs shard by target chunk "
-              f"across GPUs with no collective, see DESIGN.md)", file=sys.stderr)
+              f"across GPUs with no collective: each GPU holds its targets' spectra and takes their pairs)",
+              file=sys.stderr)
         sys.exit(2)
 
     from satsuma2_amd.api import Context, HomologyByXCorr, ResidentBatch, SlaveParams
