@@ -6,6 +6,9 @@
 
 #include <algorithm>
 #include <atomic>
+#include <exception>
+#include <mutex>
+#include <system_error>
 #include <thread>
 #include <utility>
 
@@ -371,7 +374,11 @@ void chain(const MatchSet& in, MatchSet* out) {
   // target, results concatenated in target order
   std::vector<std::vector<Match>> results((size_t)nt);
   std::atomic<int> next_target{0};
-  auto worker = [&]() {
+  // the first exception a worker meets (out of memory) stops the others and is rethrown
+  // on the calling thread once every worker has joined
+  std::exception_ptr failure;
+  std::mutex failure_mu;
+  auto chain_targets = [&]() {
     std::vector<std::pair<int, int>> tq;
     std::vector<int> ct;
     std::vector<Match> nodes;
@@ -403,12 +410,29 @@ void chain(const MatchSet& in, MatchSet* out) {
       chain_target(nodes, reps, &results[(size_t)j]);
     }
   };
+  auto worker = [&]() {
+    try {
+      chain_targets();
+    } catch (...) {
+      next_target = nt;
+      std::lock_guard<std::mutex> lock(failure_mu);
+      if (!failure) failure = std::current_exception();
+    }
+  };
   int nthreads = (int)std::min<unsigned>(16u, std::max(1u, std::thread::hardware_concurrency()));
   if (nthreads > nt) nthreads = nt;
   std::vector<std::thread> pool;
-  for (int k = 1; k < nthreads; ++k) pool.emplace_back(worker);
+  pool.reserve((size_t)std::max(0, nthreads - 1));
+  for (int k = 1; k < nthreads; ++k) {
+    try {
+      pool.emplace_back(worker);
+    } catch (const std::system_error&) {
+      break;  // no more threads to be had: the ones started (and this one) take every target
+    }
+  }
   worker();
   for (std::thread& t : pool) t.join();
+  if (failure) std::rethrow_exception(failure);
   for (const std::vector<Match>& r : results) out->m.insert(out->m.end(), r.begin(), r.end());
 }
 
