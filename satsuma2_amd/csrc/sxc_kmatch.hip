@@ -206,6 +206,18 @@ __global__ __launch_bounds__(kBlock) void k_iota(uint64_t* v, int64_t n) {
   if (i < n) v[i] = (uint64_t)i;
 }
 
+// The sweep's inputs in query order: the target position and strand of each sorted entry
+// (order is a permutation of 0 .. n - 1).
+__global__ __launch_bounds__(kBlock) void k_gather(const uint64_t* order, const int64_t* t, const uint8_t* r,
+                                                   int64_t* t_out, uint8_t* r_out, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i < n) {
+    const uint64_t j = order[i];
+    t_out[i] = t[j];
+    r_out[i] = r[j];
+  }
+}
+
 inline unsigned grid_of(int64_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
 
 // Device buffers of one run, freed on return.
@@ -441,21 +453,19 @@ int run(sxc_ctx* ctx, const Side& qs, const Side& ts, const sxc_kmatch_params& p
       void* tmp;
       KCHK(B.get((char**)&tmp, (int64_t)bytes));
       KCHK(rocprim::radix_sort_pairs(tmp, bytes, jq, jq2, ix, ix2, (size_t)nj, 0, bits, s));
-      std::vector<uint64_t> order(nj);
-      std::vector<int64_t> tt(nj);
-      std::vector<uint8_t> rr(nj);
+      int64_t* gt;
+      uint8_t* gr;
+      KCHK(B.get(&gt, (int64_t)nj));
+      KCHK(B.get(&gr, (int64_t)nj));
+      hipLaunchKernelGGL(k_gather, dim3(grid_of((int64_t)nj)), dim3(kBlock), 0, s, ix2, jt, jr, gt, gr, (int64_t)nj);
+      KCHK(hipGetLastError());
       hq.resize(nj);
-      KCHK(hipMemcpyAsync(hq.data(), jq2, 8 * nj, hipMemcpyDeviceToHost, s));
-      KCHK(hipMemcpyAsync(order.data(), ix2, 8 * nj, hipMemcpyDeviceToHost, s));
-      KCHK(hipMemcpyAsync(tt.data(), jt, 8 * nj, hipMemcpyDeviceToHost, s));
-      KCHK(hipMemcpyAsync(rr.data(), jr, nj, hipMemcpyDeviceToHost, s));
-      KCHK(hipStreamSynchronize(s));
       ht.resize(nj);
       hr.resize(nj);
-      for (uint64_t i = 0; i < nj; ++i) {
-        ht[i] = tt[order[i]];
-        hr[i] = rr[order[i]];
-      }
+      KCHK(hipMemcpyAsync(hq.data(), jq2, 8 * nj, hipMemcpyDeviceToHost, s));
+      KCHK(hipMemcpyAsync(ht.data(), gt, 8 * nj, hipMemcpyDeviceToHost, s));
+      KCHK(hipMemcpyAsync(hr.data(), gr, nj, hipMemcpyDeviceToHost, s));
+      KCHK(hipStreamSynchronize(s));
     }
     KCHK(hipEventRecord(e1, s));
     KCHK(hipEventSynchronize(e1));
