@@ -14,13 +14,37 @@
 
 namespace sxc_master {
 
-bool match_less(const Match& a, const Match& b) {
+namespace {
+
+template <class A>
+bool less_fields(const A& a, const A& b) {
   if (a.target != b.target) return a.target < b.target;
   if (a.query != b.query) return a.query < b.query;
   if (a.rc != b.rc) return !a.rc;
   if (a.start_target == b.start_target) return a.length < b.length;
   return a.start_target < b.start_target;
 }
+
+// std::sort of p[0 .. n) with match_less.  std::sort's permutation follows from the outcomes
+// of its comparisons alone, so sorting small records of the compared fields and the index,
+// then moving each match once, puts equal matches in the order sorting the matches gives.
+void sort_matches(Match* p, size_t n) {
+  if (n < 2) return;
+  struct Key {
+    int32_t target, query, start_target, length;
+    bool rc;
+    int32_t idx;
+  };
+  std::vector<Key> k(n);
+  for (size_t i = 0; i < n; ++i) k[i] = Key{p[i].target, p[i].query, p[i].start_target, p[i].length, p[i].rc, (int32_t)i};
+  std::sort(k.begin(), k.end(), [](const Key& a, const Key& b) { return less_fields(a, b); });
+  std::vector<Match> tmp(p, p + n);
+  for (size_t i = 0; i < n; ++i) p[i] = tmp[(size_t)k[i].idx];
+}
+
+}  // namespace
+
+bool match_less(const Match& a, const Match& b) { return less_fields(a, b); }
 
 void MatchSet::sort() { std::sort(m.begin(), m.end(), match_less); }
 
@@ -188,26 +212,70 @@ class MinTree {
 
 }  // namespace
 
+// Appends v[path[k]] (path: descending indices of v, which is sorted by target start) to out,
+// grouped stably by (query, strand), the groups in match_less order; true iff the appended
+// run is strictly increasing under match_less (target ids included: a set that was not sorted
+// can put other targets' matches into a target's range, and then this fails).
+bool chain_in_order(const std::vector<Match>& v, const std::vector<int>& path, std::vector<Match>* out) {
+  constexpr size_t kMaxGroups = 64;
+  using Group = std::pair<int32_t, bool>;  // (query, strand)
+  auto group_less = [](const Group& a, const Group& b) {
+    return a.first != b.first ? a.first < b.first : !a.second && b.second;
+  };
+  std::vector<Group> groups;
+  Group last{-1, false};
+  for (int i : path) {
+    const Group g{v[(size_t)i].query, v[(size_t)i].rc};
+    if (!groups.empty() && g == last) continue;
+    last = g;
+    if (std::find(groups.begin(), groups.end(), g) == groups.end()) {
+      if (groups.size() == kMaxGroups) return false;
+      groups.push_back(g);
+    }
+  }
+  std::sort(groups.begin(), groups.end(), group_less);
+  // counting scatter by group, each group's members in ascending index order
+  const size_t n = path.size();
+  std::vector<uint8_t> gid(n);
+  std::vector<size_t> slot(groups.size() + 1, 0);
+  for (size_t k = 0; k < n; ++k) {
+    const Match& x = v[(size_t)path[k]];
+    const Group g{x.query, x.rc};
+    gid[k] = (uint8_t)(std::lower_bound(groups.begin(), groups.end(), g, group_less) - groups.begin());
+    ++slot[(size_t)gid[k] + 1];
+  }
+  for (size_t g = 1; g <= groups.size(); ++g) slot[g] += slot[g - 1];
+  const size_t first = out->size();
+  out->resize(first + n);
+  Match* o = out->data() + first;
+  for (size_t k = n; k-- > 0;) o[slot[gid[k]]++] = v[(size_t)path[k]];
+  for (size_t k = 1; k < n; ++k)
+    if (!match_less(o[k - 1], o[k])) return false;
+  return true;
+}
+
 void chain_target(std::vector<Match>& v, std::vector<double>& rep_of, std::vector<Match>* out) {
   constexpr int kLookAheadDist = 250000;  // m_laDist (:173)
   constexpr int kLookAheadCount = 2000;   // la_limit (:200)
   constexpr int kNear = 4;                // sources evaluated for the bound
   const int n = (int)v.size();
   if (n == 0) return;
-  // the reference sorts its match vector by target start with std::sort; sorting the same
-  // records the same way gives the same order among equal starts
+  // the reference sorts its match vector by target start with std::sort; std::sort of
+  // (start, index) pairs by start makes the same comparisons, so equal starts come out in
+  // the same order
   {
-    struct Rec {
-      Match m;
-      double rep;
-    };
-    std::vector<Rec> r((size_t)n);
-    for (int i = 0; i < n; ++i) r[(size_t)i] = Rec{v[(size_t)i], rep_of[(size_t)i]};
-    std::sort(r.begin(), r.end(), [](const Rec& a, const Rec& b) { return a.m.start_target < b.m.start_target; });
+    std::vector<std::pair<int32_t, int32_t>> ord((size_t)n);
+    for (int i = 0; i < n; ++i) ord[(size_t)i] = {v[(size_t)i].start_target, i};
+    std::sort(ord.begin(), ord.end(),
+              [](const std::pair<int32_t, int32_t>& a, const std::pair<int32_t, int32_t>& b) { return a.first < b.first; });
+    std::vector<Match> sv((size_t)n);
+    std::vector<double> sr((size_t)n);
     for (int i = 0; i < n; ++i) {
-      v[(size_t)i] = r[(size_t)i].m;
-      rep_of[(size_t)i] = r[(size_t)i].rep;
+      sv[(size_t)i] = v[(size_t)ord[(size_t)i].second];
+      sr[(size_t)i] = rep_of[(size_t)ord[(size_t)i].second];
     }
+    v.swap(sv);
+    rep_of.swap(sr);
   }
   std::vector<int> back((size_t)n, -1), next((size_t)n);
   std::vector<double> total((size_t)n), skip((size_t)n, 0.), pre((size_t)n);
@@ -260,9 +328,20 @@ void chain_target(std::vector<Match>& v, std::vector<double>& rep_of, std::vecto
     total[(size_t)j] = best + match_penalty(two.prob);
     settle(j);
   }
+  // The chain, traced back from the last match, sorted with match_less.  Traced back it runs
+  // through descending indices, i.e. by target start; grouped stably by (query, strand) in
+  // ascending (query, strand) order, it is sorted with match_less unless two neighbours in a
+  // group have equal target starts.  When the result is strictly increasing under match_less
+  // it is the only sorted order, the one std::sort gives; otherwise std::sort of the chain in
+  // trace order decides, as in the reference.
+  std::vector<int> path;
+  for (int i = n - 1; i >= 0; i = back[(size_t)i]) path.push_back(i);
   const size_t first = out->size();
-  for (int i = n - 1; i >= 0; i = back[(size_t)i]) out->push_back(v[(size_t)i]);
-  std::sort(out->begin() + (std::ptrdiff_t)first, out->end(), match_less);
+  if (!chain_in_order(v, path, out)) {
+    out->resize(first);
+    for (int i : path) out->push_back(v[(size_t)i]);
+    sort_matches(out->data() + first, out->size() - first);
+  }
 }
 
 // Sorts v by key(x) (unsigned 32-bit): LSD radix in three passes of 11 bits, std::sort for

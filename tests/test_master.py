@@ -291,6 +291,49 @@ def test_chaining_at_look_ahead_limits_matches_reference(tmp_path, op):
     assert len(want) > 1000 and same(got, want)
 
 
+def tied_set(seed: int):
+    """A collinear run in which every 7th match has a twin: same target, query, strand, target
+    start and length, query start one base on.  Twins tie under SingleMatch::operator<, so a
+    chain holding both is put in order by std::sort's own tie order."""
+    rng = np.random.default_rng(seed)
+    tsizes = np.array([3_000_000], np.int32)
+    qsizes = np.array([3_000_000, 2_000_000], np.int32)
+    rows = []
+    pos = 1000
+    for i in range(3000):
+        q = 1 if rng.random() < 0.05 else 0
+        rc = 1 if rng.random() < 0.1 else 0
+        ln = int(rng.integers(60, 240))
+        sq = min(int(qsizes[q]) - 300, pos + int(rng.integers(-4, 5)))
+        rows.append((0, q, qsizes[q], pos, sq, ln, rc, float(rng.uniform(0.99, 1.0)), 0.9))
+        if i % 7 == 0:
+            rows.append((0, q, qsizes[q], pos, sq + 1, ln, rc, float(rng.uniform(0.99, 1.0)), 0.9))
+        pos += int(rng.integers(200, 800))
+    m = np.zeros(len(rows), MATCH_DTYPE)
+    for i, r in enumerate(rows):
+        m[i]["target_id"], m[i]["query_id"], m[i]["query_len"], m[i]["start_target"], m[i]["start_query"] = r[:5]
+        m[i]["length"], m[i]["rc"], m[i]["prob"], m[i]["ident"] = r[5:]
+        m[i]["matches"] = r[8] * r[5]
+    rng.shuffle(m)
+    return tsizes, qsizes, m
+
+
+@need_ref
+@pytest.mark.parametrize("op", ["chain", "chainmult"])
+def test_chaining_with_ties_matches_reference(tmp_path, op):
+    """The chain is put in SingleMatch::operator< order by grouping when that order is strict,
+    and by std::sort otherwise: a chain holding tied matches, against the reference."""
+    ts, qs, m = tied_set(41)
+    m = MatchSet(ts, qs).add(m).sort().matches()  # sorted, not collapsed: the twins stay
+    p = tmp_path / "tied.bin"
+    p.write_bytes(set_bytes(ts, qs, m))
+    _, _, want, _ = parse_set(run_ref(tmp_path, op, p))
+    key = ["target_id", "query_id", "rc", "start_target", "length"]
+    ties = np.all([want[k][1:] == want[k][:-1] for k in key], axis=0)
+    assert len(want) > 1000 and ties.sum() > 10  # the reference's own chain holds ties
+    assert same(mine(op, ts, qs, m), want)
+
+
 @need_ref
 def test_interpolation_adds_guesses(tmp_path):
     """A chained set with a run of >= 40 collinear matches gets two guesses on each side."""
