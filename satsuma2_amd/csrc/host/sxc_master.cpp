@@ -53,11 +53,14 @@ void MatchSet::sort() { std::sort(m.begin(), m.end(), match_less); }
 // member's target start and spans the members' query extents.  The query ids are not
 // compared.  The group still open when the list ends is not emitted, so a set of one match
 // collapses to none (analysis/SequenceMatch.cc:418-469).
+//
+// In place: before step i at most i - 1 groups were written, to slots below i - 1, so m[i - 1]
+// and m[i] are read before step i's write (to a slot below i) can reach m[i - 1].
 void MatchSet::collapse() {
-  std::vector<Match> out;
   if (m.empty()) return;
   auto near = [](int a, int b) { return a - b < 4 && b - a < 4; };  // Laps (:408-416)
   Match cur = m[0];
+  size_t w = 0;
   for (size_t i = 1; i < m.size(); ++i) {
     const Match& a = m[i - 1];
     const Match& b = m[i];
@@ -70,11 +73,12 @@ void MatchSet::collapse() {
         cur.length = b.start_query + b.length - cur.start_query;
       }
     } else {
-      out.push_back(cur);
-      cur = b;
+      const Match next = b;
+      m[w++] = cur;
+      cur = next;
     }
   }
-  m.swap(out);
+  m.resize(w);
 }
 
 void MatchSet::length_filter(uint64_t min_length) {
