@@ -82,11 +82,10 @@ void MatchSet::collapse() {
 }
 
 void MatchSet::length_filter(uint64_t min_length) {
-  std::vector<Match> out;
-  out.reserve(m.size());
-  for (const Match& x : m)
-    if ((uint64_t)(int64_t)x.length >= min_length) out.push_back(x);
-  m.swap(out);
+  size_t w = 0;  // in place, order kept
+  for (size_t i = 0; i < m.size(); ++i)
+    if ((uint64_t)(int64_t)m[i].length >= min_length) m[w++] = m[i];
+  m.resize(w);
 }
 
 int64_t MatchSet::collect(const Result* r, int64_t n, bool from_file) {
