@@ -37,7 +37,8 @@ using sxc::fail;
 namespace {
 
 constexpr int64_t kRecordStride = 10000000000LL;  // KMATCH_POSITION_CHR_CNST (KMatch.h:18)
-constexpr int kTile = 64;                          // positions per thread
+constexpr int kSpan = 16;                          // positions per lane
+constexpr int kChunk = 64 * kSpan;                 // positions per chunk (one wave)
 constexpr int kBlock = 256;
 
 // One genome's sequences as the table builder walks them.
@@ -96,50 +97,75 @@ __device__ inline int base2(unsigned char c) {
   }
 }
 
-// Canonical k-mers of positions [64 t, 64 t + 64): pass 0 counts them per thread, pass 1
-// writes them from base[t] on.  off/index: sequence starts and indices (binary search).
+// Canonical k-mers of chunk c, positions [1024 c, 1024 c + 1024): one wave per chunk, each lane
+// rolling its window over 16 positions.  Pass 0 counts a chunk's k-mers; pass 1 puts each
+// lane's k-mers into its 16 LDS slots and the wave then copies the chunk's k-mers, in position
+// order, to base[c] on as one contiguous run.  off/index: sequence starts and indices (binary
+// search).
 template <bool WRITE>
-__global__ __launch_bounds__(kBlock) void k_kmers(const unsigned char* buf, int64_t n, int K, uint64_t* count,
-                                                  const uint64_t* base, const int64_t* off, const int64_t* index,
-                                                  int nseq, uint64_t* keys, int64_t* vals) {
-  const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  const int64_t start = t * kTile;
-  if (start >= n) return;
-  const int64_t end = start + kTile < n ? start + kTile : n;
-  const uint64_t mask = (1ull << (2 * K)) - 1;
-  const int top = 2 * (K - 1);
-  const int64_t s0 = start - (K - 1) > 0 ? start - (K - 1) : 0;
-  int64_t last_unknown = s0 - 1;
-  uint64_t f = 0, r = 0, c = 0;
-  uint64_t at = WRITE ? base[t] : 0;
-  for (int64_t p = s0; p < end; ++p) {
-    const int b = base2(buf[p]);
-    const uint64_t code = b < 0 ? 0 : (uint64_t)b;  // any other letter acts as A and breaks the window
-    f = ((f << 2) + code) & mask;
-    r = (r >> 2) + ((3 - code) << top);
-    if (b < 0) last_unknown = p;
-    if (p >= start && last_unknown + K <= p) {
-      if (WRITE) {
-        int lo = 0, hi = nseq - 1;  // the last sequence starting at or before p
-        while (lo < hi) {
-          const int mid = (lo + hi + 1) >> 1;
-          if (off[mid] <= p) lo = mid; else hi = mid - 1;
+__global__ __launch_bounds__(kBlock) void k_kmers(const unsigned char* buf, int64_t n, int K, int64_t chunks,
+                                                  uint64_t* count, const uint64_t* base, const int64_t* off,
+                                                  const int64_t* index, int nseq, uint64_t* keys, int64_t* vals) {
+  constexpr int kWaves = kBlock / 64;
+  static_assert(kBlock % 64 == 0, "");
+  __shared__ uint64_t skey[WRITE ? kWaves * kChunk : 1];
+  __shared__ int64_t sval[WRITE ? kWaves * kChunk : 1];
+  __shared__ int pre[kWaves][65];
+  const int wave = (int)threadIdx.x / 64, lane = (int)threadIdx.x % 64;
+  const int64_t c = (int64_t)blockIdx.x * kWaves + wave;
+  const bool active = c < chunks;  // every wave reaches the barriers
+  int got = 0;
+  if (active) {
+    const int64_t start = c * kChunk + (int64_t)lane * kSpan;
+    const int64_t end = start + kSpan < n ? start + kSpan : n;
+    const uint64_t mask = (1ull << (2 * K)) - 1;
+    const int top = 2 * (K - 1);
+    const int64_t s0 = start - (K - 1) > 0 ? start - (K - 1) : 0;
+    int64_t last_unknown = s0 - 1;
+    uint64_t f = 0, r = 0;
+    for (int64_t p = s0; p < end; ++p) {
+      const int b = base2(buf[p]);
+      const uint64_t code = b < 0 ? 0 : (uint64_t)b;  // any other letter acts as A and breaks the window
+      f = ((f << 2) + code) & mask;
+      r = (r >> 2) + ((3 - code) << top);
+      if (b < 0) last_unknown = p;
+      if (p >= start && last_unknown + K <= p) {
+        if (WRITE) {
+          int lo = 0, hi = nseq - 1;  // the last sequence starting at or before p
+          while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (off[mid] <= p) lo = mid; else hi = mid - 1;
+          }
+          const int64_t pos = (p - off[lo]) - K + 2 + index[lo] * kRecordStride;  // 1-based start
+          const int slot = wave * kChunk + lane * kSpan + got;
+          skey[slot] = f <= r ? f : r;
+          sval[slot] = f <= r ? pos : -pos;
         }
-        const int64_t pos = (p - off[lo]) - K + 2 + index[lo] * kRecordStride;  // 1-based start
-        if (f <= r) {
-          keys[at] = f;
-          vals[at] = pos;
-        } else {
-          keys[at] = r;
-          vals[at] = -pos;
-        }
-        ++at;
-      } else {
-        ++c;
+        ++got;
       }
     }
   }
-  if (!WRITE) count[t] = c;
+  pre[wave][lane + 1] = got;
+  __syncthreads();
+  if (lane == 0) {
+    pre[wave][0] = 0;
+    for (int l = 1; l <= 64; ++l) pre[wave][l] += pre[wave][l - 1];
+    if (!WRITE && active) count[c] = (uint64_t)pre[wave][64];
+  }
+  __syncthreads();
+  if (!WRITE || !active) return;
+  const int total = pre[wave][64];
+  const uint64_t at = base[c];
+  for (int i = lane; i < total; i += 64) {
+    int lo = 0, hi = 63;  // the lane whose k-mers hold entry i: pre[lo] <= i < pre[lo + 1]
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (pre[wave][mid] <= i) lo = mid; else hi = mid - 1;
+    }
+    const int slot = wave * kChunk + lo * kSpan + (i - pre[wave][lo]);
+    keys[at + (uint64_t)i] = skey[slot];
+    vals[at + (uint64_t)i] = sval[slot];
+  }
 }
 
 // keep[i] = 1 unless entry i lies in a run of more than F equal k-mers.
@@ -279,15 +305,15 @@ int build_table(Buffers& B, const Side& side, int K, int F, hipStream_t s, uint6
   KCHK(hipMemcpyAsync(dbuf, side.buf.data(), (size_t)len, hipMemcpyHostToDevice, s));
   KCHK(hipMemcpyAsync(doff, side.off.data(), sizeof(int64_t) * nseq, hipMemcpyHostToDevice, s));
   KCHK(hipMemcpyAsync(didx, side.index.data(), sizeof(int64_t) * nseq, hipMemcpyHostToDevice, s));
-  const int64_t threads = (len + kTile - 1) / kTile;
+  const int64_t chunks = (len + kChunk - 1) / kChunk;
   uint64_t *cnt, *base;
-  KCHK(B.get(&cnt, threads));
-  KCHK(B.get(&base, threads));
-  hipLaunchKernelGGL(k_kmers<false>, dim3(grid_of(threads)), dim3(kBlock), 0, s, dbuf, len, K, cnt, nullptr, doff,
-                     didx, nseq, nullptr, nullptr);
+  KCHK(B.get(&cnt, chunks));
+  KCHK(B.get(&base, chunks));
+  hipLaunchKernelGGL(k_kmers<false>, dim3(grid_of(chunks * 64)), dim3(kBlock), 0, s, dbuf, len, K, chunks, cnt,
+                     nullptr, doff, didx, nseq, nullptr, nullptr);
   KCHK(hipGetLastError());
   uint64_t filled = 0;
-  int rc = scan(B, cnt, base, threads, &filled, s);
+  int rc = scan(B, cnt, base, chunks, &filled, s);
   if (rc != SXC_OK) return rc;
   if ((int64_t)filled > slots) return fail(SXC_ERR_ARG, "k-mer table overflow");  // cannot happen: one per slot
   uint64_t *k0, *k1;
@@ -299,8 +325,8 @@ int build_table(Buffers& B, const Side& side, int K, int F, hipStream_t s, uint6
   // slots no k-mer fills stay {0, 0}
   KCHK(hipMemsetAsync(k0, 0, sizeof(uint64_t) * slots, s));
   KCHK(hipMemsetAsync(v0, 0, sizeof(int64_t) * slots, s));
-  hipLaunchKernelGGL(k_kmers<true>, dim3(grid_of(threads)), dim3(kBlock), 0, s, dbuf, len, K, nullptr, base, doff,
-                     didx, nseq, k0, v0);
+  hipLaunchKernelGGL(k_kmers<true>, dim3(grid_of(chunks * 64)), dim3(kBlock), 0, s, dbuf, len, K, chunks, nullptr,
+                     base, doff, didx, nseq, k0, v0);
   KCHK(hipGetLastError());
   size_t bytes = 0;
   KCHK(rocprim::radix_sort_pairs(nullptr, bytes, k0, k1, v0, v1, (size_t)slots, 0, 2 * K, s));
